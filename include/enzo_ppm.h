@@ -1,0 +1,250 @@
+/*
+ * enzo_ppm.h -- C ABI of the MI355X PPM "direct Eulerian" hydro step.
+ *
+ * Drop-in for the GPU switch inside grid::SolvePPM_DE
+ * (src/enzo/Grid_SolvePPM_DE.C:63-82,99-102,118-121,137-141,148-152), which in
+ * the reference calls the cuPPM* lifecycle declared in src/enzo/cuPPM.h:53-102.
+ * This header exports the same lifecycle for the default precision-64 /
+ * integers-64 build (`float` = double, `int` = int64_t:
+ * src/enzo/macros_and_parameters.h:216-217,257), with these additions:
+ *   - every call returns a status (ENZO_PPM_SUCCESS, or a negative error
+ *     code; the reference calls exit(1) or aborts),
+ *   - an explicit hipStream_t,
+ *   - CycleNumber drives the x/y/z sweep rotation (Grid_SolvePPM_DE.C:53,86),
+ *   - per-cell CellWidthTemp arrays (the reference cuPPM takes one scalar dx
+ *     per axis; the Fortran takes dx(idim) arrays),
+ *   - SmallRho (the density floor euler.F applies, Grid_xEulerSweep.C:289-296),
+ *   - the subgrid face list is registered before the sweeps, so that face
+ *     fluxes are captured inside the sweep kernel instead of from full flux
+ *     arrays (cuPPM.cu:431-497),
+ *   - a one-call entry with grid::SolvePPM_DE's argument list
+ *     (Grid.h:1867-1871), and device-resident entries for multi-step runs.
+ * No symbol takes a torch, HIP-runtime-owned or C++ type other than
+ * hipStream_t (passed as void*).
+ */
+#ifndef ENZO_PPM_H
+#define ENZO_PPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int64_t enzo_int;      /* `int` with integers-64 (macros_and_parameters.h:216-217) */
+typedef double enzo_float;     /* `float` with precision-64 (macros_and_parameters.h:257) */
+typedef int64_t enzo_long_int; /* long_int (macros_and_parameters.h:202-217) */
+
+#define ENZO_PPM_MAX_DIMENSION 3                 /* MAX_DIMENSION (macros_and_parameters.h:113) */
+#ifndef ENZO_PPM_MAX_NUMBER_OF_BARYON_FIELDS
+#define ENZO_PPM_MAX_NUMBER_OF_BARYON_FIELDS 30  /* PARAMETER_MAX_BARYONS (Make.config.settings:50) */
+#endif
+#define ENZO_PPM_MAX_COLOR 45                    /* MAX_COLOR (fortran.def:2) */
+#define ENZO_PPM_MAX_ANY_SINGLE_DIRECTION 1031   /* fortran.def:1 */
+
+/* status codes: SUCCESS mirrors macros_and_parameters.h:348-349 */
+#define ENZO_PPM_SUCCESS 1
+#define ENZO_PPM_ERR_ARGUMENT (-1)      /* bad sizes, indices or unsupported option */
+#define ENZO_PPM_ERR_HIP (-2)           /* a HIP runtime call failed */
+#define ENZO_PPM_ERR_NEGATIVE (-3)      /* negative density/energy with fallback off
+                                           (flux_twoshock.F:438-442) or after HLL
+                                           (flux_hll.F:336-384) */
+#define ENZO_PPM_ERR_NOT_SUPPORTED (-4) /* option the reference rejects on this path */
+#define ENZO_PPM_ERR_NO_DEVICE (-5)
+
+/* Riemann solver and reconstruction enums (typedefs.h) used on this path */
+#define ENZO_PPM_RIEMANN_TWOSHOCK 0 /* TwoShock: ReadParameterFile.C:1574-1578 */
+#define ENZO_PPM_RECONSTRUCTION_PPM 1
+
+/* struct fluxes (src/enzo/Fluxes.h:29-37), identical layout */
+typedef struct enzo_fluxes {
+  enzo_long_int LeftFluxStartGlobalIndex[ENZO_PPM_MAX_DIMENSION][ENZO_PPM_MAX_DIMENSION];
+  enzo_long_int LeftFluxEndGlobalIndex[ENZO_PPM_MAX_DIMENSION][ENZO_PPM_MAX_DIMENSION];
+  enzo_long_int RightFluxStartGlobalIndex[ENZO_PPM_MAX_DIMENSION][ENZO_PPM_MAX_DIMENSION];
+  enzo_long_int RightFluxEndGlobalIndex[ENZO_PPM_MAX_DIMENSION][ENZO_PPM_MAX_DIMENSION];
+  enzo_float *LeftFluxes[ENZO_PPM_MAX_NUMBER_OF_BARYON_FIELDS][ENZO_PPM_MAX_DIMENSION];
+  enzo_float *RightFluxes[ENZO_PPM_MAX_NUMBER_OF_BARYON_FIELDS][ENZO_PPM_MAX_DIMENSION];
+} enzo_fluxes;
+
+/* Hydro globals read on the path (global_data.h; defaults in
+   SetDefaultGlobalValues.C:57,181-195,424-426,803-813). */
+typedef struct enzo_ppm_globals {
+  enzo_float Gamma;
+  enzo_int PPMDiffusionParameter;
+  enzo_int PPMFlatteningParameter;
+  enzo_int PPMSteepeningParameter;
+  enzo_int DualEnergyFormalism;
+  enzo_float DualEnergyFormalismEta1;
+  enzo_float DualEnergyFormalismEta2;
+  enzo_int PressureFree;
+  enzo_float tiny_number; /* MinimumPressure in the sweeps */
+  enzo_int RiemannSolver;
+  enzo_int RiemannSolverFallback;
+  enzo_int ReconstructionMethod;
+  enzo_int ConservativeReconstruction;
+  enzo_int PositiveReconstruction;
+  enzo_float SmallRho;
+  enzo_int EOSType;
+} enzo_ppm_globals;
+
+/* Grid state grid::SolvePPM_DE reads (Grid.h members). */
+typedef struct enzo_ppm_grid {
+  enzo_int GridRank;
+  enzo_int GridDimension[ENZO_PPM_MAX_DIMENSION];
+  enzo_int GridStartIndex[ENZO_PPM_MAX_DIMENSION]; /* 0-based, inclusive */
+  enzo_int GridEndIndex[ENZO_PPM_MAX_DIMENSION];
+  enzo_float **BaryonField;       /* MAX_NUMBER_OF_BARYON_FIELDS pointers, x fastest */
+  enzo_float **AccelerationField; /* [3], or NULL without gravity */
+  enzo_int DensNum, GENum, Vel1Num, Vel2Num, Vel3Num, TENum;
+  enzo_float dtFixed;
+} enzo_ppm_grid;
+
+/* cuPPMParameter (cuPPM.h:24-50), FP64/int64, plus the additions above. */
+typedef struct hipPPMParameter {
+  enzo_int DensNum, TENum, Vel1Num, Vel2Num, Vel3Num, GENum;
+  enzo_int MaxNumberOfBaryonFields;
+  enzo_long_int GridGlobalStart[3];
+  enzo_int GridStartIndex[3];
+  enzo_int GridEndIndex[3];
+  enzo_int GridDimension[3];
+  enzo_int GridRank;
+  enzo_float dx[3];                /* uniform widths (cuPPM.cu semantics) */
+  const enzo_float *CellWidth[3];  /* optional per-cell widths; NULL = uniform dx */
+  enzo_float Gamma;
+  enzo_int PPMDiffusionParameter;
+  enzo_int PPMFlatteningParameter;
+  enzo_int PPMSteepeningParameter;
+  enzo_int ConservativeReconstruction;
+  enzo_int PositiveReconstruction;
+  enzo_int GravityOn;
+  enzo_int DualEnergyFormalism;
+  enzo_float DualEnergyFormalismEta1;
+  enzo_float DualEnergyFormalismEta2;
+  enzo_int PressureFree;
+  enzo_float MinimumPressure;
+  enzo_int RiemannSolver;
+  enzo_int RiemannSolverFallback;
+  enzo_float SmallRho;
+  enzo_int NumberOfColours;
+  enzo_int colnum[ENZO_PPM_MAX_COLOR];
+} hipPPMParameter;
+
+/* Opaque device state (cuPPMData, cuPPM.h:9-22).  Buffers stay resident
+   between calls and are reused while the grid shape is unchanged. */
+typedef struct hipPPMData hipPPMData;
+
+/* ---- lifecycle mirroring cuPPM.h:53-102 ---------------------------------- */
+
+/* cuPPMInitParameter (cuPPM.h:53-79); adds GridRank and SmallRho. */
+int hipPPMInitParameter(hipPPMParameter *Para, enzo_int DensNum, enzo_int TENum,
+                        enzo_int Vel1Num, enzo_int Vel2Num, enzo_int Vel3Num,
+                        enzo_int GENum, enzo_int MaxNumberOfBaryonFields,
+                        const enzo_long_int GridGlobalStart[],
+                        const enzo_int GridStartIndex[],
+                        const enzo_int GridEndIndex[],
+                        const enzo_int GridDimension[], enzo_float dxx,
+                        enzo_float dxy, enzo_float dxz, enzo_float Gamma,
+                        enzo_int PPMDiffusionParameter,
+                        enzo_int PPMFlatteningParameter,
+                        enzo_int PPMSteepeningParameter,
+                        enzo_int ConservativeReconstruction,
+                        enzo_int PositiveReconstruction, enzo_int GravityOn,
+                        enzo_int DualEnergyFormalism,
+                        enzo_float DualEnergyFormalismEta1,
+                        enzo_float DualEnergyFormalismEta2,
+                        enzo_int PressureFree, enzo_float MinimumPressure,
+                        enzo_int RiemannSolver, enzo_int RiemannSolverFallback,
+                        enzo_int NumberOfColours, const enzo_int *colnum,
+                        enzo_int GridRank, enzo_float SmallRho);
+
+/* Per-cell widths CellWidthTemp[dim][0..GridDimension[dim]-1]
+   (Grid_SolveHydroEquations.C:401-409).  The array is copied at InitData. */
+int hipPPMSetCellWidth(hipPPMParameter *Para, enzo_int dim, const enzo_float *CellWidthTemp);
+
+/* cuPPMInitData (cuPPM.h:81-83).  *Data may hold a previous handle, which is
+   reused when the shape fits.  stream: hipStream_t or NULL (default). */
+int hipPPMInitData(hipPPMData **Data, const hipPPMParameter *Para, void *stream);
+
+/* cuPPMSetBaryon (cuPPM.h:88-92): host -> device.  Pressure may be NULL, in
+   which case the stale pressure of Grid_SolvePPM_DE.C:58-61 is computed on the
+   device (Grid_ComputePressure.C:96-166) with MinimumSupportEnergyCoefficient
+   0.  AccelerationField is read only when GravityOn. */
+int hipPPMSetBaryon(hipPPMData *Data, const hipPPMParameter *Para,
+                    enzo_float *BaryonField[], enzo_float *Pressure,
+                    enzo_float *AccelerationField[]);
+
+/* Face list of the subgrids whose fluxes the sweeps capture
+   (Grid_xEulerSweep.C:313-376).  Must precede hipPPMSweep for the fluxes to
+   be recorded; NumberOfSubgrids = 0 disables capture. */
+int hipPPMSetSubgridFluxes(hipPPMData *Data, const hipPPMParameter *Para,
+                           enzo_fluxes *SubgridFluxes[], enzo_int NumberOfSubgrids,
+                           const enzo_long_int GridGlobalStart[]);
+
+/* cuPPMSweep (cuPPM.h:98-99): one directional sweep, dir 0/1/2 = x/y/z. */
+int hipPPMSweep(hipPPMData *Data, const hipPPMParameter *Para, enzo_float dtFixed,
+                enzo_int dir);
+
+/* cuPPMSaveSubgridFluxes (cuPPM.h:101-103): copy the fluxes captured in the
+   last sweep along dir into the SubgridFluxes arrays registered above. */
+int hipPPMSaveSubgridFluxes(enzo_fluxes *SubgridFluxes[], enzo_int NumberOfSubgrids,
+                            const enzo_long_int GridGlobalStart[], hipPPMData *Data,
+                            const hipPPMParameter *Para, enzo_int dir);
+
+/* cuPPMGetBaryon (cuPPM.h:94-96): device -> host. */
+int hipPPMGetBaryon(hipPPMData *Data, const hipPPMParameter *Para,
+                    enzo_float *BaryonField[]);
+
+/* cuPPMDestroy (cuPPM.h:85-86). */
+int hipPPMDestroy(hipPPMData *Data);
+
+/* Number of cells that took the per-cell HLL fallback
+   (flux_twoshock.F:401-456) since the last SetBaryon; -1 on error. */
+enzo_long_int hipPPMFallbackCount(hipPPMData *Data);
+
+/* ---- one call with grid::SolvePPM_DE's argument list --------------------- */
+
+/* grid::SolvePPM_DE (Grid_SolvePPM_DE.C:36-164, Grid.h:1867-1871): the
+   pressure once, the cycle-rotated sweeps, in-place update of BaryonField and
+   the SubgridFluxes fill.  Data may be NULL (a temporary handle is used). */
+int hipPPMSolvePPM_DE(const enzo_ppm_grid *grid, const enzo_ppm_globals *globals,
+                      enzo_int CycleNumber, enzo_int NumberOfSubgrids,
+                      enzo_fluxes *SubgridFluxes[], enzo_float *CellWidthTemp[],
+                      const enzo_long_int GridGlobalStart[], enzo_int GravityOn,
+                      enzo_int NumberOfColours, const enzo_int colnum[],
+                      enzo_float MinimumSupportEnergyCoefficient,
+                      hipPPMData **Data, void *stream);
+
+/* ---- device-resident multi-step use --------------------------------------- */
+
+/* Device pointer of BaryonField[field] (current buffer) or, with field = -1,
+   of the pressure array; NULL on error.  Valid until the next sweep. */
+enzo_float *hipPPMDeviceField(hipPPMData *Data, enzo_int field);
+
+/* Fused pressure + Courant pass over the device-resident state:
+   p = ComputePressure (Grid_ComputePressure.C:96-166) into the pressure
+   array, and dt = CourantSafetyNumber * calc_dt (calc_dt.F:129-177,
+   Grid_ComputeTimeStep.C:162-209) left on the device for hipPPMStep.
+   dt_host, when non-NULL, receives dt (this synchronises the stream). */
+int hipPPMPressureAndTimeStep(hipPPMData *Data, const hipPPMParameter *Para,
+                              enzo_float CourantSafetyNumber,
+                              enzo_float MinimumSupportEnergyCoefficient,
+                              enzo_float *dt_host);
+
+/* One full step on the device-resident state: the cycle-rotated x/y/z sweeps
+   with the dt left by hipPPMPressureAndTimeStep (dtFixed <= 0) or dtFixed.
+   No host synchronisation. */
+int hipPPMStep(hipPPMData *Data, const hipPPMParameter *Para, enzo_int CycleNumber,
+               enzo_float dtFixed);
+
+/* Periodic ghost-zone fill of every baryon field on the device
+   (the periodic case of ExternalBoundary::SetExternalBoundary). */
+int hipPPMPeriodicBoundary(hipPPMData *Data, const hipPPMParameter *Para);
+
+/* Library version string and device count check. */
+const char *hipPPMVersion(void);
+int hipPPMDeviceCount(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ENZO_PPM_H */

@@ -1,0 +1,96 @@
+"""ORACLE -- test infrastructure only (tests/, smoke check, bench CPU baseline).
+
+ctypes wrapper over the two CPU builds of oracle/Makefile:
+  "port": this project's C restatement (liboracle_port.so, symbols orc_*)
+  "ref" : the reference Fortran kernels built with flang from /root/reference
+          (oracle/_ref/libppm_ref.so, symbols ref_*), same slice driver.
+Neither is ever used by the product path.
+"""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "enzo-dev_amd"))
+from enzo_ppm import abi  # noqa: E402
+from enzo_ppm.grid import make_globals  # noqa: E402
+
+PATHS = {"port": os.path.join(HERE, "liboracle_port.so"),
+         "ref": os.path.join(HERE, "_ref", "libppm_ref.so")}
+PREFIX = {"port": "orc_", "ref": "ref_"}
+_libs = {}
+
+
+def build(kind="port"):
+    target = {"port": "port", "ref": "ref"}[kind]
+    subprocess.run(["make", "-s", "-C", HERE, target], check=True)
+
+
+def available(kind):
+    return os.path.exists(PATHS[kind])
+
+
+def lib(kind="port"):
+    if kind in _libs:
+        return _libs[kind]
+    if not os.path.exists(PATHS[kind]):
+        raise RuntimeError(f"oracle library {PATHS[kind]} not built")
+    L = C.CDLL(PATHS[kind])
+    p = PREFIX[kind]
+    f = getattr(L, p + "solve_ppm_de")
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(abi.EnzoPPMGrid), C.POINTER(abi.EnzoPPMGlobals), C.c_int64, C.c_int64,
+                  C.POINTER(C.POINTER(abi.EnzoFluxes)), C.POINTER(abi.PF), abi.PI, C.c_int64,
+                  C.c_int64, abi.PI, C.c_double, C.c_int64, C.POINTER(C.c_int64)]
+    f = getattr(L, p + "compute_pressure")
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(abi.EnzoPPMGrid), C.POINTER(abi.EnzoPPMGlobals), C.c_double, abi.PF]
+    f = getattr(L, p + "compute_time_step")
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(abi.EnzoPPMGrid), C.POINTER(abi.EnzoPPMGlobals), C.POINTER(abi.PF),
+                  C.c_double, C.POINTER(C.c_double)]
+    _libs[kind] = L
+    return L
+
+
+def solve_ppm_de(kind, grid, glob, cycle=0, fluxes=None, nthreads=8, min_support=0.0):
+    """grid::SolvePPM_DE on `grid` (modified in place).  Returns (status, fallbacks)."""
+    L = lib(kind)
+    gs, glob_dict = make_globals(**glob)
+    cg, keep = grid.ctypes_grid()
+    cw = grid.ctypes_cellwidth()
+    colnum = abi.iarr(grid.colnum, max(1, len(grid.colnum)))
+    nsub = fluxes.n if fluxes is not None else 0
+    fptr = fluxes.ptrs if fluxes is not None else (C.POINTER(abi.EnzoFluxes) * 1)()
+    gstart = fluxes.global_start if fluxes is not None else abi.iarr([0, 0, 0])
+    fb = C.c_int64(0)
+    st = getattr(L, PREFIX[kind] + "solve_ppm_de")(
+        C.byref(cg), C.byref(gs), cycle, nsub, fptr, cw, gstart, int(grid.gravity),
+        grid.ncolours, colnum, min_support, nthreads, C.byref(fb))
+    return st, fb.value
+
+
+def compute_pressure(kind, grid, glob, min_support=0.0):
+    import numpy as np
+    L = lib(kind)
+    gs, _ = make_globals(**glob)
+    cg, keep = grid.ctypes_grid()
+    p = np.zeros(grid.shape)
+    getattr(L, PREFIX[kind] + "compute_pressure")(C.byref(cg), C.byref(gs), min_support,
+                                                 abi.dptr(p))
+    return p
+
+
+def compute_time_step(kind, grid, glob, cfl):
+    L = lib(kind)
+    gs, _ = make_globals(**glob)
+    cg, keep = grid.ctypes_grid()
+    cw = grid.ctypes_cellwidth()
+    dt = C.c_double(0)
+    st = getattr(L, PREFIX[kind] + "compute_time_step")(C.byref(cg), C.byref(gs), cw, cfl,
+                                                       C.byref(dt))
+    if st != abi.SUCCESS:
+        raise RuntimeError(f"compute_time_step failed {st}")
+    return dt.value
