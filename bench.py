@@ -1,0 +1,146 @@
+"""Benchmark of the PPM-DE hydro step on one MI355X.
+
+Metric: FP64 PPM cell-updates per second per GPU (active cells per grid per
+step, the reference's definition: Grid_CollectGridInformation.C:36,
+EvolveLevel.C:951-957).  A step is one full hydro step on the device-resident
+grid: periodic ghost fill, the fused pressure + Courant pass, and the three
+cycle-rotated directional sweeps (grid::SolvePPM_DE).  Inputs are resident in
+HBM before the timed region; dt stays on the device.
+
+Workload (default): uniform 256^3 grid, 3 ghost zones, FP64, gamma = 5/3,
+lognormal density (sigma_lnrho = 0.5), k^-2 Gaussian random velocity field
+on k = 1..16 at RMS Mach 3, uniform temperature, three passive colours,
+dual energy on, steepening and flattening on, diffusion off, CFL 0.4.
+
+roofline: HBM-bound.  Algorithmic bytes of one sweep launch = active cells x
+8 x [(7 + C + G) reads + (6 + C) writes] (C colours, G = 1 with gravity);
+achieved = those bytes / the sweep kernel's mean launch time, timed with HIP
+events around each launch on the library's stream.
+cpu_baseline: the reference Fortran kernels (flang build in oracle/_ref,
+driven by the oracle's slice driver on a thread pool) on one step of the same
+256^3 workload; falls back to the oracle's C restatement ("port") when the
+reference build is absent.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "enzo-dev_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256, help="active cells per axis")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    return ap.parse_args()
+
+
+def cpu_baseline(n, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from enzo_ppm import problems
+    kind = "ref" if oracle.available("ref") else "port"
+    g, glob, run = problems.turbulent(n, seed=2024)
+    g.fill_periodic()
+    t0 = time.perf_counter()
+    g.dt = oracle.compute_time_step(kind, g, glob, run["cfl"])
+    st, _ = oracle.solve_ppm_de(kind, g, glob, cycle=0, nthreads=threads)
+    t1 = time.perf_counter()
+    if st != 1:
+        raise RuntimeError(f"cpu baseline status {st}")
+    cells = n ** 3
+    return dict(value=cells / (t1 - t0), unit="cell-updates/s", cores=threads,
+                kind="reference" if kind == "ref" else "port",
+                sample=f"one full step (pressure + Courant + 3 sweeps) of the {n}^3 workload, "
+                       f"{threads} threads, {t1 - t0:.2f} s")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+    from enzo_ppm import abi, device, problems
+    lib = abi.load_library()
+    if lib.hipPPMDeviceCount() < 1:
+        raise RuntimeError("no GPU visible")
+    n = a.n
+    print(f"bench: building {n}^3 workload", file=sys.stderr, flush=True)
+    g, glob, run = problems.turbulent(n, seed=2024)
+    dg = device.DeviceGrid(g, glob, lib=lib)
+    cfl = run["cfl"]
+    cycle = 0
+    for _ in range(a.warmup):
+        dg.periodic_boundary()
+        dg.pressure_and_dt(cfl, want_host=False)
+        dg.step(cycle)
+        cycle += 1
+    dg.synchronize()
+    dg.profile(True)
+    dg.profile_read()
+    dg.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        dg.periodic_boundary()
+        dg.pressure_and_dt(cfl, want_host=False)
+        dg.step(cycle)
+        cycle += 1
+    dg.synchronize()
+    t1 = time.perf_counter()
+    sweep_ms, launches = dg.profile_read()
+    fb = dg.fallback_count()
+    dg.close()
+    cells = n ** 3
+    ms_step = (t1 - t0) * 1e3 / a.steps
+    value = cells * a.steps / (t1 - t0)
+    C = g.ncolours
+    G = 1 if g.gravity else 0
+    bytes_sweep = cells * 8 * ((7 + C + G) + (6 + C))
+    sweep_avg_s = sweep_ms / 1e3 / max(1, launches)
+    achieved = bytes_sweep / sweep_avg_s / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_turbulent{n}.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("sweep_bytes_per_launch")
+    out = {
+        "metric": "FP64 PPM cell-updates/s/GPU (batched+learned vs plain), % HBM roofline",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lognormal density, k^-2 Mach-3 velocity field)",
+        "config": {"workload": f"plain per-grid PPM-DE step, uniform {n}^3 turbulent grid, "
+                               f"3 colours, dual energy, periodic", "grid": n,
+                   "form": "plain per-grid", "fallback_cells": int(fb)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "kernel": "ppm_sweep_kernel", "kernel_ms_avg": sweep_avg_s * 1e3,
+                     "algorithmic_bytes_per_launch": bytes_sweep},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(n, a.cpu_threads)
+        except Exception as e:  # reported, not fatal
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

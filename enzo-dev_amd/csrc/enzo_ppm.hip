@@ -1,0 +1,943 @@
+// enzo_ppm.hip -- MI355X (gfx950) implementation of the C ABI in
+// include/enzo_ppm.h: device state, kernels and the cuPPM-style lifecycle.
+//
+// Device state: the baryon fields live in two buffers of NSLOT arrays each
+// (slots: d, e, vx, vy, vz, ge, colours), x fastest, ghost zones included.
+// A sweep reads buffer A and writes buffer B (tiles read halo cells owned by
+// neighbouring tiles, so the update cannot be in place); the buffers swap
+// after each sweep.  Pressure, acceleration and per-axis coefficient tables
+// are separate arrays.  Nothing is re-allocated while the grid shape fits.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/enzo_ppm.h"
+#include "ppm_geometry.hpp"
+#include "ppm_sweep.hpp"
+
+using namespace ppm;
+
+namespace {
+
+constexpr int kTP = 8;   // pencils per workgroup
+constexpr int kTS = 32;  // interfaces per pencil segment
+using SweepCfg = Cfg<kTP, kTS>;
+
+// ------------------------------------------------------------------------
+// Fused sweep kernel: see ppm_sweep.hpp for the stages.
+struct DevExec {
+  Regs r;
+  template <class F>
+  __device__ void run(F f) {
+    f((int)threadIdx.x, r);
+    __syncthreads();
+  }
+  template <class F>
+  __device__ int count(F f) {
+    int v = f((int)threadIdx.x, r);
+    return __syncthreads_count(v);
+  }
+};
+
+__global__ __launch_bounds__(kTP *kTS) void ppm_sweep_kernel(SweepParams P) {
+  __shared__ double sm[SweepCfg::SMEM];
+  DevExec ex;
+  TileCounts tc = sweep_tile<kTP, kTS>(P, (int)blockIdx.x, sm, ex);
+  if (threadIdx.x == 0) {
+    if (tc.bad) atomicAdd(&P.counters[0], (unsigned long long)tc.bad);
+    if (tc.chains) atomicAdd(&P.counters[1], (unsigned long long)tc.chains);
+    if (tc.err) atomicAdd(&P.counters[2], (unsigned long long)tc.err);
+  }
+}
+
+// ------------------------------------------------------------------------
+// Flattening of line 1 of every slice.  The reference hands `flatten` (line 1)
+// to intvar for every line of the slice (inteuler.F:241-289), so one
+// coefficient row per slice is all the sweep needs.  Thread per
+// (slice, cell in [i1-1, i2+1]).
+__global__ void flat_row_kernel(FlatParams F) {
+  const int ncell = F.i2 - F.i1 + 3;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long long)ncell * F.n_outer) return;
+  const int outer = (int)(gid / ncell);
+  const int cell = F.i1 - 1 + (int)(gid % ncell);
+  F.flat[(long long)outer * F.N + (cell - 1)] = flat_value(F, outer, cell);
+}
+
+// ------------------------------------------------------------------------
+// Fused pressure + Courant condition (Grid_ComputePressure.C:96-166,
+// calc_dt.F:129-177).  Thread per cell; active cells contribute
+// dt1 = aye / sum_d (c_s + |v_d|)/dx_d to a device-wide minimum kept as the
+// bit pattern of a positive double (ordered like the value).
+struct PresParams {
+  int rank, idual;
+  int GD[3], st[3], en[3];
+  long long cells;
+  const double *A;
+  double *pres;
+  double gamma, tiny_number, min_support;
+  const double *cw[3];  // CellWidth per axis (calc_dt widths)
+  double aye;
+  int want_dt;
+  unsigned long long *dtmin;
+};
+
+__device__ inline double pressure_of(const PresParams &Q, long long g) {
+  const double *A = Q.A;
+  double density = A[S_D * Q.cells + g];
+  double gas_energy;
+  if (Q.idual == 0) {
+    double v1 = A[S_VX * Q.cells + g];
+    double v2 = Q.rank > 1 ? A[S_VY * Q.cells + g] : 0.0;
+    double v3 = Q.rank > 2 ? A[S_VZ * Q.cells + g] : 0.0;
+    gas_energy = A[S_E * Q.cells + g] - 0.5 * (v1 * v1 + v2 * v2 + v3 * v3);
+  } else {
+    gas_energy = A[S_GE * Q.cells + g];
+  }
+  double p = (Q.gamma - 1.0) * density * gas_energy;
+  if (p < Q.tiny_number) p = Q.tiny_number;
+  double min_pressure = Q.min_support * (Q.gamma - 1.0) * density * density;
+  if (p < min_pressure) p = min_pressure;
+  return p;
+}
+
+__global__ __launch_bounds__(256) void pressure_dt_kernel(PresParams Q) {
+  double local = 1.0e20;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < Q.cells;
+       g += (long long)gridDim.x * blockDim.x) {
+    double p = pressure_of(Q, g);
+    Q.pres[g] = p;
+    if (!Q.want_dt) continue;
+    int i = (int)(g % Q.GD[0]);
+    long long r = g / Q.GD[0];
+    int j = (int)(r % Q.GD[1]);
+    int k = (int)(r / Q.GD[1]);
+    if (i < Q.st[0] || i > Q.en[0] || j < Q.st[1] || j > Q.en[1] || k < Q.st[2] || k > Q.en[2])
+      continue;
+    double pdt = p;
+    if (Q.min_support != 0.0) {  // calc_dt's pressure uses coefficient 0
+      PresParams Z = Q;
+      Z.min_support = 0.0;
+      pdt = pressure_of(Z, g);
+    }
+    double d = Q.A[S_D * Q.cells + g];
+    double cs = fmx(sqrt(Q.gamma * pdt / d), kTiny);
+    double u = Q.A[S_VX * Q.cells + g];
+    double dt1;
+    if (Q.rank == 1) {
+      dt1 = Q.cw[0][i] * Q.aye / (cs + fabs(u - 0.0));
+    } else if (Q.rank == 2) {
+      double v = Q.A[S_VY * Q.cells + g];
+      dt1 = Q.aye / ((cs + fabs(u - 0.0)) / Q.cw[0][i] + (cs + fabs(v - 0.0)) / Q.cw[1][j]);
+    } else {
+      double v = Q.A[S_VY * Q.cells + g], w = Q.A[S_VZ * Q.cells + g];
+      dt1 = Q.aye / ((cs + fabs(u - 0.0)) / Q.cw[0][i] + (cs + fabs(v - 0.0)) / Q.cw[1][j] +
+                     (cs + fabs(w - 0.0)) / Q.cw[2][k]);
+    }
+    local = fmn(local, dt1);
+  }
+  if (!Q.want_dt) return;
+  // wavefront then workgroup minimum
+  for (int off = 32; off > 0; off >>= 1) local = fmn(local, __shfl_xor(local, off, 64));
+  __shared__ double wmin[4];
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmin[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmn(m, wmin[w]);
+    atomicMin(Q.dtmin, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+__global__ void dt_init_kernel(unsigned long long *dtmin) {
+  if (threadIdx.x == 0) *dtmin = (unsigned long long)__double_as_longlong(1.0e20);
+}
+
+__global__ void dt_final_kernel(const unsigned long long *dtmin, double cfl, double *dt) {
+  if (threadIdx.x == 0) *dt = __longlong_as_double((long long)*dtmin) * cfl;
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// ------------------------------------------------------------------------
+// Periodic ghost fill along one axis for every slot (the periodic case of
+// the external boundary).  Thread per (slot, ghost cell).
+struct BCParams {
+  int axis, ghost, nslot;
+  int GD[3], act[3];
+  long long cells;
+  double *S;
+};
+
+__global__ void periodic_axis_kernel(BCParams B) {
+  const int a = B.axis;
+  const int b = (a + 1) % 3, c = (a + 2) % 3;
+  const long long per_slot = (long long)2 * B.ghost * B.GD[b] * B.GD[c];
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= per_slot * B.nslot) return;
+  const int slot = (int)(gid / per_slot);
+  long long r = gid % per_slot;
+  const int gl = (int)(r % (2 * B.ghost));
+  r /= 2 * B.ghost;
+  const int cb = (int)(r % B.GD[b]);
+  const int cc = (int)(r / B.GD[b]);
+  int dst = gl < B.ghost ? gl : B.ghost + B.act[a] + (gl - B.ghost);
+  int src = gl < B.ghost ? gl + B.act[a] : B.ghost + (gl - B.ghost);
+  int x[3];
+  x[b] = cb;
+  x[c] = cc;
+  x[a] = dst;
+  long long gd = ((long long)x[2] * B.GD[1] + x[1]) * B.GD[0] + x[0];
+  x[a] = src;
+  long long gs = ((long long)x[2] * B.GD[1] + x[1]) * B.GD[0] + x[0];
+  double *S = B.S + (long long)slot * B.cells;
+  S[gd] = S[gs];
+}
+
+}  // namespace
+
+// ==========================================================================
+// host side
+struct hipPPMData {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int GD[3] = {0, 0, 0};
+  long long cells = 0;
+  int nslot = 0, ncol = 0;
+  bool gravity = false;
+  double *A = nullptr, *B = nullptr;
+  double *pres = nullptr;
+  double *accel = nullptr;  // 3 * cells when gravity
+  double *tab[3] = {nullptr, nullptr, nullptr};
+  double *cwdev[3] = {nullptr, nullptr, nullptr};
+  double *flat = nullptr;
+  long long flat_cap = 0;
+  double *dt = nullptr;
+  unsigned long long *dtmin = nullptr;
+  unsigned long long *counters = nullptr;  // [0] fallback [1] DE chain [2] errors
+  int *segbuf = nullptr;  // per axis: o[nseg], w[nseg]
+  int segoff[3] = {0, 0, 0}, nseg[3] = {0, 0, 0};
+  // face capture
+  int *face_head[3] = {nullptr, nullptr, nullptr};
+  FaceRec *faces[3] = {nullptr, nullptr, nullptr};
+  double *face_out[3] = {nullptr, nullptr, nullptr};
+  std::vector<FaceRec> hfaces[3];
+  std::vector<int> hface_sub[3], hface_side[3];
+  long long face_total[3] = {0, 0, 0};
+  int nsub = 0;
+  long long fallback_total = 0;
+  long long chain_total = 0;
+  // optional per-launch timing of the sweep kernel (HIP events on D->stream)
+  int profile = 0;
+  std::vector<hipEvent_t> ev;  // pairs
+  size_t ev_used = 0;
+};
+
+namespace {
+
+thread_local char g_last_error[512] = "";
+
+void set_error(const char *what, hipError_t e, int line) {
+  snprintf(g_last_error, sizeof(g_last_error), "%s failed: %s (enzo_ppm.hip:%d)", what,
+           hipGetErrorString(e), line);
+  if (getenv("ENZO_PPM_VERBOSE")) fprintf(stderr, "enzo_ppm: %s\n", g_last_error);
+}
+
+#define HIPCHK(x)                            \
+  do {                                       \
+    hipError_t e_ = (x);                     \
+    if (e_ != hipSuccess) {                  \
+      set_error(#x, e_, __LINE__);           \
+      return ENZO_PPM_ERR_HIP;               \
+    }                                        \
+  } while (0)
+
+inline int nslot_of(const hipPPMParameter *P) { return S_COL0 + (int)P->NumberOfColours; }
+
+int check_para(const hipPPMParameter *P) {
+  if (!P) return ENZO_PPM_ERR_ARGUMENT;
+  if (P->GridRank < 1 || P->GridRank > 3) return ENZO_PPM_ERR_ARGUMENT;
+  if (P->NumberOfColours < 0 || P->NumberOfColours > ENZO_PPM_MAX_COLOR) return ENZO_PPM_ERR_ARGUMENT;
+  for (int d = 0; d < 3; d++) {
+    if (P->GridDimension[d] < 1 || P->GridDimension[d] > ENZO_PPM_MAX_ANY_SINGLE_DIRECTION)
+      return ENZO_PPM_ERR_ARGUMENT;
+    if (d < P->GridRank) {
+      if (P->GridStartIndex[d] < 3 || P->GridEndIndex[d] + 3 >= P->GridDimension[d] ||
+          P->GridEndIndex[d] < P->GridStartIndex[d])
+        return ENZO_PPM_ERR_ARGUMENT;
+    }
+  }
+  if (P->ConservativeReconstruction != 0 || P->PositiveReconstruction != 0)
+    return ENZO_PPM_ERR_NOT_SUPPORTED;  // off on this path (inteuler.F:232-239)
+  if (P->RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK) return ENZO_PPM_ERR_NOT_SUPPORTED;
+  if (P->PPMFlatteningParameter != 0 && P->PPMFlatteningParameter != 1)
+    return ENZO_PPM_ERR_NOT_SUPPORTED;
+  if (P->PPMDiffusionParameter != 0 && P->PPMDiffusionParameter != 1)
+    return ENZO_PPM_ERR_NOT_SUPPORTED;  // idiff = 2 reads undefined values in calcdiss.F
+  return ENZO_PPM_SUCCESS;
+}
+
+// true when the allocation failed (so that `if (alloc_dev(..) || ..)` reads
+// as "if any allocation failed")
+bool alloc_dev(void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    set_error("hipMalloc", e, __LINE__);
+    *p = nullptr;
+    return true;
+  }
+  return false;
+}
+
+void free_data(hipPPMData *D) {
+  if (!D) return;
+  for (hipEvent_t e : D->ev) (void)hipEventDestroy(e);
+  D->ev.clear();
+  void *ptrs[] = {D->A, D->B, D->pres, D->accel, D->tab[0], D->tab[1], D->tab[2], D->cwdev[0],
+                  D->cwdev[1], D->cwdev[2], D->flat, D->dt, D->dtmin, D->counters, D->segbuf,
+                  D->face_head[0], D->face_head[1], D->face_head[2], D->faces[0], D->faces[1],
+                  D->faces[2], D->face_out[0], D->face_out[1], D->face_out[2]};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+}
+
+int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const double *dtp,
+                 int dim) {
+  const int N = D->GD[dim];
+  const int i1 = (int)P->GridStartIndex[dim] + 1, i2 = (int)P->GridEndIndex[dim] + 1;
+  const int rax = (dim + 1) % 3, oax = (dim + 2) % 3;
+  const int rank = (int)P->GridRank;
+  const int iflat = (int)P->PPMFlatteningParameter;
+  // 1) flattening row of line 1 of every slice
+  if (iflat) {
+    FlatParams F;
+    F.dim = dim; F.N = N; F.i1 = i1; F.i2 = i2;
+    for (int d = 0; d < 3; d++) F.GD[d] = D->GD[d];
+    F.cells = D->cells;
+    F.n_outer = D->GD[oax];
+    F.A = D->A;
+    F.pres = D->pres;
+    F.flat = D->flat;
+    long long n = (long long)(i2 - i1 + 3) * F.n_outer;
+    hipLaunchKernelGGL(flat_row_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       D->stream, F);
+    HIPCHK(hipGetLastError());
+  }
+  SweepParams S;
+  memset(&S, 0, sizeof(S));
+  S.dim = dim; S.N = N; S.i1 = i1; S.i2 = i2; S.rank = rank;
+  for (int d = 0; d < 3; d++) S.GD[d] = D->GD[d];
+  S.cells = D->cells;
+  S.nseg = D->nseg[dim];
+  S.seg_o = D->segbuf + D->segoff[dim];
+  S.seg_w = D->segbuf + D->segoff[dim] + D->nseg[dim];
+  S.A = D->A;
+  S.B = D->B;
+  S.ncol = (int)P->NumberOfColours;
+  S.pres = D->pres;
+  S.gr = P->GravityOn ? D->accel + (long long)dim * D->cells : nullptr;
+  S.flat = D->flat;
+  S.tab = D->tab[dim];
+  S.tab_row = D->tab[rax];
+  S.tab_out = D->tab[oax];
+  S.gamma = P->Gamma;
+  S.eta1 = P->DualEnergyFormalismEta1;
+  S.eta2 = P->DualEnergyFormalismEta2;
+  S.pmin = P->MinimumPressure;
+  S.dfloor = P->SmallRho;
+  S.dt = dtFixed;
+  S.dtp = dtp;
+  S.isteep = (int)P->PPMSteepeningParameter;
+  S.iflatten = iflat;
+  S.idiff = (int)P->PPMDiffusionParameter;
+  S.idual = (int)P->DualEnergyFormalism;
+  S.ipresfree = (int)P->PressureFree;
+  S.gravity = (int)P->GravityOn;
+  S.ifallback = (int)P->RiemannSolverFallback;
+  S.nact_out = (int)(P->GridEndIndex[oax] - P->GridStartIndex[oax] + 1);
+  S.face_head = D->face_total[dim] > 0 ? D->face_head[dim] : nullptr;
+  S.faces = D->faces[dim];
+  S.face_out = D->face_out[dim];
+  S.counters = D->counters;
+  if (S.idiff == 0) {
+    // all pencils at once: t1 is the contiguous transverse axis for y/z
+    // sweeps (coalesced loads), the row axis for x sweeps
+    if (dim == 0) { S.ax_t1 = 1; S.ax_t2 = 2; }
+    else if (dim == 1) { S.ax_t1 = 0; S.ax_t2 = 2; }
+    else { S.ax_t1 = 0; S.ax_t2 = 1; }
+    S.n_t1 = D->GD[S.ax_t1];
+    S.n_t2 = D->GD[S.ax_t2];
+    S.t2_0 = 0;
+    S.nblk_t1 = (S.n_t1 + kTP - 1) / kTP;
+    long long nblk = (long long)S.nseg * S.nblk_t1 * S.n_t2;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (D->profile) {
+      if (D->ev_used + 2 > D->ev.size()) {
+        for (int k = 0; k < 2; k++) {
+          hipEvent_t e;
+          HIPCHK(hipEventCreate(&e));
+          D->ev.push_back(e);
+        }
+      }
+      e0 = D->ev[D->ev_used];
+      e1 = D->ev[D->ev_used + 1];
+      D->ev_used += 2;
+      HIPCHK(hipEventRecord(e0, D->stream));
+    }
+    hipLaunchKernelGGL(ppm_sweep_kernel, dim3((unsigned)nblk), dim3(kTP * kTS), 0, D->stream, S);
+    HIPCHK(hipGetLastError());
+    if (D->profile) HIPCHK(hipEventRecord(e1, D->stream));
+  } else {
+    // diffusion reads the live transverse velocities of the previous slice
+    // (calcdiss.F:207-288 on BaryonField): slices in order, one launch each
+    S.ax_t1 = rax;
+    S.ax_t2 = oax;
+    S.n_t1 = D->GD[rax];
+    S.n_t2 = 1;
+    S.nblk_t1 = (S.n_t1 + kTP - 1) / kTP;
+    for (int k = 0; k < D->GD[oax]; k++) {
+      S.t2_0 = k;
+      long long nblk = (long long)S.nseg * S.nblk_t1;
+      hipLaunchKernelGGL(ppm_sweep_kernel, dim3((unsigned)nblk), dim3(kTP * kTS), 0, D->stream,
+                         S);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  std::swap(D->A, D->B);
+  return ENZO_PPM_SUCCESS;
+}
+
+int launch_pressure(hipPPMData *D, const hipPPMParameter *P, double min_support, int want_dt,
+                    double cfl) {
+  PresParams Q;
+  memset(&Q, 0, sizeof(Q));
+  Q.rank = (int)P->GridRank;
+  Q.idual = (int)P->DualEnergyFormalism;
+  for (int d = 0; d < 3; d++) {
+    Q.GD[d] = D->GD[d];
+    Q.st[d] = (int)P->GridStartIndex[d];
+    Q.en[d] = (int)P->GridEndIndex[d];
+    Q.cw[d] = D->cwdev[d];
+  }
+  Q.cells = D->cells;
+  Q.A = D->A;
+  Q.pres = D->pres;
+  Q.gamma = P->Gamma;
+  Q.tiny_number = P->MinimumPressure;
+  Q.min_support = min_support;
+  Q.aye = 1.0;
+  Q.want_dt = want_dt;
+  Q.dtmin = D->dtmin;
+  if (want_dt) {
+    hipLaunchKernelGGL(dt_init_kernel, dim3(1), dim3(64), 0, D->stream, D->dtmin);
+    HIPCHK(hipGetLastError());
+  }
+  long long nb = (D->cells + 255) / 256;
+  if (nb > 8192) nb = 8192;
+  hipLaunchKernelGGL(pressure_dt_kernel, dim3((unsigned)nb), dim3(256), 0, D->stream, Q);
+  HIPCHK(hipGetLastError());
+  if (want_dt) {
+    hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(64), 0, D->stream, D->dtmin, cfl, D->dt);
+    HIPCHK(hipGetLastError());
+  }
+  return ENZO_PPM_SUCCESS;
+}
+
+// Enzo field number of each device slot (-1: none)
+void slot_fields(const hipPPMParameter *P, std::vector<long long> &f) {
+  f.assign(nslot_of(P), -1);
+  f[S_D] = P->DensNum;
+  f[S_E] = P->TENum;
+  f[S_VX] = P->Vel1Num;
+  if (P->GridRank > 1) f[S_VY] = P->Vel2Num;
+  if (P->GridRank > 2) f[S_VZ] = P->Vel3Num;
+  if (P->DualEnergyFormalism) f[S_GE] = P->GENum;
+  for (int n = 0; n < P->NumberOfColours; n++) f[S_COL0 + n] = P->colnum[n];
+}
+
+}  // namespace
+
+// ==========================================================================
+extern "C" {
+
+int hipPPMInitParameter(hipPPMParameter *Para, enzo_int DensNum, enzo_int TENum,
+                        enzo_int Vel1Num, enzo_int Vel2Num, enzo_int Vel3Num, enzo_int GENum,
+                        enzo_int MaxNumberOfBaryonFields, const enzo_long_int GridGlobalStart[],
+                        const enzo_int GridStartIndex[], const enzo_int GridEndIndex[],
+                        const enzo_int GridDimension[], enzo_float dxx, enzo_float dxy,
+                        enzo_float dxz, enzo_float Gamma, enzo_int PPMDiffusionParameter,
+                        enzo_int PPMFlatteningParameter, enzo_int PPMSteepeningParameter,
+                        enzo_int ConservativeReconstruction, enzo_int PositiveReconstruction,
+                        enzo_int GravityOn, enzo_int DualEnergyFormalism,
+                        enzo_float DualEnergyFormalismEta1, enzo_float DualEnergyFormalismEta2,
+                        enzo_int PressureFree, enzo_float MinimumPressure,
+                        enzo_int RiemannSolver, enzo_int RiemannSolverFallback,
+                        enzo_int NumberOfColours, const enzo_int *colnum, enzo_int GridRank,
+                        enzo_float SmallRho) {
+  if (!Para || !GridStartIndex || !GridEndIndex || !GridDimension) return ENZO_PPM_ERR_ARGUMENT;
+  if (NumberOfColours < 0 || NumberOfColours > ENZO_PPM_MAX_COLOR) return ENZO_PPM_ERR_ARGUMENT;
+  if (NumberOfColours > 0 && !colnum) return ENZO_PPM_ERR_ARGUMENT;
+  memset(Para, 0, sizeof(*Para));
+  Para->DensNum = DensNum; Para->TENum = TENum; Para->Vel1Num = Vel1Num;
+  Para->Vel2Num = Vel2Num; Para->Vel3Num = Vel3Num; Para->GENum = GENum;
+  Para->MaxNumberOfBaryonFields = MaxNumberOfBaryonFields;
+  for (int d = 0; d < 3; d++) {
+    Para->GridGlobalStart[d] = GridGlobalStart ? GridGlobalStart[d] : 0;
+    Para->GridStartIndex[d] = GridStartIndex[d];
+    Para->GridEndIndex[d] = GridEndIndex[d];
+    Para->GridDimension[d] = GridDimension[d];
+  }
+  Para->GridRank = GridRank;
+  Para->dx[0] = dxx; Para->dx[1] = dxy; Para->dx[2] = dxz;
+  Para->Gamma = Gamma;
+  Para->PPMDiffusionParameter = PPMDiffusionParameter;
+  Para->PPMFlatteningParameter = PPMFlatteningParameter;
+  Para->PPMSteepeningParameter = PPMSteepeningParameter;
+  Para->ConservativeReconstruction = ConservativeReconstruction;
+  Para->PositiveReconstruction = PositiveReconstruction;
+  Para->GravityOn = GravityOn;
+  Para->DualEnergyFormalism = DualEnergyFormalism;
+  Para->DualEnergyFormalismEta1 = DualEnergyFormalismEta1;
+  Para->DualEnergyFormalismEta2 = DualEnergyFormalismEta2;
+  Para->PressureFree = PressureFree;
+  Para->MinimumPressure = MinimumPressure;
+  Para->RiemannSolver = RiemannSolver;
+  Para->RiemannSolverFallback = RiemannSolverFallback;
+  Para->SmallRho = SmallRho;
+  Para->NumberOfColours = NumberOfColours;
+  for (int n = 0; n < NumberOfColours; n++) Para->colnum[n] = colnum[n];
+  return check_para(Para);
+}
+
+int hipPPMSetCellWidth(hipPPMParameter *Para, enzo_int dim, const enzo_float *CellWidthTemp) {
+  if (!Para || dim < 0 || dim > 2) return ENZO_PPM_ERR_ARGUMENT;
+  Para->CellWidth[dim] = CellWidthTemp;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMInitData(hipPPMData **Data, const hipPPMParameter *Para, void *stream) {
+  if (!Data) return ENZO_PPM_ERR_ARGUMENT;
+  int st = check_para(Para);
+  if (st != ENZO_PPM_SUCCESS) return st;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return ENZO_PPM_ERR_NO_DEVICE;
+  hipPPMData *D = *Data;
+  long long cells = 1;
+  for (int d = 0; d < 3; d++) cells *= Para->GridDimension[d];
+  const int nslot = nslot_of(Para);
+  const bool same = D && D->cells == cells && D->nslot == nslot &&
+                    D->GD[0] == Para->GridDimension[0] && D->GD[1] == Para->GridDimension[1] &&
+                    D->GD[2] == Para->GridDimension[2] && (D->gravity || !Para->GravityOn);
+  if (!same) {
+    if (D) {
+      free_data(D);
+      delete D;
+    }
+    D = new (std::nothrow) hipPPMData();
+    if (!D) return ENZO_PPM_ERR_ARGUMENT;
+    *Data = D;
+    HIPCHK(hipGetDevice(&D->device));
+    for (int d = 0; d < 3; d++) D->GD[d] = (int)Para->GridDimension[d];
+    D->cells = cells;
+    D->nslot = nslot;
+    D->ncol = (int)Para->NumberOfColours;
+    D->gravity = Para->GravityOn != 0;
+    size_t fb = sizeof(double) * (size_t)cells;
+    if (alloc_dev((void **)&D->A, fb * nslot) || alloc_dev((void **)&D->B, fb * nslot) ||
+        alloc_dev((void **)&D->pres, fb) ||
+        (D->gravity && alloc_dev((void **)&D->accel, 3 * fb)))
+      return ENZO_PPM_ERR_HIP;
+    long long fcap = 0;
+    for (int d = 0; d < 3; d++) {
+      if (alloc_dev((void **)&D->tab[d], sizeof(double) * NTAB * D->GD[d]) ||
+          alloc_dev((void **)&D->cwdev[d], sizeof(double) * D->GD[d]))
+        return ENZO_PPM_ERR_HIP;
+      fcap = std::max(fcap, (long long)D->GD[d] * D->GD[(d + 2) % 3]);
+    }
+    D->flat_cap = fcap;
+    if (alloc_dev((void **)&D->flat, sizeof(double) * fcap) ||
+        alloc_dev((void **)&D->dt, sizeof(double)) ||
+        alloc_dev((void **)&D->dtmin, sizeof(unsigned long long)) ||
+        alloc_dev((void **)&D->counters, 4 * sizeof(unsigned long long)))
+      return ENZO_PPM_ERR_HIP;
+  }
+  D->stream = (hipStream_t)stream;
+  // geometry tables and segments (cheap; rebuilt on every init)
+  std::vector<int> segs;
+  for (int d = 0; d < 3; d++) {
+    const int N = D->GD[d];
+    std::vector<double> dx(N), tab;
+    for (int i = 0; i < N; i++)
+      dx[i] = Para->CellWidth[d] ? Para->CellWidth[d][i] : (d < Para->GridRank ? Para->dx[d] : 1.0);
+    const int i1 = (int)Para->GridStartIndex[d] + 1, i2 = (int)Para->GridEndIndex[d] + 1;
+    build_tables(dx.data(), N, tab);
+    HIPCHK(hipMemcpy(D->tab[d], tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D->cwdev[d], dx.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    std::vector<int> o, w;
+    if (d < Para->GridRank && N > 1) {
+      if (!build_segments<kTP, kTS>(N, i1, i2, o, w)) return ENZO_PPM_ERR_ARGUMENT;
+    }
+    D->segoff[d] = (int)segs.size();
+    D->nseg[d] = (int)o.size();
+    segs.insert(segs.end(), o.begin(), o.end());
+    segs.insert(segs.end(), w.begin(), w.end());
+  }
+  if (D->segbuf) (void)hipFree(D->segbuf);
+  D->segbuf = nullptr;
+  if (alloc_dev((void **)&D->segbuf, sizeof(int) * std::max<size_t>(1, segs.size())))
+    return ENZO_PPM_ERR_HIP;
+  if (!segs.empty())
+    HIPCHK(hipMemcpy(D->segbuf, segs.data(), sizeof(int) * segs.size(), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, D->stream, D->counters, 4);
+  HIPCHK(hipGetLastError());
+  D->fallback_total = 0;
+  D->chain_total = 0;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMSetBaryon(hipPPMData *D, const hipPPMParameter *P, enzo_float *BaryonField[],
+                    enzo_float *Pressure, enzo_float *AccelerationField[]) {
+  if (!D || !P || !BaryonField) return ENZO_PPM_ERR_ARGUMENT;
+  std::vector<long long> f;
+  slot_fields(P, f);
+  const size_t fb = sizeof(double) * (size_t)D->cells;
+  for (int s = 0; s < D->nslot; s++) {
+    double *dst = D->A + (size_t)s * D->cells;
+    if (f[s] >= 0) {
+      if (!BaryonField[f[s]]) return ENZO_PPM_ERR_ARGUMENT;
+      HIPCHK(hipMemcpyAsync(dst, BaryonField[f[s]], fb, hipMemcpyHostToDevice, D->stream));
+    } else {
+      HIPCHK(hipMemsetAsync(dst, 0, fb, D->stream));
+    }
+  }
+  if (P->GravityOn) {
+    if (!AccelerationField || !D->accel) return ENZO_PPM_ERR_ARGUMENT;
+    for (int d = 0; d < 3; d++) {
+      if (d < P->GridRank && AccelerationField[d])
+        HIPCHK(hipMemcpyAsync(D->accel + (size_t)d * D->cells, AccelerationField[d], fb,
+                              hipMemcpyHostToDevice, D->stream));
+      else
+        HIPCHK(hipMemsetAsync(D->accel + (size_t)d * D->cells, 0, fb, D->stream));
+    }
+  }
+  if (Pressure) {
+    HIPCHK(hipMemcpyAsync(D->pres, Pressure, fb, hipMemcpyHostToDevice, D->stream));
+  } else {
+    int st = launch_pressure(D, P, 0.0, 0, 1.0);
+    if (st != ENZO_PPM_SUCCESS) return st;
+  }
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMSetSubgridFluxes(hipPPMData *D, const hipPPMParameter *P, enzo_fluxes *SubgridFluxes[],
+                           enzo_int NumberOfSubgrids, const enzo_long_int GridGlobalStart[]) {
+  if (!D || !P || NumberOfSubgrids < 0) return ENZO_PPM_ERR_ARGUMENT;
+  if (NumberOfSubgrids > 0 && (!SubgridFluxes || !GridGlobalStart)) return ENZO_PPM_ERR_ARGUMENT;
+  D->nsub = (int)NumberOfSubgrids;
+  const int nfld = nslot_of(P);
+  for (int dim = 0; dim < 3; dim++) {
+    D->hfaces[dim].clear();
+    D->hface_sub[dim].clear();
+    D->hface_side[dim].clear();
+    if (D->face_head[dim]) (void)hipFree(D->face_head[dim]);
+    if (D->faces[dim]) (void)hipFree(D->faces[dim]);
+    if (D->face_out[dim]) (void)hipFree(D->face_out[dim]);
+    D->face_head[dim] = nullptr;
+    D->faces[dim] = nullptr;
+    D->face_out[dim] = nullptr;
+    D->face_total[dim] = 0;
+    if (dim >= P->GridRank) continue;
+    const int N = D->GD[dim];
+    const int i1 = (int)P->GridStartIndex[dim] + 1, i2 = (int)P->GridEndIndex[dim] + 1;
+    const int ai = dim == 0 ? 1 : 0, aj = dim == 2 ? 1 : 2;
+    std::vector<std::pair<int, int>> order;  // (pos, record)
+    long long total = 0;
+    for (int n = 0; n < NumberOfSubgrids; n++) {
+      const enzo_fluxes *F = SubgridFluxes[n];
+      if (!F) return ENZO_PPM_ERR_ARGUMENT;
+      for (int side = 0; side < 2; side++) {
+        FaceRec R;
+        memset(&R, 0, sizeof(R));
+        long long face = side == 0 ? F->LeftFluxStartGlobalIndex[dim][dim] - GridGlobalStart[dim]
+                                   : F->RightFluxStartGlobalIndex[dim][dim] - GridGlobalStart[dim] + 1;
+        R.pos = (int)face + 1;  // 1-based interface at the left edge of cell `face`
+        R.lo_i = (int)(F->RightFluxStartGlobalIndex[dim][ai] - GridGlobalStart[ai]);
+        R.hi_i = (int)(F->RightFluxEndGlobalIndex[dim][ai] - GridGlobalStart[ai]);
+        R.lo_j = (int)(F->RightFluxStartGlobalIndex[dim][aj] - GridGlobalStart[aj]);
+        R.hi_j = (int)(F->RightFluxEndGlobalIndex[dim][aj] - GridGlobalStart[aj]);
+        if (R.pos < i1 || R.pos > i2 + 1 || R.hi_i < R.lo_i || R.hi_j < R.lo_j)
+          return ENZO_PPM_ERR_ARGUMENT;
+        R.nface = (long long)(R.hi_i - R.lo_i + 1) * (R.hi_j - R.lo_j + 1);
+        order.push_back({R.pos, (int)D->hfaces[dim].size()});
+        D->hfaces[dim].push_back(R);
+        D->hface_sub[dim].push_back(n);
+        D->hface_side[dim].push_back(side);
+      }
+    }
+    // sort records by position, assign staging offsets
+    std::stable_sort(order.begin(), order.end());
+    std::vector<FaceRec> sorted;
+    std::vector<int> sub, side;
+    for (auto &pr : order) {
+      FaceRec R = D->hfaces[dim][pr.second];
+      R.base = total;
+      total += R.nface * nfld;
+      sorted.push_back(R);
+      sub.push_back(D->hface_sub[dim][pr.second]);
+      side.push_back(D->hface_side[dim][pr.second]);
+    }
+    D->hfaces[dim] = sorted;
+    D->hface_sub[dim] = sub;
+    D->hface_side[dim] = side;
+    std::vector<int> head(N + 2, 0);
+    {
+      size_t r = 0;
+      for (int pos = 0; pos <= N + 1; pos++) {
+        while (r < sorted.size() && sorted[r].pos < pos) r++;
+        head[pos] = (int)r;
+      }
+    }
+    D->face_total[dim] = total;
+    if (total > 0) {
+      if (alloc_dev((void **)&D->face_head[dim], sizeof(int) * head.size()) ||
+          alloc_dev((void **)&D->faces[dim], sizeof(FaceRec) * sorted.size()) ||
+          alloc_dev((void **)&D->face_out[dim], sizeof(double) * total))
+        return ENZO_PPM_ERR_HIP;
+      HIPCHK(hipMemcpy(D->face_head[dim], head.data(), sizeof(int) * head.size(),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(D->faces[dim], sorted.data(), sizeof(FaceRec) * sorted.size(),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemsetAsync(D->face_out[dim], 0, sizeof(double) * total, D->stream));
+    }
+  }
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMSweep(hipPPMData *D, const hipPPMParameter *P, enzo_float dtFixed, enzo_int dir) {
+  if (!D || !P || dir < 0 || dir > 2 || dir >= P->GridRank) return ENZO_PPM_ERR_ARGUMENT;
+  if (D->nseg[dir] == 0) return ENZO_PPM_ERR_ARGUMENT;
+  return launch_sweep(D, P, dtFixed, nullptr, (int)dir);
+}
+
+int hipPPMSaveSubgridFluxes(enzo_fluxes *SubgridFluxes[], enzo_int NumberOfSubgrids,
+                            const enzo_long_int GridGlobalStart[], hipPPMData *D,
+                            const hipPPMParameter *P, enzo_int dir) {
+  (void)GridGlobalStart;
+  if (!D || !P || dir < 0 || dir > 2) return ENZO_PPM_ERR_ARGUMENT;
+  if (NumberOfSubgrids != D->nsub) return ENZO_PPM_ERR_ARGUMENT;
+  if (D->face_total[dir] == 0) return ENZO_PPM_SUCCESS;
+  std::vector<double> h((size_t)D->face_total[dir]);
+  HIPCHK(hipMemcpyAsync(h.data(), D->face_out[dir], sizeof(double) * h.size(),
+                        hipMemcpyDeviceToHost, D->stream));
+  HIPCHK(hipStreamSynchronize(D->stream));
+  const long long nact[3] = {P->GridEndIndex[0] - P->GridStartIndex[0] + 1,
+                             P->GridEndIndex[1] - P->GridStartIndex[1] + 1,
+                             P->GridEndIndex[2] - P->GridStartIndex[2] + 1};
+  std::vector<long long> f;
+  slot_fields(P, f);
+  for (size_t r = 0; r < D->hfaces[dir].size(); r++) {
+    const FaceRec &R = D->hfaces[dir][r];
+    enzo_fluxes *F = SubgridFluxes[D->hface_sub[dir][r]];
+    const int side = D->hface_side[dir][r];
+    for (int s = 0; s < (int)f.size(); s++) {
+      if (f[s] < 0) continue;
+      // velocity guards of Grid_[xyz]EulerSweep.C:343-357
+      if (s >= S_VX && s <= S_VZ && (s - S_VX) != dir && nact[s - S_VX] <= 1) continue;
+      enzo_float *out = side == 0 ? F->LeftFluxes[f[s]][dir] : F->RightFluxes[f[s]][dir];
+      if (!out) continue;
+      memcpy(out, h.data() + R.base + (size_t)s * R.nface, sizeof(double) * R.nface);
+    }
+  }
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMGetBaryon(hipPPMData *D, const hipPPMParameter *P, enzo_float *BaryonField[]) {
+  if (!D || !P || !BaryonField) return ENZO_PPM_ERR_ARGUMENT;
+  std::vector<long long> f;
+  slot_fields(P, f);
+  const size_t fb = sizeof(double) * (size_t)D->cells;
+  for (int s = 0; s < D->nslot; s++) {
+    if (f[s] < 0 || !BaryonField[f[s]]) continue;
+    HIPCHK(hipMemcpyAsync(BaryonField[f[s]], D->A + (size_t)s * D->cells, fb,
+                          hipMemcpyDeviceToHost, D->stream));
+  }
+  HIPCHK(hipStreamSynchronize(D->stream));
+  unsigned long long c[4];
+  HIPCHK(hipMemcpy(c, D->counters, sizeof(c), hipMemcpyDeviceToHost));
+  D->fallback_total = (long long)c[0];
+  D->chain_total = (long long)c[1];
+  if (c[2] != 0) return ENZO_PPM_ERR_NEGATIVE;
+  if (c[0] != 0 && P->RiemannSolverFallback == 0) return ENZO_PPM_ERR_NEGATIVE;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMDestroy(hipPPMData *D) {
+  if (!D) return ENZO_PPM_SUCCESS;
+  free_data(D);
+  delete D;
+  return ENZO_PPM_SUCCESS;
+}
+
+enzo_long_int hipPPMFallbackCount(hipPPMData *D) {
+  if (!D) return -1;
+  unsigned long long c[4];
+  if (hipMemcpy(c, D->counters, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return (enzo_long_int)c[0];
+}
+
+int hipPPMSolvePPM_DE(const enzo_ppm_grid *g, const enzo_ppm_globals *gl, enzo_int CycleNumber,
+                      enzo_int NumberOfSubgrids, enzo_fluxes *SubgridFluxes[],
+                      enzo_float *CellWidthTemp[], const enzo_long_int GridGlobalStart[],
+                      enzo_int GravityOn, enzo_int NumberOfColours, const enzo_int colnum[],
+                      enzo_float MinimumSupportEnergyCoefficient, hipPPMData **Data,
+                      void *stream) {
+  if (!g || !gl) return ENZO_PPM_ERR_ARGUMENT;
+  if (gl->EOSType != 0 || gl->ReconstructionMethod != ENZO_PPM_RECONSTRUCTION_PPM)
+    return ENZO_PPM_ERR_NOT_SUPPORTED;
+  hipPPMParameter Para;
+  enzo_long_int ggs[3] = {0, 0, 0};
+  if (GridGlobalStart)
+    for (int d = 0; d < 3; d++) ggs[d] = GridGlobalStart[d];
+  int st = hipPPMInitParameter(
+      &Para, g->DensNum, g->TENum, g->Vel1Num, g->Vel2Num, g->Vel3Num, g->GENum,
+      ENZO_PPM_MAX_NUMBER_OF_BARYON_FIELDS, ggs, g->GridStartIndex, g->GridEndIndex,
+      g->GridDimension, CellWidthTemp ? CellWidthTemp[0][0] : 1.0,
+      CellWidthTemp ? CellWidthTemp[1][0] : 1.0, CellWidthTemp ? CellWidthTemp[2][0] : 1.0,
+      gl->Gamma, gl->PPMDiffusionParameter, gl->PPMFlatteningParameter,
+      gl->PPMSteepeningParameter, gl->ConservativeReconstruction, gl->PositiveReconstruction,
+      GravityOn, gl->DualEnergyFormalism, gl->DualEnergyFormalismEta1,
+      gl->DualEnergyFormalismEta2, gl->PressureFree, gl->tiny_number, gl->RiemannSolver,
+      gl->RiemannSolverFallback, NumberOfColours, colnum, g->GridRank, gl->SmallRho);
+  if (st != ENZO_PPM_SUCCESS) return st;
+  if (CellWidthTemp)
+    for (int d = 0; d < 3; d++) hipPPMSetCellWidth(&Para, d, CellWidthTemp[d]);
+  hipPPMData *local = nullptr;
+  hipPPMData **pd = Data ? Data : &local;
+  st = hipPPMInitData(pd, &Para, stream);
+  if (st != ENZO_PPM_SUCCESS) return st;
+  hipPPMData *D = *pd;
+  // stale pressure (Grid_SolvePPM_DE.C:58-61)
+  st = hipPPMSetBaryon(D, &Para, g->BaryonField, nullptr, g->AccelerationField);
+  if (st == ENZO_PPM_SUCCESS && MinimumSupportEnergyCoefficient != 0.0)
+    st = launch_pressure(D, &Para, MinimumSupportEnergyCoefficient, 0, 1.0);
+  if (st == ENZO_PPM_SUCCESS)
+    st = hipPPMSetSubgridFluxes(D, &Para, SubgridFluxes, NumberOfSubgrids, ggs);
+  const long long nact[3] = {g->GridEndIndex[0] - g->GridStartIndex[0] + 1,
+                             g->GridEndIndex[1] - g->GridStartIndex[1] + 1,
+                             g->GridEndIndex[2] - g->GridStartIndex[2] + 1};
+  const int rank = (int)g->GridRank;
+  const int ixyz = (int)(CycleNumber % rank);
+  for (int n = ixyz; n < ixyz + rank && st == ENZO_PPM_SUCCESS; n++) {
+    const int dir = n % rank;
+    if (nact[dir] <= 1) continue;
+    st = launch_sweep(D, &Para, g->dtFixed, nullptr, dir);
+    if (st == ENZO_PPM_SUCCESS && NumberOfSubgrids > 0)
+      st = hipPPMSaveSubgridFluxes(SubgridFluxes, NumberOfSubgrids, ggs, D, &Para, dir);
+  }
+  if (st == ENZO_PPM_SUCCESS) st = hipPPMGetBaryon(D, &Para, g->BaryonField);
+  if (!Data) hipPPMDestroy(local);
+  return st;
+}
+
+enzo_float *hipPPMDeviceField(hipPPMData *D, enzo_int field) {
+  if (!D) return nullptr;
+  if (field == -1) return D->pres;
+  if (field < 0 || field >= D->nslot) return nullptr;
+  return D->A + (size_t)field * D->cells;
+}
+
+int hipPPMPressureAndTimeStep(hipPPMData *D, const hipPPMParameter *P,
+                              enzo_float CourantSafetyNumber,
+                              enzo_float MinimumSupportEnergyCoefficient, enzo_float *dt_host) {
+  if (!D || !P) return ENZO_PPM_ERR_ARGUMENT;
+  int st = launch_pressure(D, P, MinimumSupportEnergyCoefficient, 1, CourantSafetyNumber);
+  if (st != ENZO_PPM_SUCCESS) return st;
+  if (dt_host) {
+    HIPCHK(hipMemcpyAsync(dt_host, D->dt, sizeof(double), hipMemcpyDeviceToHost, D->stream));
+    HIPCHK(hipStreamSynchronize(D->stream));
+  }
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMStep(hipPPMData *D, const hipPPMParameter *P, enzo_int CycleNumber,
+               enzo_float dtFixed) {
+  if (!D || !P) return ENZO_PPM_ERR_ARGUMENT;
+  const int rank = (int)P->GridRank;
+  const int ixyz = (int)(CycleNumber % rank);
+  for (int n = ixyz; n < ixyz + rank; n++) {
+    const int dir = n % rank;
+    if (P->GridEndIndex[dir] - P->GridStartIndex[dir] + 1 <= 1) continue;
+    int st = launch_sweep(D, P, dtFixed, dtFixed > 0 ? nullptr : D->dt, dir);
+    if (st != ENZO_PPM_SUCCESS) return st;
+  }
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMPeriodicBoundary(hipPPMData *D, const hipPPMParameter *P) {
+  if (!D || !P) return ENZO_PPM_ERR_ARGUMENT;
+  for (int a = 0; a < P->GridRank; a++) {
+    BCParams B;
+    B.axis = a;
+    B.ghost = (int)P->GridStartIndex[a];
+    B.nslot = D->nslot;
+    for (int d = 0; d < 3; d++) {
+      B.GD[d] = D->GD[d];
+      B.act[d] = (int)(P->GridEndIndex[d] - P->GridStartIndex[d] + 1);
+    }
+    if (P->GridDimension[a] - P->GridEndIndex[a] - 1 != B.ghost) return ENZO_PPM_ERR_ARGUMENT;
+    B.cells = D->cells;
+    B.S = D->A;
+    long long n = (long long)2 * B.ghost * B.GD[(a + 1) % 3] * B.GD[(a + 2) % 3] * B.nslot;
+    hipLaunchKernelGGL(periodic_axis_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       D->stream, B);
+    HIPCHK(hipGetLastError());
+  }
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMProfile(hipPPMData *D, enzo_int enable) {
+  if (!D) return ENZO_PPM_ERR_ARGUMENT;
+  D->profile = (int)enable;
+  D->ev_used = 0;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMProfileRead(hipPPMData *D, enzo_float *total_ms, enzo_long_int *launches) {
+  if (!D || !total_ms || !launches) return ENZO_PPM_ERR_ARGUMENT;
+  HIPCHK(hipStreamSynchronize(D->stream));
+  double tot = 0.0;
+  for (size_t k = 0; k + 1 < D->ev_used; k += 2) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, D->ev[k], D->ev[k + 1]));
+    tot += ms;
+  }
+  *total_ms = tot;
+  *launches = (enzo_long_int)(D->ev_used / 2);
+  D->ev_used = 0;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMSynchronize(hipPPMData *D) {
+  if (!D) return ENZO_PPM_ERR_ARGUMENT;
+  HIPCHK(hipStreamSynchronize(D->stream));
+  return ENZO_PPM_SUCCESS;
+}
+
+const char *hipPPMVersion(void) { return "enzo_ppm 0.1 gfx950 fp64"; }
+
+const char *hipPPMLastError(void) { return g_last_error; }
+
+int hipPPMDeviceCount(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+}  // extern "C"

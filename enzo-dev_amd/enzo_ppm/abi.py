@@ -161,7 +161,11 @@ PROTOTYPES = {
                                             C.c_double, PF]),
     "hipPPMStep": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_int64, C.c_double]),
     "hipPPMPeriodicBoundary": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter)]),
+    "hipPPMProfile": (C.c_int, [C.c_void_p, C.c_int64]),
+    "hipPPMProfileRead": (C.c_int, [C.c_void_p, PF, PI]),
+    "hipPPMSynchronize": (C.c_int, [C.c_void_p]),
     "hipPPMVersion": (C.c_char_p, []),
+    "hipPPMLastError": (C.c_char_p, []),
     "hipPPMDeviceCount": (C.c_int, []),
 }
 

@@ -1,0 +1,113 @@
+"""Python driver over the HIP library (include/enzo_ppm.h).
+
+Every call goes through the C ABI; if the library is missing or no GPU is
+visible the calls raise -- there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .grid import make_globals
+
+
+def _check(st, what):
+    if st != abi.SUCCESS:
+        err = abi.load_library().hipPPMLastError().decode()
+        raise RuntimeError(f"{what} failed with status {st}: {err}")
+
+
+def solve_ppm_de(grid, glob, cycle=0, fluxes=None, handle=None, min_support=0.0, lib=None):
+    """grid::SolvePPM_DE on the GPU through hipPPMSolvePPM_DE; updates
+    `grid` in place and fills `fluxes` (a grid.SubgridFluxes) if given.
+    Returns the status code (SUCCESS or a negative error)."""
+    L = lib or abi.load_library()
+    gs, _ = make_globals(**glob)
+    cg, keep = grid.ctypes_grid()
+    cw = grid.ctypes_cellwidth()
+    colnum = abi.iarr(grid.colnum, max(1, len(grid.colnum)))
+    nsub = fluxes.n if fluxes is not None else 0
+    fptr = fluxes.ptrs if fluxes is not None else (C.POINTER(abi.EnzoFluxes) * 1)()
+    gstart = fluxes.global_start if fluxes is not None else abi.iarr([0, 0, 0])
+    st = L.hipPPMSolvePPM_DE(C.byref(cg), C.byref(gs), cycle, nsub, fptr, cw, gstart,
+                             int(grid.gravity), grid.ncolours, colnum, min_support,
+                             C.byref(handle) if handle is not None else None, None)
+    return st
+
+
+class DeviceGrid:
+    """Device-resident state of one grid for multi-step runs: upload once,
+    then pressure + Courant pass and rotated sweeps stay on the GPU."""
+
+    def __init__(self, grid, glob, lib=None, stream=None):
+        self.lib = L = lib or abi.load_library()
+        self.grid = grid
+        self.glob = dict(glob)
+        _, gd = make_globals(**glob)
+        self.gdict = gd
+        self.para = grid.hip_parameter(L, gd)
+        self.handle = C.c_void_p(None)
+        _check(L.hipPPMInitData(C.byref(self.handle), C.byref(self.para), stream), "InitData")
+        self._keep = None
+        self.upload()
+
+    def upload(self):
+        g = self.grid
+        bf = (abi.PF * abi.MAX_NUMBER_OF_BARYON_FIELDS)()
+        for i, f in enumerate(g.fields):
+            bf[i] = abi.dptr(f)
+        acc = (abi.PF * 3)()
+        if g.accel is not None:
+            for i in range(3):
+                acc[i] = abi.dptr(g.accel[i])
+        self._keep = (bf, acc)
+        _check(self.lib.hipPPMSetBaryon(self.handle, C.byref(self.para), bf, None, acc),
+               "SetBaryon")
+
+    def download(self):
+        g = self.grid
+        bf = (abi.PF * abi.MAX_NUMBER_OF_BARYON_FIELDS)()
+        for i, f in enumerate(g.fields):
+            bf[i] = abi.dptr(f)
+        return self.lib.hipPPMGetBaryon(self.handle, C.byref(self.para), bf)
+
+    def pressure_and_dt(self, cfl, min_support=0.0, want_host=True):
+        dt = C.c_double(0.0)
+        _check(self.lib.hipPPMPressureAndTimeStep(self.handle, C.byref(self.para), cfl,
+                                                   min_support,
+                                                   C.byref(dt) if want_host else None),
+               "PressureAndTimeStep")
+        return dt.value
+
+    def step(self, cycle, dt=0.0):
+        """Rotated sweeps; dt <= 0 uses the device dt of pressure_and_dt."""
+        _check(self.lib.hipPPMStep(self.handle, C.byref(self.para), cycle, dt), "Step")
+
+    def periodic_boundary(self):
+        _check(self.lib.hipPPMPeriodicBoundary(self.handle, C.byref(self.para)), "Boundary")
+
+    def synchronize(self):
+        _check(self.lib.hipPPMSynchronize(self.handle), "Synchronize")
+
+    def profile(self, enable=True):
+        _check(self.lib.hipPPMProfile(self.handle, int(enable)), "Profile")
+
+    def profile_read(self):
+        ms = C.c_double(0.0)
+        n = C.c_int64(0)
+        _check(self.lib.hipPPMProfileRead(self.handle, C.byref(ms), C.byref(n)), "ProfileRead")
+        return ms.value, n.value
+
+    def fallback_count(self):
+        return self.lib.hipPPMFallbackCount(self.handle)
+
+    def close(self):
+        if self.handle.value:
+            self.lib.hipPPMDestroy(self.handle)
+            self.handle = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

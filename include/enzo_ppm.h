@@ -240,8 +240,21 @@ int hipPPMStep(hipPPMData *Data, const hipPPMParameter *Para, enzo_int CycleNumb
    (the periodic case of ExternalBoundary::SetExternalBoundary). */
 int hipPPMPeriodicBoundary(hipPPMData *Data, const hipPPMParameter *Para);
 
+/* Timing of the fused sweep kernel: with profiling enabled every sweep
+   kernel launch is bracketed by HIP events on the handle's stream;
+   hipPPMProfileRead synchronises, returns the summed kernel time (ms) and the
+   number of launches since the last read, and resets the record. */
+int hipPPMProfile(hipPPMData *Data, enzo_int enable);
+int hipPPMProfileRead(hipPPMData *Data, enzo_float *total_ms, enzo_long_int *launches);
+
+/* Wait for all work queued on the handle's stream. */
+int hipPPMSynchronize(hipPPMData *Data);
+
 /* Library version string and device count check. */
 const char *hipPPMVersion(void);
+
+/* Text of the last HIP failure seen by this thread ("" if none). */
+const char *hipPPMLastError(void);
 int hipPPMDeviceCount(void);
 
 #ifdef __cplusplus

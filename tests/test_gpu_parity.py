@@ -1,0 +1,132 @@
+"""Parity of the HIP library (through the C ABI) with the oracle.
+
+The oracle's C restatement is itself pinned bit-for-bit to the reference
+Fortran (tests/test_oracle_reference.py and the golden fixtures); when the
+reference build travels with the tree (oracle/_ref), it is compared directly
+as well.  Bar: bitwise equality of every baryon field and captured face flux;
+the north-star tolerances (max relative 1e-11 after one step) are asserted
+as the hard limit and bitwise equality is reported.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from enzo_ppm import abi, device, problems
+from enzo_ppm.grid import SubgridFluxes
+from helpers import bit_equal, field_report, fill_bc, max_rel, oracle_steps
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    L = abi.load_library()
+    if L.hipPPMDeviceCount() < 1:
+        pytest.fail("HIP library loaded but no GPU is visible")
+    return L
+
+
+def _one_step(lib, g, glob, cfl, cycle, boxes=None, bc="periodic"):
+    fill_bc(g, bc)
+    g.dt = oracle.compute_time_step("port", g, glob, cfl)
+    a, b = g.copy(), g.copy()
+    fa = SubgridFluxes(a, boxes) if boxes else None
+    fb = SubgridFluxes(b, boxes) if boxes else None
+    st = device.solve_ppm_de(a, glob, cycle=cycle, fluxes=fa, lib=lib)
+    assert st == abi.SUCCESS, (st, lib.hipPPMLastError())
+    sto, fbk = oracle.solve_ppm_de("port", b, glob, cycle=cycle, fluxes=fb)
+    assert sto == abi.SUCCESS
+    return a, b, fa, fb, fbk
+
+
+CASES = {
+    "random_colours": lambda: problems.random_smooth([12, 10, 14], seed=1, ncolours=2),
+    "gravity": lambda: problems.random_smooth([10, 12, 8], seed=2, ncolours=1, gravity=True),
+    "no_dual_energy": lambda: problems.random_smooth([10, 12, 8], seed=3, dual_energy=False),
+    "diffusion": lambda: problems.random_smooth([10, 12, 8], seed=4, ncolours=1, flags=(1, 1, 1)),
+    "no_flatten_steepen": lambda: problems.random_smooth([16, 9, 11], seed=6, flags=(0, 0, 0)),
+    "sod_2d": lambda: problems.sod_x(20, rank=2, gamma=1.4, dual_energy=True, flags=(1, 1, 1)),
+    "turbulent_40": lambda: problems.turbulent(40, seed=3),
+    "turbulent_mach30": lambda: problems.turbulent(24, seed=4, mach=30, sigma_lnrho=1.5),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("cycle", [0, 1, 2])
+def test_one_step_matches_oracle(lib, name, cycle):
+    g, glob, run = CASES[name]()
+    a, b, _, _, _ = _one_step(lib, g, glob, run["cfl"], cycle)
+    rep = field_report(a, b)
+    worst = max(r[2] for r in rep)
+    print(f"{name} cycle {cycle}: bitwise={all(r[1] for r in rep)} max_rel={worst:.3e}")
+    assert worst <= 1e-11, rep
+    assert all(r[1] for r in rep), rep
+
+
+def test_config1_sod_64(lib):
+    """64^3 Sod along x, diffusion/flattening/steepening on, one step (cycle 0)."""
+    g, glob, run = problems.sod_x(64)
+    a, b, _, _, _ = _one_step(lib, g, glob, run["cfl"], 0)
+    rep = field_report(a, b)
+    assert all(r[1] for r in rep), rep
+    # mass conserved to round-off over the periodic box
+    sl = a.active_slice()
+    m0 = g.d[sl].sum()
+    assert abs(a.d[sl].sum() - m0) <= 1e-12 * m0
+
+
+def test_subgrid_face_fluxes(lib):
+    g, glob, run = problems.random_smooth([20, 18, 16], seed=5, ncolours=2)
+    boxes = [((7, 5, 4), (12, 9, 8)), ((3, 3, 3), (22, 20, 18)), ((10, 3, 3), (10, 20, 18))]
+    for cycle in range(3):
+        a, b, fa, fb, _ = _one_step(lib, g, glob, run["cfl"], cycle, boxes=boxes)
+        assert all(r[1] for r in field_report(a, b))
+        assert bit_equal(fa.flat(), fb.flat())
+        assert np.count_nonzero(fa.flat()) > 0.9 * fa.flat().size
+
+
+def test_rank1_sod(lib):
+    g, glob, run = problems.sod_x(50, rank=1, gamma=1.4, dual_energy=False, flags=(0, 0, 0))
+    for cycle in range(5):
+        a, b, _, _, _ = _one_step(lib, g, glob, 0.9, cycle, bc="outflow")
+        assert all(r[1] for r in field_report(a, b))
+        g = a
+
+
+def test_against_reference_build(lib):
+    if not oracle.available("ref"):
+        pytest.skip("oracle/_ref not built in this tree")
+    g, glob, run = problems.turbulent(32, seed=8)
+    g.fill_periodic()
+    g.dt = oracle.compute_time_step("ref", g, glob, run["cfl"])
+    a, b = g.copy(), g.copy()
+    assert device.solve_ppm_de(a, glob, cycle=1, lib=lib) == abi.SUCCESS
+    assert oracle.solve_ppm_de("ref", b, glob, cycle=1)[0] == abi.SUCCESS
+    assert all(r[1] for r in field_report(a, b))
+
+
+def test_device_resident_steps(lib):
+    """Pressure + Courant pass, rotated sweeps and periodic ghost fill all on
+    the device for several steps, against the oracle step loop."""
+    g, glob, run = problems.turbulent(32, seed=11)
+    ref = g.copy()
+    dg = device.DeviceGrid(g, glob, lib=lib)
+    nsteps = 6
+    dts = []
+    for n in range(nsteps):
+        dg.periodic_boundary()
+        dts.append(dg.pressure_and_dt(run["cfl"]))
+        dg.step(n)
+    assert dg.download() == abi.SUCCESS
+    dg.close()
+    for n in range(nsteps):
+        fill_bc(ref, "periodic")
+        ref.dt = oracle.compute_time_step("port", ref, glob, run["cfl"])
+        assert ref.dt == dts[n], (n, ref.dt, dts[n])
+        assert oracle.solve_ppm_de("port", ref, glob, cycle=n)[0] == abi.SUCCESS
+    # the device state after the last step has no ghost fill; compare actives
+    sl = g.active_slice()
+    for x, y in zip(g.fields, ref.fields):
+        assert bit_equal(x[sl], y[sl])
