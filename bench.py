@@ -41,7 +41,22 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="active cells per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of known size (1 GiB read + 1 GiB write, "
+                         "3 times) to scale HBM byte counters")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 27  # doubles = 1 GiB
+    x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8,
+                      "launches": 3}), flush=True)
 
 
 def cpu_baseline(n, threads):
@@ -66,6 +81,9 @@ def cpu_baseline(n, threads):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not built)"}))

@@ -44,10 +44,12 @@ struct DevExec {
   }
 };
 
-__global__ __launch_bounds__(kTP *kTS) void ppm_sweep_kernel(SweepParams P) {
+template <int UNI>
+__global__ __launch_bounds__(kTP *kTS) __attribute__((amdgpu_waves_per_eu(4)))
+void ppm_sweep_kernel(SweepParams P) {
   __shared__ double sm[SweepCfg::SMEM];
   DevExec ex;
-  TileCounts tc = sweep_tile<kTP, kTS>(P, (int)blockIdx.x, sm, ex);
+  TileCounts tc = sweep_tile<kTP, kTS, UNI>(P, (int)blockIdx.x, sm, ex);
   if (threadIdx.x == 0) {
     if (tc.bad) atomicAdd(&P.counters[0], (unsigned long long)tc.bad);
     if (tc.chains) atomicAdd(&P.counters[1], (unsigned long long)tc.chains);
@@ -56,16 +58,25 @@ __global__ __launch_bounds__(kTP *kTS) void ppm_sweep_kernel(SweepParams P) {
 }
 
 // ------------------------------------------------------------------------
-// Flattening of line 1 of every slice.  The reference hands `flatten` (line 1)
-// to intvar for every line of the slice (inteuler.F:241-289), so one
-// coefficient row per slice is all the sweep needs.  Thread per
-// (slice, cell in [i1-1, i2+1]).
-__global__ void flat_row_kernel(FlatParams F) {
-  const int ncell = F.i2 - F.i1 + 3;
+// Per-sweep preparation: qc(i) = dt/dx(i) for the whole axis (used by the
+// fluxes, flux_twoshock.F:366, and the gas-energy source, :393), and the
+// flattening of line 1 of every slice.  The reference hands `flatten`
+// (line 1) to intvar for every line of the slice (inteuler.F:241-289), so one
+// coefficient row per slice is all the sweep needs.  Threads [0, N) fill qc;
+// the rest one (slice, cell in [i1-1, i2+1]) each.
+__global__ void sweep_prep_kernel(FlatParams F) {
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long long)ncell * F.n_outer) return;
-  const int outer = (int)(gid / ncell);
-  const int cell = F.i1 - 1 + (int)(gid % ncell);
+  if (gid < F.N) {
+    const double dt = F.dtp ? *F.dtp : F.dt;
+    F.qc[gid] = dt / F.dx[gid];
+    return;
+  }
+  if (!F.iflatten) return;
+  const long long k = gid - F.N;
+  const int ncell = F.i2 - F.i1 + 3;
+  if (k >= (long long)ncell * F.n_outer) return;
+  const int outer = (int)(k / ncell);
+  const int cell = F.i1 - 1 + (int)(k % ncell);
   F.flat[(long long)outer * F.N + (cell - 1)] = flat_value(F, outer, cell);
 }
 
@@ -219,10 +230,13 @@ struct hipPPMData {
   double *cwdev[3] = {nullptr, nullptr, nullptr};
   double *flat = nullptr;
   long long flat_cap = 0;
+  double *qc = nullptr;  // dt/dx of the current sweep axis
   double *dt = nullptr;
   unsigned long long *dtmin = nullptr;
   unsigned long long *counters = nullptr;  // [0] fallback [1] DE chain [2] errors
   int *segbuf = nullptr;  // per axis: o[nseg], w[nseg]
+  int uni[3] = {0, 0, 0};  // axis has one cell width
+  double kt[3][NTAB];      // its coefficient entries
   int segoff[3] = {0, 0, 0}, nseg[3] = {0, 0, 0};
   // face capture
   int *face_head[3] = {nullptr, nullptr, nullptr};
@@ -301,7 +315,7 @@ void free_data(hipPPMData *D) {
   for (hipEvent_t e : D->ev) (void)hipEventDestroy(e);
   D->ev.clear();
   void *ptrs[] = {D->A, D->B, D->pres, D->accel, D->tab[0], D->tab[1], D->tab[2], D->cwdev[0],
-                  D->cwdev[1], D->cwdev[2], D->flat, D->dt, D->dtmin, D->counters, D->segbuf,
+                  D->cwdev[1], D->cwdev[2], D->flat, D->qc, D->dt, D->dtmin, D->counters, D->segbuf,
                   D->face_head[0], D->face_head[1], D->face_head[2], D->faces[0], D->faces[1],
                   D->faces[2], D->face_out[0], D->face_out[1], D->face_out[2]};
   for (void *p : ptrs)
@@ -315,18 +329,23 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
   const int rax = (dim + 1) % 3, oax = (dim + 2) % 3;
   const int rank = (int)P->GridRank;
   const int iflat = (int)P->PPMFlatteningParameter;
-  // 1) flattening row of line 1 of every slice
-  if (iflat) {
+  // 1) dt/dx table and flattening row of line 1 of every slice
+  {
     FlatParams F;
     F.dim = dim; F.N = N; F.i1 = i1; F.i2 = i2;
     for (int d = 0; d < 3; d++) F.GD[d] = D->GD[d];
     F.cells = D->cells;
     F.n_outer = D->GD[oax];
+    F.iflatten = iflat;
     F.A = D->A;
     F.pres = D->pres;
     F.flat = D->flat;
-    long long n = (long long)(i2 - i1 + 3) * F.n_outer;
-    hipLaunchKernelGGL(flat_row_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+    F.dx = D->cwdev[dim];
+    F.dt = dtFixed;
+    F.dtp = dtp;
+    F.qc = D->qc;
+    long long n = N + (iflat ? (long long)(i2 - i1 + 3) * F.n_outer : 0);
+    hipLaunchKernelGGL(sweep_prep_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        D->stream, F);
     HIPCHK(hipGetLastError());
   }
@@ -347,6 +366,15 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
   S.tab = D->tab[dim];
   S.tab_row = D->tab[rax];
   S.tab_out = D->tab[oax];
+  S.qc = D->qc;
+  S.uni[0] = D->uni[dim];
+  S.uni[1] = D->uni[rax];
+  S.uni[2] = D->uni[oax];
+  for (int t = 0; t < NTAB; t++) {
+    S.kt[t] = D->kt[dim][t];
+    S.kt_row[t] = D->kt[rax][t];
+    S.kt_out[t] = D->kt[oax][t];
+  }
   S.gamma = P->Gamma;
   S.eta1 = P->DualEnergyFormalismEta1;
   S.eta2 = P->DualEnergyFormalismEta2;
@@ -391,7 +419,10 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
       D->ev_used += 2;
       HIPCHK(hipEventRecord(e0, D->stream));
     }
-    hipLaunchKernelGGL(ppm_sweep_kernel, dim3((unsigned)nblk), dim3(kTP * kTS), 0, D->stream, S);
+    if (S.uni[0])
+      hipLaunchKernelGGL(ppm_sweep_kernel<1>, dim3((unsigned)nblk), dim3(kTP * kTS), 0, D->stream, S);
+    else
+      hipLaunchKernelGGL(ppm_sweep_kernel<0>, dim3((unsigned)nblk), dim3(kTP * kTS), 0, D->stream, S);
     HIPCHK(hipGetLastError());
     if (D->profile) HIPCHK(hipEventRecord(e1, D->stream));
   } else {
@@ -405,8 +436,12 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
     for (int k = 0; k < D->GD[oax]; k++) {
       S.t2_0 = k;
       long long nblk = (long long)S.nseg * S.nblk_t1;
-      hipLaunchKernelGGL(ppm_sweep_kernel, dim3((unsigned)nblk), dim3(kTP * kTS), 0, D->stream,
-                         S);
+      if (S.uni[0])
+        hipLaunchKernelGGL(ppm_sweep_kernel<1>, dim3((unsigned)nblk), dim3(kTP * kTS), 0,
+                           D->stream, S);
+      else
+        hipLaunchKernelGGL(ppm_sweep_kernel<0>, dim3((unsigned)nblk), dim3(kTP * kTS), 0,
+                           D->stream, S);
       HIPCHK(hipGetLastError());
     }
   }
@@ -562,7 +597,9 @@ int hipPPMInitData(hipPPMData **Data, const hipPPMParameter *Para, void *stream)
       fcap = std::max(fcap, (long long)D->GD[d] * D->GD[(d + 2) % 3]);
     }
     D->flat_cap = fcap;
+    int nmax = std::max(D->GD[0], std::max(D->GD[1], D->GD[2]));
     if (alloc_dev((void **)&D->flat, sizeof(double) * fcap) ||
+        alloc_dev((void **)&D->qc, sizeof(double) * nmax) ||
         alloc_dev((void **)&D->dt, sizeof(double)) ||
         alloc_dev((void **)&D->dtmin, sizeof(unsigned long long)) ||
         alloc_dev((void **)&D->counters, 4 * sizeof(unsigned long long)))
@@ -578,6 +615,7 @@ int hipPPMInitData(hipPPMData **Data, const hipPPMParameter *Para, void *stream)
       dx[i] = Para->CellWidth[d] ? Para->CellWidth[d][i] : (d < Para->GridRank ? Para->dx[d] : 1.0);
     const int i1 = (int)Para->GridStartIndex[d] + 1, i2 = (int)Para->GridEndIndex[d] + 1;
     build_tables(dx.data(), N, tab);
+    D->uni[d] = uniform_axis(dx.data(), N, tab, D->kt[d]) ? 1 : 0;
     HIPCHK(hipMemcpy(D->tab[d], tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(D->cwdev[d], dx.data(), sizeof(double) * N, hipMemcpyHostToDevice));
     std::vector<int> o, w;

@@ -46,11 +46,23 @@ inline void build_tables(const double *dx, int N, std::vector<double> &t) {
   }
 }
 
+// If every width of the axis is the same double, every defined table entry is
+// the same double too: returns true and the entries (taken at an interior
+// cell) in k[NTAB].
+inline bool uniform_axis(const double *dx, int N, const std::vector<double> &t, double *k) {
+  for (int i = 1; i < N; i++)
+    if (dx[i] != dx[0]) return false;
+  int c = N >= 5 ? N / 2 + 1 : 0;  // 1-based interior cell with every entry defined
+  for (int tt = 0; tt < NTAB; tt++) k[tt] = c ? t[(size_t)tt * N + (c - 1)] : 0.0;
+  if (!c) k[T_DX] = dx[0];
+  return true;
+}
+
 // Split cells 1..N into segments owned by one tile each; every tile must
 // fit its interfaces in TS threads and its loaded cells in LW.
 template <int TP, int TS>
 inline bool build_segments(int N, int i1, int i2, std::vector<int> &o, std::vector<int> &w) {
-  for (int nseg = (N + TS - 4) / (TS - 3); nseg <= N; nseg++) {
+  for (int nseg = (N + TS - 8) / (TS - 7); nseg <= N; nseg++) {
     o.clear();
     w.clear();
     int base = N / nseg, rem = N % nseg, start = 1;
@@ -62,7 +74,8 @@ inline bool build_segments(int N, int i1, int i2, std::vector<int> &o, std::vect
       int NI = IB - IA + 1;
       int L0 = std::min(IA - 3, std::max(start - 1, 1));
       int L1 = std::max(IB + 2, std::min(start + ww, N));
-      if (NI > TS || L1 - L0 + 1 > Cfg<TP, TS>::LW || L0 < 1 || L1 > N) { ok = false; break; }
+      // one slope item per thread (cells [IA-2, IB+1]) and the loaded cells in LW
+      if (NI + 4 > TS || L1 - L0 + 1 > Cfg<TP, TS>::LW || L0 < 1 || L1 > N) { ok = false; break; }
       o.push_back(start);
       w.push_back(ww);
       start += ww;

@@ -82,6 +82,12 @@ struct SweepParams {
   const double *gr;       // acceleration along dim (gravity) or null
   const double *flat;     // flattening of line 1 of each slice: [outer][N]
   const double *tab, *tab_row, *tab_out;  // tables of sweep, row, outer axes
+  const double *qc;       // dt/dx(i) of this sweep, i = 1..N (prep kernel)
+  // Uniform axes (Enzo grids have one cell width per axis): every table
+  // entry of the sweep / row / outer axis is the same double, passed here so
+  // that no table is read.  uni[]: sweep, row, outer.
+  int uni[3];
+  double kt[NTAB], kt_row[NTAB], kt_out[NTAB];
   double gamma, eta1, eta2, pmin, dfloor, dt;
   const double *dtp;      // if non-null, dt = *dtp
   int isteep, iflatten, idiff, idual, ipresfree, gravity, ifallback;
@@ -95,16 +101,17 @@ struct SweepParams {
 // ---------------------------------------------------------------------------
 // LDS layout: NARR arrays of [TP][LW] doubles.
 enum Arr {
-  A_D = 0, A_E, A_U, A_V, A_W, A_GE, A_P, A_GR, A_CH1, A_CH2, A_CM, A_C0, A_CP,
+  A_D = 0, A_E, A_U, A_V, A_W, A_GE, A_P, A_GR, A_CS, A_U0,
   A_ST, A_DQ, A_QL, A_QR, A_Q6, A_COL, A_X, NARR
 };
 // flux arrays alias arrays that are dead after the interface solve
 constexpr int F_DF = A_DQ, F_EF = A_QL, F_UF = A_QR, F_VF = A_Q6, F_WF = A_ST,
-              F_GEF = A_P, F_UB = A_X, F_COLF = A_ST, A_EOUT = A_X;
+              F_GEF = A_P, F_UB = A_X, F_COLF = A_ST, A_EOUT = A_X, A_GE1 = A_DQ,
+              A_HOT = A_QR;
 
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 struct Cfg {
-  static constexpr int LW = TS + 8;
+  static constexpr int LW = TS + 4;
   static constexpr int NT = TP * TS;
   static constexpr int SMEM = NARR * TP * LW;  // doubles
 };
@@ -115,6 +122,7 @@ struct Tile {
   int L0, L1;        // loaded cells [L0, L1]
   int t1_0, t2;      // first pencil (t1) of the block, pencil row t2
   int np;            // pencils in this block
+  double dt;         // time step, read once per block (it may live on the device)
 };
 
 struct Regs {
@@ -127,19 +135,54 @@ struct Regs {
   double dls_pre, drs_pre, dls, drs, db, ub;
   double df, ef, uf, vf, wf, gef;
   int ovl, ovr, upl, upr;  // DE overrides; upwind choice (u(i-1)<=0, u(i)>=0)
+  double flat_c;           // flattening at this thread's parabola cell
 };
 
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD double &LDS(double *sm, int arr, int p, int c) {
-  PPM_ASSERT(arr >= 0 && arr < NARR && p >= 0 && p < TP && c >= 0 && c < Cfg<TP, TS>::LW);
-  return sm[(arr * TP + p) * Cfg<TP, TS>::LW + c];
+  PPM_ASSERT(arr >= 0 && arr < NARR && p >= 0 && p < TP && c >= 0 && c < Cfg<TP, TS, UNI>::LW);
+  return sm[(arr * TP + p) * Cfg<TP, TS, UNI>::LW + c];
 }
 
 PPM_HD double dtval(const SweepParams &P) { return P.dtp ? *P.dtp : P.dt; }
 
+// characteristic distances of one cell (inteuler.F:201-208)
+struct Chars {
+  double ch1, ch2, cm, c0, cp;
+};
+PPM_HD Chars chars_of(double u, double cs, double dt, double dx2i) {
+  Chars r;
+  r.ch1 = fmx(0.0, dt * (u + cs)) * dx2i;
+  r.ch2 = fmx(0.0, -(dt * (u - cs))) * dx2i;
+  r.cm = dt * (u - cs) * dx2i;
+  r.c0 = dt * (u)*dx2i;
+  r.cp = dt * (u + cs) * dx2i;
+  return r;
+}
+
 PPM_HD double tabv(const double *tab, int N, int t, int cell) {
   PPM_ASSERT(cell >= 1 && cell <= N && t >= 0 && t < NTAB);
   return tab[t * N + (cell - 1)];
+}
+
+// coefficient t at `cell` of the sweep axis (row / outer axis for TR / TO);
+// UNI: the sweep axis has one cell width (kernel instantiated for it)
+template <int UNI>
+PPM_HD double TS_(const SweepParams &P, int t, int cell) {
+  if (UNI) return P.kt[t];
+  return tabv(P.tab, P.N, t, cell);
+}
+PPM_HD double TR_(const SweepParams &P, int t, int cell, int n) {
+  return P.uni[1] ? P.kt_row[t] : tabv(P.tab_row, n, t, cell);
+}
+PPM_HD double TO_(const SweepParams &P, int t, int cell, int n) {
+  return P.uni[2] ? P.kt_out[t] : tabv(P.tab_out, n, t, cell);
+}
+// dt/dx(i) (flux_twoshock.F:366)
+template <int UNI>
+PPM_HD double qc_of(const SweepParams &P, double dt, int cell) {
+  if (UNI) return dt / P.kt[T_DX];
+  return P.qc[cell - 1];
 }
 
 // 3-D grid index of pencil (t1, t2) at sweep cell `cell` (1-based)
@@ -157,7 +200,7 @@ PPM_HD int coord_of(const SweepParams &P, int axis, int t1, int t2) {
   return axis == P.ax_t1 ? t1 : t2;
 }
 
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD Tile make_tile(const SweepParams &P, int block) {
   Tile T;
   int seg = block % P.nseg;
@@ -175,19 +218,34 @@ PPM_HD Tile make_tile(const SweepParams &P, int block) {
   T.L0 = lo < lo2 ? lo : lo2;
   int hi = T.IB + 2, hi2 = T.o + T.w < P.N ? T.o + T.w : P.N;
   T.L1 = hi > hi2 ? hi : hi2;
+  T.dt = dtval(P);
   return T;
 }
 
 // ---------------------------------------------------------------------------
 // Phase: load the pencil segments into LDS (coalesced along the contiguous
 // grid axis: s when sweeping x, t1 otherwise).
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
+PPM_HD double flat_of(const SweepParams &P, const Tile &T, int p, int cell);
+
+// the thread's parabola cell (same mapping as ph_parab: one item per thread)
+template <int TP, int TS, int UNI = 0>
+PPM_HD void ph_flat_reg(const SweepParams &P, const Tile &T, int tid, Regs &r) {
+  const int n = T.NI + 2;
+  r.flat_c = 0.0;
+  if (P.iflatten == 0 || tid >= TP * n) return;
+  const int p = tid / n, cell = T.IA - 1 + tid % n;
+  if (p >= T.np) return;
+  r.flat_c = flat_of<TP, TS, UNI>(P, T, p, cell);
+}
+
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_load(const SweepParams &P, const Tile &T, double *sm, int tid) {
   const int nc = T.L1 - T.L0 + 1;
   const int total = TP * nc;
   const bool s_contig = (P.dim == 0);
   const int vax = (P.dim + 1) % 3, wax = (P.dim + 2) % 3;
-  for (int idx = tid; idx < total; idx += Cfg<TP, TS>::NT) {
+  for (int idx = tid; idx < total; idx += Cfg<TP, TS, UNI>::NT) {
     int p, c;
     if (s_contig) { p = idx / nc; c = idx % nc; }
     else { p = idx % TP; c = idx / TP; }
@@ -195,105 +253,102 @@ PPM_HD void ph_load(const SweepParams &P, const Tile &T, double *sm, int tid) {
     int cell = T.L0 + c;
     long long g = gidx(P, T.t1_0 + p, T.t2, cell);
     const double *A = P.A;
-    LDS<TP, TS>(sm, A_D, p, c) = A[S_D * P.cells + g];
-    LDS<TP, TS>(sm, A_E, p, c) = A[S_E * P.cells + g];
-    LDS<TP, TS>(sm, A_U, p, c) = A[(S_VX + P.dim) * P.cells + g];
-    LDS<TP, TS>(sm, A_V, p, c) = vax < P.rank ? A[(S_VX + vax) * P.cells + g] : 0.0;
-    LDS<TP, TS>(sm, A_W, p, c) = wax < P.rank ? A[(S_VX + wax) * P.cells + g] : 0.0;
-    if (P.idual) LDS<TP, TS>(sm, A_GE, p, c) = A[S_GE * P.cells + g];
-    LDS<TP, TS>(sm, A_P, p, c) = P.pres[g];
-    if (P.gravity) LDS<TP, TS>(sm, A_GR, p, c) = P.gr[g];
+    LDS<TP, TS, UNI>(sm, A_D, p, c) = A[S_D * P.cells + g];
+    LDS<TP, TS, UNI>(sm, A_E, p, c) = A[S_E * P.cells + g];
+    const double uu = A[(S_VX + P.dim) * P.cells + g];
+    LDS<TP, TS, UNI>(sm, A_U, p, c) = uu;
+    LDS<TP, TS, UNI>(sm, A_U0, p, c) = uu;
+    LDS<TP, TS, UNI>(sm, A_V, p, c) = vax < P.rank ? A[(S_VX + vax) * P.cells + g] : 0.0;
+    LDS<TP, TS, UNI>(sm, A_W, p, c) = wax < P.rank ? A[(S_VX + wax) * P.cells + g] : 0.0;
+    if (P.idual) LDS<TP, TS, UNI>(sm, A_GE, p, c) = A[S_GE * P.cells + g];
+    LDS<TP, TS, UNI>(sm, A_P, p, c) = P.pres[g];
+    if (P.gravity) LDS<TP, TS, UNI>(sm, A_GR, p, c) = P.gr[g];
   }
 }
 
-// Phase: characteristic distances (inteuler.F:197-208) at cells [IA-1, IB]
-// and density second differences (inteuler.F:166-173) at [IA-2, IB+1] (A_DQ).
-template <int TP, int TS>
+// Phase: sound speed (inteuler.F:199-200) at cells [IA-1, IB] into A_CS,
+// and density second differences (inteuler.F:166-173) at [IA-2, IB+1]
+// (A_DQ).  The characteristic distances of inteuler.F:201-208 are rebuilt
+// from (u, cs) where they are used (char_* below), bit for bit.
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_chars(const SweepParams &P, const Tile &T, double *sm, int tid) {
-  const double dt = dtval(P);
   const int nch = T.NI + 2;
-  for (int idx = tid; idx < TP * nch; idx += Cfg<TP, TS>::NT) {
+  for (int idx = tid; idx < TP * nch; idx += Cfg<TP, TS, UNI>::NT) {
     int p = idx / nch, cell = T.IA - 1 + idx % nch;
     if (p >= T.np) continue;
     int c = cell - T.L0;
-    double pp = LDS<TP, TS>(sm, A_P, p, c), dd = LDS<TP, TS>(sm, A_D, p, c);
-    double u = LDS<TP, TS>(sm, A_U, p, c);
+    double pp = LDS<TP, TS, UNI>(sm, A_P, p, c), dd = LDS<TP, TS, UNI>(sm, A_D, p, c);
     double cs = sqrt(P.gamma * pp / dd);
     if (P.ipresfree == 1) cs = kTiny;
-    double dx2i = tabv(P.tab, P.N, T_DX2I, cell);
-    LDS<TP, TS>(sm, A_CH1, p, c) = fmx(0.0, dt * (u + cs)) * dx2i;
-    LDS<TP, TS>(sm, A_CH2, p, c) = fmx(0.0, -(dt * (u - cs))) * dx2i;
-    LDS<TP, TS>(sm, A_CM, p, c) = dt * (u - cs) * dx2i;
-    LDS<TP, TS>(sm, A_C0, p, c) = dt * (u)*dx2i;
-    LDS<TP, TS>(sm, A_CP, p, c) = dt * (u + cs) * dx2i;
+    LDS<TP, TS, UNI>(sm, A_CS, p, c) = cs;
   }
   if (P.isteep) {
     const int n2 = T.NI + 4;
-    for (int idx = tid; idx < TP * n2; idx += Cfg<TP, TS>::NT) {
+    for (int idx = tid; idx < TP * n2; idx += Cfg<TP, TS, UNI>::NT) {
       int p = idx / n2, cell = T.IA - 2 + idx % n2;
       if (p >= T.np) continue;
       int c = cell - T.L0;
-      double dm = LDS<TP, TS>(sm, A_D, p, c - 1), d0 = LDS<TP, TS>(sm, A_D, p, c),
-             dp = LDS<TP, TS>(sm, A_D, p, c + 1);
-      double d2d = (dp - d0) / tabv(P.tab, P.N, T_SDXP, cell);
-      d2d = (d2d - (d0 - dm) / tabv(P.tab, P.N, T_SDXM, cell)) / tabv(P.tab, P.N, T_QA3, cell);
-      LDS<TP, TS>(sm, A_DQ, p, c) = d2d;
+      double dm = LDS<TP, TS, UNI>(sm, A_D, p, c - 1), d0 = LDS<TP, TS, UNI>(sm, A_D, p, c),
+             dp = LDS<TP, TS, UNI>(sm, A_D, p, c + 1);
+      double d2d = (dp - d0) / TS_<UNI>(P, T_SDXP, cell);
+      d2d = (d2d - (d0 - dm) / TS_<UNI>(P, T_SDXM, cell)) / TS_<UNI>(P, T_QA3, cell);
+      LDS<TP, TS, UNI>(sm, A_DQ, p, c) = d2d;
     }
   }
 }
 
 // Phase: steepening coefficients (inteuler.F:174-192) at [IA-1, IB].
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_steepen(const SweepParams &P, const Tile &T, double *sm, int tid) {
   if (!P.isteep) return;
   const int n = T.NI + 2;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS>::NT) {
+  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
     int p = idx / n, cell = T.IA - 1 + idx % n;
     if (p >= T.np) continue;
     int c = cell - T.L0;
-    double dm = LDS<TP, TS>(sm, A_D, p, c - 1), dp = LDS<TP, TS>(sm, A_D, p, c + 1);
-    double pm = LDS<TP, TS>(sm, A_P, p, c - 1), pq = LDS<TP, TS>(sm, A_P, p, c + 1);
-    double d2m = LDS<TP, TS>(sm, A_DQ, p, c - 1), d2p = LDS<TP, TS>(sm, A_DQ, p, c + 1);
+    double dm = LDS<TP, TS, UNI>(sm, A_D, p, c - 1), dp = LDS<TP, TS, UNI>(sm, A_D, p, c + 1);
+    double pm = LDS<TP, TS, UNI>(sm, A_P, p, c - 1), pq = LDS<TP, TS, UNI>(sm, A_P, p, c + 1);
+    double d2m = LDS<TP, TS, UNI>(sm, A_DQ, p, c - 1), d2p = LDS<TP, TS, UNI>(sm, A_DQ, p, c + 1);
     double qc = fabs(dp - dm) - 0.01 * fmn(fabs(dp), fabs(dm));
-    double s1 = (d2m - d2p) * tabv(P.tab, P.N, T_STNUM, cell) /
-                (tabv(P.tab, P.N, T_STDEN, cell) * (dp - dm + kTiny));
+    double s1 = (d2m - d2p) * TS_<UNI>(P, T_STNUM, cell) /
+                (TS_<UNI>(P, T_STDEN, cell) * (dp - dm + kTiny));
     if (d2p * d2m > 0.0) s1 = 0.0;
     if (qc <= 0.0) s1 = 0.0;
     double s2 = fmx(0.0, fmn(20.0 * (s1 - 0.05), 1.0));
     double qa = fabs(dp - dm) / fmn(dp, dm);
     double qb = fabs(pq - pm) / fmn(pq, pm);
-    LDS<TP, TS>(sm, A_ST, p, c) = (P.gamma * 0.1 * qa >= qb) ? s2 : 0.0;
+    LDS<TP, TS, UNI>(sm, A_ST, p, c) = (P.gamma * 0.1 * qa >= qb) ? s2 : 0.0;
   }
 }
 
 // Phase: monotonised average slopes of LDS array `arr` (intvar.F:82-94) at
 // cells [IA-2, IB+1], into A_DQ.
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_slope(const SweepParams &P, const Tile &T, double *sm, int tid, int arr) {
   const int n = T.NI + 4;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS>::NT) {
+  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
     int p = idx / n, cell = T.IA - 2 + idx % n;
     if (p >= T.np) continue;
     int c = cell - T.L0;
-    double qm = LDS<TP, TS>(sm, arr, p, c - 1), q0 = LDS<TP, TS>(sm, arr, p, c),
-           qp = LDS<TP, TS>(sm, arr, p, c + 1);
+    double qm = LDS<TP, TS, UNI>(sm, arr, p, c - 1), q0 = LDS<TP, TS, UNI>(sm, arr, p, c),
+           qp = LDS<TP, TS, UNI>(sm, arr, p, c + 1);
     double qplus = qp - q0, qmnus = q0 - qm, dq;
     if (qplus * qmnus > 0) {
-      double qcent = tabv(P.tab, P.N, T_C1, cell) * qplus + tabv(P.tab, P.N, T_C2, cell) * qmnus;
+      double qcent = TS_<UNI>(P, T_C1, cell) * qplus + TS_<UNI>(P, T_C2, cell) * qmnus;
       double qvanl = 2.0 * qplus * qmnus / (qmnus + qplus);
       double temp1 = fmn(fmn(fmn(fabs(qcent), fabs(qvanl)), 2.0 * fabs(qmnus)), 2.0 * fabs(qplus));
       dq = temp1 * sgn1(qcent);
     } else {
       dq = 0.0;
     }
-    LDS<TP, TS>(sm, A_DQ, p, c) = dq;
+    LDS<TP, TS, UNI>(sm, A_DQ, p, c) = dq;
   }
 }
 
 // flattening coefficient of `cell` for the block's pencil p: the reference
 // passes `flatten` (line 1 of the slice) to intvar for every line
 // (inteuler.F:241-289), so it depends only on the slice (outer) index.
-template <int TP, int TS>
+template <int TP, int TS, int UNI>
 PPM_HD double flat_of(const SweepParams &P, const Tile &T, int p, int cell) {
   int outer = coord_of(P, (P.dim + 2) % 3, T.t1_0 + p, T.t2);
   return P.flat[(long long)outer * P.N + (cell - 1)];
@@ -301,24 +356,25 @@ PPM_HD double flat_of(const SweepParams &P, const Tile &T, int p, int cell) {
 
 // Phase: interface values, steepening, monotonisation, flattening, CHECK_LR
 // and parabola (intvar.F:98-160) at cells [IA-1, IB] -> A_QL, A_QR, A_Q6.
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_parab(const SweepParams &P, const Tile &T, double *sm, int tid, int arr,
-                     int steep) {
+                     int steep, const Regs &rg) {
   const int n = T.NI + 2;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS>::NT) {
+  PPM_ASSERT(TP * n <= Cfg<TP, TS, UNI>::NT);  // one cell per thread (build_segments)
+  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
     int p = idx / n, cell = T.IA - 1 + idx % n;
     if (p >= T.np) continue;
     int c = cell - T.L0;
-    double qm = LDS<TP, TS>(sm, arr, p, c - 1), q = LDS<TP, TS>(sm, arr, p, c),
-           qp = LDS<TP, TS>(sm, arr, p, c + 1);
-    double dqm = LDS<TP, TS>(sm, A_DQ, p, c - 1), dq0 = LDS<TP, TS>(sm, A_DQ, p, c),
-           dqp = LDS<TP, TS>(sm, A_DQ, p, c + 1);
-    double ql = tabv(P.tab, P.N, T_C3, cell) * qm + tabv(P.tab, P.N, T_C4, cell) * q +
-                tabv(P.tab, P.N, T_C5, cell) * dqm + tabv(P.tab, P.N, T_C6, cell) * dq0;
-    double qr = tabv(P.tab, P.N, T_C3, cell + 1) * q + tabv(P.tab, P.N, T_C4, cell + 1) * qp +
-                tabv(P.tab, P.N, T_C5, cell + 1) * dq0 + tabv(P.tab, P.N, T_C6, cell + 1) * dqp;
+    double qm = LDS<TP, TS, UNI>(sm, arr, p, c - 1), q = LDS<TP, TS, UNI>(sm, arr, p, c),
+           qp = LDS<TP, TS, UNI>(sm, arr, p, c + 1);
+    double dqm = LDS<TP, TS, UNI>(sm, A_DQ, p, c - 1), dq0 = LDS<TP, TS, UNI>(sm, A_DQ, p, c),
+           dqp = LDS<TP, TS, UNI>(sm, A_DQ, p, c + 1);
+    double ql = TS_<UNI>(P, T_C3, cell) * qm + TS_<UNI>(P, T_C4, cell) * q +
+                TS_<UNI>(P, T_C5, cell) * dqm + TS_<UNI>(P, T_C6, cell) * dq0;
+    double qr = TS_<UNI>(P, T_C3, cell + 1) * q + TS_<UNI>(P, T_C4, cell + 1) * qp +
+                TS_<UNI>(P, T_C5, cell + 1) * dq0 + TS_<UNI>(P, T_C6, cell + 1) * dqp;
     if (steep) {
-      double st = LDS<TP, TS>(sm, A_ST, p, c);
+      double st = LDS<TP, TS, UNI>(sm, A_ST, p, c);
       ql = (1.0 - st) * ql + st * (qm + 0.5 * dqm);
       qr = (1.0 - st) * qr + st * (qp - 0.5 * dqp);
     }
@@ -331,7 +387,7 @@ PPM_HD void ph_parab(const SweepParams &P, const Tile &T, double *sm, int tid, i
     if (temp22 < temp23) ql = 3.0 * q - 2.0 * qr;
     if (temp22 < -temp23) qr = 3.0 * q - 2.0 * ql;
     if (P.iflatten != 0) {
-      double f = flat_of<TP, TS>(P, T, p, cell);
+      double f = rg.flat_c;
       ql = q * f + ql * (1.0 - f);
       qr = q * f + qr * (1.0 - f);
     }
@@ -339,9 +395,9 @@ PPM_HD void ph_parab(const SweepParams &P, const Tile &T, double *sm, int tid, i
     ql = fmn(fmx(q, qm), ql);
     qr = fmx(fmn(q, qp), qr);
     qr = fmn(fmx(q, qp), qr);
-    LDS<TP, TS>(sm, A_QL, p, c) = ql;
-    LDS<TP, TS>(sm, A_QR, p, c) = qr;
-    LDS<TP, TS>(sm, A_Q6, p, c) = 6.0 * (q - 0.5 * (ql + qr));
+    LDS<TP, TS, UNI>(sm, A_QL, p, c) = ql;
+    LDS<TP, TS, UNI>(sm, A_QR, p, c) = qr;
+    LDS<TP, TS, UNI>(sm, A_Q6, p, c) = 6.0 * (q - 0.5 * (ql + qr));
   }
 }
 
@@ -352,23 +408,29 @@ struct IfaceAvg {
   double qla, qra, ql0, qr0, qlm, qrm, qlp, qrp;
 };
 
-template <int TP, int TS>
-PPM_HD IfaceAvg iface_avg(double *sm, int p, int ca, int cb, int with_mp) {
+template <int TP, int TS, int UNI = 0>
+PPM_HD IfaceAvg iface_avg(const SweepParams &P, const Tile &T, double *sm, int p, int ca,
+                          int cb, int with_mp) {
   IfaceAvg r;
-  double qr_a = LDS<TP, TS>(sm, A_QR, p, ca), ql_a = LDS<TP, TS>(sm, A_QL, p, ca);
-  double q6_a = LDS<TP, TS>(sm, A_Q6, p, ca);
-  double ql_b = LDS<TP, TS>(sm, A_QL, p, cb), qr_b = LDS<TP, TS>(sm, A_QR, p, cb);
-  double q6_b = LDS<TP, TS>(sm, A_Q6, p, cb);
+  const double dt = T.dt;
+  const Chars A = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, ca), LDS<TP, TS, UNI>(sm, A_CS, p, ca), dt,
+                           TS_<UNI>(P, T_DX2I, ca + T.L0));
+  const Chars B = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, cb), LDS<TP, TS, UNI>(sm, A_CS, p, cb), dt,
+                           TS_<UNI>(P, T_DX2I, cb + T.L0));
+  double qr_a = LDS<TP, TS, UNI>(sm, A_QR, p, ca), ql_a = LDS<TP, TS, UNI>(sm, A_QL, p, ca);
+  double q6_a = LDS<TP, TS, UNI>(sm, A_Q6, p, ca);
+  double ql_b = LDS<TP, TS, UNI>(sm, A_QL, p, cb), qr_b = LDS<TP, TS, UNI>(sm, A_QR, p, cb);
+  double q6_b = LDS<TP, TS, UNI>(sm, A_Q6, p, cb);
   double dq_a = qr_a - ql_a, dq_b = qr_b - ql_b;
-  double ch1 = LDS<TP, TS>(sm, A_CH1, p, ca), ch2 = LDS<TP, TS>(sm, A_CH2, p, cb);
-  double c0a = LDS<TP, TS>(sm, A_C0, p, ca), c0b = LDS<TP, TS>(sm, A_C0, p, cb);
+  double ch1 = A.ch1, ch2 = B.ch2;
+  double c0a = A.c0, c0b = B.c0;
   r.qla = qr_a - ch1 * (dq_a - (1.0 - kFt * ch1) * q6_a);
   r.qra = ql_b + ch2 * (dq_b + (1.0 - kFt * ch2) * q6_b);
   r.ql0 = qr_a - c0a * (dq_a - (1.0 - kFt * c0a) * q6_a);
   r.qr0 = ql_b - c0b * (dq_b + (1.0 + kFt * c0b) * q6_b);
   if (with_mp) {
-    double cma = LDS<TP, TS>(sm, A_CM, p, ca), cmb = LDS<TP, TS>(sm, A_CM, p, cb);
-    double cpa = LDS<TP, TS>(sm, A_CP, p, ca), cpb = LDS<TP, TS>(sm, A_CP, p, cb);
+    double cma = A.cm, cmb = B.cm;
+    double cpa = A.cp, cpb = B.cp;
     r.qlm = qr_a - cma * (dq_a - (1.0 - kFt * cma) * q6_a);
     r.qrm = ql_b - cmb * (dq_b + (1.0 + kFt * cmb) * q6_b);
     r.qlp = qr_a - cpa * (dq_a - (1.0 - kFt * cpa) * q6_a);
@@ -381,15 +443,14 @@ PPM_HD IfaceAvg iface_avg(double *sm, int p, int ca, int cb, int with_mp) {
 
 // Phase: interface averages of variable `var` (0 d, 1 p, 2 u, 3 v, 4 w, 5 ge)
 // for the thread's interface; results into Regs.
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_iface(const SweepParams &P, const Tile &T, double *sm, int tid, int var,
                      Regs &r) {
-  (void)P;
   int p = tid / TS, q = tid % TS;
   if (p >= T.np || q >= T.NI) return;
   int i = T.IA + q;
   int ca = i - 1 - T.L0, cb = i - T.L0;
-  IfaceAvg a = iface_avg<TP, TS>(sm, p, ca, cb, var == 1 || var == 2);
+  IfaceAvg a = iface_avg<TP, TS, UNI>(P, T, sm, p, ca, cb, var == 1 || var == 2);
   if (var == 0) { r.dla = a.qla; r.dra = a.qra; r.dl0 = a.ql0; r.dr0 = a.qr0; }
   else if (var == 1) {
     r.pla = a.qla; r.pra = a.qra; r.pl0 = a.ql0; r.pr0 = a.qr0;
@@ -398,8 +459,8 @@ PPM_HD void ph_iface(const SweepParams &P, const Tile &T, double *sm, int tid, i
     r.ula = a.qla; r.ura = a.qra;
     r.ulm = a.qlm; r.urm = a.qrm; r.ulp = a.qlp; r.urp = a.qrp;
     // upwind choice of the advected states (inteuler.F:414-434)
-    r.upl = LDS<TP, TS>(sm, A_U, p, ca) <= 0.0;
-    r.upr = LDS<TP, TS>(sm, A_U, p, cb) >= 0.0;
+    r.upl = LDS<TP, TS, UNI>(sm, A_U, p, ca) <= 0.0;
+    r.upr = LDS<TP, TS, UNI>(sm, A_U, p, cb) >= 0.0;
   } else if (var == 3) {
     r.vls = r.upl ? a.qla : a.ql0;
     r.vrs = r.upr ? a.qra : a.qr0;
@@ -451,14 +512,14 @@ PPM_HD void twoshock1(double dls, double drs, double &pls, double &prs, double u
 // diffusion coefficient at interface i (calcdiss.F:207-236, 377-404);
 // transverse velocities come from the 3-D state: the current slice and the
 // next one from A, the previous slice (already swept) from B.
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD double diffcoef1(const SweepParams &P, const Tile &T, double *sm, int p, int i) {
   const int rax = (P.dim + 1) % 3, oax = (P.dim + 2) % 3;
   const int t1 = T.t1_0 + p, t2 = T.t2;
   const int row = coord_of(P, rax, t1, t2), outer = coord_of(P, oax, t1, t2);
   const int j = row + 1, k = outer + 1;  // 1-based as in calcdiss
   const int jdim = P.GD[rax], kdim = P.GD[oax];
-  double dc = LDS<TP, TS>(sm, A_U, p, i - 1 - T.L0) - LDS<TP, TS>(sm, A_U, p, i - T.L0);
+  double dc = LDS<TP, TS, UNI>(sm, A_U, p, i - 1 - T.L0) - LDS<TP, TS, UNI>(sm, A_U, p, i - T.L0);
   // grid index of sweep cell `cell` at (row, outer) coordinates
   auto at = [&](int rr, int oo, int cell) {
     int c[3];
@@ -471,47 +532,53 @@ PPM_HD double diffcoef1(const SweepParams &P, const Tile &T, double *sm, int p, 
     const double *v = P.A + (long long)(S_VX + rax) * P.cells;
     double vdiff1 = (v[at(row - 1, outer, i)] + v[at(row - 1, outer, i - 1)]) -
                     (v[at(row + 1, outer, i)] + v[at(row + 1, outer, i - 1)]);
-    dc = dc + (tabv(P.tab, P.N, T_DIFFX, i) / tabv(P.tab_row, jdim, T_DDEN, j)) * vdiff1;
+    dc = dc + (TS_<UNI>(P, T_DIFFX, i) / TR_(P, T_DDEN, j, jdim)) * vdiff1;
   }
   if (P.nact_out > 1 && 1 < k && k < kdim) {  // nzz > 1 .and. lk1 (dimz > 1 .and. lkz1)
     const double *wo = P.A + (long long)(S_VX + oax) * P.cells;
     const double *wn = P.B + (long long)(S_VX + oax) * P.cells;
     double wdiff1 = (wn[at(row, outer - 1, i)] + wn[at(row, outer - 1, i - 1)]) -
                     (wo[at(row, outer + 1, i)] + wo[at(row, outer + 1, i - 1)]);
-    dc = dc + (tabv(P.tab, P.N, T_DIFFX, i) / tabv(P.tab_out, kdim, T_DDEN, k)) * wdiff1;
+    dc = dc + (TS_<UNI>(P, T_DIFFX, i) / TO_(P, T_DDEN, k, kdim)) * wdiff1;
   }
   return 0.1 * fmx(0.0, dc);
 }
 
 // Phase: Eulerian corrections (inteuler.F:321-551), two-shock solve and
 // Godunov fluxes (flux_twoshock.F:84-396) for the thread's interface.
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, Regs &r) {
   int p = tid / TS, q = tid % TS;
   if (p >= T.np || q >= T.NI) return;
   const int i = T.IA + q;
   const int ca = i - 1 - T.L0, cb = i - T.L0;
-  const double gamma = P.gamma, dt = dtval(P);
+  const double gamma = P.gamma, dt = T.dt;
+  const Chars CA = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, ca), LDS<TP, TS, UNI>(sm, A_CS, p, ca), dt,
+                            TS_<UNI>(P, T_DX2I, i - 1));
+  const Chars CB = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, cb), LDS<TP, TS, UNI>(sm, A_CS, p, cb), dt,
+                            TS_<UNI>(P, T_DX2I, i));
   // correction terms (3.7), inteuler.F:321-383
   double cla = sqrt(fmx(gamma * r.pla * r.dla, 0.0));
   double cra = sqrt(fmx(gamma * r.pra * r.dra, 0.0));
-  double f1 = 1.0 / cla;
+  const double rcla = 1.0 / cla;
+  double f1 = rcla;
   double betalp = (r.ula - r.ulp) + (r.pla - r.plp) * f1;
   double betalm = (r.ula - r.ulm) - (r.pla - r.plm) * f1;
   double betal0 = (r.pla - r.pl0) * (f1 * f1) + 1.0 / r.dla - 1.0 / r.dl0;
   double grs = 0.0;
   if (P.gravity == 1) {
-    grs = 0.25 * dt * (LDS<TP, TS>(sm, A_GR, p, ca) + LDS<TP, TS>(sm, A_GR, p, cb));
+    grs = 0.25 * dt * (LDS<TP, TS, UNI>(sm, A_GR, p, ca) + LDS<TP, TS, UNI>(sm, A_GR, p, cb));
     betalp = betalp - grs;
     betalm = betalm - grs;
   }
-  f1 = 0.5 / cla;
+  f1 = 0.5 * rcla;  // == 0.5/cla exactly (scaling by 2 commutes with rounding)
   betalp = -(betalp * f1);
   betalm = +betalm * f1;
-  if (LDS<TP, TS>(sm, A_CP, p, ca) <= 0.0) betalp = 0.0;
-  if (LDS<TP, TS>(sm, A_CM, p, ca) <= 0.0) betalm = 0.0;
-  if (LDS<TP, TS>(sm, A_C0, p, ca) <= 0.0) betal0 = 0.0;
-  f1 = 1.0 / cra;
+  if (CA.cp <= 0.0) betalp = 0.0;
+  if (CA.cm <= 0.0) betalm = 0.0;
+  if (CA.c0 <= 0.0) betal0 = 0.0;
+  const double rcra = 1.0 / cra;
+  f1 = rcra;
   double betarp = (r.ura - r.urp) + (r.pra - r.prp) * f1;
   double betarm = (r.ura - r.urm) - (r.pra - r.prm) * f1;
   double betar0 = (r.pra - r.pr0) * (f1 * f1) + 1.0 / r.dra - 1.0 / r.dr0;
@@ -519,12 +586,12 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
     betarp = betarp - grs;
     betarm = betarm - grs;
   }
-  f1 = 0.5 / cra;
+  f1 = 0.5 * rcra;
   betarp = -(betarp * f1);
   betarm = +betarm * f1;
-  if (LDS<TP, TS>(sm, A_CP, p, cb) >= 0) betarp = 0.0;
-  if (LDS<TP, TS>(sm, A_CM, p, cb) >= 0) betarm = 0.0;
-  if (LDS<TP, TS>(sm, A_C0, p, cb) >= 0) betar0 = 0.0;
+  if (CB.cp >= 0) betarp = 0.0;
+  if (CB.cm >= 0) betarm = 0.0;
+  if (CB.c0 >= 0) betar0 = 0.0;
   // corrected states (3.6), inteuler.F:387-396
   double pls = r.pla + (betalp + betalm) * (cla * cla);
   double prs = r.pra + (betarp + betarm) * (cra * cra);
@@ -537,10 +604,8 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
   r.ovl = r.ovr = 0;
   // dual-energy override (inteuler.F:493-521)
   if (P.idual == 1) {
-    double cma = LDS<TP, TS>(sm, A_CM, p, ca), c0a = LDS<TP, TS>(sm, A_C0, p, ca),
-           cpa = LDS<TP, TS>(sm, A_CP, p, ca);
-    double cmb = LDS<TP, TS>(sm, A_CM, p, cb), c0b = LDS<TP, TS>(sm, A_C0, p, cb),
-           cpb = LDS<TP, TS>(sm, A_CP, p, cb);
+    const double cma = CA.cm, c0a = CA.c0, cpa = CA.cp;
+    const double cmb = CB.cm, c0b = CB.c0, cpb = CB.cp;
     if (gamma * r.pla / r.dla < P.eta2 * (r.ula * r.ula) ||
         fmx(fmx(fabs(cma), fabs(c0a)), fabs(cpa)) < 1.0e-3 || dls / r.dla > 5.0) {
       r.ovl = 1;
@@ -605,16 +670,16 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
   double dub = ub * db;
   double duub, duvb, duwb, dueb, dugeb = 0.0;
   if (P.idiff != 0) {  // flux_twoshock.F:291-322
-    double dc = diffcoef1<TP, TS>(P, T, sm, p, i);
-    double da = LDS<TP, TS>(sm, A_D, p, ca), dbb = LDS<TP, TS>(sm, A_D, p, cb);
-    duub = dub * ub + dc * (da * LDS<TP, TS>(sm, A_U, p, ca) - dbb * LDS<TP, TS>(sm, A_U, p, cb));
-    duvb = dub * vb + dc * (da * LDS<TP, TS>(sm, A_V, p, ca) - dbb * LDS<TP, TS>(sm, A_V, p, cb));
-    duwb = dub * wb + dc * (da * LDS<TP, TS>(sm, A_W, p, ca) - dbb * LDS<TP, TS>(sm, A_W, p, cb));
-    dueb = dub * eb + dc * (da * LDS<TP, TS>(sm, A_E, p, ca) - dbb * LDS<TP, TS>(sm, A_E, p, cb));
+    double dc = diffcoef1<TP, TS, UNI>(P, T, sm, p, i);
+    double da = LDS<TP, TS, UNI>(sm, A_D, p, ca), dbb = LDS<TP, TS, UNI>(sm, A_D, p, cb);
+    duub = dub * ub + dc * (da * LDS<TP, TS, UNI>(sm, A_U, p, ca) - dbb * LDS<TP, TS, UNI>(sm, A_U, p, cb));
+    duvb = dub * vb + dc * (da * LDS<TP, TS, UNI>(sm, A_V, p, ca) - dbb * LDS<TP, TS, UNI>(sm, A_V, p, cb));
+    duwb = dub * wb + dc * (da * LDS<TP, TS, UNI>(sm, A_W, p, ca) - dbb * LDS<TP, TS, UNI>(sm, A_W, p, cb));
+    dueb = dub * eb + dc * (da * LDS<TP, TS, UNI>(sm, A_E, p, ca) - dbb * LDS<TP, TS, UNI>(sm, A_E, p, cb));
     dub = dub + dc * (da - dbb);
     if (P.idual == 1)
       dugeb = dub * geb +
-              dc * (da * LDS<TP, TS>(sm, A_GE, p, ca) - dbb * LDS<TP, TS>(sm, A_GE, p, cb));
+              dc * (da * LDS<TP, TS, UNI>(sm, A_GE, p, ca) - dbb * LDS<TP, TS, UNI>(sm, A_GE, p, cb));
   } else {
     duub = dub * ub;
     duvb = dub * vb;
@@ -622,7 +687,7 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
     dueb = dub * eb;
     if (P.idual == 1) dugeb = dub * geb;
   }
-  double qc = dt / tabv(P.tab, P.N, T_DX, i);
+  double qc = qc_of<UNI>(P, T.dt, i);
   r.df = qc * dub;
   r.ef = qc * (dueb + upb);
   r.uf = qc * (duub + pb);
@@ -634,20 +699,20 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
 }
 
 // Phase: store the interface fluxes into LDS (indexed by the right cell).
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_store_flux(const SweepParams &P, const Tile &T, double *sm, int tid,
                           const Regs &r) {
   (void)P;
   int p = tid / TS, q = tid % TS;
   if (p >= T.np || q >= T.NI) return;
   int cb = T.IA + q - T.L0;
-  LDS<TP, TS>(sm, F_DF, p, cb) = r.df;
-  LDS<TP, TS>(sm, F_EF, p, cb) = r.ef;
-  LDS<TP, TS>(sm, F_UF, p, cb) = r.uf;
-  LDS<TP, TS>(sm, F_VF, p, cb) = r.vf;
-  LDS<TP, TS>(sm, F_WF, p, cb) = r.wf;
-  LDS<TP, TS>(sm, F_GEF, p, cb) = r.gef;
-  LDS<TP, TS>(sm, F_UB, p, cb) = r.ub;
+  LDS<TP, TS, UNI>(sm, F_DF, p, cb) = r.df;
+  LDS<TP, TS, UNI>(sm, F_EF, p, cb) = r.ef;
+  LDS<TP, TS, UNI>(sm, F_UF, p, cb) = r.uf;
+  LDS<TP, TS, UNI>(sm, F_VF, p, cb) = r.vf;
+  LDS<TP, TS, UNI>(sm, F_WF, p, cb) = r.wf;
+  LDS<TP, TS, UNI>(sm, F_GEF, p, cb) = r.gef;
+  LDS<TP, TS, UNI>(sm, F_UB, p, cb) = r.ub;
 }
 
 // updated cells of the tile: [IA, IB-1] within [i1, i2]
@@ -658,19 +723,19 @@ PPM_HD void upd_range(const SweepParams &P, const Tile &T, int &c0, int &c1) {
 
 // Phase: negative density / energy check (flux_twoshock.F:401-403).
 // Returns the number of failing cells seen by this thread.
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD int ph_check(const SweepParams &P, const Tile &T, double *sm, int tid) {
   int c0, c1;
   upd_range(P, T, c0, c1);
   const int n = c1 - c0 + 1;
   int bad = 0;
   if (n <= 0) return 0;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS>::NT) {
+  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
     int p = idx / n, cell = c0 + idx % n;
     if (p >= T.np) continue;
     int c = cell - T.L0;
-    double d = LDS<TP, TS>(sm, A_D, p, c), e = LDS<TP, TS>(sm, A_E, p, c);
-    double dfl = LDS<TP, TS>(sm, F_DF, p, c), dfr = LDS<TP, TS>(sm, F_DF, p, c + 1);
+    double d = LDS<TP, TS, UNI>(sm, A_D, p, c), e = LDS<TP, TS, UNI>(sm, A_E, p, c);
+    double dfl = LDS<TP, TS, UNI>(sm, F_DF, p, c), dfr = LDS<TP, TS, UNI>(sm, F_DF, p, c + 1);
     if (d + (dfl - dfr) <= 0.0 || e < 0.0) bad++;
   }
   return bad;
@@ -678,55 +743,55 @@ PPM_HD int ph_check(const SweepParams &P, const Tile &T, double *sm, int tid) {
 
 // Phase: conservative update (euler.F:121-217, gravity :248-273) of the
 // tile's updated cells, in place in LDS.  Returns error flags.
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD int ph_euler(const SweepParams &P, const Tile &T, double *sm, int tid) {
   int c0, c1;
   upd_range(P, T, c0, c1);
   const int n = c1 - c0 + 1;
   int err = 0;
   if (n <= 0) return 0;
-  const double dt = dtval(P), gamma = P.gamma;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS>::NT) {
+  const double dt = T.dt, gamma = P.gamma;
+  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
     int p = idx / n, cell = c0 + idx % n;
     if (p >= T.np) continue;
     int c = cell - T.L0;
-    double d = LDS<TP, TS>(sm, A_D, p, c), e = LDS<TP, TS>(sm, A_E, p, c);
-    double u = LDS<TP, TS>(sm, A_U, p, c), v = LDS<TP, TS>(sm, A_V, p, c),
-           w = LDS<TP, TS>(sm, A_W, p, c);
-    double dnu = d + (LDS<TP, TS>(sm, F_DF, p, c) - LDS<TP, TS>(sm, F_DF, p, c + 1));
+    double d = LDS<TP, TS, UNI>(sm, A_D, p, c), e = LDS<TP, TS, UNI>(sm, A_E, p, c);
+    double u = LDS<TP, TS, UNI>(sm, A_U, p, c), v = LDS<TP, TS, UNI>(sm, A_V, p, c),
+           w = LDS<TP, TS, UNI>(sm, A_W, p, c);
+    double dnu = d + (LDS<TP, TS, UNI>(sm, F_DF, p, c) - LDS<TP, TS, UNI>(sm, F_DF, p, c + 1));
     double dnuinv = 1.0 / dnu;
     if (P.dfloor > 0.0) {
       dnu = fmx(dnu, P.dfloor);
       dnuinv = 1.0 / dnu;
     }
     double uold = u;
-    u = (u * d + (LDS<TP, TS>(sm, F_UF, p, c) - LDS<TP, TS>(sm, F_UF, p, c + 1))) * dnuinv;
-    v = (v * d + (LDS<TP, TS>(sm, F_VF, p, c) - LDS<TP, TS>(sm, F_VF, p, c + 1))) * dnuinv;
-    w = (w * d + (LDS<TP, TS>(sm, F_WF, p, c) - LDS<TP, TS>(sm, F_WF, p, c + 1))) * dnuinv;
+    u = (u * d + (LDS<TP, TS, UNI>(sm, F_UF, p, c) - LDS<TP, TS, UNI>(sm, F_UF, p, c + 1))) * dnuinv;
+    v = (v * d + (LDS<TP, TS, UNI>(sm, F_VF, p, c) - LDS<TP, TS, UNI>(sm, F_VF, p, c + 1))) * dnuinv;
+    w = (w * d + (LDS<TP, TS, UNI>(sm, F_WF, p, c) - LDS<TP, TS, UNI>(sm, F_WF, p, c + 1))) * dnuinv;
     e = fmx(0.1 * e,
-            (e * d + (LDS<TP, TS>(sm, F_EF, p, c) - LDS<TP, TS>(sm, F_EF, p, c + 1))) * dnuinv);
+            (e * d + (LDS<TP, TS, UNI>(sm, F_EF, p, c) - LDS<TP, TS, UNI>(sm, F_EF, p, c + 1))) * dnuinv);
     if (P.idual == 1) {
-      double ge = LDS<TP, TS>(sm, A_GE, p, c);
+      double ge = LDS<TP, TS, UNI>(sm, A_GE, p, c);
       double pcent = fmx((gamma - 1.0) * ge * d, kTiny);
-      double qc = dt / tabv(P.tab, P.N, T_DX, cell);
-      double ges = qc * pcent * (LDS<TP, TS>(sm, F_UB, p, c) - LDS<TP, TS>(sm, F_UB, p, c + 1));
-      ge = fmx((ge * d + (LDS<TP, TS>(sm, F_GEF, p, c) - LDS<TP, TS>(sm, F_GEF, p, c + 1)) + ges) *
+      double qc = qc_of<UNI>(P, T.dt, cell);
+      double ges = qc * pcent * (LDS<TP, TS, UNI>(sm, F_UB, p, c) - LDS<TP, TS, UNI>(sm, F_UB, p, c + 1));
+      ge = fmx((ge * d + (LDS<TP, TS, UNI>(sm, F_GEF, p, c) - LDS<TP, TS, UNI>(sm, F_GEF, p, c + 1)) + ges) *
                    dnuinv,
                0.5 * ge);
       if (ge < 0.0) err |= 1;
-      LDS<TP, TS>(sm, A_GE, p, c) = ge;
+      LDS<TP, TS, UNI>(sm, A_GE, p, c) = ge;
     }
     if (P.gravity == 1) {
-      double gr = LDS<TP, TS>(sm, A_GR, p, c);
+      double gr = LDS<TP, TS, UNI>(sm, A_GR, p, c);
       u = u + dt * gr * 0.5 * (d * dnuinv + 1.0);
       e = e + dt * gr * 0.5 * (u + uold * d * dnuinv);
       e = fmx(e, kTiny);
     }
-    LDS<TP, TS>(sm, A_D, p, c) = dnu;
-    LDS<TP, TS>(sm, A_E, p, c) = e;
-    LDS<TP, TS>(sm, A_U, p, c) = u;
-    LDS<TP, TS>(sm, A_V, p, c) = v;
-    LDS<TP, TS>(sm, A_W, p, c) = w;
+    LDS<TP, TS, UNI>(sm, A_D, p, c) = dnu;
+    LDS<TP, TS, UNI>(sm, A_E, p, c) = e;
+    LDS<TP, TS, UNI>(sm, A_U, p, c) = u;
+    LDS<TP, TS, UNI>(sm, A_V, p, c) = v;
+    LDS<TP, TS, UNI>(sm, A_W, p, c) = w;
   }
   return err;
 }
@@ -737,86 +802,75 @@ PPM_HD void pg_range(const SweepParams &P, int &g0, int &g1) {
   g1 = P.i2 + 3 < P.N ? P.i2 + 3 : P.N;
 }
 
-// dual-energy sync of one cell (pgas2d_dual.F:70-112).  emax_prev is
-// d(i-1)*e(i-1) with e(i-1) already synchronised (or d(i)*e(i) at the range
-// start).  Returns e; updates ge.  *chain is set when the gas-energy choice
-// fed back into e (only possible with non-positive energies).
-PPM_HD double pgas1(double d, double e, double &ge, double u, double v, double w, double emax_m,
-                    double emax_p, double eta1, double eta2, double gamma, double pmin,
-                    int *chain) {
-  double ke = 0.5 * (u * u + v * v + w * w);
-  double ge1 = e - ke;
-  double demax = fmx(fmx(d * e, emax_m), emax_p);
-  int took = 0;
-  if (ge1 * d / demax > eta2) { ge = ge1; took = 1; }
-  double ge2;
-  if (ge1 / e > eta1) ge2 = ge1;
-  else { ge2 = ge; if (took && chain) *chain = 1; }
-  ge2 = fmx(ge2, pmin / ((gamma - 1.0) * d));
-  return e - ge1 + ge2;
-}
-
-// e after the sync without the demax test (exact whenever the test cannot
-// change e: see pgas1's chain flag)
-PPM_HD double pgas_e_spec(double d, double e, double ge, double u, double v, double w,
-                          double eta1, double gamma, double pmin) {
-  double ke = 0.5 * (u * u + v * v + w * w);
-  double ge1 = e - ke;
-  double ge2 = (ge1 / e > eta1) ? ge1 : ge;
-  ge2 = fmx(ge2, pmin / ((gamma - 1.0) * d));
-  (void)d;
-  return e - ge1 + ge2;
-}
-
-// Phase: speculative synchronised e at cells [o-1, o+w-1] (A_EOUT).
-template <int TP, int TS>
+// Phase: speculative synchronised e at cells [o-1, o+w-1] (A_EOUT), with
+// ge1 = e - ke (A_GE1) and the flag ge1/e > eta1 (A_HOT) kept for the
+// final pass.
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_pgas_spec(const SweepParams &P, const Tile &T, double *sm, int tid) {
   int g0, g1;
   pg_range(P, g0, g1);
   int a = T.o - 1 > g0 ? T.o - 1 : g0, b = T.o + T.w - 1 < g1 ? T.o + T.w - 1 : g1;
   int n = b - a + 1;
   if (n <= 0) return;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS>::NT) {
+  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
     int p = idx / n, cell = a + idx % n;
     if (p >= T.np) continue;
     int c = cell - T.L0;
-    LDS<TP, TS>(sm, A_EOUT, p, c) =
-        pgas_e_spec(LDS<TP, TS>(sm, A_D, p, c), LDS<TP, TS>(sm, A_E, p, c),
-                    LDS<TP, TS>(sm, A_GE, p, c), LDS<TP, TS>(sm, A_U, p, c),
-                    LDS<TP, TS>(sm, A_V, p, c), LDS<TP, TS>(sm, A_W, p, c), P.eta1, P.gamma,
-                    P.pmin);
+    const double d = LDS<TP, TS, UNI>(sm, A_D, p, c), e = LDS<TP, TS, UNI>(sm, A_E, p, c);
+    const double u = LDS<TP, TS, UNI>(sm, A_U, p, c), v = LDS<TP, TS, UNI>(sm, A_V, p, c),
+                 w = LDS<TP, TS, UNI>(sm, A_W, p, c);
+    const double ke = 0.5 * (u * u + v * v + w * w);
+    const double ge1 = e - ke;
+    const bool hot = ge1 / e > P.eta1;
+    double ge2 = hot ? ge1 : LDS<TP, TS, UNI>(sm, A_GE, p, c);
+    ge2 = fmx(ge2, P.pmin / ((P.gamma - 1.0) * d));
+    LDS<TP, TS, UNI>(sm, A_EOUT, p, c) = e - ge1 + ge2;
+    LDS<TP, TS, UNI>(sm, A_GE1, p, c) = ge1;
+    LDS<TP, TS, UNI>(sm, A_HOT, p, c) = hot ? 1.0 : 0.0;
   }
 }
 
-// Phase: dual-energy sync of the owned cells, then write every owned cell of
-// the hydro fields to B.  Returns the number of cells where the sync chain
-// made the speculative value inexact.
-template <int TP, int TS>
+// Phase: dual-energy sync of the owned cells (pgas2d_dual.F:70-118), then
+// write every owned cell of the hydro fields to B.  The speculative e is
+// exact unless the demax test changed ge in a cell whose e uses ge (never
+// with positive energies); those cells are recomputed here and counted.
+template <int TP, int TS, int UNI = 0>
 PPM_HD int ph_pgas_write(const SweepParams &P, const Tile &T, double *sm, int tid) {
   int g0, g1;
   pg_range(P, g0, g1);
   int chains = 0;
   const int n = T.w;
   const int vax = (P.dim + 1) % 3, wax = (P.dim + 2) % 3;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS>::NT) {
+  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
     int p, k;
     if (P.dim == 0) { p = idx / n; k = idx % n; }
     else { p = idx % TP; k = idx / TP; }
     if (p >= T.np) continue;
     int cell = T.o + k, c = cell - T.L0;
-    double d = LDS<TP, TS>(sm, A_D, p, c), e = LDS<TP, TS>(sm, A_E, p, c);
-    double u = LDS<TP, TS>(sm, A_U, p, c), v = LDS<TP, TS>(sm, A_V, p, c),
-           w = LDS<TP, TS>(sm, A_W, p, c);
-    double ge = P.idual ? LDS<TP, TS>(sm, A_GE, p, c) : 0.0;
+    double d = LDS<TP, TS, UNI>(sm, A_D, p, c), e = LDS<TP, TS, UNI>(sm, A_E, p, c);
+    double u = LDS<TP, TS, UNI>(sm, A_U, p, c), v = LDS<TP, TS, UNI>(sm, A_V, p, c),
+           w = LDS<TP, TS, UNI>(sm, A_W, p, c);
+    double ge = P.idual ? LDS<TP, TS, UNI>(sm, A_GE, p, c) : 0.0;
     if (P.idual && cell >= g0 && cell <= g1) {
       int im1 = cell - 1 > g0 ? cell - 1 : g0, ip1 = cell + 1 < g1 ? cell + 1 : g1;
       double em = im1 == cell ? d * e
-                              : LDS<TP, TS>(sm, A_D, p, im1 - T.L0) *
-                                    LDS<TP, TS>(sm, A_EOUT, p, im1 - T.L0);
-      double ep = LDS<TP, TS>(sm, A_D, p, ip1 - T.L0) * LDS<TP, TS>(sm, A_E, p, ip1 - T.L0);
-      int chain = 0;
-      e = pgas1(d, e, ge, u, v, w, em, ep, P.eta1, P.eta2, P.gamma, P.pmin, &chain);
-      chains += chain;
+                              : LDS<TP, TS, UNI>(sm, A_D, p, im1 - T.L0) *
+                                    LDS<TP, TS, UNI>(sm, A_EOUT, p, im1 - T.L0);
+      double ep = LDS<TP, TS, UNI>(sm, A_D, p, ip1 - T.L0) * LDS<TP, TS, UNI>(sm, A_E, p, ip1 - T.L0);
+      const double ge1 = LDS<TP, TS, UNI>(sm, A_GE1, p, c);
+      const bool hot = LDS<TP, TS, UNI>(sm, A_HOT, p, c) != 0.0;
+      double demax = fmx(fmx(d * e, em), ep);
+      bool took = ge1 * d / demax > P.eta2;
+      double e_new = LDS<TP, TS, UNI>(sm, A_EOUT, p, c);
+      if (took) {
+        if (!hot) {  // e uses the gas energy just replaced: not speculated
+          double ge2 = fmx(ge1, P.pmin / ((P.gamma - 1.0) * d));
+          e_new = e - ge1 + ge2;
+          chains++;
+        }
+        ge = ge1;
+      }
+      e = e_new;
     }
     long long g = gidx(P, T.t1_0 + p, T.t2, cell);
     double *B = P.B;
@@ -841,7 +895,7 @@ PPM_HD void face_coords(const SweepParams &P, int t1, int t2, int &ci, int &cj) 
 // Phase: capture of the hydro face fluxes of owned interfaces
 // (Grid_xEulerSweep.C:313-376): staging layout per face record is
 // [slot][offset] with slots d, e, vx, vy, vz, ge, colours.
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_capture(const SweepParams &P, const Tile &T, double *sm, int tid) {
   if (!P.face_head) return;
   int p = tid / TS, q = tid % TS;
@@ -858,41 +912,41 @@ PPM_HD void ph_capture(const SweepParams &P, const Tile &T, double *sm, int tid)
     const FaceRec &F = P.faces[h];
     if (ci < F.lo_i || ci > F.hi_i || cj < F.lo_j || cj > F.hi_j) continue;
     long long off = F.base + (ci - F.lo_i) + (long long)(cj - F.lo_j) * (F.hi_i - F.lo_i + 1);
-    P.face_out[off + 0 * F.nface] = LDS<TP, TS>(sm, F_DF, p, cb);
-    P.face_out[off + 1 * F.nface] = LDS<TP, TS>(sm, F_EF, p, cb);
-    P.face_out[off + (2 + P.dim) * F.nface] = LDS<TP, TS>(sm, F_UF, p, cb);
-    P.face_out[off + (2 + vax) * F.nface] = LDS<TP, TS>(sm, F_VF, p, cb);
-    P.face_out[off + (2 + wax) * F.nface] = LDS<TP, TS>(sm, F_WF, p, cb);
-    if (P.idual) P.face_out[off + 5 * F.nface] = LDS<TP, TS>(sm, F_GEF, p, cb);
+    P.face_out[off + 0 * F.nface] = LDS<TP, TS, UNI>(sm, F_DF, p, cb);
+    P.face_out[off + 1 * F.nface] = LDS<TP, TS, UNI>(sm, F_EF, p, cb);
+    P.face_out[off + (2 + P.dim) * F.nface] = LDS<TP, TS, UNI>(sm, F_UF, p, cb);
+    P.face_out[off + (2 + vax) * F.nface] = LDS<TP, TS, UNI>(sm, F_VF, p, cb);
+    P.face_out[off + (2 + wax) * F.nface] = LDS<TP, TS, UNI>(sm, F_WF, p, cb);
+    if (P.idual) P.face_out[off + 5 * F.nface] = LDS<TP, TS, UNI>(sm, F_GEF, p, cb);
   }
 }
 
 // Phase: load colour n into A_COL.
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_load_col(const SweepParams &P, const Tile &T, double *sm, int tid, int n) {
   const int nc = T.L1 - T.L0 + 1;
   const bool s_contig = (P.dim == 0);
   const double *src = P.A + (long long)(S_COL0 + n) * P.cells;
-  for (int idx = tid; idx < TP * nc; idx += Cfg<TP, TS>::NT) {
+  for (int idx = tid; idx < TP * nc; idx += Cfg<TP, TS, UNI>::NT) {
     int p, c;
     if (s_contig) { p = idx / nc; c = idx % nc; }
     else { p = idx % TP; c = idx / TP; }
     if (p >= T.np) continue;
-    LDS<TP, TS>(sm, A_COL, p, c) = src[gidx(P, T.t1_0 + p, T.t2, T.L0 + c)];
+    LDS<TP, TS, UNI>(sm, A_COL, p, c) = src[gidx(P, T.t1_0 + p, T.t2, T.L0 + c)];
   }
 }
 
 // Phase: colour interface state and flux (inteuler.F:437-450, 493-551;
 // flux_twoshock.F:251-259, 375-380; euler.F:173-177 division by dx),
 // stored as colf/dx into F_COLF, and captured for owned faces.
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_col_flux(const SweepParams &P, const Tile &T, double *sm, int tid, int n,
                         const Regs &r) {
   int p = tid / TS, q = tid % TS;
   if (p >= T.np || q >= T.NI) return;
   int i = T.IA + q;
   int ca = i - 1 - T.L0, cb = i - T.L0;
-  IfaceAvg a = iface_avg<TP, TS>(sm, p, ca, cb, 0);
+  IfaceAvg a = iface_avg<TP, TS, UNI>(P, T, sm, p, ca, cb, 0);
   double colls = r.upl ? a.qla * r.dls_pre / r.dla : a.ql0 * r.dls_pre / r.dl0;
   double colrs = r.upr ? a.qra * r.drs_pre / r.dra : a.qr0 * r.drs_pre / r.dr0;
   if (r.ovl) colls = colls * r.dla / r.dls_pre;
@@ -900,9 +954,9 @@ PPM_HD void ph_col_flux(const SweepParams &P, const Tile &T, double *sm, int tid
   colls = fmx(colls, kColorFloor);
   colrs = fmx(colrs, kColorFloor);
   double colb = r.ub > 0.0 ? colls * r.db / r.dls : colrs * r.db / r.drs;
-  double colf = dtval(P) * r.ub * colb;
-  colf = colf / tabv(P.tab, P.N, T_DX, i);
-  LDS<TP, TS>(sm, F_COLF, p, cb) = colf;
+  double colf = T.dt * r.ub * colb;
+  colf = colf / TS_<UNI>(P, T_DX, i);
+  LDS<TP, TS, UNI>(sm, F_COLF, p, cb) = colf;
   if (P.face_head && i >= T.o && i < T.o + T.w) {
     int h0 = P.face_head[i], h1 = P.face_head[i + 1];
     int ci, cj;
@@ -917,19 +971,19 @@ PPM_HD void ph_col_flux(const SweepParams &P, const Tile &T, double *sm, int tid
 }
 
 // Phase: colour update (euler.F:178-186) of owned cells and write to B.
-template <int TP, int TS>
+template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_col_update(const SweepParams &P, const Tile &T, double *sm, int tid, int n) {
   const double min_color = 1e-5 * kTiny;
   double *dst = P.B + (long long)(S_COL0 + n) * P.cells;
-  for (int idx = tid; idx < TP * T.w; idx += Cfg<TP, TS>::NT) {
+  for (int idx = tid; idx < TP * T.w; idx += Cfg<TP, TS, UNI>::NT) {
     int p, k;
     if (P.dim == 0) { p = idx / T.w; k = idx % T.w; }
     else { p = idx % TP; k = idx / TP; }
     if (p >= T.np) continue;
     int cell = T.o + k, c = cell - T.L0;
-    double col = LDS<TP, TS>(sm, A_COL, p, c);
+    double col = LDS<TP, TS, UNI>(sm, A_COL, p, c);
     if (cell >= P.i1 && cell <= P.i2) {
-      col = col + (LDS<TP, TS>(sm, F_COLF, p, c) - LDS<TP, TS>(sm, F_COLF, p, c + 1));
+      col = col + (LDS<TP, TS, UNI>(sm, F_COLF, p, c) - LDS<TP, TS, UNI>(sm, F_COLF, p, c + 1));
       col = fmx(col, min_color);
     }
     dst[gidx(P, T.t1_0 + p, T.t2, cell)] = col;
@@ -943,8 +997,13 @@ struct FlatParams {
   int GD[3];
   long long cells;
   int n_outer;
+  int iflatten;
   const double *A, *pres;
   double *flat;
+  const double *dx;   // cell widths along the sweep (N)
+  double dt;
+  const double *dtp;  // device dt if non-null
+  double *qc;         // out: dt/dx(i), i = 1..N
 };
 
 PPM_HD double flat_value(const FlatParams &F, int outer, int cell) {
@@ -989,40 +1048,43 @@ struct TileCounts {
   int bad, chains, err;
 };
 
-template <int TP, int TS, class X>
+template <int TP, int TS, int UNI, class X>
 PPM_HD TileCounts sweep_tile(const SweepParams &P, int block, double *sm, X &ex) {
-  const Tile T = make_tile<TP, TS>(P, block);
+  const Tile T = make_tile<TP, TS, UNI>(P, block);
   TileCounts tc;
-  ex.run([&](int tid, Regs &r) { (void)r; ph_load<TP, TS>(P, T, sm, tid); });
-  ex.run([&](int tid, Regs &r) { (void)r; ph_chars<TP, TS>(P, T, sm, tid); });
-  ex.run([&](int tid, Regs &r) { (void)r; ph_steepen<TP, TS>(P, T, sm, tid); });
+  ex.run([&](int tid, Regs &r) {
+    ph_flat_reg<TP, TS, UNI>(P, T, tid, r);
+    ph_load<TP, TS, UNI>(P, T, sm, tid);
+  });
+  ex.run([&](int tid, Regs &r) { (void)r; ph_chars<TP, TS, UNI>(P, T, sm, tid); });
+  ex.run([&](int tid, Regs &r) { (void)r; ph_steepen<TP, TS, UNI>(P, T, sm, tid); });
   const int nv = P.idual ? 6 : 5;
   for (int v = 0; v < nv; v++) {
     const int arr = v == 0 ? A_D : v == 1 ? A_P : v == 2 ? A_U : v == 3 ? A_V : v == 4 ? A_W : A_GE;
     const int steep = (v == 0 && P.isteep) ? 1 : 0;
-    ex.run([&](int tid, Regs &r) { (void)r; ph_slope<TP, TS>(P, T, sm, tid, arr); });
-    ex.run([&](int tid, Regs &r) { (void)r; ph_parab<TP, TS>(P, T, sm, tid, arr, steep); });
-    ex.run([&](int tid, Regs &r) { ph_iface<TP, TS>(P, T, sm, tid, v, r); });
+    ex.run([&](int tid, Regs &r) { (void)r; ph_slope<TP, TS, UNI>(P, T, sm, tid, arr); });
+    ex.run([&](int tid, Regs &r) { ph_parab<TP, TS, UNI>(P, T, sm, tid, arr, steep, r); });
+    ex.run([&](int tid, Regs &r) { ph_iface<TP, TS, UNI>(P, T, sm, tid, v, r); });
   }
   ex.run([&](int tid, Regs &r) {
-    ph_solve<TP, TS>(P, T, sm, tid, r);
-    ph_store_flux<TP, TS>(P, T, sm, tid, r);
+    ph_solve<TP, TS, UNI>(P, T, sm, tid, r);
+    ph_store_flux<TP, TS, UNI>(P, T, sm, tid, r);
   });
-  tc.bad = ex.count([&](int tid, Regs &r) { (void)r; return ph_check<TP, TS>(P, T, sm, tid); });
+  tc.bad = ex.count([&](int tid, Regs &r) { (void)r; return ph_check<TP, TS, UNI>(P, T, sm, tid); });
   tc.err = ex.count([&](int tid, Regs &r) {
     (void)r;
-    ph_capture<TP, TS>(P, T, sm, tid);
-    return ph_euler<TP, TS>(P, T, sm, tid);
+    ph_capture<TP, TS, UNI>(P, T, sm, tid);
+    return ph_euler<TP, TS, UNI>(P, T, sm, tid);
   });
-  if (P.idual) ex.run([&](int tid, Regs &r) { (void)r; ph_pgas_spec<TP, TS>(P, T, sm, tid); });
+  if (P.idual) ex.run([&](int tid, Regs &r) { (void)r; ph_pgas_spec<TP, TS, UNI>(P, T, sm, tid); });
   tc.chains =
-      ex.count([&](int tid, Regs &r) { (void)r; return ph_pgas_write<TP, TS>(P, T, sm, tid); });
+      ex.count([&](int tid, Regs &r) { (void)r; return ph_pgas_write<TP, TS, UNI>(P, T, sm, tid); });
   for (int n = 0; n < P.ncol; n++) {
-    ex.run([&](int tid, Regs &r) { (void)r; ph_load_col<TP, TS>(P, T, sm, tid, n); });
-    ex.run([&](int tid, Regs &r) { (void)r; ph_slope<TP, TS>(P, T, sm, tid, A_COL); });
-    ex.run([&](int tid, Regs &r) { (void)r; ph_parab<TP, TS>(P, T, sm, tid, A_COL, 0); });
-    ex.run([&](int tid, Regs &r) { ph_col_flux<TP, TS>(P, T, sm, tid, n, r); });
-    ex.run([&](int tid, Regs &r) { (void)r; ph_col_update<TP, TS>(P, T, sm, tid, n); });
+    ex.run([&](int tid, Regs &r) { (void)r; ph_load_col<TP, TS, UNI>(P, T, sm, tid, n); });
+    ex.run([&](int tid, Regs &r) { (void)r; ph_slope<TP, TS, UNI>(P, T, sm, tid, A_COL); });
+    ex.run([&](int tid, Regs &r) { ph_parab<TP, TS, UNI>(P, T, sm, tid, A_COL, 0, r); });
+    ex.run([&](int tid, Regs &r) { ph_col_flux<TP, TS, UNI>(P, T, sm, tid, n, r); });
+    ex.run([&](int tid, Regs &r) { (void)r; ph_col_update<TP, TS, UNI>(P, T, sm, tid, n); });
   }
   return tc;
 }

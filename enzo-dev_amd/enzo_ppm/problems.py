@@ -53,7 +53,7 @@ def gaussian_random_field(n, kmin, kmax, slope, rng, ncomp=1):
     out = []
     for _ in range(ncomp):
         ph = rng.standard_normal(k.shape) + 1j * rng.standard_normal(k.shape)
-        f = np.fft.irfftn(ph * amp, s=(n, n, n))
+        f = np.fft.irfftn(ph * amp, s=(n, n, n), axes=(0, 1, 2))
         f -= f.mean()
         f /= f.std()
         out.append(f)
@@ -139,11 +139,15 @@ def amr_level(ngrids=512, seed=7, nmin=8, nmax=64, level_dx=1.0 / 256, gamma=GAM
 
 
 def random_smooth(active, seed=0, ncolours=0, dual_energy=True, gravity=False, rank=3,
-                  amp=0.3, vamp=0.8, flags=(1, 1, 0)):
+                  amp=0.3, vamp=0.8, flags=(1, 1, 0), stretch=0.0):
     """Small seeded test state: smooth random density/pressure/velocity with
     moderate contrasts, used by the parity tests."""
     rng = np.random.default_rng(seed)
     g = Grid(active, rank=rank, ncolours=ncolours, dual_energy=dual_energy, gravity=gravity)
+    if stretch:  # non-uniform cell widths (CellWidthTemp arrays)
+        for d in range(rank):
+            w = g.cellwidth[d]
+            w *= 1.0 + stretch * np.sin(np.arange(w.size) * 0.7 + d)
     shape = g.shape
     x = g.cell_centers(0)[None, None, :]
     y = g.cell_centers(1)[None, :, None] if rank > 1 else np.zeros((1, 1, 1))

@@ -9,6 +9,7 @@
 // face-flux staging.  Used by tests to check tile indexing and arithmetic
 // against the oracle before (and independently of) a GPU run.
 #include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -77,8 +78,12 @@ extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *
   }
   std::vector<double> tabs[3];
   std::vector<int> so[3], sw[3];
+  int uni[3];
+  double kt[3][NTAB];
+  const bool force_tables = getenv("PPM_EMU_TABLES") != nullptr;
   for (int d = 0; d < 3; d++) {
     build_tables(CellWidthTemp[d], GD[d], tabs[d]);
+    uni[d] = (!force_tables && uniform_axis(CellWidthTemp[d], GD[d], tabs[d], kt[d])) ? 1 : 0;
     if (d < rank && GD[d] > 1)
       if (!build_segments<kTP, kTS>(GD[d], (int)g->GridStartIndex[d] + 1,
                                     (int)g->GridEndIndex[d] + 1, so[d], sw[d]))
@@ -102,9 +107,11 @@ extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *
     if (nact[dim] <= 1) continue;
     const int N = GD[dim], i1 = (int)g->GridStartIndex[dim] + 1, i2 = (int)g->GridEndIndex[dim] + 1;
     const int rax = (dim + 1) % 3, oax = (dim + 2) % 3;
-    std::vector<double> flat((size_t)GD[oax] * N, 0.0);
+    std::vector<double> flat((size_t)GD[oax] * N, 0.0), qc(N);
+    for (int i = 0; i < N; i++) qc[i] = g->dtFixed / CellWidthTemp[dim][i];
     if (gl->PPMFlatteningParameter) {
       FlatParams F;
+      memset(&F, 0, sizeof(F));
       F.dim = dim; F.N = N; F.i1 = i1; F.i2 = i2;
       for (int d = 0; d < 3; d++) F.GD[d] = GD[d];
       F.cells = cells; F.n_outer = GD[oax]; F.A = A.data(); F.pres = pres.data(); F.flat = flat.data();
@@ -169,6 +176,9 @@ extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *
     S.gr = GravityOn ? &accel[(size_t)dim * cells] : nullptr;
     S.flat = flat.data();
     S.tab = tabs[dim].data(); S.tab_row = tabs[rax].data(); S.tab_out = tabs[oax].data();
+    S.qc = qc.data();
+    S.uni[0] = uni[dim]; S.uni[1] = uni[rax]; S.uni[2] = uni[oax];
+    for (int t = 0; t < NTAB; t++) { S.kt[t] = kt[dim][t]; S.kt_row[t] = kt[rax][t]; S.kt_out[t] = kt[oax][t]; }
     S.gamma = gl->Gamma; S.eta1 = gl->DualEnergyFormalismEta1; S.eta2 = gl->DualEnergyFormalismEta2;
     S.pmin = gl->tiny_number; S.dfloor = gl->SmallRho; S.dt = g->dtFixed; S.dtp = nullptr;
     S.isteep = (int)gl->PPMSteepeningParameter; S.iflatten = (int)gl->PPMFlatteningParameter;
@@ -183,7 +193,8 @@ extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *
     auto run_blocks = [&](long long nblk) {
       for (long long b = 0; b < nblk; b++) {
         std::fill(sm.begin(), sm.end(), NAN);
-        TileCounts tc = sweep_tile<kTP, kTS>(S, (int)b, sm.data(), *ex);
+        TileCounts tc = S.uni[0] ? sweep_tile<kTP, kTS, 1>(S, (int)b, sm.data(), *ex)
+                                 : sweep_tile<kTP, kTS, 0>(S, (int)b, sm.data(), *ex);
         counters[0] += tc.bad; counters[1] += tc.chains; counters[2] += tc.err;
       }
     };

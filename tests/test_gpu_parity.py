@@ -50,6 +50,9 @@ CASES = {
     "sod_2d": lambda: problems.sod_x(20, rank=2, gamma=1.4, dual_energy=True, flags=(1, 1, 1)),
     "turbulent_40": lambda: problems.turbulent(40, seed=3),
     "turbulent_mach30": lambda: problems.turbulent(24, seed=4, mach=30, sigma_lnrho=1.5),
+    "stretched_grid": lambda: problems.random_smooth([12, 10, 11], seed=7, ncolours=1, stretch=0.2),
+    "stretched_diffusion": lambda: problems.random_smooth([12, 10, 11], seed=7, ncolours=1,
+                                                          stretch=0.2, flags=(1, 1, 1)),
 }
 
 
