@@ -49,11 +49,62 @@ __global__ __launch_bounds__(kTP *kTS) __attribute__((amdgpu_waves_per_eu(4)))
 void ppm_sweep_kernel(SweepParams P) {
   __shared__ double sm[SweepCfg::SMEM];
   DevExec ex;
-  TileCounts tc = sweep_tile<kTP, kTS, UNI>(P, (int)blockIdx.x, sm, ex);
+  const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  TileCounts tc = sweep_tile<kTP, kTS, UNI>(P, tile, sm, ex);
   if (threadIdx.x == 0) {
-    if (tc.bad) atomicAdd(&P.counters[0], (unsigned long long)tc.bad);
+    // a failing cell or an error: the fallback kernel redoes the group's
+    // pencils whole (the fallback chain can cross segments) and settles them
+    if (tc.bad || tc.err) {
+      const int grp = tile / P.nseg;
+      if (atomicExch(&P.fb_flag[grp], 1) == 0) P.fb_list[atomicAdd(P.fb_count, 1u)] = (unsigned)grp;
+    }
     if (tc.chains) atomicAdd(&P.counters[1], (unsigned long long)tc.chains);
-    if (tc.err) atomicAdd(&P.counters[2], (unsigned long long)tc.err);
+  }
+}
+
+// ------------------------------------------------------------------------
+// Per-cell HLL fallback (flux_twoshock.F:401-456): every pencil of each
+// group the sweep kernel flagged is recomputed whole by one workgroup of
+// kFbTS threads (one thread per interface, the pencil in LDS), with the
+// fallback applied in the reference's cell order; its results replace the
+// sweep kernel's for those pencils.  Launched after every sweep launch; with
+// no flagged group each workgroup only reads the count.  The last workgroup
+// to finish clears the count for the next launch (the group flags are
+// cleared group by group).
+constexpr int kFbTS = 1024;
+constexpr int kFbBlocks = 64;
+using FbCfg = Cfg<1, kFbTS>;
+
+template <int UNI>
+__global__ __launch_bounds__(kFbTS) void ppm_fallback_kernel(SweepParams P) {
+  __shared__ double sm[NARR_FB * FbCfg::LW];
+  DevExec ex;
+  const unsigned n = *P.fb_count;
+  for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
+    const int grp = (int)P.fb_list[k];
+    const int tb = grp % P.nblk_t1, t2 = P.t2_0 + grp / P.nblk_t1;
+    for (int p = 0; p < kTP && tb * kTP + p < P.n_t1; p++) {
+      if (!P.fb_ok) {
+        if (threadIdx.x == 0) atomicAdd(&P.counters[3], 1ull);
+        break;
+      }
+      TileCounts tc = sweep_pencil_fb<kFbTS, UNI>(P, tb * kTP + p, t2, sm, ex);
+      if (threadIdx.x == 0) {
+        if (tc.nfb) atomicAdd(&P.counters[0], (unsigned long long)tc.nfb);
+        if (tc.err) atomicAdd(&P.counters[2], (unsigned long long)tc.err);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) P.fb_flag[grp] = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(P.fb_done, 1u) == gridDim.x - 1) {
+      *P.fb_count = 0u;
+      *P.fb_done = 0u;
+      __threadfence();
+    }
   }
 }
 
@@ -233,7 +284,11 @@ struct hipPPMData {
   double *qc = nullptr;  // dt/dx of the current sweep axis
   double *dt = nullptr;
   unsigned long long *dtmin = nullptr;
-  unsigned long long *counters = nullptr;  // [0] fallback [1] DE chain [2] errors
+  unsigned long long *counters = nullptr;  // [0] fallback [1] DE chain [2] errors [3] unsupported
+  // per-cell fallback: group flags, flagged-group list, {count, done}
+  int *fb_flag = nullptr;
+  unsigned *fb_list = nullptr, *fb_state = nullptr;
+  long long fb_groups = 0;
   int *segbuf = nullptr;  // per axis: o[nseg], w[nseg]
   int uni[3] = {0, 0, 0};  // axis has one cell width
   double kt[3][NTAB];      // its coefficient entries
@@ -316,6 +371,7 @@ void free_data(hipPPMData *D) {
   D->ev.clear();
   void *ptrs[] = {D->A, D->B, D->pres, D->accel, D->tab[0], D->tab[1], D->tab[2], D->cwdev[0],
                   D->cwdev[1], D->cwdev[2], D->flat, D->qc, D->dt, D->dtmin, D->counters, D->segbuf,
+                  D->fb_flag, D->fb_list, D->fb_state,
                   D->face_head[0], D->face_head[1], D->face_head[2], D->faces[0], D->faces[1],
                   D->faces[2], D->face_out[0], D->face_out[1], D->face_out[2]};
   for (void *p : ptrs)
@@ -394,6 +450,18 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
   S.faces = D->faces[dim];
   S.face_out = D->face_out[dim];
   S.counters = D->counters;
+  S.fb_flag = D->fb_flag;
+  S.fb_list = D->fb_list;
+  S.fb_count = D->fb_state;
+  S.fb_done = D->fb_state + 1;
+  S.fb_ok = pencil_fits<kFbTS>(S) ? 1 : 0;
+  auto launch_fb = [&]() {
+    if (S.uni[0])
+      hipLaunchKernelGGL(ppm_fallback_kernel<1>, dim3(kFbBlocks), dim3(kFbTS), 0, D->stream, S);
+    else
+      hipLaunchKernelGGL(ppm_fallback_kernel<0>, dim3(kFbBlocks), dim3(kFbTS), 0, D->stream, S);
+    return hipGetLastError();
+  };
   if (S.idiff == 0) {
     // all pencils at once: t1 is the contiguous transverse axis for y/z
     // sweeps (coalesced loads), the row axis for x sweeps
@@ -425,6 +493,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
       hipLaunchKernelGGL(ppm_sweep_kernel<0>, dim3((unsigned)nblk), dim3(kTP * kTS), 0, D->stream, S);
     HIPCHK(hipGetLastError());
     if (D->profile) HIPCHK(hipEventRecord(e1, D->stream));
+    HIPCHK(launch_fb());
   } else {
     // diffusion reads the live transverse velocities of the previous slice
     // (calcdiss.F:207-288 on BaryonField): slices in order, one launch each
@@ -443,6 +512,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
         hipLaunchKernelGGL(ppm_sweep_kernel<0>, dim3((unsigned)nblk), dim3(kTP * kTS), 0,
                            D->stream, S);
       HIPCHK(hipGetLastError());
+      HIPCHK(launch_fb());  // the next slice reads this one's results
     }
   }
   std::swap(D->A, D->B);
@@ -604,6 +674,16 @@ int hipPPMInitData(hipPPMData **Data, const hipPPMParameter *Para, void *stream)
         alloc_dev((void **)&D->dtmin, sizeof(unsigned long long)) ||
         alloc_dev((void **)&D->counters, 4 * sizeof(unsigned long long)))
       return ENZO_PPM_ERR_HIP;
+    // pencil groups of any sweep launch: ceil(n_t1 / kTP) * n_t2
+    long long ngrp = 1;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        if (a != b) ngrp = std::max(ngrp, (long long)((D->GD[a] + kTP - 1) / kTP) * D->GD[b]);
+    D->fb_groups = ngrp;
+    if (alloc_dev((void **)&D->fb_flag, sizeof(int) * ngrp) ||
+        alloc_dev((void **)&D->fb_list, sizeof(unsigned) * ngrp) ||
+        alloc_dev((void **)&D->fb_state, 2 * sizeof(unsigned)))
+      return ENZO_PPM_ERR_HIP;
   }
   D->stream = (hipStream_t)stream;
   // geometry tables and segments (cheap; rebuilt on every init)
@@ -635,6 +715,8 @@ int hipPPMInitData(hipPPMData **Data, const hipPPMParameter *Para, void *stream)
     HIPCHK(hipMemcpy(D->segbuf, segs.data(), sizeof(int) * segs.size(), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, D->stream, D->counters, 4);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipMemsetAsync(D->fb_flag, 0, sizeof(int) * D->fb_groups, D->stream));
+  HIPCHK(hipMemsetAsync(D->fb_state, 0, 2 * sizeof(unsigned), D->stream));
   D->fallback_total = 0;
   D->chain_total = 0;
   return ENZO_PPM_SUCCESS;
@@ -811,6 +893,11 @@ int hipPPMGetBaryon(hipPPMData *D, const hipPPMParameter *P, enzo_float *BaryonF
   HIPCHK(hipMemcpy(c, D->counters, sizeof(c), hipMemcpyDeviceToHost));
   D->fallback_total = (long long)c[0];
   D->chain_total = (long long)c[1];
+  if (c[3] != 0) {
+    snprintf(g_last_error, sizeof(g_last_error),
+             "the per-cell fallback was needed on a pencil longer than %d cells", kFbTS + 1);
+    return ENZO_PPM_ERR_NOT_SUPPORTED;
+  }
   if (c[2] != 0) return ENZO_PPM_ERR_NEGATIVE;
   if (c[0] != 0 && P->RiemannSolverFallback == 0) return ENZO_PPM_ERR_NEGATIVE;
   return ENZO_PPM_SUCCESS;

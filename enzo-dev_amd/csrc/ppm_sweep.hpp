@@ -96,6 +96,13 @@ struct SweepParams {
   const FaceRec *faces;
   double *face_out;
   unsigned long long *counters;  // [0] fallback cells [1] dual-energy chain [2] errors
+                                 // [3] fallback needed on a pencil too long for it
+  // pencil groups (block / nseg of the main launch) to redo with the
+  // per-cell fallback: flag per group, list of flagged groups, its length,
+  // and the count of finished fallback workgroups (which clear both)
+  int *fb_flag;
+  unsigned *fb_list, *fb_count, *fb_done;
+  int fb_ok;  // a one-pencil workgroup holds the whole sweep axis
 };
 
 // ---------------------------------------------------------------------------
@@ -108,6 +115,11 @@ enum Arr {
 constexpr int F_DF = A_DQ, F_EF = A_QL, F_UF = A_QR, F_VF = A_Q6, F_WF = A_ST,
               F_GEF = A_P, F_UB = A_X, F_COLF = A_ST, A_EOUT = A_X, A_GE1 = A_DQ,
               A_HOT = A_QR;
+
+// Fallback workgroups (one pencil, TP = 1) keep two more arrays: the HLL
+// density flux for the in-order fallback decision, then the two gas-energy
+// source terms sl*bm0 and sr*bp0 of each interface (flux_hll.F:297-308).
+constexpr int A_HA = NARR, A_HB = NARR + 1, NARR_FB = NARR + 2;
 
 template <int TP, int TS, int UNI = 0>
 struct Cfg {
@@ -136,12 +148,25 @@ struct Regs {
   double df, ef, uf, vf, wf, gef;
   int ovl, ovr, upl, upr;  // DE overrides; upwind choice (u(i-1)<=0, u(i)>=0)
   double flat_c;           // flattening at this thread's parabola cell
+  // fallback workgroups only: HLL fluxes of the interface (dt/dx absorbed),
+  // HLL weights and wave-speed offsets, and whether they replace the
+  // two-shock fluxes
+  double h_df, h_uf, h_vf, h_wf, h_ef, h_gef, h_sl, h_sr, h_bm0, h_bp0;
+  int hll;
 };
 
 template <int TP, int TS, int UNI = 0>
 PPM_HD double &LDS(double *sm, int arr, int p, int c) {
   PPM_ASSERT(arr >= 0 && arr < NARR && p >= 0 && p < TP && c >= 0 && c < Cfg<TP, TS, UNI>::LW);
   return sm[(arr * TP + p) * Cfg<TP, TS, UNI>::LW + c];
+}
+
+// the fallback arrays A_HA / A_HB (one-pencil tiles only)
+template <int TP, int TS, int UNI = 0>
+PPM_HD double &LDSH(double *sm, int arr, int c) {
+  static_assert(TP == 1, "fallback arrays exist in one-pencil tiles only");
+  PPM_ASSERT((arr == A_HA || arr == A_HB) && c >= 0 && c < Cfg<TP, TS, UNI>::LW);
+  return sm[arr * Cfg<TP, TS, UNI>::LW + c];
 }
 
 PPM_HD double dtval(const SweepParams &P) { return P.dtp ? *P.dtp : P.dt; }
@@ -200,6 +225,32 @@ PPM_HD int coord_of(const SweepParams &P, int axis, int t1, int t2) {
   return axis == P.ax_t1 ? t1 : t2;
 }
 
+// Workgroups are handed to the 8 XCDs of the chip round-robin (workgroup b
+// runs on XCD b % 8, and every XCD has its own L2).  Tiles are renumbered
+// so that each XCD works through one contiguous range of them: neighbouring
+// segments of a pencil (which share halo cells) and neighbouring pencil
+// groups (which share cache lines when t1 is the contiguous axis) then meet
+// in the same L2.  A bijection of [0, nblk).
+constexpr int kXCD = 8;
+PPM_HD int xcd_tile(int b, int nblk) {
+  const int q = nblk / kXCD, rem = nblk % kXCD;
+  const int x = b % kXCD, k = b / kXCD;
+  return x * q + (x < rem ? x : rem) + k;
+}
+
+// interface and load ranges of a tile that owns cells [T.o, T.o + T.w)
+PPM_HD void tile_ranges(const SweepParams &P, Tile &T) {
+  T.IA = T.o - 1 > P.i1 ? T.o - 1 : P.i1;
+  T.IB = T.o + T.w + 1 < P.i2 + 1 ? T.o + T.w + 1 : P.i2 + 1;
+  T.NI = T.IB - T.IA + 1;
+  int lo = T.IA - 3, lo2 = T.o - 1 > 1 ? T.o - 1 : 1;
+  T.L0 = lo < lo2 ? lo : lo2;
+  int hi = T.IB + 2, hi2 = T.o + T.w < P.N ? T.o + T.w : P.N;
+  T.L1 = hi > hi2 ? hi : hi2;
+}
+
+// tile `block` (after xcd_tile) of the launch: segment fastest, then the
+// pencil group along t1, then t2.  block / nseg is the pencil group.
 template <int TP, int TS, int UNI = 0>
 PPM_HD Tile make_tile(const SweepParams &P, int block) {
   Tile T;
@@ -211,13 +262,22 @@ PPM_HD Tile make_tile(const SweepParams &P, int block) {
   T.np = P.n_t1 - T.t1_0 < TP ? P.n_t1 - T.t1_0 : TP;
   T.o = P.seg_o[seg];
   T.w = P.seg_w[seg];
-  T.IA = T.o - 1 > P.i1 ? T.o - 1 : P.i1;
-  T.IB = T.o + T.w + 1 < P.i2 + 1 ? T.o + T.w + 1 : P.i2 + 1;
-  T.NI = T.IB - T.IA + 1;
-  int lo = T.IA - 3, lo2 = T.o - 1 > 1 ? T.o - 1 : 1;
-  T.L0 = lo < lo2 ? lo : lo2;
-  int hi = T.IB + 2, hi2 = T.o + T.w < P.N ? T.o + T.w : P.N;
-  T.L1 = hi > hi2 ? hi : hi2;
+  tile_ranges(P, T);
+  T.dt = dtval(P);
+  return T;
+}
+
+// one whole pencil (t1, t2) as a single tile owning cells [1, N]: the tile
+// of the fallback workgroups.  Fits when N - 1 <= TS (interfaces and slope
+// items) and N <= TS + 4 (loaded cells).
+PPM_HD Tile make_pencil_tile(const SweepParams &P, int t1, int t2) {
+  Tile T;
+  T.t2 = t2;
+  T.t1_0 = t1;
+  T.np = 1;
+  T.o = 1;
+  T.w = P.N;
+  tile_ranges(P, T);
   T.dt = dtval(P);
   return T;
 }
@@ -544,11 +604,69 @@ PPM_HD double diffcoef1(const SweepParams &P, const Tile &T, double *sm, int p, 
   return 0.1 * fmx(0.0, dc);
 }
 
+// HLL fluxes of one interface (flux_hll.F:107-332, i1 = i2 = the cell of
+// the fallback): Roe averages, Davis wave speeds, HLL weights, left/right
+// fluxes plus diffusion, dt/dx absorbed.  Inputs are the final interface
+// states the two-shock solver used.
+struct HllIn {
+  double dls, drs, pls, prs, uls, urs, vls, vrs, wls, wrs, gels, gers;
+  double diffd, diffuu, diffuv, diffuw, diffue, diffuge;  // zero without diffusion
+  double gamma, qc;
+  int idual;
+};
+
+PPM_HD void hll_flux(const HllIn &h, Regs &r) {
+  const double gamma1 = h.gamma - 1.0;
+  const double gamma1i = 1.0 / gamma1;
+  const double sqrtdl = sqrt(h.dls);
+  const double sqrtdr = sqrt(h.drs);
+  const double isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+  const double vroe1 = (sqrtdl * h.uls + sqrtdr * h.urs) * isdlpdr;
+  const double vroe2 = (sqrtdl * h.vls + sqrtdr * h.vrs) * isdlpdr;
+  const double vroe3 = (sqrtdl * h.wls + sqrtdr * h.wrs) * isdlpdr;
+  const double v2 = vroe1 * vroe1 + vroe2 * vroe2 + vroe3 * vroe3;
+  const double el = gamma1i * h.pls + 0.5 * h.dls * (h.uls * h.uls + h.vls * h.vls + h.wls * h.wls);
+  const double er = gamma1i * h.prs + 0.5 * h.drs * (h.urs * h.urs + h.vrs * h.vrs + h.wrs * h.wrs);
+  const double hroe = ((el + h.pls) / sqrtdl + (er + h.prs) / sqrtdr) * isdlpdr;
+  const double cs = sqrt(gamma1 * fmx((hroe - 0.5 * v2), kTiny));
+  const double char1 = vroe1 - cs, char2 = vroe1 + cs;
+  const double csl0 = sqrt(h.gamma * h.pls / h.dls);
+  const double csr0 = sqrt(h.gamma * h.prs / h.drs);
+  const double csl = fmn(h.uls - csl0, char1);
+  const double csr = fmx(h.urs + csr0, char2);
+  const double bm = fmn(csl, 0.0), bp = fmx(csr, 0.0);
+  const double bm0 = h.uls - bm, bp0 = h.urs - bp;
+  const double q1 = (bp + bm) / (bp - bm);
+  const double sl = 0.5 * (1.0 + q1), sr = 0.5 * (1.0 - q1);
+  const double dubl = h.dls * h.uls, dubr = h.drs * h.urs;
+  const double dfl = h.dls * bm0, dfr = h.drs * bp0;
+  const double ufl = dubl * bm0 + h.pls, ufr = dubr * bp0 + h.prs;
+  const double vfl = h.dls * h.vls * bm0, vfr = h.drs * h.vrs * bp0;
+  const double wfl = h.dls * h.wls * bm0, wfr = h.drs * h.wrs * bp0;
+  const double efl = el * bm0 + h.pls * h.uls, efr = er * bp0 + h.prs * h.urs;
+  r.h_df = h.qc * (sl * dfl + sr * dfr + h.diffd);
+  r.h_uf = h.qc * (sl * ufl + sr * ufr + h.diffuu);
+  r.h_vf = h.qc * (sl * vfl + sr * vfr + h.diffuv);
+  r.h_wf = h.qc * (sl * wfl + sr * wfr + h.diffuw);
+  r.h_ef = h.qc * (sl * efl + sr * efr + h.diffue);
+  r.h_gef = 0.0;
+  if (h.idual == 1) {
+    const double gefl = bm0 * h.gels * h.dls, gefr = bp0 * h.gers * h.drs;
+    r.h_gef = h.qc * (sl * gefl + sr * gefr + h.diffuge);
+  }
+  r.h_sl = sl;
+  r.h_sr = sr;
+  r.h_bm0 = bm0;
+  r.h_bp0 = bp0;
+}
+
 // Phase: Eulerian corrections (inteuler.F:321-551), two-shock solve and
-// Godunov fluxes (flux_twoshock.F:84-396) for the thread's interface.
-template <int TP, int TS, int UNI = 0>
+// Godunov fluxes (flux_twoshock.F:84-396) for the thread's interface; in
+// fallback workgroups (FB) also the HLL fluxes of the interface.
+template <int TP, int TS, int UNI = 0, int FB = 0>
 PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, Regs &r) {
   int p = tid / TS, q = tid % TS;
+  if constexpr (FB != 0) r.hll = 0;
   if (p >= T.np || q >= T.NI) return;
   const int i = T.IA + q;
   const int ca = i - 1 - T.L0, cb = i - T.L0;
@@ -669,8 +787,9 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
   double upb = pb * ub;
   double dub = ub * db;
   double duub, duvb, duwb, dueb, dugeb = 0.0;
+  double dc = 0.0;
   if (P.idiff != 0) {  // flux_twoshock.F:291-322
-    double dc = diffcoef1<TP, TS, UNI>(P, T, sm, p, i);
+    dc = diffcoef1<TP, TS, UNI>(P, T, sm, p, i);
     double da = LDS<TP, TS, UNI>(sm, A_D, p, ca), dbb = LDS<TP, TS, UNI>(sm, A_D, p, cb);
     duub = dub * ub + dc * (da * LDS<TP, TS, UNI>(sm, A_U, p, ca) - dbb * LDS<TP, TS, UNI>(sm, A_U, p, cb));
     duvb = dub * vb + dc * (da * LDS<TP, TS, UNI>(sm, A_V, p, ca) - dbb * LDS<TP, TS, UNI>(sm, A_V, p, cb));
@@ -696,16 +815,37 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
   r.gef = qc * dugeb;
   r.ub = ub;
   r.db = db;
+  if constexpr (FB != 0) {
+    HllIn h;
+    h.dls = dls; h.drs = drs; h.pls = pls; h.prs = prs; h.uls = uls; h.urs = urs;
+    h.vls = r.vls; h.vrs = r.vrs; h.wls = r.wls; h.wrs = r.wrs; h.gels = r.gels; h.gers = r.gers;
+    h.diffd = h.diffuu = h.diffuv = h.diffuw = h.diffue = h.diffuge = 0.0;
+    if (P.idiff != 0) {  // flux_hll.F:166-195
+      const double da = LDS<TP, TS, UNI>(sm, A_D, p, ca), dbb = LDS<TP, TS, UNI>(sm, A_D, p, cb);
+      h.diffd = dc * (da - dbb);
+      h.diffuu = dc * (da * LDS<TP, TS, UNI>(sm, A_U, p, ca) - dbb * LDS<TP, TS, UNI>(sm, A_U, p, cb));
+      h.diffuv = dc * (da * LDS<TP, TS, UNI>(sm, A_V, p, ca) - dbb * LDS<TP, TS, UNI>(sm, A_V, p, cb));
+      h.diffuw = dc * (da * LDS<TP, TS, UNI>(sm, A_W, p, ca) - dbb * LDS<TP, TS, UNI>(sm, A_W, p, cb));
+      h.diffue = dc * (da * LDS<TP, TS, UNI>(sm, A_E, p, ca) - dbb * LDS<TP, TS, UNI>(sm, A_E, p, cb));
+      if (P.idual == 1)
+        h.diffuge = dc * (da * LDS<TP, TS, UNI>(sm, A_GE, p, ca) - dbb * LDS<TP, TS, UNI>(sm, A_GE, p, cb));
+    }
+    h.gamma = gamma;
+    h.qc = qc;
+    h.idual = P.idual;
+    hll_flux(h, r);
+  }
 }
 
 // Phase: store the interface fluxes into LDS (indexed by the right cell).
-template <int TP, int TS, int UNI = 0>
+template <int TP, int TS, int UNI = 0, int FB = 0>
 PPM_HD void ph_store_flux(const SweepParams &P, const Tile &T, double *sm, int tid,
                           const Regs &r) {
   (void)P;
   int p = tid / TS, q = tid % TS;
   if (p >= T.np || q >= T.NI) return;
   int cb = T.IA + q - T.L0;
+  if constexpr (FB != 0) LDSH<TP, TS, UNI>(sm, A_HA, cb) = r.h_df;
   LDS<TP, TS, UNI>(sm, F_DF, p, cb) = r.df;
   LDS<TP, TS, UNI>(sm, F_EF, p, cb) = r.ef;
   LDS<TP, TS, UNI>(sm, F_UF, p, cb) = r.uf;
@@ -741,10 +881,79 @@ PPM_HD int ph_check(const SweepParams &P, const Tile &T, double *sm, int tid) {
   return bad;
 }
 
-// Phase: conservative update (euler.F:121-217, gravity :248-273) of the
-// tile's updated cells, in place in LDS.  Returns error flags.
+// Fallback workgroups (one pencil).  The reference checks the cells of the
+// pencil in order (flux_twoshock.F:401-456): cell c falls back when
+// d + (df(c) - df(c+1)) <= 0 or e < 0, with df(c) already the HLL flux if
+// cell c-1 fell back; flux_hll (i1 = i2 = c) then replaces the fluxes of
+// interfaces c and c+1 and the gas-energy sources of cells c and c+1, and its
+// own check (d + df(c)) - df(c+1) <= 0 (flux_hll.F:349-351) stops the run.
+// The HLL flux of an interface does not depend on which call computes it, so
+// only the choice of cells is serial: one thread walks the pencil and writes
+// each cell's flag (1 = fell back) into A_COL, free until the colour passes.
+// Without the fallback (ifallback = 0) a failing cell is an error.
+// Returns the number of fallbacks, plus kFbError if any check failed.
+constexpr int kFbError = 1 << 24;
+
 template <int TP, int TS, int UNI = 0>
-PPM_HD int ph_euler(const SweepParams &P, const Tile &T, double *sm, int tid) {
+PPM_HD int ph_fb_decide(const SweepParams &P, const Tile &T, double *sm, int tid) {
+  static_assert(TP == 1, "one-pencil tiles only");
+  if (tid != 0) return 0;
+  int nfb = 0, err = 0, prev = 0;
+  LDS<TP, TS, UNI>(sm, A_COL, 0, P.i1 - 1 - T.L0) = 0.0;
+  LDS<TP, TS, UNI>(sm, A_COL, 0, P.i2 + 1 - T.L0) = 0.0;
+  for (int cell = P.i1; cell <= P.i2; cell++) {
+    const int c = cell - T.L0;
+    const double d = LDS<TP, TS, UNI>(sm, A_D, 0, c), e = LDS<TP, TS, UNI>(sm, A_E, 0, c);
+    const double dfl = prev ? LDSH<TP, TS, UNI>(sm, A_HA, c) : LDS<TP, TS, UNI>(sm, F_DF, 0, c);
+    const double dfr = LDS<TP, TS, UNI>(sm, F_DF, 0, c + 1);
+    int fb = 0;
+    if (d + (dfl - dfr) <= 0.0 || e < 0.0) {
+      if (P.ifallback == 0) {
+        err = 1;
+      } else {
+        fb = 1;
+        nfb++;
+        if ((d + LDSH<TP, TS, UNI>(sm, A_HA, c)) - LDSH<TP, TS, UNI>(sm, A_HA, c + 1) <= 0.0) err = 1;
+      }
+    }
+    LDS<TP, TS, UNI>(sm, A_COL, 0, c) = fb ? 1.0 : 0.0;
+    prev = fb;
+  }
+  return nfb + (err ? kFbError : 0);
+}
+
+// Phase: an interface takes its HLL fluxes when the cell on either side fell
+// back; every interface stores its gas-energy source terms sl*bm0, sr*bp0
+// (flux_hll.F:279-308) into A_HA / A_HB (A_HA's HLL density flux is spent).
+template <int TP, int TS, int UNI = 0>
+PPM_HD void ph_fb_apply(const SweepParams &P, const Tile &T, double *sm, int tid, Regs &r) {
+  (void)P;
+  r.hll = 0;
+  if (tid >= T.NI) return;
+  const int cb = T.IA + tid - T.L0;
+  r.hll = (LDS<TP, TS, UNI>(sm, A_COL, 0, cb - 1) != 0.0 || LDS<TP, TS, UNI>(sm, A_COL, 0, cb) != 0.0);
+  if (r.hll) {
+    LDS<TP, TS, UNI>(sm, F_DF, 0, cb) = r.h_df;
+    LDS<TP, TS, UNI>(sm, F_EF, 0, cb) = r.h_ef;
+    LDS<TP, TS, UNI>(sm, F_UF, 0, cb) = r.h_uf;
+    LDS<TP, TS, UNI>(sm, F_VF, 0, cb) = r.h_vf;
+    LDS<TP, TS, UNI>(sm, F_WF, 0, cb) = r.h_wf;
+    LDS<TP, TS, UNI>(sm, F_GEF, 0, cb) = r.h_gef;
+  }
+  LDSH<TP, TS, UNI>(sm, A_HA, cb) = r.h_sl * r.h_bm0;
+  LDSH<TP, TS, UNI>(sm, A_HB, cb) = r.h_sr * r.h_bp0;
+}
+
+// Phase: conservative update (euler.F:121-217, gravity :248-273) of the
+// tile's updated cells, in place in LDS.  Returns error flags.  With fbon (a
+// fallback workgroup whose pencil fell back), cells c with a fallback at c
+// or c-1 take flux_hll's gas-energy source
+// qc*pcent*(sl(c)gesl(c) + sr(c)gesr(c) - sl(c+1)gesl(c+1) - sr(c+1)gesr(c+1)).
+// For a cell c+1 after a fallback at c alone, the reference's one-cell call
+// evaluates that at c+2 from locals it never set (flux_hll.F:304-307 with
+// i2 = c); here they are the HLL terms of interface c+2, as in the oracle.
+template <int TP, int TS, int UNI = 0, int FB = 0>
+PPM_HD int ph_euler(const SweepParams &P, const Tile &T, double *sm, int tid, bool fbon = false) {
   int c0, c1;
   upd_range(P, T, c0, c1);
   const int n = c1 - c0 + 1;
@@ -775,6 +984,13 @@ PPM_HD int ph_euler(const SweepParams &P, const Tile &T, double *sm, int tid) {
       double pcent = fmx((gamma - 1.0) * ge * d, kTiny);
       double qc = qc_of<UNI>(P, T.dt, cell);
       double ges = qc * pcent * (LDS<TP, TS, UNI>(sm, F_UB, p, c) - LDS<TP, TS, UNI>(sm, F_UB, p, c + 1));
+      if constexpr (FB != 0) {
+        if (fbon &&
+            (LDS<TP, TS, UNI>(sm, A_COL, p, c) != 0.0 || LDS<TP, TS, UNI>(sm, A_COL, p, c - 1) != 0.0))
+          ges = qc * pcent *
+                (LDSH<TP, TS, UNI>(sm, A_HA, c) + LDSH<TP, TS, UNI>(sm, A_HB, c) -
+                 LDSH<TP, TS, UNI>(sm, A_HA, c + 1) - LDSH<TP, TS, UNI>(sm, A_HB, c + 1));
+      }
       ge = fmx((ge * d + (LDS<TP, TS, UNI>(sm, F_GEF, p, c) - LDS<TP, TS, UNI>(sm, F_GEF, p, c + 1)) + ges) *
                    dnuinv,
                0.5 * ge);
@@ -939,7 +1155,7 @@ PPM_HD void ph_load_col(const SweepParams &P, const Tile &T, double *sm, int tid
 // Phase: colour interface state and flux (inteuler.F:437-450, 493-551;
 // flux_twoshock.F:251-259, 375-380; euler.F:173-177 division by dx),
 // stored as colf/dx into F_COLF, and captured for owned faces.
-template <int TP, int TS, int UNI = 0>
+template <int TP, int TS, int UNI = 0, int FB = 0>
 PPM_HD void ph_col_flux(const SweepParams &P, const Tile &T, double *sm, int tid, int n,
                         const Regs &r) {
   int p = tid / TS, q = tid % TS;
@@ -955,6 +1171,8 @@ PPM_HD void ph_col_flux(const SweepParams &P, const Tile &T, double *sm, int tid
   colrs = fmx(colrs, kColorFloor);
   double colb = r.ub > 0.0 ? colls * r.db / r.dls : colrs * r.db / r.drs;
   double colf = T.dt * r.ub * colb;
+  if (FB != 0 && r.hll)  // flux_hll.F:231-236, 272-276, 327-331
+    colf = T.dt * (r.h_sl * (r.h_bm0 * colls) + r.h_sr * (r.h_bp0 * colrs));
   colf = colf / TS_<UNI>(P, T_DX, i);
   LDS<TP, TS, UNI>(sm, F_COLF, p, cb) = colf;
   if (P.face_head && i >= T.o && i < T.o + T.w) {
@@ -1045,13 +1263,13 @@ PPM_HD double flat_value(const FlatParams &F, int outer, int cell) {
 // thread followed by a workgroup barrier, ex.count(f) additionally returns the
 // workgroup sum of f's results.
 struct TileCounts {
-  int bad, chains, err;
+  int bad, chains, err, nfb;
 };
 
-template <int TP, int TS, int UNI, class X>
-PPM_HD TileCounts sweep_tile(const SweepParams &P, int block, double *sm, X &ex) {
-  const Tile T = make_tile<TP, TS, UNI>(P, block);
+template <int TP, int TS, int UNI, int FB, class X>
+PPM_HD TileCounts sweep_body(const SweepParams &P, const Tile &T, double *sm, X &ex) {
   TileCounts tc;
+  tc.nfb = 0;
   ex.run([&](int tid, Regs &r) {
     ph_flat_reg<TP, TS, UNI>(P, T, tid, r);
     ph_load<TP, TS, UNI>(P, T, sm, tid);
@@ -1067,15 +1285,30 @@ PPM_HD TileCounts sweep_tile(const SweepParams &P, int block, double *sm, X &ex)
     ex.run([&](int tid, Regs &r) { ph_iface<TP, TS, UNI>(P, T, sm, tid, v, r); });
   }
   ex.run([&](int tid, Regs &r) {
-    ph_solve<TP, TS, UNI>(P, T, sm, tid, r);
-    ph_store_flux<TP, TS, UNI>(P, T, sm, tid, r);
+    ph_solve<TP, TS, UNI, FB>(P, T, sm, tid, r);
+    ph_store_flux<TP, TS, UNI, FB>(P, T, sm, tid, r);
   });
   tc.bad = ex.count([&](int tid, Regs &r) { (void)r; return ph_check<TP, TS, UNI>(P, T, sm, tid); });
+  bool fbon = false;
+  int fberr = 0;
+  if constexpr (FB != 0) {
+    if (tc.bad) {  // block-uniform
+    const int dec = ex.count([&](int tid, Regs &r) {
+      (void)r;
+      return ph_fb_decide<TP, TS, UNI>(P, T, sm, tid);
+    });
+    tc.nfb = dec % kFbError;
+    fberr = dec / kFbError;
+    fbon = tc.nfb > 0;
+    if (fbon) ex.run([&](int tid, Regs &r) { ph_fb_apply<TP, TS, UNI>(P, T, sm, tid, r); });
+    }
+  }
   tc.err = ex.count([&](int tid, Regs &r) {
     (void)r;
     ph_capture<TP, TS, UNI>(P, T, sm, tid);
-    return ph_euler<TP, TS, UNI>(P, T, sm, tid);
+    return ph_euler<TP, TS, UNI, FB>(P, T, sm, tid, fbon);
   });
+  tc.err += fberr;
   if (P.idual) ex.run([&](int tid, Regs &r) { (void)r; ph_pgas_spec<TP, TS, UNI>(P, T, sm, tid); });
   tc.chains =
       ex.count([&](int tid, Regs &r) { (void)r; return ph_pgas_write<TP, TS, UNI>(P, T, sm, tid); });
@@ -1083,10 +1316,35 @@ PPM_HD TileCounts sweep_tile(const SweepParams &P, int block, double *sm, X &ex)
     ex.run([&](int tid, Regs &r) { (void)r; ph_load_col<TP, TS, UNI>(P, T, sm, tid, n); });
     ex.run([&](int tid, Regs &r) { (void)r; ph_slope<TP, TS, UNI>(P, T, sm, tid, A_COL); });
     ex.run([&](int tid, Regs &r) { ph_parab<TP, TS, UNI>(P, T, sm, tid, A_COL, 0, r); });
-    ex.run([&](int tid, Regs &r) { ph_col_flux<TP, TS, UNI>(P, T, sm, tid, n, r); });
+    ex.run([&](int tid, Regs &r) { ph_col_flux<TP, TS, UNI, FB>(P, T, sm, tid, n, r); });
     ex.run([&](int tid, Regs &r) { (void)r; ph_col_update<TP, TS, UNI>(P, T, sm, tid, n); });
   }
   return tc;
+}
+
+// A tile of the main launch.  bad > 0 means some cell of the tile needs the
+// fallback (or, without it, is an error); its pencils are then redone by a
+// fallback workgroup, which also settles the errors of those pencils.
+template <int TP, int TS, int UNI, class X>
+PPM_HD TileCounts sweep_tile(const SweepParams &P, int block, double *sm, X &ex) {
+  return sweep_body<TP, TS, UNI, 0>(P, make_tile<TP, TS, UNI>(P, block), sm, ex);
+}
+
+// One whole pencil in a one-pencil workgroup of TS threads, with the
+// in-order per-cell fallback; sm holds NARR_FB arrays of TS + 4 doubles.
+template <int TS, int UNI, class X>
+PPM_HD TileCounts sweep_pencil_fb(const SweepParams &P, int t1, int t2, double *sm, X &ex) {
+  return sweep_body<1, TS, UNI, 1>(P, make_pencil_tile(P, t1, t2), sm, ex);
+}
+
+// whether a one-pencil tile of TS threads holds the whole sweep axis
+template <int TS>
+PPM_HD bool pencil_fits(const SweepParams &P) {
+  Tile T;
+  T.o = 1;
+  T.w = P.N;
+  tile_ranges(P, T);
+  return T.NI + 4 <= TS && T.L1 - T.L0 + 1 <= TS + 4 && T.L0 >= 1 && T.L1 <= P.N;
 }
 
 }  // namespace ppm

@@ -178,3 +178,34 @@ def random_smooth(active, seed=0, ncolours=0, dual_energy=True, gravity=False, r
     glob = dict(Gamma=GAMMA, PPMSteepeningParameter=flags[0], PPMFlatteningParameter=flags[1],
                 PPMDiffusionParameter=flags[2], DualEnergyFormalism=int(dual_energy))
     return g, glob, dict(cfl=0.4, bc="periodic")
+
+
+def two_stream(n=32, axis=0, rank=1, seed=0, noise=0.3, p0=1e-3, transverse=(6, 5), gamma=GAMMA,
+               ncolours=0):
+    """Cold gas streaming apart from the middle of axis `axis` (u = -1 for
+    x < 0.5, +1 beyond, each times 1 + 0.2 N(0,1)), density lognormal with
+    standard deviation `noise`, pressure p0.  The central rarefaction drains
+    the middle cells; with a time step a few times the Courant one the
+    two-shock fluxes empty one of them, which is the per-cell fallback case
+    of flux_twoshock.F:401-456."""
+    rng = np.random.default_rng(seed)
+    act = [n] if rank == 1 else ([n] + list(transverse[:rank - 1]))
+    if rank > 1:  # put the streaming axis at `axis`
+        act = list(transverse[:rank - 1])
+        act.insert(axis, n)
+    g = Grid(act, rank=rank, ncolours=ncolours, dual_energy=True, dx=1.0 / n)
+    x = [g.cell_centers(0)[None, None, :], g.cell_centers(1)[None, :, None] if rank > 1 else 0.0,
+         g.cell_centers(2)[:, None, None] if rank > 2 else 0.0][axis] * np.ones(g.shape)
+    rho = np.exp(noise * rng.standard_normal(g.shape))
+    vel = np.where(x < 0.5, -1.0, 1.0) * (1 + 0.2 * rng.standard_normal(g.shape))
+    p = p0 * np.ones(g.shape)
+    z = np.zeros(g.shape)
+    v = [z, z, z]
+    v[axis] = vel
+    _set_thermo(g, rho, p, v[0], v[1], v[2], gamma)
+    for c in g.colnum:
+        g.fields[c][...] = rng.uniform(0.0, 1.0, size=g.shape)
+    g.fill_periodic()
+    glob = dict(Gamma=gamma, PPMSteepeningParameter=1, PPMFlatteningParameter=1,
+                PPMDiffusionParameter=0, DualEnergyFormalism=1)
+    return g, glob, dict(cfl=0.4, bc="periodic")

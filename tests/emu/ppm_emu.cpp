@@ -22,20 +22,23 @@ using namespace ppm;
 
 namespace {
 constexpr int kTP = 8, kTS = 32;
+constexpr int kFbTS = 1024;  // one-pencil fallback workgroups
 
-struct HostExec {
-  Regs r[kTP * kTS];
+template <int NT>
+struct HostExecT {
+  Regs r[NT];
   template <class F>
   void run(F f) {
-    for (int t = 0; t < kTP * kTS; t++) f(t, r[t]);
+    for (int t = 0; t < NT; t++) f(t, r[t]);
   }
   template <class F>
   int count(F f) {
     int s = 0;
-    for (int t = 0; t < kTP * kTS; t++) s += f(t, r[t]);
+    for (int t = 0; t < NT; t++) s += f(t, r[t]);
     return s;
   }
 };
+using HostExec = HostExecT<kTP * kTS>;
 }  // namespace
 
 extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *gl,
@@ -97,7 +100,9 @@ extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *
   }
   unsigned long long counters[4] = {0, 0, 0, 0};
   std::vector<double> sm(Cfg<kTP, kTS>::SMEM);
+  std::vector<double> smfb((size_t)NARR_FB * Cfg<1, kFbTS>::LW);
   HostExec *ex = new HostExec();
+  HostExecT<kFbTS> *exfb = new HostExecT<kFbTS>();
   const long long nact[3] = {g->GridEndIndex[0] - g->GridStartIndex[0] + 1,
                              g->GridEndIndex[1] - g->GridStartIndex[1] + 1,
                              g->GridEndIndex[2] - g->GridStartIndex[2] + 1};
@@ -190,12 +195,33 @@ extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *
     S.faces = sorted.data();
     S.face_out = fout.data();
     S.counters = counters;
+    S.fb_ok = pencil_fits<kFbTS>(S) ? 1 : 0;
     auto run_blocks = [&](long long nblk) {
+      // main launch: flag pencil groups with a failing cell or an error
+      std::vector<int> flagged;
+      std::vector<char> seen((size_t)nblk / S.nseg + 1, 0);
       for (long long b = 0; b < nblk; b++) {
         std::fill(sm.begin(), sm.end(), NAN);
-        TileCounts tc = S.uni[0] ? sweep_tile<kTP, kTS, 1>(S, (int)b, sm.data(), *ex)
-                                 : sweep_tile<kTP, kTS, 0>(S, (int)b, sm.data(), *ex);
-        counters[0] += tc.bad; counters[1] += tc.chains; counters[2] += tc.err;
+        const int t = xcd_tile((int)b, (int)nblk);  // the device's tile order
+        TileCounts tc = S.uni[0] ? sweep_tile<kTP, kTS, 1>(S, t, sm.data(), *ex)
+                                 : sweep_tile<kTP, kTS, 0>(S, t, sm.data(), *ex);
+        counters[1] += tc.chains;
+        if ((tc.bad || tc.err) && !seen[t / S.nseg]) {
+          seen[t / S.nseg] = 1;
+          flagged.push_back(t / S.nseg);
+        }
+      }
+      // fallback launch: every pencil of a flagged group, whole
+      for (int grp : flagged) {
+        const int tb = grp % S.nblk_t1, t2 = S.t2_0 + grp / S.nblk_t1;
+        for (int p = 0; p < kTP && tb * kTP + p < S.n_t1; p++) {
+          if (!S.fb_ok) { counters[3]++; break; }
+          std::fill(smfb.begin(), smfb.end(), NAN);
+          TileCounts tc = S.uni[0] ? sweep_pencil_fb<kFbTS, 1>(S, tb * kTP + p, t2, smfb.data(), *exfb)
+                                   : sweep_pencil_fb<kFbTS, 0>(S, tb * kTP + p, t2, smfb.data(), *exfb);
+          counters[0] += tc.nfb;
+          counters[2] += tc.err;
+        }
       }
     };
     if (S.idiff == 0) {
@@ -229,6 +255,9 @@ extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *
   for (int s = 0; s < nslot; s++)
     if (fnum[s] >= 0) memcpy(g->BaryonField[fnum[s]], &A[(size_t)s * cells], sizeof(double) * cells);
   delete ex;
-  for (int k = 0; k < 3; k++) counters_out[k] = (int64_t)counters[k];
+  delete exfb;
+  for (int k = 0; k < 4; k++) counters_out[k] = (int64_t)counters[k];
+  if (counters[3]) return ENZO_PPM_ERR_NOT_SUPPORTED;
+  if (counters[2]) return ENZO_PPM_ERR_NEGATIVE;
   return ENZO_PPM_SUCCESS;
 }

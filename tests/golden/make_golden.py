@@ -16,7 +16,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "enzo-dev_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 import oracle  # noqa: E402
+from helpers import FALLBACK_CASES, fallback_case  # noqa: E402
 from enzo_ppm import problems  # noqa: E402
 from enzo_ppm.grid import SubgridFluxes  # noqa: E402
 
@@ -92,6 +94,27 @@ def make_twoshock():
                         uls=uls, urs=urs, pbar=pbar, ubar=ubar)
 
 
+def make_fallback(name):
+    """One SolvePPM_DE call of a per-cell fallback case (tests/helpers.py)
+    with the reference's status, fallback count and fields.  The gas energy
+    of the cell after each fallback cell is the one value the reference
+    computes from locals it never set (flux_hll.F:304-307 with i1 = i2)."""
+    g, glob, cycle = fallback_case(name)
+    out = {"glob_keys": np.array(sorted(glob)), "glob_vals": np.array([float(glob[k]) for k in sorted(glob)]),
+           "active": np.array(g.active), "rank": g.rank, "ncolours": g.ncolours, "dx": np.array(g.dx),
+           "dt": g.dt, "cycle": cycle}
+    for d in range(3):
+        out[f"cellwidth{d}"] = g.cellwidth[d].copy()
+    for k, f in enumerate(g.fields):
+        out[f"in_f{k}"] = f.copy()
+    st, fb = oracle.solve_ppm_de("ref", g, glob, cycle=cycle)
+    assert st == FALLBACK_CASES[name][-1] and fb > 0, (name, st, fb)
+    out["status"], out["fallbacks"] = st, fb
+    for k, f in enumerate(g.fields):
+        out[f"out_f{k}"] = f.copy()
+    np.savez_compressed(os.path.join(HERE, f"fallback_{name}.npz"), **out)
+
+
 if __name__ == "__main__":
     if not oracle.available("ref"):
         oracle.build("ref")
@@ -100,3 +123,6 @@ if __name__ == "__main__":
         print("wrote", n)
     make_twoshock()
     print("wrote twoshock")
+    for n in ("1d_ok", "3d_y_ok", "3d_z_stretched_ok"):
+        make_fallback(n)
+        print("wrote fallback", n)

@@ -12,7 +12,7 @@ import pytest
 import oracle
 from enzo_ppm import abi, problems
 from enzo_ppm.grid import SubgridFluxes, make_globals
-from helpers import bit_equal, field_report, fill_bc
+from helpers import FALLBACK_CASES, bit_equal, fallback_case, field_report, fill_bc
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EMU = os.path.join(HERE, "emu", "libppm_emu.so")
@@ -98,3 +98,21 @@ def test_emulated_table_path(emu, monkeypatch):
     assert oracle.solve_ppm_de("port", a, glob, cycle=2)[0] == 1
     assert emu_solve(emu, b, glob, cycle=2)[0] == 1
     assert all(r[1] for r in field_report(b, a))
+
+
+@pytest.mark.parametrize("name", sorted(FALLBACK_CASES))
+def test_emulated_fallback(emu, name):
+    """Per-cell HLL fallback: the sweep tiles flag the failing pencils and
+    one-pencil workgroups of 1024 threads redo them with the reference's
+    in-order fallback.  Same status and fallback count as the oracle, and
+    the same fields where the step succeeds."""
+    g, glob, cycle = fallback_case(name)
+    status = FALLBACK_CASES[name][-1]
+    a, b = g.copy(), g.copy()
+    sa, fa = oracle.solve_ppm_de("port", a, glob, cycle=cycle)
+    assert sa == status and fa > 0
+    sb, cnt = emu_solve(emu, b, glob, cycle=cycle)
+    assert sb == status and cnt[0] == fa, (sb, cnt, fa)
+    if status == 1:
+        rep = field_report(b, a)
+        assert all(r[1] for r in rep), rep

@@ -1,0 +1,117 @@
+"""Per-cell HLL fallback on the GPU (flux_twoshock.F:401-456 -> flux_hll.F),
+through the C ABI, against the oracle.
+
+The sweep kernel flags every pencil group with a failing cell; the fallback
+kernel redoes those pencils whole, one 1024-thread workgroup per pencil, in
+the reference's cell order.  Bar: the oracle's status and fallback count,
+and bitwise-equal fields and face fluxes where the step succeeds.  The cases
+sit at the threshold where the first cell fails (tests/helpers.py).
+"""
+import ctypes as C
+
+import pytest
+
+import oracle
+from enzo_ppm import abi, device
+from enzo_ppm.grid import SubgridFluxes
+from helpers import FALLBACK_CASES, bit_equal, fallback_case, field_report
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    L = abi.load_library()
+    if L.hipPPMDeviceCount() < 1:
+        pytest.fail("HIP library loaded but no GPU is visible")
+    return L
+
+
+def gpu_solve(lib, g, glob, cycle, fluxes=None, handle=None):
+    """hipPPMSolvePPM_DE; returns (status, fallback count)."""
+    own = handle is None
+    h = C.c_void_p(None) if own else handle
+    try:
+        st = device.solve_ppm_de(g, glob, cycle=cycle, fluxes=fluxes, handle=h, lib=lib)
+        n = lib.hipPPMFallbackCount(h)
+    finally:
+        if own and h.value:
+            lib.hipPPMDestroy(h)
+    return st, n
+
+
+@pytest.mark.parametrize("name", sorted(FALLBACK_CASES))
+def test_fallback_matches_oracle(lib, name):
+    g, glob, cycle = fallback_case(name)
+    status = FALLBACK_CASES[name][-1]
+    a, b = g.copy(), g.copy()
+    so, fo = oracle.solve_ppm_de("port", a, glob, cycle=cycle)
+    assert so == status and fo > 0
+    sg, fg = gpu_solve(lib, b, glob, cycle)
+    assert (sg, fg) == (so, fo), lib.hipPPMLastError()
+    if status == 1:
+        rep = field_report(b, a)
+        assert all(r[1] for r in rep), rep
+
+
+def test_fallback_off_is_an_error(lib):
+    """RiemannSolverFallback = 0: the failing cell stops the step
+    (flux_twoshock.F:440-443)."""
+    g, glob, cycle = fallback_case("3d_y_ok")
+    glob = dict(glob, RiemannSolverFallback=0)
+    so, _ = oracle.solve_ppm_de("port", g.copy(), glob, cycle=cycle)
+    sg, fg = gpu_solve(lib, g.copy(), glob, cycle)
+    assert so == abi.ERR_NEGATIVE and sg == abi.ERR_NEGATIVE and fg == 0
+
+
+def test_fallback_face_fluxes(lib):
+    """Subgrid faces on the fallback interfaces capture the HLL fluxes (the
+    failing cell is y = 19: its faces are the right face of the first box
+    and the left face of the second)."""
+    g, glob, cycle = fallback_case("3d_y_ok", ncolours=2)
+    boxes = [((3, 15, 3), (8, 18, 7)), ((4, 20, 4), (6, 25, 6))]
+    a, b = g.copy(), g.copy()
+    fa, fb = SubgridFluxes(a, boxes), SubgridFluxes(b, boxes)
+    so, fo = oracle.solve_ppm_de("port", a, glob, cycle=cycle, fluxes=fa)
+    sg, fg = gpu_solve(lib, b, glob, cycle, fluxes=fb)
+    assert so == 1 and fo > 0 and (sg, fg) == (so, fo)
+    assert bit_equal(fa.flat(), fb.flat())
+    assert all(r[1] for r in field_report(b, a))
+
+
+def test_fallback_state_clears_between_calls(lib):
+    """One handle (same grid shape, so the device buffers are reused) for
+    fallback and plain steps in turn: a group flag or list entry left over
+    from an earlier step would hide a later step's fallback."""
+    h = C.c_void_p(None)
+    try:
+        for name, expect in (("3d_z_ok", True), ("3d_z_ok", False), ("3d_z_stretched_ok", True),
+                             ("3d_z_ok", True)):
+            g, glob, cycle = fallback_case(name)
+            if not expect:
+                g.dt = g.dt / 4.0  # well inside the Courant limit: no fallback
+            a, b = g.copy(), g.copy()
+            so, fo = oracle.solve_ppm_de("port", a, glob, cycle=cycle)
+            sg, fg = gpu_solve(lib, b, glob, cycle, handle=h)
+            assert (sg, fg) == (so, fo) and (fo > 0) == expect
+            assert all(r[1] for r in field_report(b, a))
+    finally:
+        if h.value:
+            lib.hipPPMDestroy(h)
+
+
+def test_fallback_device_resident_step(lib):
+    """hipPPMPressureAndTimeStep + hipPPMStep with the fallback case's dt."""
+    g, glob, cycle = fallback_case("3d_z_stretched_ok")
+    a, b = g.copy(), g.copy()
+    so, fo = oracle.solve_ppm_de("port", a, glob, cycle=cycle)
+    dg = device.DeviceGrid(b, glob, lib=lib)
+    try:
+        dg.pressure_and_dt(0.4)
+        dg.step(cycle, dt=g.dt)
+        st = dg.download()
+        assert st == so == 1
+        assert dg.fallback_count() == fo > 0
+    finally:
+        dg.close()
+    assert all(r[1] for r in field_report(b, a))

@@ -14,7 +14,7 @@ import pytest
 
 import oracle
 from enzo_ppm import problems
-from helpers import bit_equal, field_report, oracle_steps
+from helpers import FALLBACK_CASES, bit_equal, fallback_case, fallback_differences, field_report, oracle_steps
 
 pytestmark = pytest.mark.skipif(not oracle.available("ref"), reason="oracle/_ref not built")
 
@@ -114,3 +114,19 @@ def test_pgas2d_dual_routine_bitwise():
         res.append((ee, gg, pp))
     for x, y in zip(*res):
         assert bit_equal(x, y)
+
+
+@pytest.mark.parametrize("name", sorted(FALLBACK_CASES))
+def test_fallback_against_reference(name):
+    """Per-cell HLL fallback (flux_twoshock.F:401-456 -> flux_hll.F with
+    i1 = i2): same status and fallback count; where the step succeeds the
+    same fields except the gas energy of the cells after the fallback cells,
+    which the reference computes from locals it never set."""
+    g, glob, cycle = fallback_case(name)
+    a, b = g.copy(), g.copy()
+    sa, fa = oracle.solve_ppm_de("port", a, glob, cycle=cycle)
+    sb, fb = oracle.solve_ppm_de("ref", b, glob, cycle=cycle)
+    assert (sa, fa) == (sb, fb) and fa > 0
+    assert sa == FALLBACK_CASES[name][-1]
+    if sa == 1:
+        fallback_differences(a, b.fields, fa)
