@@ -42,6 +42,16 @@ struct DevExec {
     int v = f((int)threadIdx.x, r);
     return __syncthreads_count(v);
   }
+  template <class F>
+  __device__ int first(F f) {
+    __shared__ int v0;
+    int v = f((int)threadIdx.x, r);
+    if (threadIdx.x == 0) v0 = v;
+    __syncthreads();
+    v = v0;
+    __syncthreads();  // v0 is free for the next call
+    return v;
+  }
 };
 
 template <int UNI>

@@ -95,7 +95,8 @@ struct SweepParams {
   const int *face_head;   // [N+2]: records of position i are [head[i], head[i+1])
   const FaceRec *faces;
   double *face_out;
-  unsigned long long *counters;  // [0] fallback cells [1] dual-energy chain [2] errors
+  unsigned long long *counters;  // [0] fallback cells [1] threads that redid a dual-energy
+                                 // chain (diagnostic) [2] errors
                                  // [3] fallback needed on a pencil too long for it
   // pencil groups (block / nseg of the main launch) to redo with the
   // per-cell fallback: flag per group, list of flagged groups, its length,
@@ -1260,8 +1261,9 @@ PPM_HD double flat_value(const FlatParams &F, int outer, int cell) {
 
 // ---------------------------------------------------------------------------
 // The whole tile: phases in order; ex.run(f) runs f(tid, regs) on every
-// thread followed by a workgroup barrier, ex.count(f) additionally returns the
-// workgroup sum of f's results.
+// thread followed by a workgroup barrier; ex.count(f) also returns the number
+// of threads whose result is nonzero (__syncthreads_count), ex.first(f) the
+// result of thread 0.
 struct TileCounts {
   int bad, chains, err, nfb;
 };
@@ -1293,7 +1295,7 @@ PPM_HD TileCounts sweep_body(const SweepParams &P, const Tile &T, double *sm, X 
   int fberr = 0;
   if constexpr (FB != 0) {
     if (tc.bad) {  // block-uniform
-    const int dec = ex.count([&](int tid, Regs &r) {
+    const int dec = ex.first([&](int tid, Regs &r) {
       (void)r;
       return ph_fb_decide<TP, TS, UNI>(P, T, sm, tid);
     });

@@ -32,10 +32,19 @@ struct HostExecT {
     for (int t = 0; t < NT; t++) f(t, r[t]);
   }
   template <class F>
-  int count(F f) {
+  int count(F f) {  // as __syncthreads_count: threads with a nonzero result
     int s = 0;
-    for (int t = 0; t < NT; t++) s += f(t, r[t]);
+    for (int t = 0; t < NT; t++) s += f(t, r[t]) != 0;
     return s;
+  }
+  template <class F>
+  int first(F f) {  // thread 0's result
+    int v = 0;
+    for (int t = 0; t < NT; t++) {
+      int x = f(t, r[t]);
+      if (t == 0) v = x;
+    }
+    return v;
   }
 };
 using HostExec = HostExecT<kTP * kTS>;
