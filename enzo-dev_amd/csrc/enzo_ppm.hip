@@ -481,6 +481,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
     S.n_t1 = D->GD[S.ax_t1];
     S.n_t2 = D->GD[S.ax_t2];
     S.t2_0 = 0;
+    set_strides(S);
     S.nblk_t1 = (S.n_t1 + kTP - 1) / kTP;
     long long nblk = (long long)S.nseg * S.nblk_t1 * S.n_t2;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -511,6 +512,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
     S.ax_t2 = oax;
     S.n_t1 = D->GD[rax];
     S.n_t2 = 1;
+    set_strides(S);
     S.nblk_t1 = (S.n_t1 + kTP - 1) / kTP;
     for (int k = 0; k < D->GD[oax]; k++) {
       S.t2_0 = k;

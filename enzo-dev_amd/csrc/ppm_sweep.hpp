@@ -72,6 +72,7 @@ struct SweepParams {
   int GD[3];
   long long cells;
   int ax_t1, ax_t2;       // grid axes enumerated by t1 (within a block) and t2
+  int st_s, st_1, st_2;   // grid strides of the sweep, t1 and t2 axes (set_strides)
   int n_t1, n_t2, t2_0;   // pencil counts; first t2 (slab-serial launches)
   int nseg, nblk_t1;
   const int *seg_o, *seg_w;  // owned cells of each segment: [o, o+w)
@@ -211,15 +212,28 @@ PPM_HD double qc_of(const SweepParams &P, double dt, int cell) {
   return P.qc[cell - 1];
 }
 
-// 3-D grid index of pencil (t1, t2) at sweep cell `cell` (1-based)
+// grid strides of the axes of a sweep (x fastest); after dim, ax_t1, ax_t2
+inline void set_strides(SweepParams &S) {
+  const int st[3] = {1, S.GD[0], S.GD[0] * S.GD[1]};
+  S.st_s = st[S.dim];
+  S.st_1 = st[S.ax_t1];
+  S.st_2 = st[S.ax_t2];
+}
+
+// 3-D grid index of pencil (t1, t2) at sweep cell `cell` (1-based).  Every
+// index is below prod(GridDimension) <= 1031^3 < 2^31, so 32-bit arithmetic.
 PPM_HD long long gidx(const SweepParams &P, int t1, int t2, int cell) {
+#ifdef PPM_EMU_CHECKS
   int c[3];
   c[P.dim] = cell - 1;
   c[P.ax_t1] = t1;
   c[P.ax_t2] = t2;
   PPM_ASSERT(c[0] >= 0 && c[0] < P.GD[0] && c[1] >= 0 && c[1] < P.GD[1] && c[2] >= 0 &&
              c[2] < P.GD[2]);
-  return ((long long)c[2] * P.GD[1] + c[1]) * P.GD[0] + c[0];
+  PPM_ASSERT((cell - 1) * P.st_s + t1 * P.st_1 + t2 * P.st_2 ==
+             (c[2] * P.GD[1] + c[1]) * P.GD[0] + c[0]);
+#endif
+  return (long long)((cell - 1) * P.st_s + t1 * P.st_1 + t2 * P.st_2);
 }
 
 PPM_HD int coord_of(const SweepParams &P, int axis, int t1, int t2) {
@@ -283,6 +297,25 @@ PPM_HD Tile make_pencil_tile(const SweepParams &P, int t1, int t2) {
   return T;
 }
 
+// Items k in [0, n) of every pencil of the tile, spread over the workgroup
+// and coalesced along the contiguous grid axis: along the items when
+// sweeping x (thread tid takes pencil tid / TS and items tid % TS + m TS),
+// along the pencils otherwise (consecutive threads, consecutive pencils).
+// TP and TS are compile-time constants, so no integer division remains.
+template <int TP, int TS, class F>
+PPM_HD void per_cell_coalesced(const SweepParams &P, const Tile &T, int tid, int n, F f) {
+  if (P.dim == 0) {
+    const int p = tid / TS;
+    if (p < T.np)
+      for (int k = tid % TS; k < n; k += TS) f(p, k);
+  } else {
+    for (int idx = tid; idx < TP * n; idx += TP * TS) {
+      const int p = idx % TP, k = idx / TP;
+      if (p < T.np) f(p, k);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Phase: load the pencil segments into LDS (coalesced along the contiguous
 // grid axis: s when sweeping x, t1 otherwise).
@@ -294,23 +327,16 @@ template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_flat_reg(const SweepParams &P, const Tile &T, int tid, Regs &r) {
   const int n = T.NI + 2;
   r.flat_c = 0.0;
-  if (P.iflatten == 0 || tid >= TP * n) return;
-  const int p = tid / n, cell = T.IA - 1 + tid % n;
-  if (p >= T.np) return;
-  r.flat_c = flat_of<TP, TS, UNI>(P, T, p, cell);
+  const int p = tid / TS, k = tid % TS;
+  if (P.iflatten == 0 || p >= T.np || k >= n) return;
+  r.flat_c = flat_of<TP, TS, UNI>(P, T, p, T.IA - 1 + k);
 }
 
 template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_load(const SweepParams &P, const Tile &T, double *sm, int tid) {
   const int nc = T.L1 - T.L0 + 1;
-  const int total = TP * nc;
-  const bool s_contig = (P.dim == 0);
   const int vax = (P.dim + 1) % 3, wax = (P.dim + 2) % 3;
-  for (int idx = tid; idx < total; idx += Cfg<TP, TS, UNI>::NT) {
-    int p, c;
-    if (s_contig) { p = idx / nc; c = idx % nc; }
-    else { p = idx % TP; c = idx / TP; }
-    if (p >= T.np) continue;
+  per_cell_coalesced<TP, TS>(P, T, tid, nc, [&](int p, int c) {
     int cell = T.L0 + c;
     long long g = gidx(P, T.t1_0 + p, T.t2, cell);
     const double *A = P.A;
@@ -324,7 +350,7 @@ PPM_HD void ph_load(const SweepParams &P, const Tile &T, double *sm, int tid) {
     if (P.idual) LDS<TP, TS, UNI>(sm, A_GE, p, c) = A[S_GE * P.cells + g];
     LDS<TP, TS, UNI>(sm, A_P, p, c) = P.pres[g];
     if (P.gravity) LDS<TP, TS, UNI>(sm, A_GR, p, c) = P.gr[g];
-  }
+  });
 }
 
 // Phase: sound speed (inteuler.F:199-200) at cells [IA-1, IB] into A_CS,
@@ -334,9 +360,8 @@ PPM_HD void ph_load(const SweepParams &P, const Tile &T, double *sm, int tid) {
 template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_chars(const SweepParams &P, const Tile &T, double *sm, int tid) {
   const int nch = T.NI + 2;
-  for (int idx = tid; idx < TP * nch; idx += Cfg<TP, TS, UNI>::NT) {
-    int p = idx / nch, cell = T.IA - 1 + idx % nch;
-    if (p >= T.np) continue;
+  for (int p = tid / TS, k = tid % TS; p < T.np && k < nch; k += TS) {
+    int cell = T.IA - 1 + k;
     int c = cell - T.L0;
     double pp = LDS<TP, TS, UNI>(sm, A_P, p, c), dd = LDS<TP, TS, UNI>(sm, A_D, p, c);
     double cs = sqrt(P.gamma * pp / dd);
@@ -345,9 +370,8 @@ PPM_HD void ph_chars(const SweepParams &P, const Tile &T, double *sm, int tid) {
   }
   if (P.isteep) {
     const int n2 = T.NI + 4;
-    for (int idx = tid; idx < TP * n2; idx += Cfg<TP, TS, UNI>::NT) {
-      int p = idx / n2, cell = T.IA - 2 + idx % n2;
-      if (p >= T.np) continue;
+    for (int p = tid / TS, k = tid % TS; p < T.np && k < n2; k += TS) {
+      int cell = T.IA - 2 + k;
       int c = cell - T.L0;
       double dm = LDS<TP, TS, UNI>(sm, A_D, p, c - 1), d0 = LDS<TP, TS, UNI>(sm, A_D, p, c),
              dp = LDS<TP, TS, UNI>(sm, A_D, p, c + 1);
@@ -363,9 +387,8 @@ template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_steepen(const SweepParams &P, const Tile &T, double *sm, int tid) {
   if (!P.isteep) return;
   const int n = T.NI + 2;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
-    int p = idx / n, cell = T.IA - 1 + idx % n;
-    if (p >= T.np) continue;
+  for (int p = tid / TS, k = tid % TS; p < T.np && k < n; k += TS) {
+    int cell = T.IA - 1 + k;
     int c = cell - T.L0;
     double dm = LDS<TP, TS, UNI>(sm, A_D, p, c - 1), dp = LDS<TP, TS, UNI>(sm, A_D, p, c + 1);
     double pm = LDS<TP, TS, UNI>(sm, A_P, p, c - 1), pq = LDS<TP, TS, UNI>(sm, A_P, p, c + 1);
@@ -387,9 +410,8 @@ PPM_HD void ph_steepen(const SweepParams &P, const Tile &T, double *sm, int tid)
 template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_slope(const SweepParams &P, const Tile &T, double *sm, int tid, int arr) {
   const int n = T.NI + 4;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
-    int p = idx / n, cell = T.IA - 2 + idx % n;
-    if (p >= T.np) continue;
+  for (int p = tid / TS, k = tid % TS; p < T.np && k < n; k += TS) {
+    int cell = T.IA - 2 + k;
     int c = cell - T.L0;
     double qm = LDS<TP, TS, UNI>(sm, arr, p, c - 1), q0 = LDS<TP, TS, UNI>(sm, arr, p, c),
            qp = LDS<TP, TS, UNI>(sm, arr, p, c + 1);
@@ -421,10 +443,9 @@ template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_parab(const SweepParams &P, const Tile &T, double *sm, int tid, int arr,
                      int steep, const Regs &rg) {
   const int n = T.NI + 2;
-  PPM_ASSERT(TP * n <= Cfg<TP, TS, UNI>::NT);  // one cell per thread (build_segments)
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
-    int p = idx / n, cell = T.IA - 1 + idx % n;
-    if (p >= T.np) continue;
+  PPM_ASSERT(n <= TS);  // one cell per thread, as ph_flat_reg (build_segments)
+  for (int p = tid / TS, k = tid % TS; p < T.np && k < n; k += TS) {
+    int cell = T.IA - 1 + k;
     int c = cell - T.L0;
     double qm = LDS<TP, TS, UNI>(sm, arr, p, c - 1), q = LDS<TP, TS, UNI>(sm, arr, p, c),
            qp = LDS<TP, TS, UNI>(sm, arr, p, c + 1);
@@ -871,9 +892,8 @@ PPM_HD int ph_check(const SweepParams &P, const Tile &T, double *sm, int tid) {
   const int n = c1 - c0 + 1;
   int bad = 0;
   if (n <= 0) return 0;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
-    int p = idx / n, cell = c0 + idx % n;
-    if (p >= T.np) continue;
+  for (int p = tid / TS, k = tid % TS; p < T.np && k < n; k += TS) {
+    int cell = c0 + k;
     int c = cell - T.L0;
     double d = LDS<TP, TS, UNI>(sm, A_D, p, c), e = LDS<TP, TS, UNI>(sm, A_E, p, c);
     double dfl = LDS<TP, TS, UNI>(sm, F_DF, p, c), dfr = LDS<TP, TS, UNI>(sm, F_DF, p, c + 1);
@@ -961,9 +981,8 @@ PPM_HD int ph_euler(const SweepParams &P, const Tile &T, double *sm, int tid, bo
   int err = 0;
   if (n <= 0) return 0;
   const double dt = T.dt, gamma = P.gamma;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
-    int p = idx / n, cell = c0 + idx % n;
-    if (p >= T.np) continue;
+  for (int p = tid / TS, k = tid % TS; p < T.np && k < n; k += TS) {
+    int cell = c0 + k;
     int c = cell - T.L0;
     double d = LDS<TP, TS, UNI>(sm, A_D, p, c), e = LDS<TP, TS, UNI>(sm, A_E, p, c);
     double u = LDS<TP, TS, UNI>(sm, A_U, p, c), v = LDS<TP, TS, UNI>(sm, A_V, p, c),
@@ -1029,9 +1048,8 @@ PPM_HD void ph_pgas_spec(const SweepParams &P, const Tile &T, double *sm, int ti
   int a = T.o - 1 > g0 ? T.o - 1 : g0, b = T.o + T.w - 1 < g1 ? T.o + T.w - 1 : g1;
   int n = b - a + 1;
   if (n <= 0) return;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
-    int p = idx / n, cell = a + idx % n;
-    if (p >= T.np) continue;
+  for (int p = tid / TS, k = tid % TS; p < T.np && k < n; k += TS) {
+    int cell = a + k;
     int c = cell - T.L0;
     const double d = LDS<TP, TS, UNI>(sm, A_D, p, c), e = LDS<TP, TS, UNI>(sm, A_E, p, c);
     const double u = LDS<TP, TS, UNI>(sm, A_U, p, c), v = LDS<TP, TS, UNI>(sm, A_V, p, c),
@@ -1058,11 +1076,7 @@ PPM_HD int ph_pgas_write(const SweepParams &P, const Tile &T, double *sm, int ti
   int chains = 0;
   const int n = T.w;
   const int vax = (P.dim + 1) % 3, wax = (P.dim + 2) % 3;
-  for (int idx = tid; idx < TP * n; idx += Cfg<TP, TS, UNI>::NT) {
-    int p, k;
-    if (P.dim == 0) { p = idx / n; k = idx % n; }
-    else { p = idx % TP; k = idx / TP; }
-    if (p >= T.np) continue;
+  per_cell_coalesced<TP, TS>(P, T, tid, n, [&](int p, int k) {
     int cell = T.o + k, c = cell - T.L0;
     double d = LDS<TP, TS, UNI>(sm, A_D, p, c), e = LDS<TP, TS, UNI>(sm, A_E, p, c);
     double u = LDS<TP, TS, UNI>(sm, A_U, p, c), v = LDS<TP, TS, UNI>(sm, A_V, p, c),
@@ -1097,7 +1111,7 @@ PPM_HD int ph_pgas_write(const SweepParams &P, const Tile &T, double *sm, int ti
     B[(S_VX + vax) * P.cells + g] = vax < P.rank ? v : P.A[(S_VX + vax) * P.cells + g];
     B[(S_VX + wax) * P.cells + g] = wax < P.rank ? w : P.A[(S_VX + wax) * P.cells + g];
     if (P.idual) B[S_GE * P.cells + g] = ge;
-  }
+  });
   return chains;
 }
 
@@ -1142,15 +1156,10 @@ PPM_HD void ph_capture(const SweepParams &P, const Tile &T, double *sm, int tid)
 template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_load_col(const SweepParams &P, const Tile &T, double *sm, int tid, int n) {
   const int nc = T.L1 - T.L0 + 1;
-  const bool s_contig = (P.dim == 0);
   const double *src = P.A + (long long)(S_COL0 + n) * P.cells;
-  for (int idx = tid; idx < TP * nc; idx += Cfg<TP, TS, UNI>::NT) {
-    int p, c;
-    if (s_contig) { p = idx / nc; c = idx % nc; }
-    else { p = idx % TP; c = idx / TP; }
-    if (p >= T.np) continue;
+  per_cell_coalesced<TP, TS>(P, T, tid, nc, [&](int p, int c) {
     LDS<TP, TS, UNI>(sm, A_COL, p, c) = src[gidx(P, T.t1_0 + p, T.t2, T.L0 + c)];
-  }
+  });
 }
 
 // Phase: colour interface state and flux (inteuler.F:437-450, 493-551;
@@ -1194,11 +1203,7 @@ template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_col_update(const SweepParams &P, const Tile &T, double *sm, int tid, int n) {
   const double min_color = 1e-5 * kTiny;
   double *dst = P.B + (long long)(S_COL0 + n) * P.cells;
-  for (int idx = tid; idx < TP * T.w; idx += Cfg<TP, TS, UNI>::NT) {
-    int p, k;
-    if (P.dim == 0) { p = idx / T.w; k = idx % T.w; }
-    else { p = idx % TP; k = idx / TP; }
-    if (p >= T.np) continue;
+  per_cell_coalesced<TP, TS>(P, T, tid, T.w, [&](int p, int k) {
     int cell = T.o + k, c = cell - T.L0;
     double col = LDS<TP, TS, UNI>(sm, A_COL, p, c);
     if (cell >= P.i1 && cell <= P.i2) {
@@ -1206,7 +1211,7 @@ PPM_HD void ph_col_update(const SweepParams &P, const Tile &T, double *sm, int t
       col = fmx(col, min_color);
     }
     dst[gidx(P, T.t1_0 + p, T.t2, cell)] = col;
-  }
+  });
 }
 
 // flattening coefficient of line 1 of slice `outer` at `cell` (calcdiss.F:193-202,

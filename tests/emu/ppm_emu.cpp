@@ -238,10 +238,12 @@ extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *
       else if (dim == 1) { S.ax_t1 = 0; S.ax_t2 = 2; }
       else { S.ax_t1 = 0; S.ax_t2 = 1; }
       S.n_t1 = GD[S.ax_t1]; S.n_t2 = GD[S.ax_t2]; S.t2_0 = 0;
+      set_strides(S);
       S.nblk_t1 = (S.n_t1 + kTP - 1) / kTP;
       run_blocks((long long)S.nseg * S.nblk_t1 * S.n_t2);
     } else {
       S.ax_t1 = rax; S.ax_t2 = oax; S.n_t1 = GD[rax]; S.n_t2 = 1;
+      set_strides(S);
       S.nblk_t1 = (S.n_t1 + kTP - 1) / kTP;
       for (int k = 0; k < GD[oax]; k++) {
         S.t2_0 = k;

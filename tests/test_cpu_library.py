@@ -1,0 +1,117 @@
+"""The library's own HIP source, built for the CPU (tests/emu/hipcpu), against
+the oracle through the same C ABI the GPU tests use.
+
+This runs the library's host orchestration -- allocations, launch shapes,
+sweep order, slice-serial diffusion launches, fallback lists -- and its
+kernels work-item by work-item, with every device buffer against a guard
+page, so an out-of-bounds index stops the test.  Bar: bitwise equality with
+the oracle, and its status and fallback count.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import pytest
+
+import oracle
+from enzo_ppm import abi, device, problems
+from enzo_ppm.grid import SubgridFluxes
+from helpers import FALLBACK_CASES, bit_equal, fallback_case, field_report, fill_bc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "emu", "libenzo_ppm_cpu.so")
+SRC = [os.path.join(ROOT, "enzo-dev_amd", "csrc", f) for f in ("enzo_ppm.hip", "ppm_sweep.hpp",
+                                                                "ppm_geometry.hpp")]
+
+
+@pytest.fixture(scope="module")
+def cpulib():
+    if not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(f) for f in SRC):
+        subprocess.run(["g++", "-x", "c++", "-O1", "-g", "-std=c++17", "-fPIC", "-shared",
+                        "-Wno-attributes", "-U_FORTIFY_SOURCE", "-Wl,-Bsymbolic", "-ffp-contract=off", "-fno-fast-math", "-I",
+                        os.path.join(HERE, "emu", "hipcpu"), "-o", LIB, SRC[0]], check=True)
+    return abi.load_library(LIB)
+
+
+def run(lib, g, glob, cycle, fluxes=None):
+    h = C.c_void_p(None)
+    try:
+        st = device.solve_ppm_de(g, glob, cycle=cycle, fluxes=fluxes, handle=h, lib=lib)
+        return st, lib.hipPPMFallbackCount(h)
+    finally:
+        if h.value:
+            lib.hipPPMDestroy(h)
+
+
+# guard after every buffer for all cases; before every buffer for a few
+GUARDED = [(n, "high") for n in sorted(FALLBACK_CASES)] + [
+    (n, "low") for n in ("1d_ok", "3d_y_diffusion_hll_fails", "3d_z_stretched_ok")]
+
+
+@pytest.mark.parametrize("name,guard", GUARDED)
+def test_fallback_cases(cpulib, name, guard, monkeypatch):
+    monkeypatch.setenv("HIPCPU_GUARD", guard)
+    g, glob, cycle = fallback_case(name)
+    a, b = g.copy(), g.copy()
+    so, fo = oracle.solve_ppm_de("port", a, glob, cycle=cycle)
+    assert (so, fo > 0) == (FALLBACK_CASES[name][-1], True)
+    assert run(cpulib, b, glob, cycle) == (so, fo)
+    if so == 1:
+        assert all(r[1] for r in field_report(b, a))
+
+
+CASES = {
+    "diffusion": lambda: problems.random_smooth([10, 12, 8], seed=4, ncolours=1, flags=(1, 1, 1)),
+    "stretched_diffusion": lambda: problems.random_smooth([12, 10, 11], seed=7, ncolours=1,
+                                                          stretch=0.2, flags=(1, 1, 1)),
+    "gravity": lambda: problems.random_smooth([10, 12, 8], seed=2, ncolours=1, gravity=True),
+    "sod_2d": lambda: problems.sod_x(20, rank=2, gamma=1.4, flags=(1, 1, 1)),
+    "turbulent": lambda: problems.turbulent(12, seed=3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_steps(cpulib, name):
+    g, glob, _ = CASES[name]()
+    for cycle in (0, 2):
+        fill_bc(g, "periodic")
+        g.dt = oracle.compute_time_step("port", g, glob, 0.4)
+        a, b = g.copy(), g.copy()
+        assert oracle.solve_ppm_de("port", a, glob, cycle=cycle)[0] == 1
+        assert run(cpulib, b, glob, cycle) == (1, 0)
+        assert all(r[1] for r in field_report(b, a))
+        g = a
+
+
+def test_face_fluxes(cpulib):
+    g, glob, _ = problems.random_smooth([20, 18, 16], seed=5, ncolours=2)
+    boxes = [((7, 5, 4), (12, 9, 8)), ((3, 3, 3), (22, 20, 18)), ((10, 3, 3), (10, 20, 18))]
+    g.fill_periodic()
+    g.dt = oracle.compute_time_step("port", g, glob, 0.4)
+    a, b = g.copy(), g.copy()
+    fa, fb = SubgridFluxes(a, boxes), SubgridFluxes(b, boxes)
+    assert oracle.solve_ppm_de("port", a, glob, cycle=1, fluxes=fa)[0] == 1
+    assert run(cpulib, b, glob, 1, fluxes=fb)[0] == 1
+    assert bit_equal(fa.flat(), fb.flat())
+
+
+def test_device_resident_steps(cpulib):
+    """Boundary fill, pressure + Courant pass and rotated sweeps on the
+    device state, three steps, against the oracle with the same time steps."""
+    g, glob, _ = problems.turbulent(12, seed=8)
+    a = g.copy()
+    dg = device.DeviceGrid(g, glob, lib=cpulib)
+    try:
+        for cycle in range(3):
+            dg.periodic_boundary()
+            dt = dg.pressure_and_dt(0.4)
+            dg.step(cycle)
+            a.fill_periodic()
+            a.dt = oracle.compute_time_step("port", a, glob, 0.4)
+            assert dt == a.dt
+            assert oracle.solve_ppm_de("port", a, glob, cycle=cycle)[0] == 1
+        assert dg.download() == 1
+    finally:
+        dg.close()
+    assert all(r[1] for r in field_report(g, a))
