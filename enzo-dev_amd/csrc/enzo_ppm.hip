@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/enzo_ppm.h"
+#include "ppm_device.hpp"
 #include "ppm_geometry.hpp"
 #include "ppm_sweep.hpp"
 
@@ -30,30 +31,6 @@ using SweepCfg = Cfg<kTP, kTS>;
 
 // ------------------------------------------------------------------------
 // Fused sweep kernel: see ppm_sweep.hpp for the stages.
-struct DevExec {
-  Regs r;
-  template <class F>
-  __device__ void run(F f) {
-    f((int)threadIdx.x, r);
-    __syncthreads();
-  }
-  template <class F>
-  __device__ int count(F f) {
-    int v = f((int)threadIdx.x, r);
-    return __syncthreads_count(v);
-  }
-  template <class F>
-  __device__ int first(F f) {
-    __shared__ int v0;
-    int v = f((int)threadIdx.x, r);
-    if (threadIdx.x == 0) v0 = v;
-    __syncthreads();
-    v = v0;
-    __syncthreads();  // v0 is free for the next call
-    return v;
-  }
-};
-
 template <int UNI>
 __global__ __launch_bounds__(kTP *kTS) __attribute__((amdgpu_waves_per_eu(4)))
 void ppm_sweep_kernel(SweepParams P) {
@@ -64,10 +41,7 @@ void ppm_sweep_kernel(SweepParams P) {
   if (threadIdx.x == 0) {
     // a failing cell or an error: the fallback kernel redoes the group's
     // pencils whole (the fallback chain can cross segments) and settles them
-    if (tc.bad || tc.err) {
-      const int grp = tile / P.nseg;
-      if (atomicExch(&P.fb_flag[grp], 1) == 0) P.fb_list[atomicAdd(P.fb_count, 1u)] = (unsigned)grp;
-    }
+    if (tc.bad || tc.err) flag_group(P, tile / P.nseg);
     if (tc.chains) atomicAdd(&P.counters[1], (unsigned long long)tc.chains);
   }
 }
@@ -78,152 +52,30 @@ void ppm_sweep_kernel(SweepParams P) {
 // kFbTS threads (one thread per interface, the pencil in LDS), with the
 // fallback applied in the reference's cell order; its results replace the
 // sweep kernel's for those pencils.  Launched after every sweep launch; with
-// no flagged group each workgroup only reads the count.  The last workgroup
-// to finish clears the count for the next launch (the group flags are
-// cleared group by group).
-constexpr int kFbTS = 1024;
-constexpr int kFbBlocks = 64;
-using FbCfg = Cfg<1, kFbTS>;
-
+// no flagged group each workgroup only reads the count.
 template <int UNI>
 __global__ __launch_bounds__(kFbTS) void ppm_fallback_kernel(SweepParams P) {
   __shared__ double sm[NARR_FB * FbCfg::LW];
   DevExec ex;
   const unsigned n = *P.fb_count;
-  for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
-    const int grp = (int)P.fb_list[k];
-    const int tb = grp % P.nblk_t1, t2 = P.t2_0 + grp / P.nblk_t1;
-    for (int p = 0; p < kTP && tb * kTP + p < P.n_t1; p++) {
-      if (!P.fb_ok) {
-        if (threadIdx.x == 0) atomicAdd(&P.counters[3], 1ull);
-        break;
-      }
-      TileCounts tc = sweep_pencil_fb<kFbTS, UNI>(P, tb * kTP + p, t2, sm, ex);
-      if (threadIdx.x == 0) {
-        if (tc.nfb) atomicAdd(&P.counters[0], (unsigned long long)tc.nfb);
-        if (tc.err) atomicAdd(&P.counters[2], (unsigned long long)tc.err);
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) P.fb_flag[grp] = 0;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    if (atomicAdd(P.fb_done, 1u) == gridDim.x - 1) {
-      *P.fb_count = 0u;
-      *P.fb_done = 0u;
-      __threadfence();
-    }
-  }
+  for (unsigned k = blockIdx.x; k < n; k += gridDim.x) fallback_group<UNI>(P, (int)P.fb_list[k], sm, ex);
+  fallback_launch_done(P.fb_count, P.fb_done);
 }
 
 // ------------------------------------------------------------------------
-// Per-sweep preparation: qc(i) = dt/dx(i) for the whole axis (used by the
-// fluxes, flux_twoshock.F:366, and the gas-energy source, :393), and the
-// flattening of line 1 of every slice.  The reference hands `flatten`
-// (line 1) to intvar for every line of the slice (inteuler.F:241-289), so one
-// coefficient row per slice is all the sweep needs.  Threads [0, N) fill qc;
-// the rest one (slice, cell in [i1-1, i2+1]) each.
+// Per-sweep preparation (prep_item in ppm_device.hpp), one item per thread.
 __global__ void sweep_prep_kernel(FlatParams F) {
-  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid < F.N) {
-    const double dt = F.dtp ? *F.dtp : F.dt;
-    F.qc[gid] = dt / F.dx[gid];
-    return;
-  }
-  if (!F.iflatten) return;
-  const long long k = gid - F.N;
-  const int ncell = F.i2 - F.i1 + 3;
-  if (k >= (long long)ncell * F.n_outer) return;
-  const int outer = (int)(k / ncell);
-  const int cell = F.i1 - 1 + (int)(k % ncell);
-  F.flat[(long long)outer * F.N + (cell - 1)] = flat_value(F, outer, cell);
+  prep_item(F, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ------------------------------------------------------------------------
-// Fused pressure + Courant condition (Grid_ComputePressure.C:96-166,
-// calc_dt.F:129-177).  Thread per cell; active cells contribute
-// dt1 = aye / sum_d (c_s + |v_d|)/dx_d to a device-wide minimum kept as the
-// bit pattern of a positive double (ordered like the value).
-struct PresParams {
-  int rank, idual;
-  int GD[3], st[3], en[3];
-  long long cells;
-  const double *A;
-  double *pres;
-  double gamma, tiny_number, min_support;
-  const double *cw[3];  // CellWidth per axis (calc_dt widths)
-  double aye;
-  int want_dt;
-  unsigned long long *dtmin;
-};
-
-__device__ inline double pressure_of(const PresParams &Q, long long g) {
-  const double *A = Q.A;
-  double density = A[S_D * Q.cells + g];
-  double gas_energy;
-  if (Q.idual == 0) {
-    double v1 = A[S_VX * Q.cells + g];
-    double v2 = Q.rank > 1 ? A[S_VY * Q.cells + g] : 0.0;
-    double v3 = Q.rank > 2 ? A[S_VZ * Q.cells + g] : 0.0;
-    gas_energy = A[S_E * Q.cells + g] - 0.5 * (v1 * v1 + v2 * v2 + v3 * v3);
-  } else {
-    gas_energy = A[S_GE * Q.cells + g];
-  }
-  double p = (Q.gamma - 1.0) * density * gas_energy;
-  if (p < Q.tiny_number) p = Q.tiny_number;
-  double min_pressure = Q.min_support * (Q.gamma - 1.0) * density * density;
-  if (p < min_pressure) p = min_pressure;
-  return p;
-}
-
+// Fused pressure + Courant condition, grid-stride over the cells.
 __global__ __launch_bounds__(256) void pressure_dt_kernel(PresParams Q) {
   double local = 1.0e20;
   for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < Q.cells;
-       g += (long long)gridDim.x * blockDim.x) {
-    double p = pressure_of(Q, g);
-    Q.pres[g] = p;
-    if (!Q.want_dt) continue;
-    int i = (int)(g % Q.GD[0]);
-    long long r = g / Q.GD[0];
-    int j = (int)(r % Q.GD[1]);
-    int k = (int)(r / Q.GD[1]);
-    if (i < Q.st[0] || i > Q.en[0] || j < Q.st[1] || j > Q.en[1] || k < Q.st[2] || k > Q.en[2])
-      continue;
-    double pdt = p;
-    if (Q.min_support != 0.0) {  // calc_dt's pressure uses coefficient 0
-      PresParams Z = Q;
-      Z.min_support = 0.0;
-      pdt = pressure_of(Z, g);
-    }
-    double d = Q.A[S_D * Q.cells + g];
-    double cs = fmx(sqrt(Q.gamma * pdt / d), kTiny);
-    double u = Q.A[S_VX * Q.cells + g];
-    double dt1;
-    if (Q.rank == 1) {
-      dt1 = Q.cw[0][i] * Q.aye / (cs + fabs(u - 0.0));
-    } else if (Q.rank == 2) {
-      double v = Q.A[S_VY * Q.cells + g];
-      dt1 = Q.aye / ((cs + fabs(u - 0.0)) / Q.cw[0][i] + (cs + fabs(v - 0.0)) / Q.cw[1][j]);
-    } else {
-      double v = Q.A[S_VY * Q.cells + g], w = Q.A[S_VZ * Q.cells + g];
-      dt1 = Q.aye / ((cs + fabs(u - 0.0)) / Q.cw[0][i] + (cs + fabs(v - 0.0)) / Q.cw[1][j] +
-                     (cs + fabs(w - 0.0)) / Q.cw[2][k]);
-    }
-    local = fmn(local, dt1);
-  }
-  if (!Q.want_dt) return;
-  // wavefront then workgroup minimum
-  for (int off = 32; off > 0; off >>= 1) local = fmn(local, __shfl_xor(local, off, 64));
-  __shared__ double wmin[4];
-  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = local;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = wmin[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmn(m, wmin[w]);
-    atomicMin(Q.dtmin, (unsigned long long)__double_as_longlong(m));
-  }
+       g += (long long)gridDim.x * blockDim.x)
+    local = fmn(local, pressure_dt_cell(Q, g));
+  if (Q.want_dt) block_min_dt(local, Q.dtmin);
 }
 
 __global__ void dt_init_kernel(unsigned long long *dtmin) {
@@ -464,6 +316,8 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
   S.fb_list = D->fb_list;
   S.fb_count = D->fb_state;
   S.fb_done = D->fb_state + 1;
+  S.fb_tag = 0;
+  S.tp = kTP;
   S.fb_ok = pencil_fits<kFbTS>(S) ? 1 : 0;
   auto launch_fb = [&]() {
     if (S.uni[0])

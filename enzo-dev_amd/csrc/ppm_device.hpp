@@ -1,0 +1,189 @@
+// ppm_device.hpp -- device-side pieces shared by the per-grid library
+// (enzo_ppm.hip) and the batched AMR-level update (ppm_level.hip): the
+// workgroup executor of the sweep tile, the fallback flagging and the
+// per-group fallback body, the fused pressure + Courant cell, and the
+// per-sweep preparation item.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ppm_sweep.hpp"
+
+namespace ppm {
+
+// Workgroup executor of the tile phases (ppm_sweep.hpp): run = phase then
+// barrier; count = number of work-items with a nonzero result; first = the
+// result of work-item 0, broadcast.
+struct DevExec {
+  Regs r;
+  template <class F>
+  __device__ void run(F f) {
+    f((int)threadIdx.x, r);
+    __syncthreads();
+  }
+  template <class F>
+  __device__ int count(F f) {
+    int v = f((int)threadIdx.x, r);
+    return __syncthreads_count(v);
+  }
+  template <class F>
+  __device__ int first(F f) {
+    __shared__ int v0;
+    int v = f((int)threadIdx.x, r);
+    if (threadIdx.x == 0) v0 = v;
+    __syncthreads();
+    v = v0;
+    __syncthreads();  // v0 is free for the next call
+    return v;
+  }
+};
+
+// One-pencil fallback workgroups: kFbTS threads, the whole pencil in LDS.
+constexpr int kFbTS = 1024;
+constexpr int kFbBlocks = 64;
+using FbCfg = Cfg<1, kFbTS>;
+
+// Work-item 0 of a sweep workgroup whose tile saw a failing cell or an
+// error: put the tile's pencil group on the fallback list once (the group
+// flag deduplicates the tiles of one group); the entry carries P.fb_tag
+// (the grid of a batched level, 0 otherwise).
+__device__ inline void flag_group(const SweepParams &P, int grp) {
+  if (atomicExch(&P.fb_flag[grp], 1) == 0)
+    P.fb_list[atomicAdd(P.fb_count, 1u)] = P.fb_tag | (unsigned)grp;
+}
+
+// Every pencil of group `grp` (P.tp pencils along t1) redone whole with the
+// per-cell fallback, then the group flag cleared.
+template <int UNI>
+__device__ inline void fallback_group(const SweepParams &P, int grp, double *sm, DevExec &ex) {
+  const int tb = grp % P.nblk_t1, t2 = P.t2_0 + grp / P.nblk_t1;
+  for (int p = 0; p < P.tp && tb * P.tp + p < P.n_t1; p++) {
+    if (!P.fb_ok) {
+      if (threadIdx.x == 0) atomicAdd(&P.counters[3], 1ull);
+      break;
+    }
+    TileCounts tc = sweep_pencil_fb<kFbTS, UNI>(P, tb * P.tp + p, t2, sm, ex);
+    if (threadIdx.x == 0) {
+      if (tc.nfb) atomicAdd(&P.counters[0], (unsigned long long)tc.nfb);
+      if (tc.err) atomicAdd(&P.counters[2], (unsigned long long)tc.err);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) P.fb_flag[grp] = 0;
+}
+
+// End of a fallback launch: the last workgroup to finish (every workgroup has
+// read the list length by then) clears the length for the next launch.
+__device__ inline void fallback_launch_done(unsigned *count, unsigned *done) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(done, 1u) == gridDim.x - 1) {
+      *count = 0u;
+      *done = 0u;
+      __threadfence();
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Fused pressure + Courant condition (Grid_ComputePressure.C:96-166,
+// calc_dt.F:129-177).  Active cells contribute
+// dt1 = aye / sum_d (c_s + |v_d|)/dx_d to a device-wide minimum kept as the
+// bit pattern of a positive double (ordered like the value).
+struct PresParams {
+  int rank, idual;
+  int GD[3], st[3], en[3];
+  long long cells;
+  const double *A;
+  double *pres;
+  double gamma, tiny_number, min_support;
+  const double *cw[3];  // CellWidth per axis (calc_dt widths)
+  double aye;
+  int want_dt;
+  unsigned long long *dtmin;
+};
+
+__device__ inline double pressure_of(const PresParams &Q, long long g, double min_support) {
+  const double *A = Q.A;
+  double density = A[S_D * Q.cells + g];
+  double gas_energy;
+  if (Q.idual == 0) {
+    double v1 = A[S_VX * Q.cells + g];
+    double v2 = Q.rank > 1 ? A[S_VY * Q.cells + g] : 0.0;
+    double v3 = Q.rank > 2 ? A[S_VZ * Q.cells + g] : 0.0;
+    gas_energy = A[S_E * Q.cells + g] - 0.5 * (v1 * v1 + v2 * v2 + v3 * v3);
+  } else {
+    gas_energy = A[S_GE * Q.cells + g];
+  }
+  double p = (Q.gamma - 1.0) * density * gas_energy;
+  if (p < Q.tiny_number) p = Q.tiny_number;
+  double min_pressure = min_support * (Q.gamma - 1.0) * density * density;
+  if (p < min_pressure) p = min_pressure;
+  return p;
+}
+
+// pressure of cell g into Q.pres; returns the cell's Courant step (1e20 for
+// ghost cells or without the time step)
+__device__ inline double pressure_dt_cell(const PresParams &Q, long long g) {
+  double p = pressure_of(Q, g, Q.min_support);
+  Q.pres[g] = p;
+  if (!Q.want_dt) return 1.0e20;
+  int i = (int)(g % Q.GD[0]);
+  long long r = g / Q.GD[0];
+  int j = (int)(r % Q.GD[1]);
+  int k = (int)(r / Q.GD[1]);
+  if (i < Q.st[0] || i > Q.en[0] || j < Q.st[1] || j > Q.en[1] || k < Q.st[2] || k > Q.en[2])
+    return 1.0e20;
+  // calc_dt's pressure uses minimum-support coefficient 0
+  double pdt = Q.min_support != 0.0 ? pressure_of(Q, g, 0.0) : p;
+  double d = Q.A[S_D * Q.cells + g];
+  double cs = fmx(sqrt(Q.gamma * pdt / d), kTiny);
+  double u = Q.A[S_VX * Q.cells + g];
+  if (Q.rank == 1) return Q.cw[0][i] * Q.aye / (cs + fabs(u - 0.0));
+  if (Q.rank == 2) {
+    double v = Q.A[S_VY * Q.cells + g];
+    return Q.aye / ((cs + fabs(u - 0.0)) / Q.cw[0][i] + (cs + fabs(v - 0.0)) / Q.cw[1][j]);
+  }
+  double v = Q.A[S_VY * Q.cells + g], w = Q.A[S_VZ * Q.cells + g];
+  return Q.aye / ((cs + fabs(u - 0.0)) / Q.cw[0][i] + (cs + fabs(v - 0.0)) / Q.cw[1][j] +
+                  (cs + fabs(w - 0.0)) / Q.cw[2][k]);
+}
+
+// wavefront then workgroup minimum of `local` into *dtmin (blockDim a
+// multiple of 64, at most 1024)
+__device__ inline void block_min_dt(double local, unsigned long long *dtmin) {
+  for (int off = 32; off > 0; off >>= 1) local = fmn(local, __shfl_xor(local, off, 64));
+  __shared__ double wmin[16];
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmin[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmn(m, wmin[w]);
+    atomicMin(dtmin, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// ------------------------------------------------------------------------
+// Per-sweep preparation item `gid`: qc(i) = dt/dx(i) for gid < N (used by
+// the fluxes, flux_twoshock.F:366, and the gas-energy source, :393), then the
+// flattening of line 1 of every slice, one (slice, cell in [i1-1, i2+1]) per
+// item.  The reference hands `flatten` (line 1) to intvar for every line of
+// the slice (inteuler.F:241-289), so one coefficient row per slice is all the
+// sweep needs.  Items: N + (i2 - i1 + 3) * n_outer.
+__device__ inline void prep_item(const FlatParams &F, long long gid) {
+  if (gid < F.N) {
+    const double dt = F.dtp ? *F.dtp : F.dt;
+    F.qc[gid] = dt / F.dx[gid];
+    return;
+  }
+  if (!F.iflatten) return;
+  const long long k = gid - F.N;
+  const int ncell = F.i2 - F.i1 + 3;
+  if (k >= (long long)ncell * F.n_outer) return;
+  const int outer = (int)(k / ncell);
+  const int cell = F.i1 - 1 + (int)(k % ncell);
+  F.flat[(long long)outer * F.N + (cell - 1)] = flat_value(F, outer, cell);
+}
+
+}  // namespace ppm

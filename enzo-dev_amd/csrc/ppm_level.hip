@@ -1,0 +1,855 @@
+// ppm_level.hip -- batched update of one AMR level (include/enzo_ppm.h,
+// hipPPMLevel*): every grid of the level in one device arena, each sweep one
+// launch per tile shape over a work list of (grid, tile) items, the fallback
+// over a level-wide list, and the launches of a step captured once in a
+// hipGraph and replayed.
+//
+// It computes what a loop of grid::SolvePPM_DE calls over the level's grids
+// computes (EvolveLevel.C -> Grid_SolveHydroEquations.C:513-517), grid for
+// grid bitwise: every grid keeps its own geometry, coefficient tables,
+// segments and flattening rows; the level shares the step's dt.
+//
+// Layout: arena A and B hold, grid after grid, the NSLOT slots of each grid
+// (slot s of grid g at off[g] + s * cells[g]); a sweep reads A and writes B
+// and the arenas swap.  Pressure, acceleration, tables, widths, flattening
+// rows and dt/dx live in arenas with per-grid offsets too.  SweepParams of
+// every (parity, axis, grid) sit in device memory and the kernels read them
+// there.  With diffusion the sweep runs slice by slice (the next slice reads
+// this one's results), slice k of every grid in one launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "../../include/enzo_ppm.h"
+#include "ppm_device.hpp"
+#include "ppm_geometry.hpp"
+#include "ppm_sweep.hpp"
+
+using namespace ppm;
+
+namespace {
+
+struct Work {
+  int grid, tile;
+};
+
+// tile shapes of the level kernels (256 work-items each)
+constexpr int kShapes = 3;
+constexpr int kShapeTP[kShapes] = {16, 8, 4};
+constexpr int kShapeTS[kShapes] = {16, 32, 64};
+
+template <int TP, int TS, int UNI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void level_sweep_kernel(const SweepParams *params, const Work *work) {
+  __shared__ double sm[Cfg<TP, TS>::SMEM];
+  DevExec ex;
+  const Work w = work[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
+  const SweepParams &P = params[w.grid];
+  TileCounts tc = sweep_tile<TP, TS, UNI>(P, w.tile, sm, ex);
+  if (threadIdx.x == 0) {
+    if (tc.bad || tc.err) flag_group(P, w.tile / P.nseg);
+    if (tc.chains) atomicAdd(&P.counters[1], (unsigned long long)tc.chains);
+  }
+}
+
+// fallback over the level's list: entry = grid << 16 | group
+__global__ __launch_bounds__(kFbTS) void level_fallback_kernel(const SweepParams *params,
+                                                              const unsigned *list,
+                                                              unsigned *count, unsigned *done) {
+  __shared__ double sm[NARR_FB * FbCfg::LW];
+  DevExec ex;
+  const unsigned n = *count;
+  for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
+    const unsigned e = list[k];
+    const SweepParams &P = params[e >> 16];
+    if (P.uni[0]) fallback_group<1>(P, (int)(e & 0xffffu), sm, ex);
+    else fallback_group<0>(P, (int)(e & 0xffffu), sm, ex);
+  }
+  fallback_launch_done(count, done);
+}
+
+// per-sweep preparation of every grid: one workgroup per grid
+__global__ void level_prep_kernel(const FlatParams *F) {
+  const FlatParams &f = F[blockIdx.x];
+  const long long n = f.N + (f.iflatten ? (long long)(f.i2 - f.i1 + 3) * f.n_outer : 0);
+  for (long long gid = threadIdx.x; gid < n; gid += blockDim.x) prep_item(f, gid);
+}
+
+// pressure + Courant over chunks of kPresChunk cells: work = (grid, first cell)
+constexpr int kPresChunk = 4096;
+__global__ __launch_bounds__(256) void level_pressure_kernel(const PresParams *Q, const Work *work) {
+  const Work w = work[blockIdx.x];
+  const PresParams &q = Q[w.grid];
+  double local = 1.0e20;
+  const long long lim = (long long)w.tile + kPresChunk;
+  const long long end = q.cells < lim ? q.cells : lim;
+  for (long long g = (long long)w.tile + threadIdx.x; g < end; g += blockDim.x)
+    local = fmn(local, pressure_dt_cell(q, g));
+  if (q.want_dt) block_min_dt(local, q.dtmin);
+}
+
+__global__ void level_dt_init_kernel(unsigned long long *dtmin) {
+  if (threadIdx.x == 0) *dtmin = (unsigned long long)__double_as_longlong(1.0e20);
+}
+
+__global__ void level_dt_final_kernel(const unsigned long long *dtmin, double cfl, double *dt) {
+  if (threadIdx.x == 0) *dt = __longlong_as_double((long long)*dtmin) * cfl;
+}
+
+__global__ void level_zero_kernel(unsigned long long *c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0ull;
+}
+
+thread_local char g_level_error[512] = "";
+
+void level_error(const char *what, hipError_t e, int line) {
+  snprintf(g_level_error, sizeof(g_level_error), "%s failed: %s (ppm_level.hip:%d)", what,
+           hipGetErrorString(e), line);
+}
+
+#define LCHK(x)                              \
+  do {                                       \
+    hipError_t e_ = (x);                     \
+    if (e_ != hipSuccess) {                  \
+      level_error(#x, e_, __LINE__);         \
+      return ENZO_PPM_ERR_HIP;               \
+    }                                        \
+  } while (0)
+
+template <class T>
+bool dev_alloc(T **p, size_t n) {
+  *p = nullptr;
+  return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess;
+}
+
+template <int TP, int TS>
+bool segments_for(int N, int i1, int i2, std::vector<int> &o, std::vector<int> &w) {
+  return build_segments<TP, TS>(N, i1, i2, o, w);
+}
+
+bool segments_shape(int s, int N, int i1, int i2, std::vector<int> &o, std::vector<int> &w) {
+  if (s == 0) return segments_for<16, 16>(N, i1, i2, o, w);
+  if (s == 1) return segments_for<8, 32>(N, i1, i2, o, w);
+  return segments_for<4, 64>(N, i1, i2, o, w);
+}
+
+}  // namespace
+
+struct hipPPMLevel {
+  int ngrids = 0, rank = 0, nslot = 0, ncol = 0;
+  bool gravity = false, diffusion = false;
+  hipPPMParameter para0;            // the level's hydro switches (grid 0's block)
+  std::vector<hipPPMParameter> para;
+  // per grid
+  std::vector<int> GD;              // 3 per grid
+  std::vector<long long> cells, off, poff;
+  std::vector<long long> tab_off, cw_off;  // 3 per grid (doubles into tabs / cws)
+  std::vector<long long> flat_off, qc_off;
+  std::vector<int> uni;             // 3 per grid
+  std::vector<double> kt;           // 3 * NTAB per grid
+  std::vector<int> shape;           // 3 per grid: tile shape index per sweep axis
+  // segments per (grid, axis, shape): o then w, in segs
+  std::vector<int> hsegs;
+  std::vector<long long> seg_off;   // ((g * 3 + d) * kShapes + s) -> offset into segs
+  std::vector<int> seg_n;
+  std::vector<long long> grp_off;   // fallback group flags per grid
+  long long ngroups = 0;
+  // device memory
+  double *A = nullptr, *B = nullptr, *pres = nullptr, *accel = nullptr;
+  double *tabs = nullptr, *cws = nullptr, *flat = nullptr, *qc = nullptr;
+  int *segs = nullptr;
+  double *dt = nullptr;
+  unsigned long long *dtmin = nullptr, *counters = nullptr;
+  int *fb_flag = nullptr;
+  unsigned *fb_list = nullptr, *fb_state = nullptr;
+  SweepParams *dparams = nullptr;   // [parity 2][axis 3][grid]
+  FlatParams *dflat = nullptr;      // [parity 2][axis 3][grid]
+  PresParams *dpres = nullptr;      // [parity 2][grid]
+  Work *dwork = nullptr;            // all work lists
+  Work *dpwork = nullptr;           // pressure chunks
+  long long npwork = 0;
+  // work lists: per axis, per launch: (shape, uni, first item, count, slice)
+  struct Launch {
+    int shape, uni;
+    long long first, count;
+  };
+  std::vector<Launch> launches[3];
+  std::vector<long long> launch_slices[3];  // index of the first launch of each slice
+  std::vector<SweepParams> hparams;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int parity = 0;                   // 0: state in A
+  bool use_graph = true;
+  std::map<int, hipGraphExec_t> graphs;
+  double graph_cfl = -1.0;          // cfl the Courant-step graphs were captured with
+  long long fallback_total = 0;
+};
+
+namespace {
+
+void drop_graphs(hipPPMLevel *L) {
+  for (auto &kv : L->graphs) (void)hipGraphExecDestroy(kv.second);
+  L->graphs.clear();
+}
+
+void free_level(hipPPMLevel *L) {
+  drop_graphs(L);
+  void *ptrs[] = {L->A, L->B, L->pres, L->accel, L->tabs, L->cws, L->flat, L->qc, L->segs, L->dt,
+                  L->dtmin, L->counters, L->fb_flag, L->fb_list, L->fb_state, L->dparams,
+                  L->dflat, L->dpres, L->dwork, L->dpwork};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (L->own_stream && L->stream) (void)hipStreamDestroy(L->stream);
+}
+
+// the level's switches must agree grid to grid
+bool same_switches(const hipPPMParameter &a, const hipPPMParameter &b) {
+  if (a.GridRank != b.GridRank || a.NumberOfColours != b.NumberOfColours) return false;
+  if (a.DensNum != b.DensNum || a.TENum != b.TENum || a.Vel1Num != b.Vel1Num ||
+      a.Vel2Num != b.Vel2Num || a.Vel3Num != b.Vel3Num || a.GENum != b.GENum)
+    return false;
+  for (int n = 0; n < a.NumberOfColours; n++)
+    if (a.colnum[n] != b.colnum[n]) return false;
+  return a.Gamma == b.Gamma && a.PPMDiffusionParameter == b.PPMDiffusionParameter &&
+         a.PPMFlatteningParameter == b.PPMFlatteningParameter &&
+         a.PPMSteepeningParameter == b.PPMSteepeningParameter && a.GravityOn == b.GravityOn &&
+         a.DualEnergyFormalism == b.DualEnergyFormalism &&
+         a.DualEnergyFormalismEta1 == b.DualEnergyFormalismEta1 &&
+         a.DualEnergyFormalismEta2 == b.DualEnergyFormalismEta2 &&
+         a.PressureFree == b.PressureFree && a.MinimumPressure == b.MinimumPressure &&
+         a.RiemannSolver == b.RiemannSolver && a.RiemannSolverFallback == b.RiemannSolverFallback &&
+         a.SmallRho == b.SmallRho && a.ConservativeReconstruction == b.ConservativeReconstruction &&
+         a.PositiveReconstruction == b.PositiveReconstruction;
+}
+
+// sweep orientation: t1/t2 axes (t1 the contiguous transverse axis, or the
+// row axis with diffusion), as the per-grid library does
+void orient(int dim, bool diffusion, int &ax1, int &ax2) {
+  const int rax = (dim + 1) % 3, oax = (dim + 2) % 3;
+  if (diffusion) { ax1 = rax; ax2 = oax; return; }
+  if (dim == 0) { ax1 = 1; ax2 = 2; }
+  else if (dim == 1) { ax1 = 0; ax2 = 2; }
+  else { ax1 = 0; ax2 = 1; }
+}
+
+// SweepParams, FlatParams and work lists for the current shape choice
+int build_plan(hipPPMLevel *L) {
+  const hipPPMParameter &P0 = L->para0;
+  const int G = L->ngrids;
+  L->hparams.assign((size_t)2 * 3 * G, SweepParams());
+  std::vector<FlatParams> hflat((size_t)2 * 3 * G);
+  std::vector<Work> work;
+  long long ngroups = 0;
+  L->grp_off.assign(G, 0);
+  for (int g = 0; g < G; g++) {  // flags per grid: the largest group count of any axis
+    long long m = 1;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        if (a != b) m = std::max(m, (long long)((L->GD[3 * g + a] + 3) / 4) * L->GD[3 * g + b]);
+    L->grp_off[g] = ngroups;
+    ngroups += m;
+  }
+  if (ngroups != L->ngroups) return ENZO_PPM_ERR_ARGUMENT;
+  for (int d = 0; d < 3; d++) {
+    L->launches[d].clear();
+    L->launch_slices[d].clear();
+  }
+  for (int d = 0; d < L->rank; d++) {
+    // parameters of every grid for axis d
+    int max_slices = 1;
+    for (int g = 0; g < G; g++) {
+      const int *gd = &L->GD[3 * g];
+      const int rax = (d + 1) % 3, oax = (d + 2) % 3;
+      const hipPPMParameter &Pg = L->para[g];
+      const int s = L->shape[3 * g + d];
+      for (int par = 0; par < 2; par++) {
+        SweepParams &S = L->hparams[((size_t)par * 3 + d) * G + g];
+        memset(&S, 0, sizeof(S));
+        S.dim = d;
+        S.N = gd[d];
+        S.i1 = (int)Pg.GridStartIndex[d] + 1;
+        S.i2 = (int)Pg.GridEndIndex[d] + 1;
+        S.rank = L->rank;
+        for (int k = 0; k < 3; k++) S.GD[k] = gd[k];
+        S.cells = L->cells[g];
+        orient(d, L->diffusion, S.ax_t1, S.ax_t2);
+        S.n_t1 = gd[S.ax_t1];
+        S.n_t2 = gd[S.ax_t2];
+        S.t2_0 = 0;
+        set_strides(S);
+        S.tp = kShapeTP[s];
+        S.nblk_t1 = (S.n_t1 + S.tp - 1) / S.tp;
+        const long long so = L->seg_off[((size_t)g * 3 + d) * kShapes + s];
+        S.nseg = L->seg_n[((size_t)g * 3 + d) * kShapes + s];
+        S.seg_o = L->segs + so;
+        S.seg_w = L->segs + so + S.nseg;
+        double *src = par == 0 ? L->A : L->B, *dst = par == 0 ? L->B : L->A;
+        S.A = src + L->off[g];
+        S.B = dst + L->off[g];
+        S.ncol = L->ncol;
+        S.pres = L->pres + L->poff[g];
+        S.gr = L->gravity ? L->accel + 3 * L->poff[g] + (long long)d * L->cells[g] : nullptr;
+        S.flat = L->flat + L->flat_off[g];
+        S.tab = L->tabs + L->tab_off[3 * g + d];
+        S.tab_row = L->tabs + L->tab_off[3 * g + rax];
+        S.tab_out = L->tabs + L->tab_off[3 * g + oax];
+        S.qc = L->qc + L->qc_off[g];
+        S.uni[0] = L->uni[3 * g + d];
+        S.uni[1] = L->uni[3 * g + rax];
+        S.uni[2] = L->uni[3 * g + oax];
+        for (int t = 0; t < NTAB; t++) {
+          S.kt[t] = L->kt[((size_t)3 * g + d) * NTAB + t];
+          S.kt_row[t] = L->kt[((size_t)3 * g + rax) * NTAB + t];
+          S.kt_out[t] = L->kt[((size_t)3 * g + oax) * NTAB + t];
+        }
+        S.gamma = P0.Gamma;
+        S.eta1 = P0.DualEnergyFormalismEta1;
+        S.eta2 = P0.DualEnergyFormalismEta2;
+        S.pmin = P0.MinimumPressure;
+        S.dfloor = P0.SmallRho;
+        S.dt = 0.0;
+        S.dtp = L->dt;
+        S.isteep = (int)P0.PPMSteepeningParameter;
+        S.iflatten = (int)P0.PPMFlatteningParameter;
+        S.idiff = (int)P0.PPMDiffusionParameter;
+        S.idual = (int)P0.DualEnergyFormalism;
+        S.ipresfree = (int)P0.PressureFree;
+        S.gravity = (int)P0.GravityOn;
+        S.ifallback = (int)P0.RiemannSolverFallback;
+        S.nact_out = (int)(Pg.GridEndIndex[oax] - Pg.GridStartIndex[oax] + 1);
+        S.face_head = nullptr;
+        S.faces = nullptr;
+        S.face_out = nullptr;
+        S.counters = L->counters;
+        S.fb_flag = L->fb_flag + L->grp_off[g];
+        S.fb_list = L->fb_list;
+        S.fb_count = L->fb_state;
+        S.fb_done = L->fb_state + 1;
+        S.fb_tag = (unsigned)g << 16;
+        S.fb_ok = pencil_fits<kFbTS>(S) ? 1 : 0;
+        if ((long long)S.nblk_t1 * S.n_t2 > 65535) return ENZO_PPM_ERR_ARGUMENT;
+        FlatParams &F = hflat[((size_t)par * 3 + d) * G + g];
+        memset(&F, 0, sizeof(F));
+        F.dim = d;
+        F.N = gd[d];
+        F.i1 = S.i1;
+        F.i2 = S.i2;
+        for (int k = 0; k < 3; k++) F.GD[k] = gd[k];
+        F.cells = L->cells[g];
+        F.n_outer = gd[oax];
+        F.iflatten = S.iflatten;
+        F.A = S.A;
+        F.pres = S.pres;
+        F.flat = L->flat + L->flat_off[g];
+        F.dx = L->cws + L->cw_off[3 * g + d];
+        F.dt = 0.0;
+        F.dtp = L->dt;
+        F.qc = L->qc + L->qc_off[g];
+      }
+      if (L->diffusion) max_slices = std::max(max_slices, gd[(d + 2) % 3]);
+    }
+    // work lists: per slice (diffusion) or one, then per (shape, uni)
+    const int nslices = L->diffusion ? max_slices : 1;
+    for (int k = 0; k < nslices; k++) {
+      L->launch_slices[d].push_back((long long)L->launches[d].size());
+      for (int s = 0; s < kShapes; s++)
+        for (int u = 1; u >= 0; u--) {
+          const long long first = (long long)work.size();
+          for (int g = 0; g < G; g++) {
+            if (L->shape[3 * g + d] != s || L->uni[3 * g + d] != u) continue;
+            const SweepParams &S = L->hparams[((size_t)0 * 3 + d) * G + g];
+            if (L->diffusion) {
+              if (k >= S.n_t2) continue;
+              for (int r = 0; r < S.nblk_t1; r++)
+                for (int seg = 0; seg < S.nseg; seg++)
+                  work.push_back(Work{g, seg + S.nseg * (r + S.nblk_t1 * k)});
+            } else {
+              const long long nt = (long long)S.nseg * S.nblk_t1 * S.n_t2;
+              for (long long t = 0; t < nt; t++) work.push_back(Work{g, (int)t});
+            }
+          }
+          const long long cnt = (long long)work.size() - first;
+          if (cnt > 0) L->launches[d].push_back(hipPPMLevel::Launch{s, u, first, cnt});
+        }
+    }
+    L->launch_slices[d].push_back((long long)L->launches[d].size());
+  }
+  // host check of every item against its grid's tiles
+  for (const Work &w : work) {
+    if (w.grid < 0 || w.grid >= G) return ENZO_PPM_ERR_ARGUMENT;
+  }
+  for (int d = 0; d < L->rank; d++)
+    for (const auto &ln : L->launches[d])
+      for (long long i = ln.first; i < ln.first + ln.count; i++) {
+        const Work &w = work[i];
+        const SweepParams &S = L->hparams[((size_t)0 * 3 + d) * G + w.grid];
+        if (w.tile < 0 || (long long)w.tile >= (long long)S.nseg * S.nblk_t1 * S.n_t2)
+          return ENZO_PPM_ERR_ARGUMENT;
+      }
+  if (L->dparams) (void)hipFree(L->dparams);
+  if (L->dflat) (void)hipFree(L->dflat);
+  if (L->dwork) (void)hipFree(L->dwork);
+  L->dparams = nullptr;
+  L->dflat = nullptr;
+  L->dwork = nullptr;
+  if (dev_alloc(&L->dparams, L->hparams.size()) || dev_alloc(&L->dflat, hflat.size()) ||
+      dev_alloc(&L->dwork, work.size()))
+    return ENZO_PPM_ERR_HIP;
+  LCHK(hipMemcpy(L->dparams, L->hparams.data(), sizeof(SweepParams) * L->hparams.size(),
+                 hipMemcpyHostToDevice));
+  LCHK(hipMemcpy(L->dflat, hflat.data(), sizeof(FlatParams) * hflat.size(), hipMemcpyHostToDevice));
+  if (!work.empty())
+    LCHK(hipMemcpy(L->dwork, work.data(), sizeof(Work) * work.size(), hipMemcpyHostToDevice));
+  drop_graphs(L);
+  return ENZO_PPM_SUCCESS;
+}
+
+int launch_level_sweep(hipPPMLevel *L, int d) {
+  const int G = L->ngrids;
+  const SweepParams *P = L->dparams + ((size_t)L->parity * 3 + d) * G;
+  hipLaunchKernelGGL(level_prep_kernel, dim3((unsigned)G), dim3(256), 0, L->stream,
+                     L->dflat + ((size_t)L->parity * 3 + d) * G);
+  LCHK(hipGetLastError());
+  const auto &sl = L->launch_slices[d];
+  for (size_t k = 0; k + 1 < sl.size(); k++) {
+    for (long long li = sl[k]; li < sl[k + 1]; li++) {
+      const auto &ln = L->launches[d][li];
+      const Work *w = L->dwork + ln.first;
+      const dim3 grid((unsigned)ln.count), blk(256);
+#define LEVEL_LAUNCH(S, TP, TS)                                                                 \
+  if (ln.shape == S) {                                                                        \
+    if (ln.uni) hipLaunchKernelGGL((level_sweep_kernel<TP, TS, 1>), grid, blk, 0, L->stream, P, w); \
+    else hipLaunchKernelGGL((level_sweep_kernel<TP, TS, 0>), grid, blk, 0, L->stream, P, w);        \
+  }
+      LEVEL_LAUNCH(0, 16, 16)
+      LEVEL_LAUNCH(1, 8, 32)
+      LEVEL_LAUNCH(2, 4, 64)
+#undef LEVEL_LAUNCH
+      LCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(level_fallback_kernel, dim3(kFbBlocks), dim3(kFbTS), 0, L->stream, P,
+                       L->fb_list, L->fb_state, L->fb_state + 1);
+    LCHK(hipGetLastError());
+  }
+  L->parity ^= 1;
+  return ENZO_PPM_SUCCESS;
+}
+
+// one level step: pressure (+ Courant dt) of every grid, then the sweeps in
+// the rotated order of CycleNumber
+int enqueue_step(hipPPMLevel *L, int cycle, int want_dt, double cfl) {
+  if (want_dt)
+    hipLaunchKernelGGL(level_dt_init_kernel, dim3(1), dim3(64), 0, L->stream, L->dtmin);
+  hipLaunchKernelGGL(level_pressure_kernel, dim3((unsigned)L->npwork), dim3(256), 0, L->stream,
+                     L->dpres + (size_t)L->parity * L->ngrids, L->dpwork);
+  LCHK(hipGetLastError());
+  if (want_dt)
+    hipLaunchKernelGGL(level_dt_final_kernel, dim3(1), dim3(64), 0, L->stream, L->dtmin, cfl, L->dt);
+  const int ixyz = cycle % L->rank;
+  for (int n = ixyz; n < ixyz + L->rank; n++) {
+    int st = launch_level_sweep(L, n % L->rank);
+    if (st != ENZO_PPM_SUCCESS) return st;
+  }
+  return ENZO_PPM_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMParameter *Para,
+                      void *stream) {
+  if (!Level || !Para || NumberOfGrids < 1 || NumberOfGrids > 65535) return ENZO_PPM_ERR_ARGUMENT;
+  *Level = nullptr;
+  for (enzo_int g = 0; g < NumberOfGrids; g++) {
+    const hipPPMParameter &P = Para[g];
+    if (P.GridRank < 1 || P.GridRank > 3 || P.NumberOfColours < 0 ||
+        P.NumberOfColours > ENZO_PPM_MAX_COLOR || !same_switches(P, Para[0]))
+      return ENZO_PPM_ERR_ARGUMENT;
+    for (int d = 0; d < 3; d++) {
+      if (P.GridDimension[d] < 1 || P.GridDimension[d] > ENZO_PPM_MAX_ANY_SINGLE_DIRECTION)
+        return ENZO_PPM_ERR_ARGUMENT;
+      // every swept axis has more than one active cell: the level swaps its
+      // buffers for all grids at once, so no grid may skip a sweep
+      if (d < P.GridRank && (P.GridStartIndex[d] < 3 || P.GridEndIndex[d] + 3 >= P.GridDimension[d] ||
+                             P.GridEndIndex[d] <= P.GridStartIndex[d]))
+        return ENZO_PPM_ERR_ARGUMENT;
+    }
+    if (P.ConservativeReconstruction != 0 || P.PositiveReconstruction != 0 ||
+        P.RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK ||
+        (P.PPMFlatteningParameter != 0 && P.PPMFlatteningParameter != 1) ||
+        (P.PPMDiffusionParameter != 0 && P.PPMDiffusionParameter != 1))
+      return ENZO_PPM_ERR_NOT_SUPPORTED;
+  }
+  hipPPMLevel *L = new (std::nothrow) hipPPMLevel();
+  if (!L) return ENZO_PPM_ERR_ARGUMENT;
+  const int G = (int)NumberOfGrids;
+  L->ngrids = G;
+  L->para.assign(Para, Para + G);
+  L->para0 = Para[0];
+  L->rank = (int)Para[0].GridRank;
+  L->ncol = (int)Para[0].NumberOfColours;
+  L->nslot = S_COL0 + L->ncol;
+  L->gravity = Para[0].GravityOn != 0;
+  L->diffusion = Para[0].PPMDiffusionParameter != 0;
+  L->GD.resize(3 * G);
+  L->cells.resize(G);
+  L->off.resize(G);
+  L->poff.resize(G);
+  L->tab_off.resize(3 * G);
+  L->cw_off.resize(3 * G);
+  L->flat_off.resize(G);
+  L->qc_off.resize(G);
+  L->uni.resize(3 * G);
+  L->kt.assign((size_t)3 * G * NTAB, 0.0);
+  L->shape.assign(3 * G, 1);
+  L->seg_off.assign((size_t)G * 3 * kShapes, 0);
+  L->seg_n.assign((size_t)G * 3 * kShapes, 0);
+  long long tot = 0, ptot = 0, ttot = 0, wtot = 0, ftot = 0, qtot = 0;
+  std::vector<double> htabs, hcw;
+  for (int g = 0; g < G; g++) {
+    const hipPPMParameter &P = Para[g];
+    long long c = 1;
+    for (int d = 0; d < 3; d++) {
+      L->GD[3 * g + d] = (int)P.GridDimension[d];
+      c *= P.GridDimension[d];
+    }
+    L->cells[g] = c;
+    L->off[g] = tot;
+    L->poff[g] = ptot;
+    tot += c * L->nslot;
+    ptot += c;
+    long long fcap = 0;
+    int nmax = 1;
+    for (int d = 0; d < 3; d++) {
+      const int N = L->GD[3 * g + d];
+      std::vector<double> dx(N), tab;
+      for (int i = 0; i < N; i++)
+        dx[i] = P.CellWidth[d] ? P.CellWidth[d][i] : (d < P.GridRank ? P.dx[d] : 1.0);
+      build_tables(dx.data(), N, tab);
+      L->uni[3 * g + d] = uniform_axis(dx.data(), N, tab, &L->kt[((size_t)3 * g + d) * NTAB]) ? 1 : 0;
+      L->tab_off[3 * g + d] = ttot;
+      htabs.insert(htabs.end(), tab.begin(), tab.end());
+      ttot += (long long)tab.size();
+      L->cw_off[3 * g + d] = wtot;
+      hcw.insert(hcw.end(), dx.begin(), dx.end());
+      wtot += N;
+      fcap = std::max(fcap, (long long)N * L->GD[3 * g + (d + 2) % 3]);
+      nmax = std::max(nmax, N);
+    }
+    L->flat_off[g] = ftot;
+    ftot += fcap;
+    L->qc_off[g] = qtot;
+    qtot += nmax;
+  }
+  // segments of every (grid, axis, shape); a shape that cannot tile an axis
+  // is marked with nseg 0 and never chosen
+  for (int g = 0; g < G; g++)
+    for (int d = 0; d < L->rank; d++) {
+      const int N = L->GD[3 * g + d];
+      const int i1 = (int)Para[g].GridStartIndex[d] + 1, i2 = (int)Para[g].GridEndIndex[d] + 1;
+      for (int s = 0; s < kShapes; s++) {
+        std::vector<int> o, w;
+        const size_t k = ((size_t)g * 3 + d) * kShapes + s;
+        L->seg_off[k] = (long long)L->hsegs.size();
+        if (!segments_shape(s, N, i1, i2, o, w)) continue;
+        L->seg_n[k] = (int)o.size();
+        L->hsegs.insert(L->hsegs.end(), o.begin(), o.end());
+        L->hsegs.insert(L->hsegs.end(), w.begin(), w.end());
+      }
+      if (L->seg_n[((size_t)g * 3 + d) * kShapes + 1] == 0) {
+        delete L;
+        return ENZO_PPM_ERR_ARGUMENT;
+      }
+    }
+  long long ngroups = 0;
+  for (int g = 0; g < G; g++) {
+    long long m = 1;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        if (a != b) m = std::max(m, (long long)((L->GD[3 * g + a] + 3) / 4) * L->GD[3 * g + b]);
+    ngroups += m;
+  }
+  L->ngroups = ngroups;
+  // pressure chunks
+  std::vector<Work> pw;
+  for (int g = 0; g < G; g++)
+    for (long long s = 0; s < L->cells[g]; s += kPresChunk) pw.push_back(Work{g, (int)s});
+  L->npwork = (long long)pw.size();
+  int st = ENZO_PPM_SUCCESS;
+  auto fail = [&](int code) {
+    free_level(L);
+    delete L;
+    return code;
+  };
+  if (dev_alloc(&L->A, tot) || dev_alloc(&L->B, tot) || dev_alloc(&L->pres, ptot) ||
+      (L->gravity && dev_alloc(&L->accel, 3 * ptot)) || dev_alloc(&L->tabs, ttot) ||
+      dev_alloc(&L->cws, wtot) || dev_alloc(&L->flat, ftot) || dev_alloc(&L->qc, qtot) ||
+      dev_alloc(&L->segs, L->hsegs.size()) || dev_alloc(&L->dt, 1) || dev_alloc(&L->dtmin, 1) ||
+      dev_alloc(&L->counters, 4) || dev_alloc(&L->fb_flag, ngroups) ||
+      dev_alloc(&L->fb_list, ngroups) || dev_alloc(&L->fb_state, 2) ||
+      dev_alloc(&L->dpres, (size_t)2 * G) || dev_alloc(&L->dpwork, pw.size()))
+    return fail(ENZO_PPM_ERR_HIP);
+  if (stream) {
+    L->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreate(&L->stream) != hipSuccess) return fail(ENZO_PPM_ERR_HIP);
+    L->own_stream = true;
+  }
+  if (hipMemcpy(L->tabs, htabs.data(), sizeof(double) * htabs.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(L->cws, hcw.data(), sizeof(double) * hcw.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      (!L->hsegs.empty() && hipMemcpy(L->segs, L->hsegs.data(), sizeof(int) * L->hsegs.size(),
+                                      hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(L->dpwork, pw.data(), sizeof(Work) * pw.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(L->fb_flag, 0, sizeof(int) * ngroups) != hipSuccess ||
+      hipMemset(L->fb_state, 0, 2 * sizeof(unsigned)) != hipSuccess ||
+      hipMemset(L->counters, 0, 4 * sizeof(unsigned long long)) != hipSuccess)
+    return fail(ENZO_PPM_ERR_HIP);
+  if (L->gravity && hipMemset(L->accel, 0, sizeof(double) * 3 * ptot) != hipSuccess)
+    return fail(ENZO_PPM_ERR_HIP);
+  // pressure parameters of both parities
+  std::vector<PresParams> hp((size_t)2 * G);
+  for (int par = 0; par < 2; par++)
+    for (int g = 0; g < G; g++) {
+      PresParams &Q = hp[(size_t)par * G + g];
+      memset(&Q, 0, sizeof(Q));
+      const hipPPMParameter &P = Para[g];
+      Q.rank = L->rank;
+      Q.idual = (int)P.DualEnergyFormalism;
+      for (int d = 0; d < 3; d++) {
+        Q.GD[d] = L->GD[3 * g + d];
+        Q.st[d] = (int)P.GridStartIndex[d];
+        Q.en[d] = (int)P.GridEndIndex[d];
+        Q.cw[d] = L->cws + L->cw_off[3 * g + d];
+      }
+      Q.cells = L->cells[g];
+      Q.A = (par == 0 ? L->A : L->B) + L->off[g];
+      Q.pres = L->pres + L->poff[g];
+      Q.gamma = P.Gamma;
+      Q.tiny_number = P.MinimumPressure;
+      Q.min_support = 0.0;
+      Q.aye = 1.0;
+      Q.want_dt = 1;
+      Q.dtmin = L->dtmin;
+    }
+  if (hipMemcpy(L->dpres, hp.data(), sizeof(PresParams) * hp.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(ENZO_PPM_ERR_HIP);
+  st = build_plan(L);
+  if (st != ENZO_PPM_SUCCESS) return fail(st);
+  *Level = L;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelSetBaryon(hipPPMLevel *L, enzo_int grid, enzo_float *BaryonField[],
+                         enzo_float *AccelerationField[]) {
+  if (!L || !BaryonField || grid < 0 || grid >= L->ngrids) return ENZO_PPM_ERR_ARGUMENT;
+  const hipPPMParameter &P = L->para[grid];
+  long long fnum[S_COL0 + ENZO_PPM_MAX_COLOR];
+  for (int s = 0; s < L->nslot; s++) fnum[s] = -1;
+  fnum[S_D] = P.DensNum;
+  fnum[S_E] = P.TENum;
+  fnum[S_VX] = P.Vel1Num;
+  if (P.GridRank > 1) fnum[S_VY] = P.Vel2Num;
+  if (P.GridRank > 2) fnum[S_VZ] = P.Vel3Num;
+  if (P.DualEnergyFormalism) fnum[S_GE] = P.GENum;
+  for (int n = 0; n < L->ncol; n++) fnum[S_COL0 + n] = P.colnum[n];
+  double *state = L->parity == 0 ? L->A : L->B;
+  const size_t fb = sizeof(double) * (size_t)L->cells[grid];
+  for (int s = 0; s < L->nslot; s++) {
+    double *dst = state + L->off[grid] + (long long)s * L->cells[grid];
+    if (fnum[s] >= 0 && fnum[s] < ENZO_PPM_MAX_NUMBER_OF_BARYON_FIELDS && BaryonField[fnum[s]])
+      LCHK(hipMemcpyAsync(dst, BaryonField[fnum[s]], fb, hipMemcpyHostToDevice, L->stream));
+    else
+      LCHK(hipMemsetAsync(dst, 0, fb, L->stream));
+  }
+  if (L->gravity && AccelerationField)
+    for (int d = 0; d < L->rank; d++)
+      if (AccelerationField[d])
+        LCHK(hipMemcpyAsync(L->accel + 3 * L->poff[grid] + (long long)d * L->cells[grid],
+                            AccelerationField[d], fb, hipMemcpyHostToDevice, L->stream));
+  LCHK(hipStreamSynchronize(L->stream));
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelGetBaryon(hipPPMLevel *L, enzo_int grid, enzo_float *BaryonField[]) {
+  if (!L || !BaryonField || grid < 0 || grid >= L->ngrids) return ENZO_PPM_ERR_ARGUMENT;
+  const hipPPMParameter &P = L->para[grid];
+  long long fnum[S_COL0 + ENZO_PPM_MAX_COLOR];
+  for (int s = 0; s < L->nslot; s++) fnum[s] = -1;
+  fnum[S_D] = P.DensNum;
+  fnum[S_E] = P.TENum;
+  fnum[S_VX] = P.Vel1Num;
+  if (P.GridRank > 1) fnum[S_VY] = P.Vel2Num;
+  if (P.GridRank > 2) fnum[S_VZ] = P.Vel3Num;
+  if (P.DualEnergyFormalism) fnum[S_GE] = P.GENum;
+  for (int n = 0; n < L->ncol; n++) fnum[S_COL0 + n] = P.colnum[n];
+  const double *state = L->parity == 0 ? L->A : L->B;
+  const size_t fb = sizeof(double) * (size_t)L->cells[grid];
+  for (int s = 0; s < L->nslot; s++)
+    if (fnum[s] >= 0 && fnum[s] < ENZO_PPM_MAX_NUMBER_OF_BARYON_FIELDS && BaryonField[fnum[s]])
+      LCHK(hipMemcpyAsync(BaryonField[fnum[s]], state + L->off[grid] + (long long)s * L->cells[grid],
+                          fb, hipMemcpyDeviceToHost, L->stream));
+  LCHK(hipStreamSynchronize(L->stream));
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelStep(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dtFixed, enzo_float cfl) {
+  if (!L || CycleNumber < 0) return ENZO_PPM_ERR_ARGUMENT;
+  const int want_dt = dtFixed > 0.0 ? 0 : 1;
+  if (!want_dt) LCHK(hipMemcpyAsync(L->dt, &dtFixed, sizeof(double), hipMemcpyHostToDevice, L->stream));
+  if (!L->use_graph) return enqueue_step(L, (int)(CycleNumber % L->rank), want_dt, cfl);
+  // the launches of a step depend on the rotation, the buffer parity and the
+  // dt mode
+  const int rot = (int)(CycleNumber % L->rank);
+  const int key = (rot * 2 + L->parity) * 2 + want_dt;
+  if (want_dt && cfl != L->graph_cfl) {  // cfl is a kernel argument of those graphs
+    for (auto g = L->graphs.begin(); g != L->graphs.end();) {
+      if (g->first % 2) {
+        (void)hipGraphExecDestroy(g->second);
+        g = L->graphs.erase(g);
+      } else {
+        ++g;
+      }
+    }
+    L->graph_cfl = cfl;
+  }
+  auto it = L->graphs.find(key);
+  if (it == L->graphs.end()) {
+    const int parity0 = L->parity;
+    hipGraph_t graph;
+    LCHK(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
+    int st = enqueue_step(L, rot, want_dt, cfl);
+    hipError_t ec = hipStreamEndCapture(L->stream, &graph);
+    if (st != ENZO_PPM_SUCCESS) return st;
+    LCHK(ec);
+    hipGraphExec_t exec;
+    LCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    it = L->graphs.emplace(key, exec).first;
+    L->parity = parity0;  // capture ran no work
+  }
+  LCHK(hipGraphLaunch(it->second, L->stream));
+  L->parity ^= (L->rank & 1);  // one swap per sweep
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelStatus(hipPPMLevel *L, enzo_float *dt_used, enzo_long_int *fallback_cells) {
+  if (!L) return ENZO_PPM_ERR_ARGUMENT;
+  LCHK(hipStreamSynchronize(L->stream));
+  unsigned long long c[4];
+  LCHK(hipMemcpy(c, L->counters, sizeof(c), hipMemcpyDeviceToHost));
+  if (dt_used) LCHK(hipMemcpy(dt_used, L->dt, sizeof(double), hipMemcpyDeviceToHost));
+  if (fallback_cells) *fallback_cells = (enzo_long_int)c[0];
+  hipLaunchKernelGGL(level_zero_kernel, dim3(1), dim3(64), 0, L->stream, L->counters, 4);
+  LCHK(hipGetLastError());
+  LCHK(hipStreamSynchronize(L->stream));
+  if (c[3] != 0) return ENZO_PPM_ERR_NOT_SUPPORTED;
+  if (c[2] != 0) return ENZO_PPM_ERR_NEGATIVE;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelUseGraph(hipPPMLevel *L, enzo_int enable) {
+  if (!L) return ENZO_PPM_ERR_ARGUMENT;
+  L->use_graph = enable != 0;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelSetShape(hipPPMLevel *L, enzo_int grid, enzo_int dim, enzo_int shape) {
+  if (!L || grid < 0 || grid >= L->ngrids || dim < 0 || dim >= L->rank || shape < 0 ||
+      shape >= kShapes)
+    return ENZO_PPM_ERR_ARGUMENT;
+  if (L->seg_n[((size_t)grid * 3 + dim) * kShapes + shape] == 0) return ENZO_PPM_ERR_ARGUMENT;
+  L->shape[3 * grid + dim] = (int)shape;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelCommitShapes(hipPPMLevel *L) {
+  if (!L) return ENZO_PPM_ERR_ARGUMENT;
+  LCHK(hipStreamSynchronize(L->stream));
+  return build_plan(L);
+}
+
+int hipPPMLevelTune(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dt, enzo_int repeats,
+                    enzo_int *shape_by_length) {
+  // Autotuner: pencils are classed by length (along the sweep axis, ghosts
+  // included) into [0,20], [21,40], [41,80], [81,...]; for each class in turn
+  // every tile shape that can tile all of the class's pencils is timed on
+  // full level steps (graph replay, `repeats` of them) with the other classes
+  // at their current choice, and the fastest kept.  The level's state is
+  // saved before and restored after.  shape_by_length (4 entries) receives
+  // the choice per class.
+  if (!L || dt <= 0.0 || repeats < 1) return ENZO_PPM_ERR_ARGUMENT;
+  const int bounds[4] = {20, 40, 80, 1 << 30};
+  auto cls = [&](int N) { int c = 0; while (N > bounds[c]) c++; return c; };
+  long long tot = 0;
+  for (int g = 0; g < L->ngrids; g++) tot += L->cells[g] * L->nslot;
+  double *save = nullptr;
+  if (dev_alloc(&save, tot)) return ENZO_PPM_ERR_HIP;
+  LCHK(hipStreamSynchronize(L->stream));
+  const int parity0 = L->parity;
+  LCHK(hipMemcpy(save, parity0 == 0 ? L->A : L->B, sizeof(double) * tot, hipMemcpyDeviceToDevice));
+  int choice[4];
+  for (int c = 0; c < 4; c++) choice[c] = 1;
+  auto apply = [&]() {
+    for (int g = 0; g < L->ngrids; g++)
+      for (int d = 0; d < L->rank; d++) {
+        int s = choice[cls(L->GD[3 * g + d])];
+        if (L->seg_n[((size_t)g * 3 + d) * kShapes + s] == 0) s = 1;
+        L->shape[3 * g + d] = s;
+      }
+    return build_plan(L);
+  };
+  int st = ENZO_PPM_SUCCESS;
+  for (int c = 0; c < 4 && st == ENZO_PPM_SUCCESS; c++) {
+    bool present = false;
+    for (int g = 0; g < L->ngrids && !present; g++)
+      for (int d = 0; d < L->rank; d++) present |= cls(L->GD[3 * g + d]) == c;
+    if (!present) continue;
+    double best = 1e300;
+    int best_s = choice[c];
+    for (int s = 0; s < kShapes; s++) {
+      choice[c] = s;
+      if ((st = apply()) != ENZO_PPM_SUCCESS) break;
+      // warm-up (captures the graph), then timed repeats
+      if ((st = hipPPMLevelStep(L, CycleNumber, dt, 0.0)) != ENZO_PPM_SUCCESS) break;
+      if (hipStreamSynchronize(L->stream) != hipSuccess) { st = ENZO_PPM_ERR_HIP; break; }
+      auto t0 = std::chrono::steady_clock::now();
+      for (int r = 0; r < repeats && st == ENZO_PPM_SUCCESS; r++)
+        st = hipPPMLevelStep(L, CycleNumber, dt, 0.0);
+      if (hipStreamSynchronize(L->stream) != hipSuccess) st = ENZO_PPM_ERR_HIP;
+      const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (st == ENZO_PPM_SUCCESS && t < best) {
+        best = t;
+        best_s = s;
+      }
+    }
+    choice[c] = best_s;
+  }
+  if (st == ENZO_PPM_SUCCESS) st = apply();
+  // restore the state and the counters
+  L->parity = parity0;
+  if (hipMemcpy(parity0 == 0 ? L->A : L->B, save, sizeof(double) * tot, hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemset(L->counters, 0, 4 * sizeof(unsigned long long)) != hipSuccess)
+    st = ENZO_PPM_ERR_HIP;
+  (void)hipFree(save);
+  if (shape_by_length)
+    for (int c = 0; c < 4; c++) shape_by_length[c] = choice[c];
+  return st;
+}
+
+int hipPPMLevelDestroy(hipPPMLevel *L) {
+  if (!L) return ENZO_PPM_SUCCESS;
+  free_level(L);
+  delete L;
+  return ENZO_PPM_SUCCESS;
+}
+
+const char *hipPPMLevelLastError(void) { return g_level_error; }
+
+}  // extern "C"

@@ -104,7 +104,9 @@ struct SweepParams {
   // and the count of finished fallback workgroups (which clear both)
   int *fb_flag;
   unsigned *fb_list, *fb_count, *fb_done;
-  int fb_ok;  // a one-pencil workgroup holds the whole sweep axis
+  unsigned fb_tag;  // or-ed into list entries (grid << 16 in a batched level)
+  int fb_ok;        // a one-pencil workgroup holds the whole sweep axis
+  int tp;           // pencils per group (the sweep tile's TP)
 };
 
 // ---------------------------------------------------------------------------

@@ -167,6 +167,19 @@ PROTOTYPES = {
     "hipPPMVersion": (C.c_char_p, []),
     "hipPPMLastError": (C.c_char_p, []),
     "hipPPMDeviceCount": (C.c_int, []),
+    # batched AMR level
+    "hipPPMLevelCreate": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64, C.POINTER(HipPPMParameter),
+                                    C.c_void_p]),
+    "hipPPMLevelSetBaryon": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(PF), C.POINTER(PF)]),
+    "hipPPMLevelGetBaryon": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(PF)]),
+    "hipPPMLevelStep": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_double]),
+    "hipPPMLevelStatus": (C.c_int, [C.c_void_p, PF, PI]),
+    "hipPPMLevelUseGraph": (C.c_int, [C.c_void_p, C.c_int64]),
+    "hipPPMLevelSetShape": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int64]),
+    "hipPPMLevelCommitShapes": (C.c_int, [C.c_void_p]),
+    "hipPPMLevelTune": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_int64, PI]),
+    "hipPPMLevelDestroy": (C.c_int, [C.c_void_p]),
+    "hipPPMLevelLastError": (C.c_char_p, []),
 }
 
 

@@ -1,0 +1,84 @@
+"""Batched AMR-level update through the C ABI (hipPPMLevel*, include/enzo_ppm.h).
+
+A Level holds every grid of one AMR level on the device and advances them
+together; it computes, grid for grid, what a loop of hipPPMSolvePPM_DE calls
+with the level's dt computes.  There is no CPU path: calls raise when the
+library is missing or a call fails.
+"""
+import ctypes as C
+
+from . import abi
+from .grid import make_globals
+
+
+class Level:
+    def __init__(self, grids, glob, lib=None, stream=None):
+        self.lib = L = lib or abi.load_library()
+        self.grids = list(grids)
+        _, gd = make_globals(**glob)
+        self.paras = (abi.HipPPMParameter * len(self.grids))()
+        for i, g in enumerate(self.grids):
+            self.paras[i] = g.hip_parameter(L, gd)
+        self.handle = C.c_void_p(None)
+        self._check(L.hipPPMLevelCreate(C.byref(self.handle), len(self.grids), self.paras, stream),
+                    "LevelCreate")
+        self.upload()
+
+    def _check(self, st, what):
+        if st != abi.SUCCESS:
+            raise RuntimeError(f"{what} failed with status {st}: "
+                               f"{self.lib.hipPPMLevelLastError().decode()}")
+
+    def upload(self):
+        for i, g in enumerate(self.grids):
+            bf = (abi.PF * abi.MAX_NUMBER_OF_BARYON_FIELDS)()
+            for k, f in enumerate(g.fields):
+                bf[k] = abi.dptr(f)
+            acc = (abi.PF * 3)()
+            if g.accel is not None:
+                for d in range(3):
+                    acc[d] = abi.dptr(g.accel[d])
+            self._check(self.lib.hipPPMLevelSetBaryon(self.handle, i, bf, acc), "LevelSetBaryon")
+
+    def download(self):
+        for i, g in enumerate(self.grids):
+            bf = (abi.PF * abi.MAX_NUMBER_OF_BARYON_FIELDS)()
+            for k, f in enumerate(g.fields):
+                bf[k] = abi.dptr(f)
+            self._check(self.lib.hipPPMLevelGetBaryon(self.handle, i, bf), "LevelGetBaryon")
+
+    def step(self, cycle, dt=0.0, cfl=0.4):
+        """One level step; dt <= 0 uses the level's Courant step."""
+        self._check(self.lib.hipPPMLevelStep(self.handle, cycle, dt, cfl), "LevelStep")
+
+    def status(self):
+        """(status, dt of the last step, fallback cells since the last call)."""
+        dt = C.c_double(0.0)
+        n = C.c_int64(0)
+        st = self.lib.hipPPMLevelStatus(self.handle, C.byref(dt), C.byref(n))
+        return st, dt.value, n.value
+
+    def use_graph(self, on=True):
+        self._check(self.lib.hipPPMLevelUseGraph(self.handle, int(on)), "LevelUseGraph")
+
+    def set_shape(self, grid, dim, shape):
+        self._check(self.lib.hipPPMLevelSetShape(self.handle, grid, dim, shape), "LevelSetShape")
+
+    def commit_shapes(self):
+        self._check(self.lib.hipPPMLevelCommitShapes(self.handle), "LevelCommitShapes")
+
+    def tune(self, cycle, dt, repeats=2):
+        out = (C.c_int64 * 4)()
+        self._check(self.lib.hipPPMLevelTune(self.handle, cycle, dt, repeats, out), "LevelTune")
+        return list(out)
+
+    def close(self):
+        if self.handle.value:
+            self.lib.hipPPMLevelDestroy(self.handle)
+            self.handle = C.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -257,6 +257,63 @@ const char *hipPPMVersion(void);
 const char *hipPPMLastError(void);
 int hipPPMDeviceCount(void);
 
+/* ---- batched update of one AMR level --------------------------------------
+   No counterpart entry in the reference: Enzo updates a level grid by grid
+   (EvolveLevel.C's loop over grids -> grid::SolveHydroEquations ->
+   grid::SolvePPM_DE, Grid_SolveHydroEquations.C:513-517).  These entries do
+   that loop's work for all grids of a level at once -- each grid bitwise as
+   its own hipPPMSolvePPM_DE with the level's dt would -- with the grids
+   resident in one device arena, one launch per tile shape per sweep over all
+   grids, and each step's launches replayed from a captured hipGraph.
+   Subgrid face fluxes are not captured on this path. */
+typedef struct hipPPMLevel hipPPMLevel;
+
+/* Para: NumberOfGrids parameter blocks, one per grid (own GridDimension,
+   GridStartIndex/EndIndex, dx / CellWidth), with identical hydro switches
+   and field numbers.  Every axis below GridRank needs more than one active
+   cell.  stream: hipStream_t or NULL (the level then makes its own, which
+   stream capture requires). */
+int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids,
+                      const hipPPMParameter *Para, void *stream);
+
+/* Host -> device (and back) for grid `grid` of the level; synchronous. */
+int hipPPMLevelSetBaryon(hipPPMLevel *Level, enzo_int grid, enzo_float *BaryonField[],
+                         enzo_float *AccelerationField[]);
+int hipPPMLevelGetBaryon(hipPPMLevel *Level, enzo_int grid, enzo_float *BaryonField[]);
+
+/* One level step, queued on the level's stream: the stale pressure of every
+   grid (Grid_SolvePPM_DE.C:58-61), then -- with dtFixed <= 0 -- the level's
+   Courant step cfl * min over the grids of calc_dt (calc_dt.F:129-177), and
+   the x/y/z sweeps of every grid in the rotation of CycleNumber. */
+int hipPPMLevelStep(hipPPMLevel *Level, enzo_int CycleNumber, enzo_float dtFixed,
+                    enzo_float cfl);
+
+/* Waits for the level's stream; returns ENZO_PPM_SUCCESS or the error of
+   any step since the last call, the dt of the last step and the fallback
+   cells since the last call (the counts then restart). */
+int hipPPMLevelStatus(hipPPMLevel *Level, enzo_float *dt_used, enzo_long_int *fallback_cells);
+
+/* Replay captured hipGraphs (default) or launch each step directly. */
+int hipPPMLevelUseGraph(hipPPMLevel *Level, enzo_int enable);
+
+/* Tile shape of grid `grid` along sweep axis `dim`: 0 = 16 pencils x 16
+   interfaces, 1 = 8 x 32 (default), 2 = 4 x 64; hipPPMLevelCommitShapes
+   rebuilds the work lists. */
+int hipPPMLevelSetShape(hipPPMLevel *Level, enzo_int grid, enzo_int dim, enzo_int shape);
+int hipPPMLevelCommitShapes(hipPPMLevel *Level);
+
+/* Autotuner: chooses a tile shape per pencil-length class ([0,20], [21,40],
+   [41,80], longer; ghosts included) by timing `repeats` level steps of
+   length dt for each choice; the level's state is restored afterwards.
+   shape_by_length[4] receives the choices. */
+int hipPPMLevelTune(hipPPMLevel *Level, enzo_int CycleNumber, enzo_float dt, enzo_int repeats,
+                    enzo_int *shape_by_length);
+
+int hipPPMLevelDestroy(hipPPMLevel *Level);
+
+/* Text of the last HIP failure of a level call on this thread. */
+const char *hipPPMLevelLastError(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,8 +125,12 @@ inline void run_block(int nthreads) {
   }
 }
 
+// stream capture: launches and copies made while capturing are recorded and
+// replayed by hipGraphLaunch
+inline std::vector<std::function<void()>> *g_capture = nullptr;
+
 template <class K, class... A>
-void launch(K kernel, dim3 grid, dim3 block, A... args) {
+void launch_now(K kernel, dim3 grid, dim3 block, A... args) {
   g_gdim = grid;
   g_bdim = block;
   const int nt = (int)(block.x * block.y * block.z);
@@ -137,6 +141,12 @@ void launch(K kernel, dim3 grid, dim3 block, A... args) {
         g_body = [&]() { kernel(args...); };
         run_block(nt);
       }
+}
+
+template <class K, class... A>
+void launch(K kernel, dim3 grid, dim3 block, A... args) {
+  if (g_capture) g_capture->push_back([=]() { launch_now(kernel, grid, block, args...); });
+  else launch_now(kernel, grid, block, args...);
 }
 
 // guarded allocations: base of the mapping and its size, by user pointer
@@ -207,6 +217,10 @@ inline hipError_t hipFree(void *p) {
 }
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { memmove(d, s, n); return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k, hipStream_t = nullptr) {
+  if (hipcpu::g_capture) {
+    hipcpu::g_capture->push_back([=]() { memmove(d, s, n); });
+    return hipSuccess;
+  }
   return hipMemcpy(d, s, n, k);
 }
 inline hipError_t hipMemset(void *d, int v, size_t n) { memset(d, v, n); return hipSuccess; }
@@ -231,3 +245,31 @@ inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) {
   *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
   return hipSuccess;
 }
+
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+struct hipcpu_graph { std::vector<std::function<void()>> ops; };
+typedef hipcpu_graph *hipGraph_t;
+typedef hipcpu_graph *hipGraphExec_t;
+typedef struct hipcpu_node *hipGraphNode_t;
+inline hipError_t hipStreamCreate(hipStream_t *s) { *s = (hipStream_t) new int(0); return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t s) { delete (int *)s; return hipSuccess; }
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+  hipcpu::g_capture = new std::vector<std::function<void()>>();
+  return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) {
+  *g = new hipcpu_graph{std::move(*hipcpu::g_capture)};
+  delete hipcpu::g_capture;
+  hipcpu::g_capture = nullptr;
+  return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, hipGraphNode_t *, char *, size_t) {
+  *e = new hipcpu_graph{g->ops};
+  return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+  for (auto &op : e->ops) op();
+  return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }

@@ -205,6 +205,8 @@ extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *
     S.face_out = fout.data();
     S.counters = counters;
     S.fb_ok = pencil_fits<kFbTS>(S) ? 1 : 0;
+    S.tp = kTP;
+    S.fb_tag = 0;
     auto run_blocks = [&](long long nblk) {
       // main launch: flag pencil groups with a failing cell or an error
       std::vector<int> flagged;
@@ -223,11 +225,11 @@ extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *
       // fallback launch: every pencil of a flagged group, whole
       for (int grp : flagged) {
         const int tb = grp % S.nblk_t1, t2 = S.t2_0 + grp / S.nblk_t1;
-        for (int p = 0; p < kTP && tb * kTP + p < S.n_t1; p++) {
+        for (int p = 0; p < S.tp && tb * S.tp + p < S.n_t1; p++) {
           if (!S.fb_ok) { counters[3]++; break; }
           std::fill(smfb.begin(), smfb.end(), NAN);
-          TileCounts tc = S.uni[0] ? sweep_pencil_fb<kFbTS, 1>(S, tb * kTP + p, t2, smfb.data(), *exfb)
-                                   : sweep_pencil_fb<kFbTS, 0>(S, tb * kTP + p, t2, smfb.data(), *exfb);
+          TileCounts tc = S.uni[0] ? sweep_pencil_fb<kFbTS, 1>(S, tb * S.tp + p, t2, smfb.data(), *exfb)
+                                   : sweep_pencil_fb<kFbTS, 0>(S, tb * S.tp + p, t2, smfb.data(), *exfb);
           counters[0] += tc.nfb;
           counters[2] += tc.err;
         }

@@ -87,3 +87,50 @@ def fallback_differences(grid, fields_ref, nfb):
             assert d == 0, (k, d)
     assert n <= nfb, (n, nfb)
     return n
+
+
+def level_reference(grids, glob, steps, cfl=0.4, dt=None):
+    """The oracle's per-grid loop for a level: each step every grid takes the
+    level's dt (the minimum over the grids of the Courant step, or `dt`) and
+    one SolvePPM_DE call; ghosts are left as they are.  Returns the dts."""
+    dts = []
+    for s in range(steps):
+        d = dt if dt is not None else min(oracle.compute_time_step("port", g, glob, cfl) for g in grids)
+        for g in grids:
+            g.dt = d
+            st, _ = oracle.solve_ppm_de("port", g, glob, cycle=s)
+            assert st == 1
+        dts.append(d)
+    return dts
+
+
+def level_grids(kind, n=4, seed=7):
+    """Small seeded levels for the batched-update tests."""
+    from enzo_ppm import problems
+    if kind == "blast":
+        return problems.amr_level(n, seed=seed, nmin=8, nmax=20, ncolours=1)[:2]
+    if kind == "gravity":
+        out = []
+        for i in range(n):
+            g, glob, _ = problems.random_smooth([8 + 2 * i, 10, 9 + i], seed=seed + i, ncolours=1,
+                                                gravity=True)
+            g.fill_periodic()
+            out.append(g)
+        return out, glob
+    if kind == "stretched":
+        out = []
+        for i in range(n):
+            g, glob, _ = problems.random_smooth([12, 9 + i, 10], seed=seed + i, ncolours=2,
+                                                stretch=0.15)
+            g.fill_periodic()
+            out.append(g)
+        return out, glob
+    if kind == "diffusion":
+        out = []
+        for i in range(n):
+            g, glob, _ = problems.random_smooth([9 + i, 8, 10], seed=seed + i, ncolours=1,
+                                                flags=(1, 1, 1))
+            g.fill_periodic()
+            out.append(g)
+        return out, glob
+    raise ValueError(kind)

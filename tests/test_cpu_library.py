@@ -21,8 +21,9 @@ from helpers import FALLBACK_CASES, bit_equal, fallback_case, field_report, fill
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "emu", "libenzo_ppm_cpu.so")
-SRC = [os.path.join(ROOT, "enzo-dev_amd", "csrc", f) for f in ("enzo_ppm.hip", "ppm_sweep.hpp",
-                                                                "ppm_geometry.hpp")]
+SRC = [os.path.join(ROOT, "enzo-dev_amd", "csrc", f) for f in ("enzo_ppm.hip", "ppm_level.hip",
+                                                                "ppm_sweep.hpp", "ppm_geometry.hpp",
+                                                                "ppm_device.hpp")]
 
 
 @pytest.fixture(scope="module")
@@ -30,7 +31,7 @@ def cpulib():
     if not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(f) for f in SRC):
         subprocess.run(["g++", "-x", "c++", "-O1", "-g", "-std=c++17", "-fPIC", "-shared",
                         "-Wno-attributes", "-U_FORTIFY_SOURCE", "-Wl,-Bsymbolic", "-ffp-contract=off", "-fno-fast-math", "-I",
-                        os.path.join(HERE, "emu", "hipcpu"), "-o", LIB, SRC[0]], check=True)
+                        os.path.join(HERE, "emu", "hipcpu"), "-o", LIB, SRC[0], SRC[1]], check=True)
     return abi.load_library(LIB)
 
 
@@ -115,3 +116,94 @@ def test_device_resident_steps(cpulib):
     finally:
         dg.close()
     assert all(r[1] for r in field_report(g, a))
+
+
+# ---- batched AMR level (hipPPMLevel*) --------------------------------------
+from enzo_ppm.level import Level  # noqa: E402
+from helpers import level_grids, level_reference  # noqa: E402
+
+
+def level_check(lib, grids, glob, steps, shapes=None, graph=True, dt=None):
+    ref = [g.copy() for g in grids]
+    dts = level_reference(ref, glob, steps, dt=dt)
+    lv = Level([g.copy() for g in grids], glob, lib=lib)
+    try:
+        lv.use_graph(graph)
+        if shapes is not None:
+            for i, g in enumerate(lv.grids):
+                for d in range(g.rank):
+                    lv.set_shape(i, d, shapes)
+            lv.commit_shapes()
+        for s in range(steps):
+            lv.step(s, dt if dt is not None else 0.0, 0.4)
+            st, dtg, nfb = lv.status()
+            assert st == 1 and dtg == dts[s], (st, dtg, dts[s])
+        lv.download()
+        for a, b in zip(lv.grids, ref):
+            rep = field_report(a, b)
+            assert all(r[1] for r in rep), (a.active, rep)
+    finally:
+        lv.close()
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_level_blast_shapes(cpulib, shape):
+    grids, glob = level_grids("blast", 4)
+    level_check(cpulib, grids, glob, 3, shapes=shape)
+
+
+def test_level_direct_launches(cpulib):
+    grids, glob = level_grids("blast", 3, seed=9)
+    level_check(cpulib, grids, glob, 2, graph=False)
+
+
+@pytest.mark.parametrize("kind", ["gravity", "stretched", "diffusion"])
+def test_level_kinds(cpulib, kind):
+    grids, glob = level_grids(kind, 3)
+    level_check(cpulib, grids, glob, 2)
+
+
+def test_level_fallback(cpulib):
+    """A level whose grids need the per-cell fallback: same fields and the
+    summed fallback count of the per-grid oracle calls."""
+    grids = []
+    for name in ("3d_x_ok", "3d_x_stretched_ok"):
+        g, glob, cycle = fallback_case(name)
+        grids.append(g)
+    dt = min(g.dt for g in grids)
+    # both at the smaller dt: recompute the per-grid fallback counts there
+    ref = [g.copy() for g in grids]
+    nfb = 0
+    for g in ref:
+        g.dt = dt
+        st, n = oracle.solve_ppm_de("port", g, glob, cycle=0)
+        assert st == 1
+        nfb += n
+    lv = Level([g.copy() for g in grids], glob, lib=cpulib)
+    try:
+        lv.step(0, dt, 0.4)
+        st, dtg, n = lv.status()
+        assert (st, n) == (1, nfb)
+        lv.download()
+        for a, b in zip(lv.grids, ref):
+            assert all(r[1] for r in field_report(a, b))
+    finally:
+        lv.close()
+
+
+def test_level_tune_restores_state(cpulib):
+    grids, glob = level_grids("blast", 3, seed=11)
+    dt = min(oracle.compute_time_step("port", g, glob, 0.4) for g in grids)
+    lv = Level([g.copy() for g in grids], glob, lib=cpulib)
+    try:
+        choice = lv.tune(0, dt, repeats=1)
+        assert all(0 <= c <= 2 for c in choice)
+        lv.step(0, dt, 0.4)
+        assert lv.status()[0] == 1
+        lv.download()
+    finally:
+        lv.close()
+    ref = [g.copy() for g in grids]
+    level_reference(ref, glob, 1, dt=dt)
+    for a, b in zip(lv.grids, ref):
+        assert all(r[1] for r in field_report(a, b))

@@ -1,0 +1,109 @@
+"""Batched AMR-level update on the GPU (hipPPMLevel*) against the oracle's
+per-grid loop: every grid bitwise equal to its own SolvePPM_DE with the
+level's dt, the level dt equal to the minimum Courant step, fallback counts
+summed over the grids."""
+import pytest
+
+import oracle
+from enzo_ppm import abi, problems
+from enzo_ppm.level import Level
+from helpers import fallback_case, field_report, level_grids, level_reference
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    L = abi.load_library()
+    if L.hipPPMDeviceCount() < 1:
+        pytest.fail("HIP library loaded but no GPU is visible")
+    return L
+
+
+def level_check(lib, grids, glob, steps, shapes=None, graph=True):
+    ref = [g.copy() for g in grids]
+    dts = level_reference(ref, glob, steps)
+    lv = Level([g.copy() for g in grids], glob, lib=lib)
+    try:
+        lv.use_graph(graph)
+        if shapes is not None:
+            for i, g in enumerate(lv.grids):
+                for d in range(g.rank):
+                    lv.set_shape(i, d, shapes)
+            lv.commit_shapes()
+        for s in range(steps):
+            lv.step(s, 0.0, 0.4)
+            st, dtg, nfb = lv.status()
+            assert st == 1 and dtg == dts[s], (st, dtg, dts[s])
+        lv.download()
+        for a, b in zip(lv.grids, ref):
+            rep = field_report(a, b)
+            assert all(r[1] for r in rep), (a.active, rep)
+    finally:
+        lv.close()
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_level_blast_shapes(lib, shape):
+    grids, glob = level_grids("blast", 4)
+    level_check(lib, grids, glob, 3, shapes=shape)
+
+
+def test_level_direct_launches(lib):
+    grids, glob = level_grids("blast", 3, seed=9)
+    level_check(lib, grids, glob, 2, graph=False)
+
+
+@pytest.mark.parametrize("kind", ["gravity", "stretched", "diffusion"])
+def test_level_kinds(lib, kind):
+    grids, glob = level_grids(kind, 3)
+    level_check(lib, grids, glob, 2)
+
+
+def test_level_many_grids(lib):
+    """48 grids, axes log-uniform in [8, 64]: the AMR-level workload shape."""
+    grids, glob, _ = problems.amr_level(48, seed=21, nmin=8, nmax=64, ncolours=0)
+    level_check(lib, grids, glob, 2)
+
+
+def test_level_fallback(lib):
+    grids = []
+    for name in ("3d_x_ok", "3d_x_stretched_ok"):
+        g, glob, _ = fallback_case(name)
+        grids.append(g)
+    dt = min(g.dt for g in grids)
+    ref = [g.copy() for g in grids]
+    nfb = 0
+    for g in ref:
+        g.dt = dt
+        st, n = oracle.solve_ppm_de("port", g, glob, cycle=0)
+        assert st == 1
+        nfb += n
+    lv = Level([g.copy() for g in grids], glob, lib=lib)
+    try:
+        lv.step(0, dt, 0.4)
+        st, _, n = lv.status()
+        assert (st, n) == (1, nfb)
+        lv.download()
+        for a, b in zip(lv.grids, ref):
+            assert all(r[1] for r in field_report(a, b))
+    finally:
+        lv.close()
+
+
+def test_level_tune(lib):
+    grids, glob, _ = problems.amr_level(24, seed=5, nmin=8, nmax=64, ncolours=0)
+    dt = min(oracle.compute_time_step("port", g, glob, 0.4) for g in grids)
+    lv = Level([g.copy() for g in grids], glob, lib=lib)
+    try:
+        choice = lv.tune(0, dt, repeats=2)
+        assert all(0 <= c <= 2 for c in choice)
+        lv.step(0, dt, 0.4)
+        assert lv.status()[0] == 1
+        lv.download()
+    finally:
+        lv.close()
+    ref = [g.copy() for g in grids]
+    level_reference(ref, glob, 1, dt=dt)
+    for a, b in zip(lv.grids, ref):
+        assert all(r[1] for r in field_report(a, b))
