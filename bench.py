@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="active cells per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--level-grids", type=int, default=512,
+                    help="grids of the batched AMR-level comparison (0: skip it)")
+    ap.add_argument("--level-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (1 GiB read + 1 GiB write, "
                          "3 times) to scale HBM byte counters")
@@ -77,6 +80,74 @@ def cpu_baseline(n, threads):
                 kind="reference" if kind == "ref" else "port",
                 sample=f"one full step (pressure + Courant + 3 sweeps) of the {n}^3 workload, "
                        f"{threads} threads, {t1 - t0:.2f} s")
+
+
+def level_bench(lib, ngrids, steps, cfl=0.4):
+    """The AMR-level workload: `ngrids` grids, each axis log-uniform in
+    [8, 64] active cells (seeded), smooth sinusoidal density and velocity with
+    a Sedov-like blast (pressure x 1e4 inside radius 0.1 of a random centre),
+    no colours.  Both forms are device-resident and take the same steps at
+    the level's Courant dt: the plain form calls the per-grid entries
+    (hipPPMPressureAndTimeStep + hipPPMStep) grid by grid; the batched form
+    is one hipPPMLevelStep (graph replay, autotuned tile shapes).  The two
+    end states are compared bitwise."""
+    import numpy as np
+    from enzo_ppm import device, problems
+    from enzo_ppm.level import Level
+    grids, glob, _ = problems.amr_level(ngrids, seed=7, nmin=8, nmax=64, ncolours=0)
+    cells = int(sum(int(np.prod(g.active)) for g in grids))
+    lv = Level([g.copy() for g in grids], glob, lib=lib)
+    dgs = [device.DeviceGrid(g.copy(), glob, lib=lib) for g in grids]
+    try:
+        lv.step(0, 0.0, cfl)
+        st, dt, _ = lv.status()
+        if st != 1:
+            raise RuntimeError(f"level step status {st}")
+        for d in dgs:
+            d.pressure_and_dt(cfl, want_host=False)
+            d.step(0, dt)
+        shapes = lv.tune(1, dt, repeats=2)
+        lv.step(1, dt, cfl)
+        for d in dgs:
+            d.pressure_and_dt(cfl, want_host=False)
+            d.step(1, dt)
+        for d in dgs:
+            d.synchronize()
+        lv.status()
+        t0 = time.perf_counter()
+        for c in range(2, 2 + steps):
+            for d in dgs:
+                d.pressure_and_dt(cfl, want_host=False)
+                d.step(c, dt)
+        for d in dgs:
+            d.synchronize()
+        t1 = time.perf_counter()
+        for c in range(2, 2 + steps):
+            lv.step(c, dt, cfl)
+        st, _, _ = lv.status()
+        t2 = time.perf_counter()
+        if st != 1:
+            raise RuntimeError(f"level step status {st}")
+        lv.download()
+        same = True
+        for d, g in zip(dgs, lv.grids):
+            if d.download() != 1:
+                raise RuntimeError("per-grid step failed")
+            for a, b in zip(d.grid.fields, g.fields):
+                same &= bool(np.array_equal(a.view(np.int64), b.view(np.int64)))
+    finally:
+        lv.close()
+        for d in dgs:
+            d.close()
+    plain = cells * steps / (t1 - t0)
+    batched = cells * steps / (t2 - t1)
+    return {"workload": f"AMR level: {ngrids} grids, axes log-uniform in [8,64] active cells, "
+                        "Sedov blast in sinusoidal flow, no colours",
+            "cells_per_step": cells, "steps": steps, "dt": dt,
+            "plain_per_grid": {"value": plain, "ms_per_step": (t1 - t0) * 1e3 / steps},
+            "batched": {"value": batched, "ms_per_step": (t2 - t1) * 1e3 / steps,
+                        "tile_shape_by_pencil_length": shapes},
+            "unit": "cell-updates/s", "speedup": batched / plain, "bitwise_equal": same}
 
 
 def main():
@@ -152,6 +223,12 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_sweep},
         "cpu_baseline": None,
     }
+    if a.level_grids > 0:
+        print(f"bench: AMR level of {a.level_grids} grids", file=sys.stderr, flush=True)
+        try:
+            out["level"] = level_bench(lib, a.level_grids, a.level_steps)
+        except Exception as e:  # reported, not fatal
+            out["level"] = {"error": str(e)}
     if not a.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(n, a.cpu_threads)

@@ -25,8 +25,8 @@ using namespace ppm;
 
 namespace {
 
-constexpr int kTP = 8;   // pencils per workgroup
-constexpr int kTS = 32;  // interfaces per pencil segment
+constexpr int kTP = kGridTP;  // pencils per workgroup
+constexpr int kTS = kGridTS;  // interfaces per pencil segment
 using SweepCfg = Cfg<kTP, kTS>;
 
 // ------------------------------------------------------------------------

@@ -125,6 +125,11 @@ constexpr int F_DF = A_DQ, F_EF = A_QL, F_UF = A_QR, F_VF = A_Q6, F_WF = A_ST,
 // source terms sl*bm0 and sr*bp0 of each interface (flux_hll.F:297-308).
 constexpr int A_HA = NARR, A_HB = NARR + 1, NARR_FB = NARR + 2;
 
+// Tile shape of the per-grid sweep kernel: 4 pencils x 64 interfaces (256
+// work-items).  Longer segments cut the halo recomputation (segments of up to
+// 57 owned cells); on the 256^3 workload 4 x 64 beat 8 x 32 and 2 x 128.
+constexpr int kGridTP = 4, kGridTS = 64;
+
 template <int TP, int TS, int UNI = 0>
 struct Cfg {
   static constexpr int LW = TS + 4;

@@ -21,7 +21,7 @@
 using namespace ppm;
 
 namespace {
-constexpr int kTP = 8, kTS = 32;
+constexpr int kTP = kGridTP, kTS = kGridTS;  // the device kernel's tile shape
 constexpr int kFbTS = 1024;  // one-pencil fallback workgroups
 
 template <int NT>
