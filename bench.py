@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--level-grids", type=int, default=512,
                     help="grids of the batched AMR-level comparison (0: skip it)")
     ap.add_argument("--level-steps", type=int, default=5)
+    ap.add_argument("--learn-epochs", type=int, default=4,
+                    help="training passes of the learned start (0: skip that part)")
+    ap.add_argument("--learn-records", type=int, default=10_000_000,
+                    help="harvested interface states to train on")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (1 GiB read + 1 GiB write, "
                          "3 times) to scale HBM byte counters")
@@ -82,7 +86,7 @@ def cpu_baseline(n, threads):
                        f"{threads} threads, {t1 - t0:.2f} s")
 
 
-def level_bench(lib, ngrids, steps, cfl=0.4):
+def level_bench(lib, ngrids, steps, cfl=0.4, weights=None):
     """The AMR-level workload: `ngrids` grids, each axis log-uniform in
     [8, 64] active cells (seeded), smooth sinusoidal density and velocity with
     a Sedov-like blast (pressure x 1e4 inside radius 0.1 of a random centre),
@@ -90,7 +94,8 @@ def level_bench(lib, ngrids, steps, cfl=0.4):
     the level's Courant dt: the plain form calls the per-grid entries
     (hipPPMPressureAndTimeStep + hipPPMStep) grid by grid; the batched form
     is one hipPPMLevelStep (graph replay, autotuned tile shapes).  The two
-    end states are compared bitwise."""
+    end states are compared bitwise.  With `weights`, the batched form is
+    then timed again with the learned two-shock start (batched+learned)."""
     import numpy as np
     from enzo_ppm import device, problems
     from enzo_ppm.level import Level
@@ -135,6 +140,20 @@ def level_bench(lib, ngrids, steps, cfl=0.4):
                 raise RuntimeError("per-grid step failed")
             for a, b in zip(d.grid.fields, g.fields):
                 same &= bool(np.array_equal(a.view(np.int64), b.view(np.int64)))
+        batched_learned = None
+        if weights is not None:
+            lv.set_learned(weights)
+            lv.step(2 + steps, dt, cfl)          # graph capture outside the timing
+            lv.status()
+            t3 = time.perf_counter()
+            for c in range(3 + steps, 3 + 2 * steps):
+                lv.step(c, dt, cfl)
+            st, _, _ = lv.status()
+            t4 = time.perf_counter()
+            if st != 1:
+                raise RuntimeError(f"learned level step status {st}")
+            batched_learned = cells * steps / (t4 - t3)
+            lv.set_learned(None)
     finally:
         lv.close()
         for d in dgs:
@@ -147,7 +166,87 @@ def level_bench(lib, ngrids, steps, cfl=0.4):
             "plain_per_grid": {"value": plain, "ms_per_step": (t1 - t0) * 1e3 / steps},
             "batched": {"value": batched, "ms_per_step": (t2 - t1) * 1e3 / steps,
                         "tile_shape_by_pencil_length": shapes},
-            "unit": "cell-updates/s", "speedup": batched / plain, "bitwise_equal": same}
+            "unit": "cell-updates/s", "speedup": batched / plain, "bitwise_equal": same,
+            **({"batched_learned": {"value": batched_learned,
+                                    "ms_per_step": cells / batched_learned * 1e3,
+                                    "speedup_vs_plain_per_grid": batched_learned / plain}}
+               if batched_learned else {})}
+
+
+def time_grid(lib, n, steps, warmup, weights=None):
+    """Device-resident steps of the n^3 turbulent workload: `warmup` untimed,
+    then `steps` timed, with HIP events around every sweep launch.  With
+    `weights` the sweeps use the learned two-shock start."""
+    from enzo_ppm import device, problems
+    g, glob, run = problems.turbulent(n, seed=2024)
+    dg = device.DeviceGrid(g, glob, lib=lib)
+    try:
+        if weights is not None:
+            dg.set_learned(weights)
+        cfl = run["cfl"]
+        cycle = 0
+        for _ in range(warmup):
+            dg.periodic_boundary()
+            dg.pressure_and_dt(cfl, want_host=False)
+            dg.step(cycle)
+            cycle += 1
+        dg.synchronize()
+        dg.profile(True)
+        dg.profile_read()
+        dg.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dg.periodic_boundary()
+            dg.pressure_and_dt(cfl, want_host=False)
+            dg.step(cycle)
+            cycle += 1
+        dg.synchronize()
+        t1 = time.perf_counter()
+        sweep_ms, launches = dg.profile_read()
+        fb = dg.fallback_count()
+    finally:
+        dg.close()
+    return dict(value=n ** 3 * steps / (t1 - t0), ms_per_step=(t1 - t0) * 1e3 / steps,
+                sweep_ms_avg=sweep_ms / max(1, launches), fallback_cells=int(fb), grid=g)
+
+
+def learned_bench(lib, n, steps, warmup, epochs, cap):
+    """The learned two-shock start end to end: harvest of plain-solver
+    interface states from the 64^3 Sod tube (one step, every interface) and
+    the n^3 turbulent box (one step, one interface in `keep` so that about
+    `cap` records are kept); training on the GPU; Newton statistics on a
+    held-out n^3 field (another seed, one interface in 16); and the n^3 step
+    timed with the learned start."""
+    from enzo_ppm import learn, problems
+    t0 = time.perf_counter()
+    g1, glob1, _ = problems.sod_x(64)
+    r1, _ = learn.harvest(g1, glob1, steps=1, lib=lib)
+    g2, glob2, _ = problems.turbulent(n, seed=2024)
+    swept = 3 * (n + 1) * (n + 6) ** 2
+    keep = max(1, -(-swept // max(1, cap - len(r1))))
+    r2, _ = learn.harvest(g2, glob2, steps=1, keep=keep, cap=cap, lib=lib)
+    import numpy as np
+    rec = np.concatenate([r1, r2])
+    del r1, r2
+    t1 = time.perf_counter()
+    gamma = glob2["Gamma"]
+    w, mse = learn.train(rec, gamma, epochs=epochs, batch=8192, lr=3e-3, seed=0, lib=lib)
+    t2 = time.perf_counter()
+    n_train = len(rec)
+    del rec
+    gh, globh, _ = problems.turbulent(n, seed=4242)
+    rh, _ = learn.harvest(gh, globh, steps=1, keep=16, cap=cap, lib=lib)
+    _, stats = learn.newton_stats(rh, gamma, 1e-20, w, lib=lib)
+    del rh
+    print("bench: learned-start timing", file=sys.stderr, flush=True)
+    t = time_grid(lib, n, steps, warmup, weights=w)
+    t.pop("grid")
+    return {"harvest": {"records": n_train, "sources": f"64^3 Sod (1 step, all interfaces) + "
+                                                       f"{n}^3 turbulent (1 step, 1 in {keep})",
+                        "seconds": t1 - t0},
+            "training": {"epochs": epochs, "batch": 8192, "lr": 3e-3, "seconds": t2 - t1,
+                         "mse_per_epoch": [float(x) for x in mse]},
+            "held_out_newton": stats, "grid_step_learned": t}, w
 
 
 def main():
@@ -159,43 +258,19 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not built)"}))
         sys.exit(2)
-    from enzo_ppm import abi, device, problems
+    from enzo_ppm import abi
     lib = abi.load_library()
     if lib.hipPPMDeviceCount() < 1:
         raise RuntimeError("no GPU visible")
     n = a.n
-    print(f"bench: building {n}^3 workload", file=sys.stderr, flush=True)
-    g, glob, run = problems.turbulent(n, seed=2024)
-    dg = device.DeviceGrid(g, glob, lib=lib)
-    cfl = run["cfl"]
-    cycle = 0
-    for _ in range(a.warmup):
-        dg.periodic_boundary()
-        dg.pressure_and_dt(cfl, want_host=False)
-        dg.step(cycle)
-        cycle += 1
-    dg.synchronize()
-    dg.profile(True)
-    dg.profile_read()
-    dg.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        dg.periodic_boundary()
-        dg.pressure_and_dt(cfl, want_host=False)
-        dg.step(cycle)
-        cycle += 1
-    dg.synchronize()
-    t1 = time.perf_counter()
-    sweep_ms, launches = dg.profile_read()
-    fb = dg.fallback_count()
-    dg.close()
+    print(f"bench: {n}^3 workload, plain start", file=sys.stderr, flush=True)
+    t = time_grid(lib, n, a.steps, a.warmup)
+    g = t["grid"]
     cells = n ** 3
-    ms_step = (t1 - t0) * 1e3 / a.steps
-    value = cells * a.steps / (t1 - t0)
     C = g.ncolours
     G = 1 if g.gravity else 0
     bytes_sweep = cells * 8 * ((7 + C + G) + (6 + C))
-    sweep_avg_s = sweep_ms / 1e3 / max(1, launches)
+    sweep_avg_s = t["sweep_ms_avg"] / 1e3
     achieved = bytes_sweep / sweep_avg_s / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_turbulent{n}.json")
@@ -203,12 +278,12 @@ def main():
         traffic = json.load(open(tf)).get("sweep_bytes_per_launch")
     out = {
         "metric": "FP64 PPM cell-updates/s/GPU (batched+learned vs plain), % HBM roofline",
-        "value": value,
+        "value": t["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": t["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -216,20 +291,29 @@ def main():
         "data": "synthetic (seeded lognormal density, k^-2 Mach-3 velocity field)",
         "config": {"workload": f"plain per-grid PPM-DE step, uniform {n}^3 turbulent grid, "
                                f"3 colours, dual energy, periodic", "grid": n,
-                   "form": "plain per-grid", "fallback_cells": int(fb)},
+                   "form": "plain per-grid", "fallback_cells": t["fallback_cells"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": "ppm_sweep_kernel", "kernel_ms_avg": sweep_avg_s * 1e3,
                      "algorithmic_bytes_per_launch": bytes_sweep},
         "cpu_baseline": None,
     }
+    weights = None
+    if a.learn_epochs > 0:
+        print("bench: learned start (harvest, training, statistics)", file=sys.stderr, flush=True)
+        try:
+            out["learned"], weights = learned_bench(lib, n, a.steps, a.warmup, a.learn_epochs,
+                                                    a.learn_records)
+        except Exception as e:  # reported, not fatal
+            out["learned"] = {"error": str(e)}
     if a.level_grids > 0:
         print(f"bench: AMR level of {a.level_grids} grids", file=sys.stderr, flush=True)
         try:
-            out["level"] = level_bench(lib, a.level_grids, a.level_steps)
+            out["level"] = level_bench(lib, a.level_grids, a.level_steps, weights=weights)
         except Exception as e:  # reported, not fatal
             out["level"] = {"error": str(e)}
     if not a.no_cpu_baseline:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
         try:
             out["cpu_baseline"] = cpu_baseline(n, a.cpu_threads)
         except Exception as e:  # reported, not fatal

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -31,13 +32,13 @@ using SweepCfg = Cfg<kTP, kTS>;
 
 // ------------------------------------------------------------------------
 // Fused sweep kernel: see ppm_sweep.hpp for the stages.
-template <int UNI>
+template <int UNI, int LEARN>
 __global__ __launch_bounds__(kTP *kTS) __attribute__((amdgpu_waves_per_eu(4)))
 void ppm_sweep_kernel(SweepParams P) {
   __shared__ double sm[SweepCfg::SMEM];
   DevExec ex;
   const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  TileCounts tc = sweep_tile<kTP, kTS, UNI>(P, tile, sm, ex);
+  TileCounts tc = sweep_tile<kTP, kTS, UNI, DevExec, LEARN>(P, tile, sm, ex);
   if (threadIdx.x == 0) {
     // a failing cell or an error: the fallback kernel redoes the group's
     // pencils whole (the fallback chain can cross segments) and settles them
@@ -53,12 +54,12 @@ void ppm_sweep_kernel(SweepParams P) {
 // fallback applied in the reference's cell order; its results replace the
 // sweep kernel's for those pencils.  Launched after every sweep launch; with
 // no flagged group each workgroup only reads the count.
-template <int UNI>
+template <int UNI, int LEARN>
 __global__ __launch_bounds__(kFbTS) void ppm_fallback_kernel(SweepParams P) {
   __shared__ double sm[NARR_FB * FbCfg::LW];
   DevExec ex;
   const unsigned n = *P.fb_count;
-  for (unsigned k = blockIdx.x; k < n; k += gridDim.x) fallback_group<UNI>(P, (int)P.fb_list[k], sm, ex);
+  for (unsigned k = blockIdx.x; k < n; k += gridDim.x) fallback_group<UNI, LEARN>(P, (int)P.fb_list[k], sm, ex);
   fallback_launch_done(P.fb_count, P.fb_done);
 }
 
@@ -165,6 +166,14 @@ struct hipPPMData {
   int nsub = 0;
   long long fallback_total = 0;
   long long chain_total = 0;
+  // learned two-shock start (network parameters, kMlpFloats floats) when set
+  float *learn = nullptr;
+  bool learned = false;
+  // harvest of interface states (SweepParams::harvest) when harvest_cap > 0
+  double *harvest = nullptr;
+  unsigned long long *harvest_n = nullptr;
+  long long harvest_cap = 0;
+  unsigned harvest_keep = 1;
   // optional per-launch timing of the sweep kernel (HIP events on D->stream)
   int profile = 0;
   std::vector<hipEvent_t> ev;  // pairs
@@ -233,7 +242,7 @@ void free_data(hipPPMData *D) {
   D->ev.clear();
   void *ptrs[] = {D->A, D->B, D->pres, D->accel, D->tab[0], D->tab[1], D->tab[2], D->cwdev[0],
                   D->cwdev[1], D->cwdev[2], D->flat, D->qc, D->dt, D->dtmin, D->counters, D->segbuf,
-                  D->fb_flag, D->fb_list, D->fb_state,
+                  D->fb_flag, D->fb_list, D->fb_state, D->learn, D->harvest, D->harvest_n,
                   D->face_head[0], D->face_head[1], D->face_head[2], D->faces[0], D->faces[1],
                   D->faces[2], D->face_out[0], D->face_out[1], D->face_out[2]};
   for (void *p : ptrs)
@@ -319,11 +328,33 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
   S.fb_tag = 0;
   S.tp = kTP;
   S.fb_ok = pencil_fits<kFbTS>(S) ? 1 : 0;
+  S.learn = D->learned ? D->learn : nullptr;
+  if (D->harvest_cap > 0) {
+    S.harvest = D->harvest;
+    S.harvest_n = D->harvest_n;
+    S.harvest_cap = D->harvest_cap;
+    S.harvest_keep = D->harvest_keep;
+  }
+  // instantiation by (one cell width, learned start)
+  const int variant = (S.uni[0] ? 1 : 0) | (D->learned ? 2 : 0);
+  auto launch_main = [&](unsigned nblk) {
+    const dim3 g(nblk), b(kTP * kTS);
+    switch (variant) {
+      case 0: hipLaunchKernelGGL((ppm_sweep_kernel<0, 0>), g, b, 0, D->stream, S); break;
+      case 1: hipLaunchKernelGGL((ppm_sweep_kernel<1, 0>), g, b, 0, D->stream, S); break;
+      case 2: hipLaunchKernelGGL((ppm_sweep_kernel<0, 1>), g, b, 0, D->stream, S); break;
+      default: hipLaunchKernelGGL((ppm_sweep_kernel<1, 1>), g, b, 0, D->stream, S); break;
+    }
+    return hipGetLastError();
+  };
   auto launch_fb = [&]() {
-    if (S.uni[0])
-      hipLaunchKernelGGL(ppm_fallback_kernel<1>, dim3(kFbBlocks), dim3(kFbTS), 0, D->stream, S);
-    else
-      hipLaunchKernelGGL(ppm_fallback_kernel<0>, dim3(kFbBlocks), dim3(kFbTS), 0, D->stream, S);
+    const dim3 g(kFbBlocks), b(kFbTS);
+    switch (variant) {
+      case 0: hipLaunchKernelGGL((ppm_fallback_kernel<0, 0>), g, b, 0, D->stream, S); break;
+      case 1: hipLaunchKernelGGL((ppm_fallback_kernel<1, 0>), g, b, 0, D->stream, S); break;
+      case 2: hipLaunchKernelGGL((ppm_fallback_kernel<0, 1>), g, b, 0, D->stream, S); break;
+      default: hipLaunchKernelGGL((ppm_fallback_kernel<1, 1>), g, b, 0, D->stream, S); break;
+    }
     return hipGetLastError();
   };
   if (S.idiff == 0) {
@@ -352,11 +383,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
       D->ev_used += 2;
       HIPCHK(hipEventRecord(e0, D->stream));
     }
-    if (S.uni[0])
-      hipLaunchKernelGGL(ppm_sweep_kernel<1>, dim3((unsigned)nblk), dim3(kTP * kTS), 0, D->stream, S);
-    else
-      hipLaunchKernelGGL(ppm_sweep_kernel<0>, dim3((unsigned)nblk), dim3(kTP * kTS), 0, D->stream, S);
-    HIPCHK(hipGetLastError());
+    HIPCHK(launch_main((unsigned)nblk));
     if (D->profile) HIPCHK(hipEventRecord(e1, D->stream));
     HIPCHK(launch_fb());
   } else {
@@ -371,13 +398,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
     for (int k = 0; k < D->GD[oax]; k++) {
       S.t2_0 = k;
       long long nblk = (long long)S.nseg * S.nblk_t1;
-      if (S.uni[0])
-        hipLaunchKernelGGL(ppm_sweep_kernel<1>, dim3((unsigned)nblk), dim3(kTP * kTS), 0,
-                           D->stream, S);
-      else
-        hipLaunchKernelGGL(ppm_sweep_kernel<0>, dim3((unsigned)nblk), dim3(kTP * kTS), 0,
-                           D->stream, S);
-      HIPCHK(hipGetLastError());
+      HIPCHK(launch_main((unsigned)nblk));
       HIPCHK(launch_fb());  // the next slice reads this one's results
     }
   }
@@ -918,6 +939,62 @@ int hipPPMProfileRead(hipPPMData *D, enzo_float *total_ms, enzo_long_int *launch
 int hipPPMSynchronize(hipPPMData *D) {
   if (!D) return ENZO_PPM_ERR_ARGUMENT;
   HIPCHK(hipStreamSynchronize(D->stream));
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMSetLearnedGuess(hipPPMData *D, const float *weights) {
+  if (!D) return ENZO_PPM_ERR_ARGUMENT;
+  if (!weights) {
+    D->learned = false;
+    return ENZO_PPM_SUCCESS;
+  }
+  for (int k = 0; k < kMlpFloats; k++)
+    if (!std::isfinite(weights[k])) return ENZO_PPM_ERR_ARGUMENT;
+  if (!D->learn && alloc_dev((void **)&D->learn, sizeof(float) * kMlpFloats))
+    return ENZO_PPM_ERR_HIP;
+  // ordered after the stream's earlier work, visible to its later launches
+  HIPCHK(hipStreamSynchronize(D->stream));
+  HIPCHK(hipMemcpy(D->learn, weights, sizeof(float) * kMlpFloats, hipMemcpyHostToDevice));
+  D->learned = true;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMSetHarvest(hipPPMData *D, enzo_long_int capacity, enzo_int keep) {
+  if (!D || capacity < 0 || keep < 1) return ENZO_PPM_ERR_ARGUMENT;
+  HIPCHK(hipStreamSynchronize(D->stream));
+  if (D->harvest) (void)hipFree(D->harvest);
+  D->harvest = nullptr;
+  D->harvest_cap = 0;
+  if (capacity == 0) return ENZO_PPM_SUCCESS;
+  if (!D->harvest_n && alloc_dev((void **)&D->harvest_n, sizeof(unsigned long long)))
+    return ENZO_PPM_ERR_HIP;
+  if (alloc_dev((void **)&D->harvest, sizeof(double) * kHarvestDoubles * (size_t)capacity))
+    return ENZO_PPM_ERR_HIP;
+  HIPCHK(hipMemset(D->harvest_n, 0, sizeof(unsigned long long)));
+  D->harvest_cap = capacity;
+  D->harvest_keep = (unsigned)keep;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMGetHarvest(hipPPMData *D, enzo_float *records, enzo_long_int max_records,
+                     enzo_long_int *seen, enzo_long_int *stored) {
+  if (!D || max_records < 0 || (max_records > 0 && !records) || !seen || !stored)
+    return ENZO_PPM_ERR_ARGUMENT;
+  if (D->harvest_cap == 0) {
+    *seen = *stored = 0;
+    return ENZO_PPM_SUCCESS;
+  }
+  HIPCHK(hipStreamSynchronize(D->stream));
+  unsigned long long n = 0;
+  HIPCHK(hipMemcpy(&n, D->harvest_n, sizeof(n), hipMemcpyDeviceToHost));
+  const long long kept = std::min<long long>((long long)n, D->harvest_cap);
+  const long long m = std::min<long long>(kept, max_records);
+  if (m > 0)
+    HIPCHK(hipMemcpy(records, D->harvest, sizeof(double) * kHarvestDoubles * (size_t)m,
+                     hipMemcpyDeviceToHost));
+  HIPCHK(hipMemset(D->harvest_n, 0, sizeof(unsigned long long)));
+  *seen = (enzo_long_int)n;
+  *stored = (enzo_long_int)m;
   return ENZO_PPM_SUCCESS;
 }
 

@@ -54,7 +54,7 @@ __device__ inline void flag_group(const SweepParams &P, int grp) {
 
 // Every pencil of group `grp` (P.tp pencils along t1) redone whole with the
 // per-cell fallback, then the group flag cleared.
-template <int UNI>
+template <int UNI, int LEARN = 0>
 __device__ inline void fallback_group(const SweepParams &P, int grp, double *sm, DevExec &ex) {
   const int tb = grp % P.nblk_t1, t2 = P.t2_0 + grp / P.nblk_t1;
   for (int p = 0; p < P.tp && tb * P.tp + p < P.n_t1; p++) {
@@ -62,7 +62,7 @@ __device__ inline void fallback_group(const SweepParams &P, int grp, double *sm,
       if (threadIdx.x == 0) atomicAdd(&P.counters[3], 1ull);
       break;
     }
-    TileCounts tc = sweep_pencil_fb<kFbTS, UNI>(P, tb * P.tp + p, t2, sm, ex);
+    TileCounts tc = sweep_pencil_fb<kFbTS, UNI, DevExec, LEARN>(P, tb * P.tp + p, t2, sm, ex);
     if (threadIdx.x == 0) {
       if (tc.nfb) atomicAdd(&P.counters[0], (unsigned long long)tc.nfb);
       if (tc.err) atomicAdd(&P.counters[2], (unsigned long long)tc.err);

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -44,14 +45,14 @@ constexpr int kShapes = 3;
 constexpr int kShapeTP[kShapes] = {16, 8, 4};
 constexpr int kShapeTS[kShapes] = {16, 32, 64};
 
-template <int TP, int TS, int UNI>
+template <int TP, int TS, int UNI, int LEARN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void level_sweep_kernel(const SweepParams *params, const Work *work) {
   __shared__ double sm[Cfg<TP, TS>::SMEM];
   DevExec ex;
   const Work w = work[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
   const SweepParams &P = params[w.grid];
-  TileCounts tc = sweep_tile<TP, TS, UNI>(P, w.tile, sm, ex);
+  TileCounts tc = sweep_tile<TP, TS, UNI, DevExec, LEARN>(P, w.tile, sm, ex);
   if (threadIdx.x == 0) {
     if (tc.bad || tc.err) flag_group(P, w.tile / P.nseg);
     if (tc.chains) atomicAdd(&P.counters[1], (unsigned long long)tc.chains);
@@ -59,6 +60,7 @@ void level_sweep_kernel(const SweepParams *params, const Work *work) {
 }
 
 // fallback over the level's list: entry = grid << 16 | group
+template <int LEARN>
 __global__ __launch_bounds__(kFbTS) void level_fallback_kernel(const SweepParams *params,
                                                               const unsigned *list,
                                                               unsigned *count, unsigned *done) {
@@ -68,8 +70,8 @@ __global__ __launch_bounds__(kFbTS) void level_fallback_kernel(const SweepParams
   for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
     const unsigned e = list[k];
     const SweepParams &P = params[e >> 16];
-    if (P.uni[0]) fallback_group<1>(P, (int)(e & 0xffffu), sm, ex);
-    else fallback_group<0>(P, (int)(e & 0xffffu), sm, ex);
+    if (P.uni[0]) fallback_group<1, LEARN>(P, (int)(e & 0xffffu), sm, ex);
+    else fallback_group<0, LEARN>(P, (int)(e & 0xffffu), sm, ex);
   }
   fallback_launch_done(count, done);
 }
@@ -189,6 +191,8 @@ struct hipPPMLevel {
   std::map<int, hipGraphExec_t> graphs;
   double graph_cfl = -1.0;          // cfl the Courant-step graphs were captured with
   long long fallback_total = 0;
+  float *learn = nullptr;           // learned two-shock start, when `learned`
+  bool learned = false;
 };
 
 namespace {
@@ -202,7 +206,7 @@ void free_level(hipPPMLevel *L) {
   drop_graphs(L);
   void *ptrs[] = {L->A, L->B, L->pres, L->accel, L->tabs, L->cws, L->flat, L->qc, L->segs, L->dt,
                   L->dtmin, L->counters, L->fb_flag, L->fb_list, L->fb_state, L->dparams,
-                  L->dflat, L->dpres, L->dwork, L->dpwork};
+                  L->dflat, L->dpres, L->dwork, L->dpwork, L->learn};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (L->own_stream && L->stream) (void)hipStreamDestroy(L->stream);
@@ -293,6 +297,7 @@ int build_plan(hipPPMLevel *L) {
         S.A = src + L->off[g];
         S.B = dst + L->off[g];
         S.ncol = L->ncol;
+        S.learn = L->learned ? L->learn : nullptr;
         S.pres = L->pres + L->poff[g];
         S.gr = L->gravity ? L->accel + 3 * L->poff[g] + (long long)d * L->cells[g] : nullptr;
         S.flat = L->flat + L->flat_off[g];
@@ -422,10 +427,19 @@ int launch_level_sweep(hipPPMLevel *L, int d) {
       const auto &ln = L->launches[d][li];
       const Work *w = L->dwork + ln.first;
       const dim3 grid((unsigned)ln.count), blk(256);
-#define LEVEL_LAUNCH(S, TP, TS)                                                                 \
-  if (ln.shape == S) {                                                                        \
-    if (ln.uni) hipLaunchKernelGGL((level_sweep_kernel<TP, TS, 1>), grid, blk, 0, L->stream, P, w); \
-    else hipLaunchKernelGGL((level_sweep_kernel<TP, TS, 0>), grid, blk, 0, L->stream, P, w);        \
+#define LEVEL_LAUNCH(S, TP, TS)                                                             \
+  if (ln.shape == S) {                                                                    \
+    if (L->learned) {                                                                     \
+      if (ln.uni)                                                                         \
+        hipLaunchKernelGGL((level_sweep_kernel<TP, TS, 1, 1>), grid, blk, 0, L->stream, P, w); \
+      else                                                                                \
+        hipLaunchKernelGGL((level_sweep_kernel<TP, TS, 0, 1>), grid, blk, 0, L->stream, P, w); \
+    } else {                                                                              \
+      if (ln.uni)                                                                         \
+        hipLaunchKernelGGL((level_sweep_kernel<TP, TS, 1, 0>), grid, blk, 0, L->stream, P, w); \
+      else                                                                                \
+        hipLaunchKernelGGL((level_sweep_kernel<TP, TS, 0, 0>), grid, blk, 0, L->stream, P, w); \
+    }                                                                                     \
   }
       LEVEL_LAUNCH(0, 16, 16)
       LEVEL_LAUNCH(1, 8, 32)
@@ -433,8 +447,12 @@ int launch_level_sweep(hipPPMLevel *L, int d) {
 #undef LEVEL_LAUNCH
       LCHK(hipGetLastError());
     }
-    hipLaunchKernelGGL(level_fallback_kernel, dim3(kFbBlocks), dim3(kFbTS), 0, L->stream, P,
-                       L->fb_list, L->fb_state, L->fb_state + 1);
+    if (L->learned)
+      hipLaunchKernelGGL(level_fallback_kernel<1>, dim3(kFbBlocks), dim3(kFbTS), 0, L->stream, P,
+                         L->fb_list, L->fb_state, L->fb_state + 1);
+    else
+      hipLaunchKernelGGL(level_fallback_kernel<0>, dim3(kFbBlocks), dim3(kFbTS), 0, L->stream, P,
+                         L->fb_list, L->fb_state, L->fb_state + 1);
     LCHK(hipGetLastError());
   }
   L->parity ^= 1;
@@ -841,6 +859,19 @@ int hipPPMLevelTune(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dt, enzo_in
   if (shape_by_length)
     for (int c = 0; c < 4; c++) shape_by_length[c] = choice[c];
   return st;
+}
+
+int hipPPMLevelSetLearnedGuess(hipPPMLevel *L, const float *weights) {
+  if (!L) return ENZO_PPM_ERR_ARGUMENT;
+  LCHK(hipStreamSynchronize(L->stream));
+  if (weights) {
+    for (int k = 0; k < kMlpFloats; k++)
+      if (!std::isfinite(weights[k])) return ENZO_PPM_ERR_ARGUMENT;
+    if (!L->learn && dev_alloc(&L->learn, kMlpFloats)) return ENZO_PPM_ERR_HIP;
+    LCHK(hipMemcpy(L->learn, weights, sizeof(float) * kMlpFloats, hipMemcpyHostToDevice));
+  }
+  L->learned = weights != nullptr;
+  return build_plan(L);  // parameters carry the network pointer; graphs are recaptured
 }
 
 int hipPPMLevelDestroy(hipPPMLevel *L) {

@@ -39,6 +39,21 @@
 #define PPM_ASSERT(...) ((void)0)
 #endif
 
+#include "ppm_learn.hpp"
+
+// atomic add of the harvest counter: device atomics in the HIP and CPU-HIP
+// builds, a plain add in the block emulator (one work-item at a time)
+#if defined(__HIPCC__) || defined(HIPCPU_RUNTIME)
+#define PPM_ATOMIC_ADD_U64(p, v) atomicAdd((p), (unsigned long long)(v))
+#else
+inline unsigned long long ppm_add_u64(unsigned long long *p, unsigned long long v) {
+  unsigned long long o = *p;
+  *p = o + v;
+  return o;
+}
+#define PPM_ATOMIC_ADD_U64(p, v) ppm_add_u64((p), (unsigned long long)(v))
+#endif
+
 namespace ppm {
 
 constexpr double kTiny = 1e-20;        // fortran_types.def:7-12
@@ -107,7 +122,21 @@ struct SweepParams {
   unsigned fb_tag;  // or-ed into list entries (grid << 16 in a batched level)
   int fb_ok;        // a one-pencil workgroup holds the whole sweep axis
   int tp;           // pencils per group (the sweep tile's TP)
+  // learned two-shock start: network parameters (ppm_learn.hpp), read by
+  // the LEARN instantiation only
+  const float *learn;
+  // harvest of interface states: records of kHarvestDoubles doubles for
+  // one in `harvest_keep` owned interfaces (by a hash of their position),
+  // up to harvest_cap records; harvest_n counts them.  Off when null.
+  double *harvest;
+  unsigned long long *harvest_n;
+  long long harvest_cap;
+  unsigned harvest_keep;
 };
+
+// harvest record: dls, drs, pls, prs, uls, urs (the two-shock inputs),
+// p* (converged), starting pressure, Newton updates, restarted (0/1)
+constexpr int kHarvestDoubles = 10;
 
 // ---------------------------------------------------------------------------
 // LDS layout: NARR arrays of [TP][LW] doubles.
@@ -562,25 +591,18 @@ PPM_HD void ph_iface(const SweepParams &P, const Tile &T, double *sm, int tid, i
   }
 }
 
-// two-shock Riemann solver for one interface (twoshock.F:141-254)
-PPM_HD void twoshock1(double dls, double drs, double &pls, double &prs, double uls, double urs,
-                      double gamma, double pmin, int ipresfree, double &pbar, double &ubar) {
-  if (ipresfree == 1) {
-    pbar = pmin;
-    ubar = 0.5 * (uls + urs);
-    pls = pmin;
-    prs = pmin;
-    return;
-  }
+// Newton iteration of twoshock.F:141-254 for one interface from the start
+// ps (floored at pmin).  Returns the number of updates taken; 8 means the
+// seven updates did not meet the tolerance.
+PPM_HD int twoshock_newton(double dls, double drs, double pls, double prs, double uls, double urs,
+                           double cl, double cr, double gamma, double pmin, double ps, double &pbar,
+                           double &ubar) {
   const double qa = (gamma + 1.0) / (2.0 * gamma);
   const double tolerance = 1.e-14;
-  double cl = sqrt(gamma * pls * dls);
-  double cr = sqrt(gamma * prs * drs);
-  double ps = (cr * pls + cl * prs + cr * cl * (uls - urs)) / (cr + cl);
-  if (ps < pmin) ps = pmin;
   double old_ps = ps;
   double ubl = 0, ubr = 0, dpdul = 0, dpdur = 0;
-  for (int n = 2; n <= 8; n++) {
+  int n;
+  for (n = 2; n <= 8; n++) {
     double zl = cl * sqrt((1.0 + qa * (ps / pls - 1.0)));
     double zr = cr * sqrt((1.0 + qa * (ps / prs - 1.0)));
     ubl = uls - (ps - pls) / zl;
@@ -596,6 +618,71 @@ PPM_HD void twoshock1(double dls, double drs, double &pls, double &prs, double u
   if (ps < pmin) ps = fmn(pls, prs);
   pbar = ps;
   ubar = ubl + (ubr - ubl) * dpdur / (dpdur - dpdul);
+  return n - 1;
+}
+
+// two-shock Riemann solver for one interface (twoshock.F:141-254): the
+// reference's acoustic starting guess, then Newton.  ps0 receives the start;
+// returns the Newton updates taken.
+PPM_HD int twoshock1(double dls, double drs, double &pls, double &prs, double uls, double urs,
+                     double gamma, double pmin, int ipresfree, double &pbar, double &ubar,
+                     double &ps0) {
+  if (ipresfree == 1) {
+    pbar = pmin;
+    ubar = 0.5 * (uls + urs);
+    pls = pmin;
+    prs = pmin;
+    ps0 = pmin;
+    return 0;
+  }
+  double cl = sqrt(gamma * pls * dls);
+  double cr = sqrt(gamma * prs * drs);
+  double ps = (cr * pls + cl * prs + cr * cl * (uls - urs)) / (cr + cl);
+  if (ps < pmin) ps = pmin;
+  ps0 = ps;
+  return twoshock_newton(dls, drs, pls, prs, uls, urs, cl, cr, gamma, pmin, ps, pbar, ubar);
+}
+
+// The same solver started from the learned pressure (ppm_learn.hpp).  Where
+// the reference start is at the floor pmin (the acoustic estimate predicts
+// cavitation) it is kept: the network is not asked about vacuum states.
+// When Newton does not meet the tolerance from the learned start, the
+// reference start starts it again (restarted = 1), so a non-converging
+// interface gets the reference method's result.  Returns all updates taken.
+PPM_HD int twoshock_learned(const float *w, double dls, double drs, double &pls, double &prs,
+                            double uls, double urs, double gamma, double pmin, int ipresfree,
+                            double &pbar, double &ubar, double &ps0, int &restarted) {
+  restarted = 0;
+  if (ipresfree == 1) return twoshock1(dls, drs, pls, prs, uls, urs, gamma, pmin, ipresfree, pbar, ubar, ps0);
+  const double cl = sqrt(gamma * pls * dls);
+  const double cr = sqrt(gamma * prs * drs);
+  double ps_ref = (cr * pls + cl * prs + cr * cl * (uls - urs)) / (cr + cl);
+  if (ps_ref < pmin) ps_ref = pmin;
+  double ps = ps_ref;
+  if (ps_ref > pmin) {
+    float x[kMlpIn];
+    mlp_features(dls, drs, pls, prs, uls, urs, gamma, x);
+    ps = pls * (double)mlp_exp(mlp_eval(w, x));
+    if (!(ps >= pmin)) ps = pmin;  // NaN too
+  }
+  ps0 = ps;
+  int it = twoshock_newton(dls, drs, pls, prs, uls, urs, cl, cr, gamma, pmin, ps, pbar, ubar);
+  if (it > 7 && ps != ps_ref) {
+    restarted = 1;
+    it += twoshock_newton(dls, drs, pls, prs, uls, urs, cl, cr, gamma, pmin, ps_ref, pbar, ubar);
+  }
+  return it;
+}
+
+// one harvest record (see SweepParams::harvest) for interface i of pencil
+// (t1, t2), kept when a hash of its position falls on 0 mod harvest_keep
+PPM_HD void harvest_record(const SweepParams &P, int t1, int t2, int i, const double *v) {
+  const unsigned h = (unsigned)i * 73856093u ^ (unsigned)t1 * 19349663u ^ (unsigned)t2 * 83492791u ^
+                     (unsigned)P.dim * 2654435761u;
+  if (P.harvest_keep > 1 && (h % P.harvest_keep) != 0) return;
+  const unsigned long long k = PPM_ATOMIC_ADD_U64(P.harvest_n, 1);
+  if ((long long)k >= P.harvest_cap) return;
+  for (int m = 0; m < kHarvestDoubles; m++) P.harvest[k * kHarvestDoubles + m] = v[m];
 }
 
 // diffusion coefficient at interface i (calcdiss.F:207-236, 377-404);
@@ -692,7 +779,7 @@ PPM_HD void hll_flux(const HllIn &h, Regs &r) {
 // Phase: Eulerian corrections (inteuler.F:321-551), two-shock solve and
 // Godunov fluxes (flux_twoshock.F:84-396) for the thread's interface; in
 // fallback workgroups (FB) also the HLL fluxes of the interface.
-template <int TP, int TS, int UNI = 0, int FB = 0>
+template <int TP, int TS, int UNI = 0, int FB = 0, int LEARN = 0>
 PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, Regs &r) {
   int p = tid / TS, q = tid % TS;
   if constexpr (FB != 0) r.hll = 0;
@@ -780,8 +867,19 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
   r.dls = dls;
   r.drs = drs;
   // two-shock solution
-  double pbar, ubar;
-  twoshock1(dls, drs, pls, prs, uls, urs, gamma, P.pmin, P.ipresfree, pbar, ubar);
+  double pbar, ubar, ps0;
+  const double pls_in = pls, prs_in = prs;
+  int iters, restarted = 0;
+  if constexpr (LEARN != 0)
+    iters = twoshock_learned(P.learn, dls, drs, pls, prs, uls, urs, gamma, P.pmin, P.ipresfree,
+                             pbar, ubar, ps0, restarted);
+  else
+    iters = twoshock1(dls, drs, pls, prs, uls, urs, gamma, P.pmin, P.ipresfree, pbar, ubar, ps0);
+  if (FB == 0 && P.harvest && i >= T.o && i < T.o + T.w) {
+    const double v[kHarvestDoubles] = {dls, drs, pls_in, prs_in, uls, urs, pbar, ps0,
+                                       (double)iters, (double)restarted};
+    harvest_record(P, T.t1_0 + p, T.t2, i, v);
+  }
   // Godunov state (flux_twoshock.F:84-218)
   const double qa = (gamma + 1.0) / (2.0 * gamma);
   double sn = sgn1(-ubar);
@@ -1280,7 +1378,7 @@ struct TileCounts {
   int bad, chains, err, nfb;
 };
 
-template <int TP, int TS, int UNI, int FB, class X>
+template <int TP, int TS, int UNI, int FB, class X, int LEARN = 0>
 PPM_HD TileCounts sweep_body(const SweepParams &P, const Tile &T, double *sm, X &ex) {
   TileCounts tc;
   tc.nfb = 0;
@@ -1299,7 +1397,7 @@ PPM_HD TileCounts sweep_body(const SweepParams &P, const Tile &T, double *sm, X 
     ex.run([&](int tid, Regs &r) { ph_iface<TP, TS, UNI>(P, T, sm, tid, v, r); });
   }
   ex.run([&](int tid, Regs &r) {
-    ph_solve<TP, TS, UNI, FB>(P, T, sm, tid, r);
+    ph_solve<TP, TS, UNI, FB, LEARN>(P, T, sm, tid, r);
     ph_store_flux<TP, TS, UNI, FB>(P, T, sm, tid, r);
   });
   tc.bad = ex.count([&](int tid, Regs &r) { (void)r; return ph_check<TP, TS, UNI>(P, T, sm, tid); });
@@ -1339,16 +1437,16 @@ PPM_HD TileCounts sweep_body(const SweepParams &P, const Tile &T, double *sm, X 
 // A tile of the main launch.  bad > 0 means some cell of the tile needs the
 // fallback (or, without it, is an error); its pencils are then redone by a
 // fallback workgroup, which also settles the errors of those pencils.
-template <int TP, int TS, int UNI, class X>
+template <int TP, int TS, int UNI, class X, int LEARN = 0>
 PPM_HD TileCounts sweep_tile(const SweepParams &P, int block, double *sm, X &ex) {
-  return sweep_body<TP, TS, UNI, 0>(P, make_tile<TP, TS, UNI>(P, block), sm, ex);
+  return sweep_body<TP, TS, UNI, 0, X, LEARN>(P, make_tile<TP, TS, UNI>(P, block), sm, ex);
 }
 
 // One whole pencil in a one-pencil workgroup of TS threads, with the
 // in-order per-cell fallback; sm holds NARR_FB arrays of TS + 4 doubles.
-template <int TS, int UNI, class X>
+template <int TS, int UNI, class X, int LEARN = 0>
 PPM_HD TileCounts sweep_pencil_fb(const SweepParams &P, int t1, int t2, double *sm, X &ex) {
-  return sweep_body<1, TS, UNI, 1>(P, make_pencil_tile(P, t1, t2), sm, ex);
+  return sweep_body<1, TS, UNI, 1, X, LEARN>(P, make_pencil_tile(P, t1, t2), sm, ex);
 }
 
 // whether a one-pencil tile of TS threads holds the whole sweep axis

@@ -178,7 +178,18 @@ PROTOTYPES = {
     "hipPPMLevelSetShape": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int64]),
     "hipPPMLevelCommitShapes": (C.c_int, [C.c_void_p]),
     "hipPPMLevelTune": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_int64, PI]),
+    "hipPPMLevelSetLearnedGuess": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hipPPMLevelDestroy": (C.c_int, [C.c_void_p]),
+    "hipPPMSetLearnedGuess": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hipPPMSetHarvest": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64]),
+    "hipPPMGetHarvest": (C.c_int, [C.c_void_p, PF, C.c_int64, PI, PI]),
+    "hipPPMLearnTrain": (C.c_int, [PF, C.c_int64, C.c_double, C.c_int64, C.c_int64, C.c_double,
+                                   C.c_int64, C.c_void_p, PF]),
+    "hipPPMLearnGradient": (C.c_int, [PF, C.c_int64, C.c_double, C.c_void_p, PF, PF]),
+    "hipPPMLearnPredict": (C.c_int, [PF, C.c_int64, C.c_double, C.c_void_p, C.c_void_p, PF]),
+    "hipPPMLearnNewtonStats": (C.c_int, [PF, C.c_int64, C.c_double, C.c_double, C.c_void_p, PF]),
+    "hipPPMLearnParameterCount": (C.c_int, []),
+    "hipPPMLearnLastError": (C.c_char_p, []),
     "hipPPMLevelLastError": (C.c_char_p, []),
 }
 

@@ -101,6 +101,30 @@ class DeviceGrid:
     def fallback_count(self):
         return self.lib.hipPPMFallbackCount(self.handle)
 
+    def set_learned(self, weights):
+        """Learned two-shock start from the next sweep on (float32 weights of
+        learn.N_FLOATS values); None restores the reference start."""
+        if weights is None:
+            _check(self.lib.hipPPMSetLearnedGuess(self.handle, None), "SetLearnedGuess")
+            return
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        _check(self.lib.hipPPMSetLearnedGuess(self.handle, w.ctypes.data_as(C.c_void_p)),
+               "SetLearnedGuess")
+
+    def set_harvest(self, capacity, keep=1):
+        """Record the two-shock inputs and result of one owned interface in
+        `keep`, up to `capacity` records (0 switches the harvest off)."""
+        _check(self.lib.hipPPMSetHarvest(self.handle, int(capacity), int(keep)), "SetHarvest")
+
+    def get_harvest(self, max_records):
+        """(records[n, 10] float64, interfaces selected since the last call)."""
+        out = np.zeros((max(1, int(max_records)), 10))
+        seen = C.c_int64(0)
+        stored = C.c_int64(0)
+        _check(self.lib.hipPPMGetHarvest(self.handle, abi.dptr(out), int(max_records),
+                                         C.byref(seen), C.byref(stored)), "GetHarvest")
+        return out[:stored.value].copy(), seen.value
+
     def close(self):
         if self.handle.value:
             self.lib.hipPPMDestroy(self.handle)

@@ -7,6 +7,8 @@ library is missing or a call fails.
 """
 import ctypes as C
 
+import numpy as np
+
 from . import abi
 from .grid import make_globals
 
@@ -71,6 +73,15 @@ class Level:
         out = (C.c_int64 * 4)()
         self._check(self.lib.hipPPMLevelTune(self.handle, cycle, dt, repeats, out), "LevelTune")
         return list(out)
+
+    def set_learned(self, weights):
+        """Learned two-shock start for every grid (None: reference start)."""
+        if weights is None:
+            self._check(self.lib.hipPPMLevelSetLearnedGuess(self.handle, None), "LevelSetLearned")
+            return
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        self._check(self.lib.hipPPMLevelSetLearnedGuess(self.handle, w.ctypes.data_as(C.c_void_p)),
+                    "LevelSetLearned")
 
     def close(self):
         if self.handle.value:

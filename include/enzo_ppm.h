@@ -250,6 +250,60 @@ int hipPPMProfileRead(hipPPMData *Data, enzo_float *total_ms, enzo_long_int *lau
 /* Wait for all work queued on the handle's stream. */
 int hipPPMSynchronize(hipPPMData *Data);
 
+/* ---- learned two-shock starting pressure -----------------------------------
+   No counterpart in the reference: twoshock.F:141-254 always starts Newton
+   from the acoustic estimate.  These entries serve callers that want the
+   learned start: a 6 -> 32 -> 32 -> 1 FP32 tanh network predicts
+   log(p* / p_left) per interface; Newton then runs unchanged to the same
+   tolerance from that start and restarts from the acoustic estimate when it
+   does not converge.  Weights are hipPPMLearnParameterCount() floats:
+   W1[32][6] b1[32] W2[32][32] b2[32] W3[32] b3, input mean[6], 1/std[6].
+
+   hipPPMSetLearnedGuess: use the network from the next sweep on (copied;
+   weights NULL switches back to the reference start). */
+int hipPPMSetLearnedGuess(hipPPMData *Data, const float *weights);
+
+/* Harvest of interface states: the sweeps record, for one owned interface in
+   `keep` (by a hash of its position), the two-shock inputs and result as
+   10 doubles {rho_l, rho_r, p_l, p_r, u_l, u_r, p*, starting pressure,
+   Newton updates, restarted}, up to `capacity` records (0 switches it off).
+   hipPPMGetHarvest copies up to max_records of them to the host, returns how
+   many interfaces were selected (`seen`, may exceed the capacity) and how
+   many were copied (`stored`), and restarts the count. */
+int hipPPMSetHarvest(hipPPMData *Data, enzo_long_int capacity, enzo_int keep);
+int hipPPMGetHarvest(hipPPMData *Data, enzo_float *records, enzo_long_int max_records,
+                     enzo_long_int *seen, enzo_long_int *stored);
+
+/* Training on the device (FP32, mean squared error of log(p* / p_left),
+   minibatch Adam): records as harvested (n >= 256), `epochs` passes of
+   shuffled minibatches of `batch` (>= 256) records.  The input
+   standardisation is set from the records; `weights` holds the initial
+   parameters on entry and the trained ones on return.  mse_per_epoch
+   (optional, `epochs` values) receives the loss after each pass. */
+int hipPPMLearnTrain(const enzo_float *records, enzo_long_int n, enzo_float gamma, enzo_int epochs,
+                     enzo_int batch, enzo_float lr, enzo_int seed, float *weights,
+                     enzo_float *mse_per_epoch);
+
+/* Full-batch gradient (hipPPMLearnParameterCount() - 12 values, the trained
+   parameters) and mean squared error, computed by the training kernels. */
+int hipPPMLearnGradient(const enzo_float *records, enzo_long_int n, enzo_float gamma,
+                        const float *weights, enzo_float *grad, enzo_float *mse);
+
+/* Network predictions log(p* / p_left) for n records and their mean squared
+   error. */
+int hipPPMLearnPredict(const enzo_float *records, enzo_long_int n, enzo_float gamma,
+                       const float *weights, float *predictions, enzo_float *mse);
+
+/* Per record, out[7] = {Newton updates from the reference start, updates
+   from the learned start (restart included), restarted, p* from the
+   reference start, p* from the learned start, the learned start, the
+   reference start}. */
+int hipPPMLearnNewtonStats(const enzo_float *records, enzo_long_int n, enzo_float gamma,
+                           enzo_float pmin, const float *weights, enzo_float *out);
+
+int hipPPMLearnParameterCount(void);
+const char *hipPPMLearnLastError(void);
+
 /* Library version string and device count check. */
 const char *hipPPMVersion(void);
 
@@ -308,6 +362,10 @@ int hipPPMLevelCommitShapes(hipPPMLevel *Level);
    shape_by_length[4] receives the choices. */
 int hipPPMLevelTune(hipPPMLevel *Level, enzo_int CycleNumber, enzo_float dt, enzo_int repeats,
                     enzo_int *shape_by_length);
+
+/* The learned two-shock start for every grid of the level (see
+   hipPPMSetLearnedGuess; NULL switches back to the reference start). */
+int hipPPMLevelSetLearnedGuess(hipPPMLevel *Level, const float *weights);
 
 int hipPPMLevelDestroy(hipPPMLevel *Level);
 

@@ -30,6 +30,7 @@
 #include <map>
 #include <vector>
 
+#define HIPCPU_RUNTIME 1
 #define __global__
 #define __device__
 #define __host__

@@ -1,7 +1,30 @@
 """Shared helpers of the tests (comparisons and run loops)."""
+import os
+import subprocess
+
 import numpy as np
 
 import oracle
+from enzo_ppm import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CPU_LIB = os.path.join(HERE, "emu", "libenzo_ppm_cpu.so")
+CSRC = os.path.join(ROOT, "enzo-dev_amd", "csrc")
+HIP_UNITS = ["enzo_ppm.hip", "ppm_level.hip", "ppm_learn.hip"]
+
+
+def cpu_library():
+    """The library's own HIP source built for the CPU (tests/emu/hipcpu),
+    rebuilt when a source is newer; loaded through the same C ABI."""
+    src = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".hpp"))]
+    if not os.path.exists(CPU_LIB) or os.path.getmtime(CPU_LIB) < max(map(os.path.getmtime, src)):
+        subprocess.run(["g++", "-x", "c++", "-O1", "-g", "-std=c++17", "-fPIC", "-shared",
+                        "-Wno-attributes", "-U_FORTIFY_SOURCE", "-Wl,-Bsymbolic",
+                        "-ffp-contract=off", "-fno-fast-math", "-I",
+                        os.path.join(HERE, "emu", "hipcpu"), "-o", CPU_LIB]
+                       + [os.path.join(CSRC, f) for f in HIP_UNITS], check=True)
+    return abi.load_library(CPU_LIB)
 
 
 def bit_equal(a, b):

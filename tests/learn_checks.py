@@ -1,0 +1,123 @@
+"""Checks of the learned two-shock start shared by the CPU-library tests and
+the GPU tests (same C ABI, different library and sizes)."""
+import numpy as np
+
+import learn_oracle as lo
+import oracle
+from enzo_ppm import device, learn, problems
+from helpers import field_report, oracle_steps
+
+GAMMA = 5.0 / 3.0
+PMIN = 1e-20   # MinimumPressure of the problem generators' globals
+
+
+def harvest(lib, g, glob, steps=1, keep=1, cap=1 << 22, cfl=0.4):
+    """Records and the interfaces selected of `steps` harvested steps."""
+    return learn.harvest(g, glob, steps=steps, keep=keep, cap=cap, cfl=cfl, lib=lib)
+
+
+def swept_interfaces(g):
+    """Interfaces one full step solves: per axis (i2 - i1 + 2) per pencil,
+    every transverse pencil (ghosts included) swept."""
+    n = 0
+    for d in range(g.rank):
+        pencils = int(np.prod([g.dims[k] for k in range(3) if k != d]))
+        n += (g.active[d] + 1) * pencils
+    return n
+
+
+def check_harvest_pstar(records):
+    """The harvested p* is the reference twoshock.F's on the harvested
+    inputs, bit for bit (and the oracle port's)."""
+    kinds = ["port"] + (["ref"] if oracle.available("ref") else [])
+    for kind in kinds:
+        pbar, _ = lo.twoshock(kind, records, GAMMA, PMIN)
+        assert np.array_equal(pbar, records[:, 6]), kind
+
+
+def check_gradient(lib, records, seed=1):
+    rng = np.random.default_rng(seed)
+    w = learn.init_weights(seed)
+    X, Y = lo.features(records, GAMMA, fp32=True)   # the kernels' input rounding
+    w[lo.OFF_MEAN:lo.OFF_INV_STD] = X.mean(0)
+    w[lo.OFF_INV_STD:] = 1.0 / X.std(0)
+    w[:lo.N_PARAMS] += rng.normal(0, 0.05, lo.N_PARAMS).astype(np.float32)
+    g_dev, mse_dev = learn.gradient(records, GAMMA, w, lib=lib)
+    g_ref = lo.gradient(w.astype(np.float64), X, Y)
+    mse_ref = lo.loss(w.astype(np.float64), X, Y)
+    # FP32 forward/backward summed over the records: relative to the
+    # largest gradient component
+    err = np.max(np.abs(g_dev - g_ref)) / np.max(np.abs(g_ref))
+    assert err < 2e-4, err
+    assert abs(mse_dev - mse_ref) <= 1e-4 * mse_ref
+    y_dev, _ = learn.predict(records, GAMMA, w, lib=lib)
+    y_ref, _ = lo.forward(w.astype(np.float64), X)
+    assert np.max(np.abs(y_dev - y_ref)) < 1e-5 * max(1.0, np.max(np.abs(y_ref))), \
+        np.max(np.abs(y_dev - y_ref))
+    return err
+
+
+def reference_start_mse(records):
+    """Mean squared error of the reference (acoustic) start in the network's
+    clipped target, log(p_start / p_left), from plain-solver records."""
+    c = lo.Y_CLIP
+    return float(np.mean((np.clip(np.log(records[:, 7] / records[:, 2]), -c, c)
+                          - np.clip(np.log(records[:, 6] / records[:, 2]), -c, c)) ** 2))
+
+
+def check_newton_stats(lib, records, w):
+    out, summary = learn.newton_stats(records, GAMMA, PMIN, w, lib=lib)
+    # the reference start reproduces the sweep's own solve
+    assert np.array_equal(out[:, 3], records[:, 6])
+    assert np.array_equal(out[:, 0], records[:, 8])
+    assert np.array_equal(out[:, 6], records[:, 7])
+    # from the learned start Newton meets the same tolerance (1e-14 on the
+    # last update); a restart reproduces the reference solve exactly
+    rel = np.abs(out[:, 4] - out[:, 3]) / out[:, 3]
+    assert rel.max() < 1e-12, rel.max()
+    rs = out[:, 2] == 1
+    assert np.array_equal(out[rs, 4], out[rs, 3])
+    return summary
+
+
+def learned_vs_oracle(lib, g, glob, w, steps, cfl=0.4):
+    """`steps` device-resident steps with the learned start, and the same
+    steps on the oracle (each with its own Courant dt, both periodic).
+    Returns (oracle grid, learned grid)."""
+    ref = g.copy()
+    oracle_steps("port", ref, glob, steps, cfl)
+    h = g.copy()
+    dg = device.DeviceGrid(h, glob, lib=lib)
+    try:
+        dg.set_learned(w)
+        for cycle in range(steps):
+            dg.periodic_boundary()
+            dg.pressure_and_dt(cfl)
+            dg.step(cycle)
+        assert dg.download() == 1
+    finally:
+        dg.close()
+    return ref, h
+
+
+def l1_rel(a, b):
+    return float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+
+
+def check_learned_run(ref, learned, one_step):
+    """One step: max relative difference <= 1e-11 in every field; any
+    number of steps: L1 relative difference <= 1e-9 per field, and the
+    same total mass to round-off.  Returns the per-field L1 differences."""
+    rep = field_report(learned, ref)
+    if one_step:
+        assert all(r[2] <= 1e-11 for r in rep), rep
+    l1 = [l1_rel(x, y) for x, y in zip(learned.fields, ref.fields)]
+    assert max(l1) <= 1e-9, l1
+    sl = learned.active_slice()
+    m0, m1 = ref.fields[0][sl].sum(), learned.fields[0][sl].sum()
+    assert abs(m1 - m0) <= 1e-13 * abs(m0)
+    return l1
+
+
+def turbulent(n, seed):
+    return problems.turbulent(n, seed=seed)

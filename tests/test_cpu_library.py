@@ -8,31 +8,18 @@ page, so an out-of-bounds index stops the test.  Bar: bitwise equality with
 the oracle, and its status and fallback count.
 """
 import ctypes as C
-import os
-import subprocess
 
 import pytest
 
 import oracle
-from enzo_ppm import abi, device, problems
+from enzo_ppm import device, problems
 from enzo_ppm.grid import SubgridFluxes
-from helpers import FALLBACK_CASES, bit_equal, fallback_case, field_report, fill_bc
-
-HERE = os.path.dirname(os.path.abspath(__file__))
-ROOT = os.path.dirname(HERE)
-LIB = os.path.join(HERE, "emu", "libenzo_ppm_cpu.so")
-SRC = [os.path.join(ROOT, "enzo-dev_amd", "csrc", f) for f in ("enzo_ppm.hip", "ppm_level.hip",
-                                                                "ppm_sweep.hpp", "ppm_geometry.hpp",
-                                                                "ppm_device.hpp")]
+from helpers import FALLBACK_CASES, bit_equal, cpu_library, fallback_case, field_report, fill_bc
 
 
 @pytest.fixture(scope="module")
 def cpulib():
-    if not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(f) for f in SRC):
-        subprocess.run(["g++", "-x", "c++", "-O1", "-g", "-std=c++17", "-fPIC", "-shared",
-                        "-Wno-attributes", "-U_FORTIFY_SOURCE", "-Wl,-Bsymbolic", "-ffp-contract=off", "-fno-fast-math", "-I",
-                        os.path.join(HERE, "emu", "hipcpu"), "-o", LIB, SRC[0], SRC[1]], check=True)
-    return abi.load_library(LIB)
+    return cpu_library()
 
 
 def run(lib, g, glob, cycle, fluxes=None):
