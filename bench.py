@@ -232,6 +232,7 @@ def learned_bench(lib, n, steps, warmup, epochs, cap):
     gamma = glob2["Gamma"]
     w, mse = learn.train(rec, gamma, epochs=epochs, batch=8192, lr=3e-3, seed=0, lib=lib)
     t2 = time.perf_counter()
+    w, gate_table = learn.fit_gate(rec[::16], gamma, 1e-20, w, lib=lib)
     n_train = len(rec)
     del rec
     gh, globh, _ = problems.turbulent(n, seed=4242)
@@ -245,7 +246,10 @@ def learned_bench(lib, n, steps, warmup, epochs, cap):
                                                        f"{n}^3 turbulent (1 step, 1 in {keep})",
                         "seconds": t1 - t0},
             "training": {"epochs": epochs, "batch": 8192, "lr": 3e-3, "seconds": t2 - t1,
-                         "mse_per_epoch": [float(x) for x in mse]},
+                         "mse_per_epoch": [float(x) for x in mse],
+                         "gate": float(w[learn.OFF_GATE]),
+                         "gate_choice": {str(k): {"mean_updates": v[0], "evaluated_fraction": v[1]}
+                                         for k, v in gate_table.items()}},
             "held_out_newton": stats, "grid_step_learned": t}, w
 
 

@@ -24,7 +24,8 @@ namespace {
 constexpr int kB = 256;  // samples per workgroup
 
 // records (kHarvestDoubles doubles each) -> features X[n][6], targets Y[n]
-// (log(p*/p_left), clipped to [-kYClip, kYClip])
+// (log(p* / p_ac), clipped to [-kYClip, kYClip]; NaN where the acoustic
+// start or p* is not positive -- training leaves those records out)
 __global__ void learn_features_kernel(const double *rec, long long n, double gamma, float *X,
                                       float *Y) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -33,7 +34,10 @@ __global__ void learn_features_kernel(const double *rec, long long n, double gam
   float x[kMlpIn];
   mlp_features(r[0], r[1], r[2], r[3], r[4], r[5], gamma, x);
   for (int k = 0; k < kMlpIn; k++) X[s * kMlpIn + k] = x[k];
-  Y[s] = fminf(fmaxf(mlp_log((float)(r[6] / r[2])), -kYClip), kYClip);
+  const double cl = sqrt(gamma * r[2] * r[0]), cr = sqrt(gamma * r[3] * r[1]);
+  const double pac = acoustic_start(r[0], r[1], r[2], r[3], r[4], r[5], cl, cr);
+  Y[s] = pac > 0.0 && r[6] > 0.0 ? fminf(fmaxf(mlp_log((float)(r[6] / pac)), -kYClip), kYClip)
+                                 : __builtin_nanf("");
 }
 
 // forward of one sample: standardised input, hidden layers, output
@@ -141,24 +145,33 @@ __global__ void learn_zero_kernel(float *p, int n) {
   if (i < n) p[i] = 0.0f;
 }
 
-// forward only: predictions into out (if non-null), squared errors summed
+// forward only: predictions into out (if non-null); squared errors over
+// the finite targets summed into acc[0], their number into acc[1]
 __global__ __launch_bounds__(kB) void learn_eval_kernel(const float *X, const float *Y, long long n,
-                                                        const float *w, float *out, float *sse) {
-  __shared__ float es[kB];
+                                                        const float *w, float *out, float *acc) {
+  __shared__ float es[kB], cs[kB];
   const long long s = (long long)blockIdx.x * kB + threadIdx.x;
-  float e2 = 0.0f;
+  float e2 = 0.0f, c = 0.0f;
   if (s < n) {
     float xn[kMlpIn], h1[kMlpHidden], h2[kMlpHidden];
     const float y = forward(w, X + s * kMlpIn, xn, h1, h2);
     if (out) out[s] = y;
-    e2 = (y - Y[s]) * (y - Y[s]);
+    if (Y[s] == Y[s]) {
+      e2 = (y - Y[s]) * (y - Y[s]);
+      c = 1.0f;
+    }
   }
   es[threadIdx.x] = e2;
+  cs[threadIdx.x] = c;
   __syncthreads();
   if (threadIdx.x == 0) {
-    float a = 0.0f;
-    for (int k = 0; k < kB; k++) a += es[k];
-    atomicAdd(sse, a);
+    float a = 0.0f, b = 0.0f;
+    for (int k = 0; k < kB; k++) {
+      a += es[k];
+      b += cs[k];
+    }
+    atomicAdd(&acc[0], a);
+    atomicAdd(&acc[1], b);
   }
 }
 
@@ -200,46 +213,80 @@ thread_local char g_learn_error[512] = "";
     }                                                                                      \
   } while (0)
 
-// device copies of the features and targets of n records
-struct Dataset {
-  float *X = nullptr, *Y = nullptr;
-  long long n = 0;
-  ~Dataset() {
-    if (X) (void)hipFree(X);
-    if (Y) (void)hipFree(Y);
+// device buffer freed at scope exit
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
   }
+  hipError_t alloc(size_t n) { return hipMalloc((void **)&p, sizeof(T) * (n ? n : 1)); }
 };
 
-int make_dataset(const enzo_float *records, long long n, double gamma, Dataset &D) {
-  double *drec = nullptr;
-  LNCHK(hipMalloc((void **)&drec, sizeof(double) * kHarvestDoubles * n));
-  int st = ENZO_PPM_SUCCESS;
-  if (hipMemcpy(drec, records, sizeof(double) * kHarvestDoubles * n, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMalloc((void **)&D.X, sizeof(float) * kMlpIn * n) != hipSuccess ||
-      hipMalloc((void **)&D.Y, sizeof(float) * n) != hipSuccess) {
-    st = ENZO_PPM_ERR_HIP;
-  } else {
-    D.n = n;
-    hipLaunchKernelGGL(learn_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0,
-                       drec, n, gamma, D.X, D.Y);
-    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) st = ENZO_PPM_ERR_HIP;
-  }
-  (void)hipFree(drec);
-  return st;
+// device features and targets of n records
+struct Dataset {
+  DevBuf<float> X, Y;
+  long long n = 0;
+};
+
+// a record the network is trained on: positive states, positive p* and a
+// positive acoustic start (cavitating interfaces keep the reference start)
+bool learnable(const double *r, double gamma) {
+  if (!(r[0] > 0.0 && r[1] > 0.0 && r[2] > 0.0 && r[3] > 0.0 && r[6] > 0.0)) return false;
+  const double cl = std::sqrt(gamma * r[2] * r[0]), cr = std::sqrt(gamma * r[3] * r[1]);
+  return acoustic_start(r[0], r[1], r[2], r[3], r[4], r[5], cl, cr) > 0.0;
 }
 
-// full-dataset mean squared error (and predictions) with parameters dw
-int evaluate(const Dataset &D, const float *dw, float *dout, double *mse) {
-  float *dsse = nullptr;
-  LNCHK(hipMalloc((void **)&dsse, sizeof(float)));
-  LNCHK(hipMemset(dsse, 0, sizeof(float)));
-  hipLaunchKernelGGL(learn_eval_kernel, dim3((unsigned)((D.n + kB - 1) / kB)), dim3(kB), 0, 0, D.X,
-                     D.Y, D.n, dw, dout, dsse);
+// features and targets on the device; with `filter` only of the learnable
+// records (their order kept), else of all (targets of the others are
+// meaningless and left out of the error by evaluate())
+int make_dataset(const enzo_float *records, long long n, double gamma, bool filter, Dataset &D) {
+  std::vector<double> kept;
+  if (filter) {
+    long long m = 0;
+    for (long long i = 0; i < n; i++) m += learnable(records + i * kHarvestDoubles, gamma) ? 1 : 0;
+    if (m == 0) {
+      snprintf(g_learn_error, sizeof(g_learn_error), "no learnable records");
+      return ENZO_PPM_ERR_ARGUMENT;
+    }
+    if (m < n) {
+      kept.reserve((size_t)m * kHarvestDoubles);
+      for (long long i = 0; i < n; i++)
+        if (learnable(records + i * kHarvestDoubles, gamma))
+          kept.insert(kept.end(), records + i * kHarvestDoubles,
+                      records + (i + 1) * kHarvestDoubles);
+      records = kept.data();
+      n = m;
+    }
+  }
+  DevBuf<double> drec;
+  LNCHK(drec.alloc((size_t)kHarvestDoubles * n));
+  LNCHK(D.X.alloc((size_t)kMlpIn * n));
+  LNCHK(D.Y.alloc((size_t)n));
+  LNCHK(hipMemcpy(drec.p, records, sizeof(double) * kHarvestDoubles * n, hipMemcpyHostToDevice));
+  D.n = n;
+  hipLaunchKernelGGL(learn_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0,
+                     drec.p, n, gamma, D.X.p, D.Y.p);
   LNCHK(hipGetLastError());
-  float sse = 0.0f;
-  LNCHK(hipMemcpy(&sse, dsse, sizeof(float), hipMemcpyDeviceToHost));
-  (void)hipFree(dsse);
-  *mse = (double)sse / (double)std::max<long long>(1, D.n);
+  LNCHK(hipDeviceSynchronize());
+  return ENZO_PPM_SUCCESS;
+}
+
+// mean squared error over the records with a finite target (and the
+// predictions of all) with device parameters dw
+int evaluate(const Dataset &D, const float *dw, float *dout, double *mse) {
+  DevBuf<float> acc;  // {sum of squared errors, count}
+  LNCHK(acc.alloc(2));
+  LNCHK(hipMemset(acc.p, 0, sizeof(float) * 2));
+  hipLaunchKernelGGL(learn_eval_kernel, dim3((unsigned)((D.n + kB - 1) / kB)), dim3(kB), 0, 0,
+                     D.X.p, D.Y.p, D.n, dw, dout, acc.p);
+  LNCHK(hipGetLastError());
+  float h[2] = {0.0f, 0.0f};
+  LNCHK(hipMemcpy(h, acc.p, sizeof(h), hipMemcpyDeviceToHost));
+  *mse = h[1] > 0.0f ? (double)h[0] / (double)h[1] : 0.0;
   return ENZO_PPM_SUCCESS;
 }
 
@@ -250,144 +297,141 @@ extern "C" {
 int hipPPMLearnTrain(const enzo_float *records, enzo_long_int n, enzo_float gamma, enzo_int epochs,
                      enzo_int batch, enzo_float lr, enzo_int seed, float *weights,
                      enzo_float *mse_per_epoch) {
-  if (!records || n < kB || epochs < 0 || batch < kB || !weights || lr <= 0.0)
+  if (!records || n < 1 || epochs < 0 || batch < kB || !weights || !(lr > 0.0))
     return ENZO_PPM_ERR_ARGUMENT;
   Dataset D;
-  int st = make_dataset(records, n, gamma, D);
+  int st = make_dataset(records, n, gamma, true, D);
   if (st != ENZO_PPM_SUCCESS) return st;
-  // standardisation of the inputs: mean and standard deviation (on the host
-  // from the device-computed features; 6 numbers)
-  std::vector<float> hx((size_t)kMlpIn * n);
-  LNCHK(hipMemcpy(hx.data(), D.X, sizeof(float) * hx.size(), hipMemcpyDeviceToHost));
-  for (int k = 0; k < kMlpIn; k++) {
-    double s1 = 0.0, s2 = 0.0;
-    for (long long i = 0; i < n; i++) {
-      s1 += hx[i * kMlpIn + k];
-      s2 += (double)hx[i * kMlpIn + k] * hx[i * kMlpIn + k];
-    }
-    const double mu = s1 / n, var = std::max(s2 / n - mu * mu, 1e-12);
-    weights[kOffMean + k] = (float)mu;
-    weights[kOffInvStd + k] = (float)(1.0 / std::sqrt(var));
+  const long long m = D.n;
+  if (m < batch) {
+    snprintf(g_learn_error, sizeof(g_learn_error), "%lld learnable records, fewer than a batch", m);
+    return ENZO_PPM_ERR_ARGUMENT;
   }
-  float *dw = nullptr, *dg = nullptr, *dm = nullptr, *dv = nullptr, *dsse = nullptr;
-  int *didx = nullptr;
-  LNCHK(hipMalloc((void **)&dw, sizeof(float) * kMlpFloats));
-  LNCHK(hipMalloc((void **)&dg, sizeof(float) * kMlpParams));
-  LNCHK(hipMalloc((void **)&dm, sizeof(float) * kMlpParams));
-  LNCHK(hipMalloc((void **)&dv, sizeof(float) * kMlpParams));
-  LNCHK(hipMalloc((void **)&dsse, sizeof(float)));
-  LNCHK(hipMalloc((void **)&didx, sizeof(int) * n));
-  LNCHK(hipMemcpy(dw, weights, sizeof(float) * kMlpFloats, hipMemcpyHostToDevice));
-  LNCHK(hipMemset(dm, 0, sizeof(float) * kMlpParams));
-  LNCHK(hipMemset(dv, 0, sizeof(float) * kMlpParams));
-  std::vector<int> perm((size_t)n);
+  // standardisation of the inputs: mean and standard deviation of the
+  // device-computed features (6 numbers, on the host)
+  {
+    std::vector<float> hx((size_t)kMlpIn * m);
+    LNCHK(hipMemcpy(hx.data(), D.X.p, sizeof(float) * hx.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < kMlpIn; k++) {
+      double s1 = 0.0, s2 = 0.0;
+      for (long long i = 0; i < m; i++) {
+        s1 += hx[i * kMlpIn + k];
+        s2 += (double)hx[i * kMlpIn + k] * hx[i * kMlpIn + k];
+      }
+      const double mu = s1 / m, var = std::max(s2 / m - mu * mu, 1e-12);
+      weights[kOffMean + k] = (float)mu;
+      weights[kOffInvStd + k] = (float)(1.0 / std::sqrt(var));
+    }
+  }
+  DevBuf<float> dw, dg, dm, dv, dsse;
+  DevBuf<int> didx;
+  LNCHK(dw.alloc(kMlpFloats));
+  LNCHK(dg.alloc(kMlpParams));
+  LNCHK(dm.alloc(kMlpParams));
+  LNCHK(dv.alloc(kMlpParams));
+  LNCHK(dsse.alloc(1));
+  LNCHK(didx.alloc((size_t)m));
+  LNCHK(hipMemcpy(dw.p, weights, sizeof(float) * kMlpFloats, hipMemcpyHostToDevice));
+  LNCHK(hipMemset(dm.p, 0, sizeof(float) * kMlpParams));
+  LNCHK(hipMemset(dv.p, 0, sizeof(float) * kMlpParams));
+  std::vector<int> perm((size_t)m);
   std::iota(perm.begin(), perm.end(), 0);
   std::mt19937_64 rng((unsigned long long)seed);
   const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+  const long long steps_total = (long long)epochs * (m / batch);
   long long t = 0;
-  for (int ep = 0; ep < epochs && st == ENZO_PPM_SUCCESS; ep++) {
+  for (int ep = 0; ep < epochs; ep++) {
     std::shuffle(perm.begin(), perm.end(), rng);
-    if (hipMemcpy(didx, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
-      st = ENZO_PPM_ERR_HIP;
-      break;
-    }
-    for (long long b0 = 0; b0 + batch <= n; b0 += batch) {
+    LNCHK(hipMemcpy(didx.p, perm.data(), sizeof(int) * m, hipMemcpyHostToDevice));
+    for (long long b0 = 0; b0 + batch <= m; b0 += batch) {
       t++;
-      hipLaunchKernelGGL(learn_zero_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, 0, dg,
+      hipLaunchKernelGGL(learn_zero_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, 0, dg.p,
                          kMlpParams);
-      hipLaunchKernelGGL(learn_zero_kernel, dim3(1), dim3(64), 0, 0, dsse, 1);
+      hipLaunchKernelGGL(learn_zero_kernel, dim3(1), dim3(64), 0, 0, dsse.p, 1);
       hipLaunchKernelGGL(learn_grad_kernel, dim3((unsigned)((batch + kB - 1) / kB)), dim3(kB), 0, 0,
-                         D.X, D.Y, didx, b0, (long long)batch, dw, 1.0f / (float)batch, dg, dsse);
+                         D.X.p, D.Y.p, didx.p, b0, (long long)batch, dw.p, 1.0f / (float)batch,
+                         dg.p, dsse.p);
       const float c1 = 1.0f - std::pow(beta1, (float)t), c2 = 1.0f - std::pow(beta2, (float)t);
-      hipLaunchKernelGGL(learn_adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, 0, dw, dg,
-                         dm, dv, (float)lr, beta1, beta2, eps, c1, c2);
+      // learning rate decayed linearly to a tenth over the run
+      const float lr_t = (float)(lr * (1.0 - 0.9 * (double)(t - 1) / (double)steps_total));
+      hipLaunchKernelGGL(learn_adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, 0, dw.p,
+                         dg.p, dm.p, dv.p, lr_t, beta1, beta2, eps, c1, c2);
     }
-    if (hipGetLastError() != hipSuccess) st = ENZO_PPM_ERR_HIP;
-    if (st == ENZO_PPM_SUCCESS && mse_per_epoch) {
+    LNCHK(hipGetLastError());
+    if (mse_per_epoch) {
       double mse = 0.0;
-      st = evaluate(D, dw, nullptr, &mse);
+      st = evaluate(D, dw.p, nullptr, &mse);
+      if (st != ENZO_PPM_SUCCESS) return st;
       mse_per_epoch[ep] = mse;
     }
   }
-  if (st == ENZO_PPM_SUCCESS &&
-      hipMemcpy(weights, dw, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost) != hipSuccess)
-    st = ENZO_PPM_ERR_HIP;
-  (void)hipFree(dw);
-  (void)hipFree(dg);
-  (void)hipFree(dm);
-  (void)hipFree(dv);
-  (void)hipFree(dsse);
-  (void)hipFree(didx);
-  return st;
+  LNCHK(hipMemcpy(weights, dw.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+  return ENZO_PPM_SUCCESS;
 }
 
-// full-batch gradient of the mean squared error, for checking the backward
-// kernel (grad: kMlpParams doubles, in the parameter order of ppm_learn.hpp)
+// full-batch gradient of the mean squared error over the learnable records,
+// for checking the backward kernel (grad: kMlpParams doubles, in the
+// parameter order of ppm_learn.hpp)
 int hipPPMLearnGradient(const enzo_float *records, enzo_long_int n, enzo_float gamma,
                         const float *weights, enzo_float *grad, enzo_float *mse) {
   if (!records || n < 1 || !weights || !grad) return ENZO_PPM_ERR_ARGUMENT;
   Dataset D;
-  int st = make_dataset(records, n, gamma, D);
+  int st = make_dataset(records, n, gamma, true, D);
   if (st != ENZO_PPM_SUCCESS) return st;
-  float *dw = nullptr, *dg = nullptr, *dsse = nullptr;
-  LNCHK(hipMalloc((void **)&dw, sizeof(float) * kMlpFloats));
-  LNCHK(hipMalloc((void **)&dg, sizeof(float) * kMlpParams));
-  LNCHK(hipMalloc((void **)&dsse, sizeof(float)));
-  LNCHK(hipMemcpy(dw, weights, sizeof(float) * kMlpFloats, hipMemcpyHostToDevice));
-  LNCHK(hipMemset(dg, 0, sizeof(float) * kMlpParams));
-  LNCHK(hipMemset(dsse, 0, sizeof(float)));
-  hipLaunchKernelGGL(learn_grad_kernel, dim3((unsigned)((n + kB - 1) / kB)), dim3(kB), 0, 0, D.X, D.Y,
-                     (const int *)nullptr, 0ll, (long long)n, dw, 1.0f / (float)n, dg, dsse);
+  DevBuf<float> dw, dg, dsse;
+  LNCHK(dw.alloc(kMlpFloats));
+  LNCHK(dg.alloc(kMlpParams));
+  LNCHK(dsse.alloc(1));
+  LNCHK(hipMemcpy(dw.p, weights, sizeof(float) * kMlpFloats, hipMemcpyHostToDevice));
+  LNCHK(hipMemset(dg.p, 0, sizeof(float) * kMlpParams));
+  LNCHK(hipMemset(dsse.p, 0, sizeof(float)));
+  hipLaunchKernelGGL(learn_grad_kernel, dim3((unsigned)((D.n + kB - 1) / kB)), dim3(kB), 0, 0,
+                     D.X.p, D.Y.p, (const int *)nullptr, 0ll, D.n, dw.p, 1.0f / (float)D.n, dg.p,
+                     dsse.p);
   LNCHK(hipGetLastError());
   std::vector<float> g(kMlpParams);
   float sse = 0.0f;
-  LNCHK(hipMemcpy(g.data(), dg, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
-  LNCHK(hipMemcpy(&sse, dsse, sizeof(float), hipMemcpyDeviceToHost));
+  LNCHK(hipMemcpy(g.data(), dg.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+  LNCHK(hipMemcpy(&sse, dsse.p, sizeof(float), hipMemcpyDeviceToHost));
   for (int p = 0; p < kMlpParams; p++) grad[p] = g[p];
-  if (mse) *mse = (double)sse / (double)n;
-  (void)hipFree(dw);
-  (void)hipFree(dg);
-  (void)hipFree(dsse);
+  if (mse) *mse = (double)sse / (double)D.n;
   return ENZO_PPM_SUCCESS;
 }
 
-// predictions y = log(p* / p_left) of the network for n records
+// network outputs (the correction log(p* / p_ac)) for n records, and their
+// mean squared error over the learnable ones
 int hipPPMLearnPredict(const enzo_float *records, enzo_long_int n, enzo_float gamma,
                        const float *weights, float *y, enzo_float *mse) {
   if (!records || n < 1 || !weights) return ENZO_PPM_ERR_ARGUMENT;
   Dataset D;
-  int st = make_dataset(records, n, gamma, D);
+  int st = make_dataset(records, n, gamma, false, D);
   if (st != ENZO_PPM_SUCCESS) return st;
-  float *dw = nullptr, *dy = nullptr;
-  LNCHK(hipMalloc((void **)&dw, sizeof(float) * kMlpFloats));
-  LNCHK(hipMalloc((void **)&dy, sizeof(float) * n));
-  LNCHK(hipMemcpy(dw, weights, sizeof(float) * kMlpFloats, hipMemcpyHostToDevice));
+  DevBuf<float> dw, dy;
+  LNCHK(dw.alloc(kMlpFloats));
+  LNCHK(dy.alloc((size_t)n));
+  LNCHK(hipMemcpy(dw.p, weights, sizeof(float) * kMlpFloats, hipMemcpyHostToDevice));
   double m = 0.0;
-  st = evaluate(D, dw, dy, &m);
-  if (st == ENZO_PPM_SUCCESS && y) LNCHK(hipMemcpy(y, dy, sizeof(float) * n, hipMemcpyDeviceToHost));
+  st = evaluate(D, dw.p, dy.p, &m);
+  if (st != ENZO_PPM_SUCCESS) return st;
+  if (y) LNCHK(hipMemcpy(y, dy.p, sizeof(float) * n, hipMemcpyDeviceToHost));
   if (mse) *mse = m;
-  (void)hipFree(dw);
-  (void)hipFree(dy);
-  return st;
+  return ENZO_PPM_SUCCESS;
 }
 
 int hipPPMLearnNewtonStats(const enzo_float *records, enzo_long_int n, enzo_float gamma,
                            enzo_float pmin, const float *weights, enzo_float *out) {
   if (!records || n < 1 || !weights || !out) return ENZO_PPM_ERR_ARGUMENT;
-  double *drec = nullptr, *dout = nullptr;
-  float *dw = nullptr;
-  LNCHK(hipMalloc((void **)&drec, sizeof(double) * kHarvestDoubles * n));
-  LNCHK(hipMalloc((void **)&dout, sizeof(double) * 7 * n));
-  LNCHK(hipMalloc((void **)&dw, sizeof(float) * kMlpFloats));
-  LNCHK(hipMemcpy(drec, records, sizeof(double) * kHarvestDoubles * n, hipMemcpyHostToDevice));
-  LNCHK(hipMemcpy(dw, weights, sizeof(float) * kMlpFloats, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(learn_newton_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, drec,
-                     (long long)n, gamma, pmin, dw, dout);
+  DevBuf<double> drec, dout;
+  DevBuf<float> dw;
+  LNCHK(drec.alloc((size_t)kHarvestDoubles * n));
+  LNCHK(dout.alloc((size_t)7 * n));
+  LNCHK(dw.alloc(kMlpFloats));
+  LNCHK(hipMemcpy(drec.p, records, sizeof(double) * kHarvestDoubles * n, hipMemcpyHostToDevice));
+  LNCHK(hipMemcpy(dw.p, weights, sizeof(float) * kMlpFloats, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(learn_newton_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0,
+                     drec.p, (long long)n, gamma, pmin, dw.p, dout.p);
   LNCHK(hipGetLastError());
-  LNCHK(hipMemcpy(out, dout, sizeof(double) * 7 * n, hipMemcpyDeviceToHost));
-  (void)hipFree(drec);
-  (void)hipFree(dout);
-  (void)hipFree(dw);
+  LNCHK(hipMemcpy(out, dout.p, sizeof(double) * 7 * n, hipMemcpyDeviceToHost));
   return ENZO_PPM_SUCCESS;
 }
 

@@ -1,8 +1,13 @@
 // ppm_learn.hpp -- the learned starting pressure of the two-shock Newton
 // iteration: a 6 -> 32 -> 32 -> 1 FP32 network with tanh hidden layers that
-// predicts y = log(p* / p_left) from the interface states.  Shared by the
-// sweep kernel (inference per interface), the training kernels
-// (ppm_learn.hip) and the CPU builds.
+// predicts the correction c = log(p* / p_ac) to the reference's acoustic
+// start p_ac (twoshock.F:156-170), so that log(p_ac / p_left) + c predicts
+// log(p* / p_left).  The start is p_ac e^c at interfaces whose acoustic wave
+// strength passes the gate (a calibrated threshold stored with the
+// weights), p_ac elsewhere: for weak waves the acoustic start's error,
+// second order in the strength, is already below the network's own.
+// Shared by the sweep kernel (inference per interface), the training
+// kernels (ppm_learn.hip) and the CPU builds.
 //
 // Inputs (dimensionless, so the guess scales with the states):
 //   x0 = log(p_r / p_l), x1 = log(rho_r / rho_l),
@@ -19,7 +24,8 @@
 namespace ppm {
 
 constexpr int kMlpIn = 6, kMlpHidden = 32;
-// parameter block (floats): W1[H][IN] b1[H] W2[H][H] b2[H] W3[H] b3, mean[IN], inv_std[IN]
+// parameter block (floats): W1[H][IN] b1[H] W2[H][H] b2[H] W3[H] b3, mean[IN], inv_std[IN],
+// gate
 constexpr int kOffW1 = 0;
 constexpr int kOffB1 = kOffW1 + kMlpHidden * kMlpIn;
 constexpr int kOffW2 = kOffB1 + kMlpHidden;
@@ -29,9 +35,10 @@ constexpr int kOffB3 = kOffW3 + kMlpHidden;
 constexpr int kMlpParams = kOffB3 + 1;  // trained parameters: 1313
 constexpr int kOffMean = kMlpParams;
 constexpr int kOffInvStd = kOffMean + kMlpIn;
-constexpr int kMlpFloats = kOffInvStd + kMlpIn;
+constexpr int kOffGate = kOffInvStd + kMlpIn;  // wave-strength threshold for the network
+constexpr int kMlpFloats = kOffGate + 1;
 // training targets are clipped to [-kYClip, kYClip]: p* at the pressure
-// floor gives log(p*/p_left) near -46 and would dominate the squared error
+// floor gives corrections near -46 and would dominate the squared error
 constexpr float kYClip = 8.0f;
 
 // FP32 exponential and logarithm of the network: the hardware's on the
@@ -61,6 +68,15 @@ PPM_HD float mlp_tanh(float a) {
 #else
   return 1.0f - 2.0f / (1.0f + expf(2.0f * a));
 #endif
+}
+
+// the reference's acoustic start (twoshock.F:156-170) before the floor,
+// cl and cr being the Lagrangian wave speeds sqrt(gamma p rho)
+PPM_HD double acoustic_start(double dl, double dr, double pl, double pr, double ul, double ur,
+                             double cl, double cr) {
+  (void)dl;
+  (void)dr;
+  return (cr * pl + cl * pr + cr * cl * (ul - ur)) / (cr + cl);
 }
 
 // features of one interface (FP32 after the FP64 ratios)

@@ -643,12 +643,15 @@ PPM_HD int twoshock1(double dls, double drs, double &pls, double &prs, double ul
   return twoshock_newton(dls, drs, pls, prs, uls, urs, cl, cr, gamma, pmin, ps, pbar, ubar);
 }
 
-// The same solver started from the learned pressure (ppm_learn.hpp).  Where
-// the reference start is at the floor pmin (the acoustic estimate predicts
-// cavitation) it is kept: the network is not asked about vacuum states.
-// When Newton does not meet the tolerance from the learned start, the
-// reference start starts it again (restarted = 1), so a non-converging
-// interface gets the reference method's result.  Returns all updates taken.
+// The same solver started from the learned pressure (ppm_learn.hpp): the
+// acoustic start p_ac times e^c, the network's correction, at interfaces
+// whose acoustic wave strength passes the gate g (|p_ac - p_l| > g p_l or
+// |p_ac - p_r| > g p_r); elsewhere p_ac, whose error is second order in
+// that strength, is kept and the network is not evaluated.  Where p_ac is
+// at the floor pmin (it predicts cavitation) it is kept too.  When Newton
+// does not meet the tolerance from the learned start, p_ac starts it again
+// (restarted = 1), so a non-converging interface gets the reference
+// method's result.  Returns all updates taken.
 PPM_HD int twoshock_learned(const float *w, double dls, double drs, double &pls, double &prs,
                             double uls, double urs, double gamma, double pmin, int ipresfree,
                             double &pbar, double &ubar, double &ps0, int &restarted) {
@@ -656,13 +659,14 @@ PPM_HD int twoshock_learned(const float *w, double dls, double drs, double &pls,
   if (ipresfree == 1) return twoshock1(dls, drs, pls, prs, uls, urs, gamma, pmin, ipresfree, pbar, ubar, ps0);
   const double cl = sqrt(gamma * pls * dls);
   const double cr = sqrt(gamma * prs * drs);
-  double ps_ref = (cr * pls + cl * prs + cr * cl * (uls - urs)) / (cr + cl);
+  double ps_ref = acoustic_start(dls, drs, pls, prs, uls, urs, cl, cr);
   if (ps_ref < pmin) ps_ref = pmin;
   double ps = ps_ref;
-  if (ps_ref > pmin) {
+  const double g = w[kOffGate];
+  if (ps_ref > pmin && (fabs(ps_ref - pls) > g * pls || fabs(ps_ref - prs) > g * prs)) {
     float x[kMlpIn];
     mlp_features(dls, drs, pls, prs, uls, urs, gamma, x);
-    ps = pls * (double)mlp_exp(mlp_eval(w, x));
+    ps = ps_ref * (double)mlp_exp(mlp_eval(w, x));
     if (!(ps >= pmin)) ps = pmin;  // NaN too
   }
   ps0 = ps;

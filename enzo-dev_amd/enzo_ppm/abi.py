@@ -215,7 +215,7 @@ def load_library(path=None):
     if not os.path.exists(p):
         raise RuntimeError(
             f"{p} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
-    lib = C.CDLL(p, mode=C.RTLD_GLOBAL)
+    lib = C.CDLL(p)  # local binding: several builds of the ABI can share a process
     for name, (res, args) in PROTOTYPES.items():
         f = getattr(lib, name)
         f.restype = res

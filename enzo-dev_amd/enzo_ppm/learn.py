@@ -1,11 +1,14 @@
 """Learned two-shock starting pressure through the C ABI (hipPPMLearn*,
 include/enzo_ppm.h).
 
-The network is 6 -> 32 -> 32 -> 1, FP32, tanh hidden layers, predicting
-y = log(p* / p_left) from six dimensionless interface features; its inputs
-are standardised with the training set's mean and 1/std, which travel with
-the weights.  Training, gradients, predictions and the Newton statistics
-all run in the library's HIP kernels; there is no CPU path.
+The network is 6 -> 32 -> 32 -> 1, FP32, tanh hidden layers, predicting the
+correction c = log(p* / p_ac) to the reference's acoustic start p_ac from
+six dimensionless interface features; its inputs are standardised with the
+training set's mean and 1/std, and the start is p_ac e^c only where the
+acoustic wave strength passes the gate g (|p_ac - p_l| > g p_l or
+|p_ac - p_r| > g p_r) -- all of which travel with the weights.  Training,
+gradients, predictions and the Newton statistics run in the library's HIP
+kernels; there is no CPU path.
 
 Harvest records (DeviceGrid.get_harvest) are rows of 10 float64 values:
 rho_l, rho_r, p_l, p_r, u_l, u_r, p*, starting pressure, Newton updates,
@@ -27,7 +30,8 @@ OFF_B3 = OFF_W3 + N_HIDDEN
 N_PARAMS = OFF_B3 + 1            # trained parameters
 OFF_MEAN = N_PARAMS
 OFF_INV_STD = OFF_MEAN + N_IN
-N_FLOATS = OFF_INV_STD + N_IN    # whole block
+OFF_GATE = OFF_INV_STD + N_IN    # |c| threshold for using the network
+N_FLOATS = OFF_GATE + 1          # whole block
 RECORD = 10
 
 
@@ -55,7 +59,7 @@ def _fptr(w):
 
 
 def init_weights(seed=0):
-    """Glorot-uniform weights, zero biases, identity standardisation."""
+    """Glorot-uniform weights, zero biases, identity standardisation, gate 0."""
     rng = np.random.default_rng(seed)
     w = np.zeros(N_FLOATS, np.float32)
 
@@ -66,7 +70,7 @@ def init_weights(seed=0):
     w[OFF_W1:OFF_B1] = glorot(N_HIDDEN, N_IN)
     w[OFF_W2:OFF_B2] = glorot(N_HIDDEN, N_HIDDEN)
     w[OFF_W3:OFF_B3] = glorot(1, N_HIDDEN)
-    w[OFF_INV_STD:] = 1.0
+    w[OFF_INV_STD:OFF_GATE] = 1.0
     return w
 
 
@@ -97,7 +101,8 @@ def harvest(grid, glob, steps=1, keep=1, cap=1 << 22, cfl=0.4, lib=None, weights
 
 
 def train(records, gamma, epochs=4, batch=4096, lr=2e-3, seed=0, weights=None, lib=None):
-    """Minibatch Adam on the device.  Returns (weights, mse after each epoch)."""
+    """Minibatch Adam on the device.  Returns (weights, mse after each epoch);
+    the gate is left as given (0 from init_weights): see fit_gate."""
     L = _lib(lib)
     r = _records(records)
     w = np.array(init_weights(seed) if weights is None else weights, dtype=np.float32, copy=True)
@@ -121,7 +126,7 @@ def gradient(records, gamma, weights, lib=None):
 
 
 def predict(records, gamma, weights, lib=None):
-    """(predictions of log(p*/p_left) as float32, mean squared error)."""
+    """(network outputs c = log(p*/p_ac) as float32, mean squared error)."""
     L = _lib(lib)
     r = _records(records)
     w = np.ascontiguousarray(weights, dtype=np.float32)
@@ -141,7 +146,35 @@ def newton_stats(records, gamma, pmin, weights, lib=None):
     out = np.zeros((len(r), 7))
     _check(L, L.hipPPMLearnNewtonStats(abi.dptr(r), len(r), gamma, pmin, _fptr(w),
                                        abi.dptr(out)), "LearnNewtonStats")
-    return out, summarize(out)
+    s = summarize(out)
+    s["network_evaluated_fraction"] = float(evaluated(r, out, w[OFF_GATE], pmin).mean())
+    return out, s
+
+
+def evaluated(records, out, gate, pmin=1e-20):
+    """Which records pass the gate (the network is evaluated there): the
+    floored acoustic start out[:, 6] against the left and right pressures."""
+    p_ac, pl, pr = out[:, 6], records[:, 2], records[:, 3]
+    return (p_ac > pmin) & ((np.abs(p_ac - pl) > gate * pl) | (np.abs(p_ac - pr) > gate * pr))
+
+
+def fit_gate(records, gamma, pmin, weights,
+             candidates=(0.0, 1e-3, 3e-3, 1e-2, 3e-2, 0.1, 0.3, 1.0), rel_tol=0.01, lib=None):
+    """Gate choice on `records`: the largest candidate whose mean Newton
+    updates are within `rel_tol` of the fewest any candidate gives (nearly
+    the whole gain for the fewest network evaluations).  Returns (weights
+    with that gate, {gate: (mean updates, fraction evaluated)})."""
+    w = np.array(weights, dtype=np.float32, copy=True)
+    r = np.ascontiguousarray(records, dtype=np.float64)
+    table = {}
+    for c in candidates:
+        w[OFF_GATE] = c
+        out, s = newton_stats(r, gamma, pmin, w, lib=lib)
+        table[float(c)] = (s["newton_updates_learned_start"]["mean"],
+                           float(evaluated(r, out, np.float32(c), pmin).mean()))
+    best = min(v[0] for v in table.values())
+    w[OFF_GATE] = max(c for c, v in table.items() if v[0] <= best * (1.0 + rel_tol))
+    return w, table
 
 
 def summarize(out):

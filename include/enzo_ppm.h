@@ -252,12 +252,15 @@ int hipPPMSynchronize(hipPPMData *Data);
 
 /* ---- learned two-shock starting pressure -----------------------------------
    No counterpart in the reference: twoshock.F:141-254 always starts Newton
-   from the acoustic estimate.  These entries serve callers that want the
-   learned start: a 6 -> 32 -> 32 -> 1 FP32 tanh network predicts
-   log(p* / p_left) per interface; Newton then runs unchanged to the same
-   tolerance from that start and restarts from the acoustic estimate when it
-   does not converge.  Weights are hipPPMLearnParameterCount() floats:
-   W1[32][6] b1[32] W2[32][32] b2[32] W3[32] b3, input mean[6], 1/std[6].
+   from the acoustic estimate p_ac.  These entries serve callers that want
+   the learned start: a 6 -> 32 -> 32 -> 1 FP32 tanh network predicts the
+   correction c = log(p* / p_ac) per interface (so log(p_ac / p_left) + c
+   predicts log(p* / p_left)); Newton starts from p_ac e^c where the
+   acoustic wave strength passes the gate g (|p_ac - p_l| > g p_l or
+   |p_ac - p_r| > g p_r), from p_ac elsewhere, runs unchanged to the same
+   tolerance, and restarts from p_ac when it does not converge.  Weights are
+   hipPPMLearnParameterCount() floats: W1[32][6] b1[32] W2[32][32] b2[32]
+   W3[32] b3, input mean[6], 1/std[6], gate.
 
    hipPPMSetLearnedGuess: use the network from the next sweep on (copied;
    weights NULL switches back to the reference start). */
@@ -274,23 +277,26 @@ int hipPPMSetHarvest(hipPPMData *Data, enzo_long_int capacity, enzo_int keep);
 int hipPPMGetHarvest(hipPPMData *Data, enzo_float *records, enzo_long_int max_records,
                      enzo_long_int *seen, enzo_long_int *stored);
 
-/* Training on the device (FP32, mean squared error of log(p* / p_left),
-   minibatch Adam): records as harvested (n >= 256), `epochs` passes of
-   shuffled minibatches of `batch` (>= 256) records.  The input
-   standardisation is set from the records; `weights` holds the initial
-   parameters on entry and the trained ones on return.  mse_per_epoch
-   (optional, `epochs` values) receives the loss after each pass. */
+/* Training on the device (FP32, mean squared error of the correction
+   log(p* / p_ac) clipped to [-8, 8], minibatch Adam with the learning rate
+   decayed linearly from lr to lr / 10) on the records with
+   positive states, p* and p_ac: `epochs` passes of shuffled minibatches of
+   `batch` (>= 256) records.  The input standardisation is set from the
+   records; `weights` holds the initial parameters on entry and the trained
+   ones on return (the gate is left as given).  mse_per_epoch (optional,
+   `epochs` values) receives the loss after each pass. */
 int hipPPMLearnTrain(const enzo_float *records, enzo_long_int n, enzo_float gamma, enzo_int epochs,
                      enzo_int batch, enzo_float lr, enzo_int seed, float *weights,
                      enzo_float *mse_per_epoch);
 
-/* Full-batch gradient (hipPPMLearnParameterCount() - 12 values, the trained
-   parameters) and mean squared error, computed by the training kernels. */
+/* Full-batch gradient (hipPPMLearnParameterCount() - 13 values, the trained
+   parameters) and mean squared error over the same records as training,
+   computed by the training kernels. */
 int hipPPMLearnGradient(const enzo_float *records, enzo_long_int n, enzo_float gamma,
                         const float *weights, enzo_float *grad, enzo_float *mse);
 
-/* Network predictions log(p* / p_left) for n records and their mean squared
-   error. */
+/* Network outputs (the correction c) for all n records, and their mean
+   squared error over the records training would use. */
 int hipPPMLearnPredict(const enzo_float *records, enzo_long_int n, enzo_float gamma,
                        const float *weights, float *predictions, enzo_float *mse);
 

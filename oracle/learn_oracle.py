@@ -1,19 +1,20 @@
 """ORACLE -- test infrastructure only (tests/, smoke check, bench CPU baseline).
 
 Float64 numpy restatement of the learned two-shock start's network: the six
-interface features, the standardised 6 -> 32 -> 32 -> 1 tanh forward pass,
-the mean squared error of y = log(p* / p_left) and its gradient by
-hand-written backpropagation.  The library's FP32 HIP kernels
-(enzo-dev_amd/csrc/ppm_learn.hip) are checked against it.
+interface features, the target (the correction log(p* / p_ac) to the
+reference's acoustic start p_ac, twoshock.F:156-170, clipped), the
+standardised 6 -> 32 -> 32 -> 1 tanh forward pass, the mean squared error and
+its gradient by hand-written backpropagation.  The library's FP32 HIP
+kernels (enzo-dev_amd/csrc/ppm_learn.hip) are checked against it.
 
 The reference has no counterpart (twoshock.F:141-254 always starts from the
 acoustic estimate); what pins this restatement is the mathematics: the
 backward pass is checked against central finite differences of the forward
-pass (tests/test_learn_oracle.py), and the features against their
-closed forms on hand-computed states.
+pass (tests/test_learn_oracle.py), and the features and targets against
+their closed forms on hand-computed states.
 
 Weight block layout (float32 on the library side): W1[32][6] b1[32]
-W2[32][32] b2[32] W3[32] b3, mean[6], inv_std[6].
+W2[32][32] b2[32] W3[32] b3, mean[6], inv_std[6], gate.
 """
 import ctypes as C
 
@@ -31,14 +32,34 @@ OFF_B3 = OFF_W3 + N_HIDDEN
 N_PARAMS = OFF_B3 + 1
 OFF_MEAN = N_PARAMS
 OFF_INV_STD = OFF_MEAN + N_IN
-N_FLOATS = OFF_INV_STD + N_IN
+OFF_GATE = OFF_INV_STD + N_IN
+N_FLOATS = OFF_GATE + 1
 Y_CLIP = 8.0   # targets clipped to [-Y_CLIP, Y_CLIP]
+
+
+def acoustic_start(records, gamma):
+    """p_ac = (c_r p_l + c_l p_r + c_r c_l (u_l - u_r)) / (c_r + c_l) with the
+    Lagrangian wave speeds c = sqrt(gamma p rho) (twoshock.F:156-170), before
+    the floor."""
+    r = np.asarray(records, dtype=np.float64)
+    dl, dr, pl, pr, ul, ur = (r[:, k] for k in range(6))
+    cl = np.sqrt(gamma * pl * dl)
+    cr = np.sqrt(gamma * pr * dr)
+    return (cr * pl + cl * pr + cr * cl * (ul - ur)) / (cr + cl)
+
+
+def learnable(records, gamma):
+    """Records training uses: positive states, p* and acoustic start."""
+    r = np.asarray(records, dtype=np.float64)
+    ok = np.all(r[:, [0, 1, 2, 3, 6]] > 0, axis=1)
+    with np.errstate(invalid="ignore"):
+        return ok & (acoustic_start(r, gamma) > 0)
 
 
 def features(records, gamma, fp32=False):
     """X[n, 6]: log(p_r/p_l), log(rho_r/rho_l), (u_l-u_r)/c_l, (u_l-u_r)/c_r,
     c_l/(c_l+c_r), c_r/(c_l+c_r) with c = sqrt(gamma p / rho); targets
-    y = log(p*/p_l) clipped to [-Y_CLIP, Y_CLIP].
+    y = log(p*/p_ac) clipped to [-Y_CLIP, Y_CLIP] (records: learnable ones).
     records: rows (rho_l, rho_r, p_l, p_r, u_l, u_r, p*, ...).
     fp32: round as the library's kernels do -- ratios and quotients in
     float64, rounded to float32, logarithms taken in float32."""
@@ -47,22 +68,23 @@ def features(records, gamma, fp32=False):
     cl = np.sqrt(gamma * pl / dl)
     cr = np.sqrt(gamma * pr / dr)
     du = ul - ur
+    pac = acoustic_start(r, gamma)
     if fp32:
         f = np.float32
         X = np.stack([np.log((pr / pl).astype(f)), np.log((dr / dl).astype(f)),
                       (du / cl).astype(f), (du / cr).astype(f), (cl / (cl + cr)).astype(f),
                       (cr / (cl + cr)).astype(f)], axis=1).astype(np.float64)
-        return X, np.clip(np.log((ps / pl).astype(f)), -Y_CLIP, Y_CLIP).astype(np.float64)
+        return X, np.clip(np.log((ps / pac).astype(f)), -Y_CLIP, Y_CLIP).astype(np.float64)
     X = np.stack([np.log(pr / pl), np.log(dr / dl), du / cl, du / cr, cl / (cl + cr),
                   cr / (cl + cr)], axis=1)
-    return X, np.clip(np.log(ps / pl), -Y_CLIP, Y_CLIP)
+    return X, np.clip(np.log(ps / pac), -Y_CLIP, Y_CLIP)
 
 
 def unpack(w):
     w = np.asarray(w, dtype=np.float64)
     return (w[OFF_W1:OFF_B1].reshape(N_HIDDEN, N_IN), w[OFF_B1:OFF_W2],
             w[OFF_W2:OFF_B2].reshape(N_HIDDEN, N_HIDDEN), w[OFF_B2:OFF_W3], w[OFF_W3:OFF_B3],
-            w[OFF_B3], w[OFF_MEAN:OFF_INV_STD], w[OFF_INV_STD:N_FLOATS])
+            w[OFF_B3], w[OFF_MEAN:OFF_INV_STD], w[OFF_INV_STD:OFF_GATE])
 
 
 def forward(w, X):

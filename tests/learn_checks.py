@@ -38,9 +38,10 @@ def check_harvest_pstar(records):
 def check_gradient(lib, records, seed=1):
     rng = np.random.default_rng(seed)
     w = learn.init_weights(seed)
-    X, Y = lo.features(records, GAMMA, fp32=True)   # the kernels' input rounding
+    ok = lo.learnable(records, GAMMA)
+    X, Y = lo.features(records[ok], GAMMA, fp32=True)   # the kernels' input rounding
     w[lo.OFF_MEAN:lo.OFF_INV_STD] = X.mean(0)
-    w[lo.OFF_INV_STD:] = 1.0 / X.std(0)
+    w[lo.OFF_INV_STD:lo.OFF_GATE] = 1.0 / X.std(0)
     w[:lo.N_PARAMS] += rng.normal(0, 0.05, lo.N_PARAMS).astype(np.float32)
     g_dev, mse_dev = learn.gradient(records, GAMMA, w, lib=lib)
     g_ref = lo.gradient(w.astype(np.float64), X, Y)
@@ -50,19 +51,19 @@ def check_gradient(lib, records, seed=1):
     err = np.max(np.abs(g_dev - g_ref)) / np.max(np.abs(g_ref))
     assert err < 2e-4, err
     assert abs(mse_dev - mse_ref) <= 1e-4 * mse_ref
-    y_dev, _ = learn.predict(records, GAMMA, w, lib=lib)
+    y_dev, mse_pred = learn.predict(records, GAMMA, w, lib=lib)
     y_ref, _ = lo.forward(w.astype(np.float64), X)
-    assert np.max(np.abs(y_dev - y_ref)) < 1e-5 * max(1.0, np.max(np.abs(y_ref))), \
-        np.max(np.abs(y_dev - y_ref))
+    assert np.max(np.abs(y_dev[ok] - y_ref)) < 1e-5 * max(1.0, np.max(np.abs(y_ref))), \
+        np.max(np.abs(y_dev[ok] - y_ref))
+    assert abs(mse_pred - mse_ref) <= 1e-4 * mse_ref
     return err
 
 
 def reference_start_mse(records):
     """Mean squared error of the reference (acoustic) start in the network's
-    clipped target, log(p_start / p_left), from plain-solver records."""
-    c = lo.Y_CLIP
-    return float(np.mean((np.clip(np.log(records[:, 7] / records[:, 2]), -c, c)
-                          - np.clip(np.log(records[:, 6] / records[:, 2]), -c, c)) ** 2))
+    target, i.e. of the correction 0, over the learnable records."""
+    ok = lo.learnable(records, GAMMA)
+    return float(np.mean(lo.features(records[ok], GAMMA)[1] ** 2))
 
 
 def check_newton_stats(lib, records, w):
@@ -72,11 +73,14 @@ def check_newton_stats(lib, records, w):
     assert np.array_equal(out[:, 0], records[:, 8])
     assert np.array_equal(out[:, 6], records[:, 7])
     # from the learned start Newton meets the same tolerance (1e-14 on the
-    # last update); a restart reproduces the reference solve exactly
+    # last update); a restart reproduces the reference solve exactly; where
+    # the start is the reference one (gate, floor) so is everything
     rel = np.abs(out[:, 4] - out[:, 3]) / out[:, 3]
     assert rel.max() < 1e-12, rel.max()
     rs = out[:, 2] == 1
     assert np.array_equal(out[rs, 4], out[rs, 3])
+    same = out[:, 5] == out[:, 6]
+    assert np.array_equal(out[same, 4], out[same, 3]) and np.array_equal(out[same, 1], out[same, 0])
     return summary
 
 

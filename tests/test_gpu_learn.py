@@ -40,7 +40,10 @@ def harvested(lib):
 
 @pytest.fixture(scope="module")
 def trained(lib, harvested):
-    return learn.train(harvested, lc.GAMMA, epochs=8, batch=4096, lr=3e-3, seed=2, lib=lib)
+    w, mse = learn.train(harvested, lc.GAMMA, epochs=8, batch=4096, lr=3e-3, seed=2, lib=lib)
+    w, table = learn.fit_gate(harvested[::8], lc.GAMMA, lc.PMIN, w, lib=lib)
+    print("gate", w[learn.OFF_GATE], "(mean updates, evaluated fraction) by gate", table)
+    return w, mse
 
 
 @pytest.fixture(scope="module")

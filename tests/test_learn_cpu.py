@@ -26,7 +26,8 @@ def harvested(cpulib):
 @pytest.fixture(scope="module")
 def trained(cpulib, harvested):
     _, _, rec, _ = harvested
-    w, mse = learn.train(rec, lc.GAMMA, epochs=6, batch=512, lr=3e-3, seed=1, lib=cpulib)
+    w, mse = learn.train(rec, lc.GAMMA, epochs=8, batch=512, lr=1e-2, seed=1, lib=cpulib)
+    w, _ = learn.fit_gate(rec[::3], lc.GAMMA, lc.PMIN, w, lib=cpulib)
     return w, mse
 
 
@@ -85,7 +86,7 @@ def test_untrained_network_still_converges(cpulib):
     start where it does not converge, and the step stays within tolerance."""
     g, glob, _ = problems.turbulent(12, seed=12)
     w = learn.init_weights(7)
-    w[learn.OFF_B3] = 6.0          # p_start = e^6 p_left: far off
+    w[learn.OFF_B3] = 6.0          # start e^6 times the acoustic one: far off
     ref, got = lc.learned_vs_oracle(cpulib, g, glob, w, steps=1)
     lc.check_learned_run(ref, got, one_step=True)
 

@@ -13,7 +13,8 @@ import oracle
 def random_weights(rng):
     w = rng.normal(0.0, 0.4, lo.N_FLOATS)
     w[lo.OFF_MEAN:lo.OFF_INV_STD] = rng.normal(0.0, 0.5, lo.N_IN)
-    w[lo.OFF_INV_STD:] = rng.uniform(0.5, 2.0, lo.N_IN)
+    w[lo.OFF_INV_STD:lo.OFF_GATE] = rng.uniform(0.5, 2.0, lo.N_IN)
+    w[lo.OFF_GATE] = 0.0
     return w
 
 
@@ -30,15 +31,25 @@ def random_records(rng, n):
 
 
 def test_features_closed_form():
-    # equal states: no jumps, equal sound speeds
+    # equal states: no jumps, equal sound speeds, acoustic start = p
     X, Y = lo.features([[2.0, 2.0, 3.0, 3.0, 0.5, 0.5, 3.0]], 5.0 / 3.0)
     assert np.array_equal(X[0], [0.0, 0.0, 0.0, 0.0, 0.5, 0.5]) and Y[0] == 0.0
     # rho 1 | 4, p 1 | 2, u 1 | 0, gamma 1.4: c_l = sqrt(1.4), c_r = sqrt(0.7),
-    # so c_l / (c_l + c_r) = sqrt2 / (sqrt2 + 1) = 2 - sqrt2; p* = e p_l
-    X, Y = lo.features([[1.0, 4.0, 1.0, 2.0, 1.0, 0.0, 2.718281828459045]], 1.4)
+    # so c_l / (c_l + c_r) = sqrt2 / (sqrt2 + 1) = 2 - sqrt2
+    X, _ = lo.features([[1.0, 4.0, 1.0, 2.0, 1.0, 0.0, 1.0]], 1.4)
     expect = [0.6931471805599453, 1.3862943611198906, 0.8451542547285166, 1.1952286093343936,
               0.5857864376269049, 0.4142135623730951]
-    assert np.allclose(X[0], expect, rtol=1e-14, atol=0) and abs(Y[0] - 1.0) < 1e-15
+    assert np.allclose(X[0], expect, rtol=1e-14, atol=0)
+    # rho = p = 1 both sides, u_l - u_r = 2 / sqrt(1.4): the Lagrangian wave
+    # speeds are sqrt(1.4), so the acoustic start is 1 + 1 = 2; p* = 2e
+    # gives the target 1
+    u = 0.8451542547285166
+    r = [[1.0, 1.0, 1.0, 1.0, u, -u, 2.0 * 2.718281828459045]]
+    assert abs(lo.acoustic_start(r, 1.4)[0] - 2.0) < 1e-15
+    assert abs(lo.features(r, 1.4)[1][0] - 1.0) < 1e-15
+    # cavitation: the acoustic start is negative, the record is not learnable
+    r = [[1.0, 1.0, 1.0, 1.0, -5.0, 5.0, 1e-20], [1.0, 1.0, 1.0, 1.0, 0.0, 0.0, 1.0]]
+    assert list(lo.learnable(r, 1.4)) == [False, True]
 
 
 def test_forward_standardises_inputs():
@@ -49,7 +60,7 @@ def test_forward_standardises_inputs():
     w[lo.OFF_W2 + 0] = 1.0            # W2[0][0]
     w[lo.OFF_W3 + 0] = 1.0            # W3[0]
     w[lo.OFF_MEAN] = 0.25
-    w[lo.OFF_INV_STD:] = 2.0
+    w[lo.OFF_INV_STD:lo.OFF_GATE] = 2.0
     X = np.array([[0.75, 9, 9, 9, 9, 9]])
     y, _ = lo.forward(w, X)
     assert abs(y[0] - np.tanh(np.tanh(1.0))) < 1e-15
@@ -58,7 +69,8 @@ def test_forward_standardises_inputs():
 def test_gradient_matches_finite_differences():
     rng = np.random.default_rng(3)
     w = random_weights(rng)
-    X, Y = lo.features(random_records(rng, 48), 5.0 / 3.0)
+    r = random_records(rng, 64)
+    X, Y = lo.features(r[lo.learnable(r, 5.0 / 3.0)], 5.0 / 3.0)
     g = lo.gradient(w, X, Y)
     h = 1e-6
     fd = np.zeros(lo.N_PARAMS)
@@ -76,7 +88,8 @@ def test_finite_differences_catch_a_dropped_term():
     derivative of the first layer (planted in the oracle, on the CPU)."""
     rng = np.random.default_rng(4)
     w = random_weights(rng)
-    X, Y = lo.features(random_records(rng, 32), 5.0 / 3.0)
+    r = random_records(rng, 48)
+    X, Y = lo.features(r[lo.learnable(r, 5.0 / 3.0)], 5.0 / 3.0)
 
     def broken_gradient(w, X, Y):
         W1, b1, W2, b2, W3, b3, mean, inv_std = lo.unpack(w)
