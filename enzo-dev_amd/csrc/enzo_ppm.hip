@@ -79,6 +79,12 @@ __global__ __launch_bounds__(256) void pressure_dt_kernel(PresParams Q) {
   if (Q.want_dt) block_min_dt(local, Q.dtmin);
 }
 
+__global__ void expand_kernel(ExpandParams X) {
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < X.cells;
+       g += (long long)gridDim.x * blockDim.x)
+    expand_cell(X, g);
+}
+
 __global__ void dt_init_kernel(unsigned long long *dtmin) {
   if (threadIdx.x == 0) *dtmin = (unsigned long long)__double_as_longlong(1.0e20);
 }
@@ -169,6 +175,14 @@ struct hipPPMData {
   // learned two-shock start (network parameters, kMlpFloats floats) when set
   float *learn = nullptr;
   bool learned = false;
+  // comoving coordinates: sweeps use widths a_sweep * CellWidth, the Courant
+  // pass CellWidth (cwdt) with aye; the state at each step's start is kept
+  // in `old` (slots S_D..S_GE) for the time-centred expansion pressure
+  bool comoving = false;
+  double a_sweep = 1.0, aye = 1.0;
+  double *cwdt[3] = {nullptr, nullptr, nullptr};
+  double *old = nullptr;
+  bool old_valid = false;
   // harvest of interface states (SweepParams::harvest) when harvest_cap > 0
   double *harvest = nullptr;
   unsigned long long *harvest_n = nullptr;
@@ -243,6 +257,7 @@ void free_data(hipPPMData *D) {
   void *ptrs[] = {D->A, D->B, D->pres, D->accel, D->tab[0], D->tab[1], D->tab[2], D->cwdev[0],
                   D->cwdev[1], D->cwdev[2], D->flat, D->qc, D->dt, D->dtmin, D->counters, D->segbuf,
                   D->fb_flag, D->fb_list, D->fb_state, D->learn, D->harvest, D->harvest_n,
+                  D->cwdt[0], D->cwdt[1], D->cwdt[2], D->old,
                   D->face_head[0], D->face_head[1], D->face_head[2], D->faces[0], D->faces[1],
                   D->faces[2], D->face_out[0], D->face_out[1], D->face_out[2]};
   for (void *p : ptrs)
@@ -406,6 +421,32 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
   return ENZO_PPM_SUCCESS;
 }
 
+// cell widths of the sweeps (a_sweep * CellWidth in comoving coordinates,
+// Grid_SolveHydroEquations.C:386-408, the product taken as the reference
+// takes it) and their coefficient tables; the Courant pass's widths
+// (CellWidth itself) when comoving
+int upload_geometry(hipPPMData *D, const hipPPMParameter *Para) {
+  for (int d = 0; d < 3; d++) {
+    const int N = D->GD[d];
+    std::vector<double> cw(N), dx(N), tab;
+    for (int i = 0; i < N; i++)
+      cw[i] = Para->CellWidth[d] ? Para->CellWidth[d][i] : (d < Para->GridRank ? Para->dx[d] : 1.0);
+    for (int i = 0; i < N; i++)
+      dx[i] = D->comoving && d < Para->GridRank ? D->a_sweep * cw[i] : cw[i];
+    build_tables(dx.data(), N, tab);
+    D->uni[d] = uniform_axis(dx.data(), N, tab, D->kt[d]) ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(D->tab[d], tab.data(), sizeof(double) * tab.size(),
+                          hipMemcpyHostToDevice, D->stream));
+    HIPCHK(hipMemcpyAsync(D->cwdev[d], dx.data(), sizeof(double) * N, hipMemcpyHostToDevice,
+                          D->stream));
+    if (D->comoving)
+      HIPCHK(hipMemcpyAsync(D->cwdt[d], cw.data(), sizeof(double) * N, hipMemcpyHostToDevice,
+                            D->stream));
+    HIPCHK(hipStreamSynchronize(D->stream));  // host vectors go out of scope
+  }
+  return ENZO_PPM_SUCCESS;
+}
+
 int launch_pressure(hipPPMData *D, const hipPPMParameter *P, double min_support, int want_dt,
                     double cfl) {
   PresParams Q;
@@ -416,7 +457,7 @@ int launch_pressure(hipPPMData *D, const hipPPMParameter *P, double min_support,
     Q.GD[d] = D->GD[d];
     Q.st[d] = (int)P->GridStartIndex[d];
     Q.en[d] = (int)P->GridEndIndex[d];
-    Q.cw[d] = D->cwdev[d];
+    Q.cw[d] = D->comoving ? D->cwdt[d] : D->cwdev[d];
   }
   Q.cells = D->cells;
   Q.A = D->A;
@@ -424,7 +465,7 @@ int launch_pressure(hipPPMData *D, const hipPPMParameter *P, double min_support,
   Q.gamma = P->Gamma;
   Q.tiny_number = P->MinimumPressure;
   Q.min_support = min_support;
-  Q.aye = 1.0;
+  Q.aye = D->comoving ? D->aye : 1.0;
   Q.want_dt = want_dt;
   Q.dtmin = D->dtmin;
   if (want_dt) {
@@ -574,17 +615,12 @@ int hipPPMInitData(hipPPMData **Data, const hipPPMParameter *Para, void *stream)
   }
   D->stream = (hipStream_t)stream;
   // geometry tables and segments (cheap; rebuilt on every init)
+  st = upload_geometry(D, Para);
+  if (st != ENZO_PPM_SUCCESS) return st;
   std::vector<int> segs;
   for (int d = 0; d < 3; d++) {
     const int N = D->GD[d];
-    std::vector<double> dx(N), tab;
-    for (int i = 0; i < N; i++)
-      dx[i] = Para->CellWidth[d] ? Para->CellWidth[d][i] : (d < Para->GridRank ? Para->dx[d] : 1.0);
     const int i1 = (int)Para->GridStartIndex[d] + 1, i2 = (int)Para->GridEndIndex[d] + 1;
-    build_tables(dx.data(), N, tab);
-    D->uni[d] = uniform_axis(dx.data(), N, tab, D->kt[d]) ? 1 : 0;
-    HIPCHK(hipMemcpy(D->tab[d], tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(D->cwdev[d], dx.data(), sizeof(double) * N, hipMemcpyHostToDevice));
     std::vector<int> o, w;
     if (d < Para->GridRank && N > 1) {
       if (!build_segments<kTP, kTS>(N, i1, i2, o, w)) return ENZO_PPM_ERR_ARGUMENT;
@@ -882,6 +918,11 @@ int hipPPMStep(hipPPMData *D, const hipPPMParameter *P, enzo_int CycleNumber,
                enzo_float dtFixed) {
   if (!D || !P) return ENZO_PPM_ERR_ARGUMENT;
   const int rank = (int)P->GridRank;
+  if (D->comoving) {  // OldBaryonField of the time-centred expansion pressure
+    HIPCHK(hipMemcpyAsync(D->old, D->A, sizeof(double) * (size_t)(S_GE + 1) * D->cells,
+                          hipMemcpyDeviceToDevice, D->stream));
+    D->old_valid = true;
+  }
   const int ixyz = (int)(CycleNumber % rank);
   for (int n = ixyz; n < ixyz + rank; n++) {
     const int dir = n % rank;
@@ -889,6 +930,53 @@ int hipPPMStep(hipPPMData *D, const hipPPMParameter *P, enzo_int CycleNumber,
     int st = launch_sweep(D, P, dtFixed, dtFixed > 0 ? nullptr : D->dt, dir);
     if (st != ENZO_PPM_SUCCESS) return st;
   }
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMSetComoving(hipPPMData *D, const hipPPMParameter *P, enzo_int enable) {
+  if (!D || !P) return ENZO_PPM_ERR_ARGUMENT;
+  HIPCHK(hipStreamSynchronize(D->stream));
+  if (enable) {
+    for (int d = 0; d < 3; d++)
+      if (!D->cwdt[d] && alloc_dev((void **)&D->cwdt[d], sizeof(double) * D->GD[d]))
+        return ENZO_PPM_ERR_HIP;
+    if (!D->old && alloc_dev((void **)&D->old, sizeof(double) * (size_t)(S_GE + 1) * D->cells))
+      return ENZO_PPM_ERR_HIP;
+  }
+  D->comoving = enable != 0;
+  D->old_valid = false;
+  if (!D->comoving) D->a_sweep = D->aye = 1.0;
+  return upload_geometry(D, P);
+}
+
+int hipPPMSetExpansionFactors(hipPPMData *D, const hipPPMParameter *P, enzo_float a_courant,
+                              enzo_float a_sweep) {
+  if (!D || !P || !D->comoving || !(a_courant > 0.0) || !(a_sweep > 0.0))
+    return ENZO_PPM_ERR_ARGUMENT;
+  D->aye = a_courant;
+  if (a_sweep == D->a_sweep) return ENZO_PPM_SUCCESS;
+  D->a_sweep = a_sweep;
+  return upload_geometry(D, P);
+}
+
+int hipPPMComovingExpansionTerms(hipPPMData *D, const hipPPMParameter *P, enzo_float coef,
+                                 enzo_float Coefficient) {
+  if (!D || !P || !D->comoving) return ENZO_PPM_ERR_ARGUMENT;
+  ExpandParams X;
+  X.cells = D->cells;
+  X.rank = (int)P->GridRank;
+  X.idual = (int)P->DualEnergyFormalism;
+  X.interp = D->old_valid && coef != 1.0 ? 1 : 0;
+  X.gamma = P->Gamma;
+  X.tiny_number = P->MinimumPressure;
+  X.coef = coef;
+  X.Coefficient = Coefficient;
+  X.A = D->A;
+  X.old = D->old;
+  long long nb = (D->cells + 255) / 256;
+  if (nb > 16384) nb = 16384;
+  hipLaunchKernelGGL(expand_kernel, dim3((unsigned)nb), dim3(256), 0, D->stream, X);
+  HIPCHK(hipGetLastError());
   return ENZO_PPM_SUCCESS;
 }
 

@@ -165,6 +165,68 @@ __device__ inline void block_min_dt(double local, unsigned long long *dtmin) {
 }
 
 // ------------------------------------------------------------------------
+// Comoving expansion terms of one cell (Grid_ComovingExpansionTerms.C:55-130
+// with expand_terms.F, ENERGY_METHOD3 / VELOCITY_METHOD3, HydroMethod 0):
+// the pressure at the interpolation weight `coef` between the state kept at
+// the step's start (`old`, slots S_D, S_E, S_VX..S_VZ, S_GE as in A) and the
+// current state (Grid_ComputePressure.C:56-61, :170-240), or the current
+// state's pressure when `interp` is 0 (its time == Time branch); then the
+// semi-implicit decay of gas energy, total energy and velocities with
+// Coefficient = dt (da/dt) / a.  Every cell, ghosts included.
+struct ExpandParams {
+  long long cells;
+  int rank, idual, interp;
+  double gamma, tiny_number, coef, Coefficient;
+  double *A;          // current state (slots as in the sweeps)
+  const double *old;  // state at the step's start, same slot layout (interp only)
+};
+
+__device__ inline void expand_cell(const ExpandParams &X, long long g) {
+  double *A = X.A;
+  const long long n = X.cells;
+  const double d = A[S_D * n + g];
+  double density, gas_energy;
+  if (X.interp) {
+    const double *O = X.old;
+    const double c = X.coef, co = 1 - X.coef;
+    const double total_energy = c * A[S_E * n + g] + co * O[S_E * n + g];
+    density = c * d + co * O[S_D * n + g];
+    const double velocity1 = c * A[S_VX * n + g] + co * O[S_VX * n + g];
+    double velocity2 = 0, velocity3 = 0;
+    if (X.rank > 1) velocity2 = c * A[S_VY * n + g] + co * O[S_VY * n + g];
+    if (X.rank > 2) velocity3 = c * A[S_VZ * n + g] + co * O[S_VZ * n + g];
+    if (X.idual == 0) {
+      const double kineticE =
+          0.5 * (velocity1 * velocity1 + velocity2 * velocity2 + velocity3 * velocity3);
+      gas_energy = total_energy - kineticE;
+    } else {
+      gas_energy = c * A[S_GE * n + g] + co * O[S_GE * n + g];
+    }
+  } else {
+    density = d;
+    if (X.idual == 0) {
+      const double v1 = A[S_VX * n + g];
+      const double v2 = X.rank > 1 ? A[S_VY * n + g] : 0.0;
+      const double v3 = X.rank > 2 ? A[S_VZ * n + g] : 0.0;
+      gas_energy = A[S_E * n + g] - 0.5 * (v1 * v1 + v2 * v2 + v3 * v3);
+    } else {
+      gas_energy = A[S_GE * n + g];
+    }
+  }
+  double p = (X.gamma - 1.0) * density * gas_energy;
+  if (p < X.tiny_number) p = X.tiny_number;
+  // expand_terms.F:66-166
+  const double cf = X.Coefficient;
+  if (X.idual == 1) A[S_GE * n + g] = A[S_GE * n + g] * (1.0 - cf) / (1.0 + cf);
+  double e = A[S_E * n + g] * (1.0 - cf) / (1.0 + cf);
+  e = fmx(e - cf * (3.0 - 2.0 / (X.gamma - 1.0)) * p / d, 0.5 * e);
+  A[S_E * n + g] = e;
+  A[S_VX * n + g] = A[S_VX * n + g] * (1.0 - 0.5 * cf) / (1.0 + 0.5 * cf);
+  if (X.rank > 1) A[S_VY * n + g] = A[S_VY * n + g] * (1.0 - 0.5 * cf) / (1.0 + 0.5 * cf);
+  if (X.rank > 2) A[S_VZ * n + g] = A[S_VZ * n + g] * (1.0 - 0.5 * cf) / (1.0 + 0.5 * cf);
+}
+
+// ------------------------------------------------------------------------
 // Per-sweep preparation item `gid`: qc(i) = dt/dx(i) for gid < N (used by
 // the fluxes, flux_twoshock.F:366, and the gas-energy source, :393), then the
 // flattening of line 1 of every slice, one (slice, cell in [i1-1, i2+1]) per

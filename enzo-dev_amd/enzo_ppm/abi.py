@@ -181,6 +181,11 @@ PROTOTYPES = {
     "hipPPMLevelSetLearnedGuess": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hipPPMLevelDestroy": (C.c_int, [C.c_void_p]),
     "hipPPMSetLearnedGuess": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hipPPMSetComoving": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_int64]),
+    "hipPPMSetExpansionFactors": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_double,
+                                            C.c_double]),
+    "hipPPMComovingExpansionTerms": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter),
+                                               C.c_double, C.c_double]),
     "hipPPMSetHarvest": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64]),
     "hipPPMGetHarvest": (C.c_int, [C.c_void_p, PF, C.c_int64, PI, PI]),
     "hipPPMLearnTrain": (C.c_int, [PF, C.c_int64, C.c_double, C.c_int64, C.c_int64, C.c_double,

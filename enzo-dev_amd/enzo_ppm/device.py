@@ -49,6 +49,7 @@ class DeviceGrid:
         self.handle = C.c_void_p(None)
         _check(L.hipPPMInitData(C.byref(self.handle), C.byref(self.para), stream), "InitData")
         self._keep = None
+        self.a_sweep = 1.0
         self.upload()
 
     def upload(self):
@@ -100,6 +101,24 @@ class DeviceGrid:
 
     def fallback_count(self):
         return self.lib.hipPPMFallbackCount(self.handle)
+
+    def set_comoving(self, enable=True):
+        """Comoving handling on the device state (hipPPMSetComoving)."""
+        _check(self.lib.hipPPMSetComoving(self.handle, C.byref(self.para), int(enable)),
+               "SetComoving")
+        self.a_sweep = 1.0
+
+    def set_expansion_factors(self, a_courant, a_sweep):
+        """a(Time) for the Courant pass, a(Time + dt/2) for the sweeps."""
+        _check(self.lib.hipPPMSetExpansionFactors(self.handle, C.byref(self.para), a_courant,
+                                                  a_sweep), "SetExpansionFactors")
+        self.a_sweep = a_sweep
+
+    def comoving_expansion_terms(self, coef, coefficient):
+        """Expansion terms after a step: pressure weight `coef` between the
+        step's start and now, Coefficient = dt (da/dt) / a."""
+        _check(self.lib.hipPPMComovingExpansionTerms(self.handle, C.byref(self.para), coef,
+                                                     coefficient), "ComovingExpansionTerms")
 
     def set_learned(self, weights):
         """Learned two-shock start from the next sweep on (float32 weights of

@@ -7,7 +7,7 @@ ge = p / ((gamma-1) rho), specific total energy E = ge + v^2/2
 """
 import numpy as np
 
-from .grid import Grid
+from .grid import Grid, periodic_fill
 
 GAMMA = 5.0 / 3.0
 
@@ -84,6 +84,51 @@ def turbulent(n=256, seed=2024, sigma_lnrho=0.5, mach=3.0, T=1.0, ncolours=3,
     for c in g.colnum:
         g.fields[c][a] = rng.uniform(0.0, 1.0, size=(n, n, n))
     g.fill_periodic()
+    glob = dict(Gamma=gamma, PPMSteepeningParameter=flags[0], PPMFlatteningParameter=flags[1],
+                PPMDiffusionParameter=flags[2], DualEnergyFormalism=1)
+    return g, glob, dict(cfl=cfl, bc="periodic")
+
+
+def cosmological(n=512, seed=5, sigma_lnrho=2.0, mach=100.0, T=1.0, ncolours=6, g_frac=0.05,
+                 gamma=GAMMA, flags=(1, 1, 0), cfl=0.4, log=None):
+    """Comoving-box state: lognormal density (ln rho a k^-2 Gaussian random
+    field on k = 1..16 with standard deviation sigma_lnrho, mean rho 1),
+    k^-2 Gaussian random velocity at RMS Mach `mach` for c_s = sqrt(gamma T),
+    p = rho T (so the thermal energy is ~1e-4 of the kinetic one and the
+    dual-energy switches decide), `ncolours` species fractions uniform in
+    [0, 1], and a static seeded 3-vector acceleration (a k^-2 Gaussian random
+    field per component, RMS g_frac * v_rms^2 / L over a unit box).  Cell
+    widths are comoving (1/n); the caller scales them by a."""
+    rng = np.random.default_rng(seed)
+    say = log or (lambda *a: None)
+    g = Grid([n] * 3, ncolours=ncolours, dual_energy=True, gravity=True, dx=1.0 / n)
+    a = g.active_slice()
+    say("density")
+    lnrho = gaussian_random_field(n, 1, 16, -2.0, rng)[0] * sigma_lnrho
+    rho = np.exp(lnrho)
+    del lnrho
+    rho /= rho.mean()
+    cs = np.sqrt(gamma * T)
+    say("velocity")
+    vel = gaussian_random_field(n, 1, 16, -2.0, rng, ncomp=3)
+    vrms = np.sqrt(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2))
+    for v in vel:
+        v *= mach * cs / vrms
+    full = lambda x: _embed(g, x)  # noqa: E731
+    _set_thermo(g, full(rho), full(rho * T), full(vel[0]), full(vel[1]), full(vel[2]), gamma)
+    del vel, rho
+    say("colours")
+    for c in g.colnum:
+        g.fields[c][a] = rng.uniform(0.0, 1.0, size=(n, n, n))
+    say("acceleration")
+    acc = gaussian_random_field(n, 1, 16, -2.0, rng, ncomp=3)
+    arms = np.sqrt(np.mean(acc[0] ** 2 + acc[1] ** 2 + acc[2] ** 2))
+    for d in range(3):
+        g.accel[d][a] = acc[d] * (g_frac * (mach * cs) ** 2 / arms)
+    del acc
+    g.fill_periodic()
+    for d in range(3):
+        periodic_fill(g.accel[d], g.ghost, g.rank)
     glob = dict(Gamma=gamma, PPMSteepeningParameter=flags[0], PPMFlatteningParameter=flags[1],
                 PPMDiffusionParameter=flags[2], DualEnergyFormalism=1)
     return g, glob, dict(cfl=cfl, bc="periodic")

@@ -240,6 +240,32 @@ int hipPPMStep(hipPPMData *Data, const hipPPMParameter *Para, enzo_int CycleNumb
    (the periodic case of ExternalBoundary::SetExternalBoundary). */
 int hipPPMPeriodicBoundary(hipPPMData *Data, const hipPPMParameter *Para);
 
+/* ---- comoving coordinates ----------------------------------------------
+   The reference recasts the solver in comoving form by scaling the cell
+   widths it hands to SolvePPM_DE by a(t + dt/2)
+   (Grid_SolveHydroEquations.C:386-408), takes the Courant step with a(t)
+   (Grid_ComputeTimeStep.C:103-106 -> calc_dt's aye) and adds the expansion
+   terms after the step (Grid_ComovingExpansionTerms.C:55-130 ->
+   expand_terms.F, ENERGY_METHOD3 and VELOCITY_METHOD3).  On the
+   device-resident state:
+   hipPPMSetComoving: switch comoving handling on or off.  While on, every
+     hipPPMStep first keeps a copy of the state (OldBaryonField), the Courant
+     pass uses the unscaled widths times a_courant, the sweeps the widths
+     times a_sweep.
+   hipPPMSetExpansionFactors: a_courant = a(Time) for hipPPMPressureAndTimeStep,
+     a_sweep = a(Time + dt/2) for hipPPMStep (coefficient tables rebuilt when
+     it changes).
+   hipPPMComovingExpansionTerms: after hipPPMStep, the expansion terms over
+     every cell with Coefficient = dt (da/dt) / a at the step's mid-time, the
+     pressure taken at interpolation weight `coef` = (PressureTime - OldTime)
+     / (Time - OldTime) between the kept copy and the current state (coef = 1:
+     the current state, the reference's time == Time branch). */
+int hipPPMSetComoving(hipPPMData *Data, const hipPPMParameter *Para, enzo_int enable);
+int hipPPMSetExpansionFactors(hipPPMData *Data, const hipPPMParameter *Para, enzo_float a_courant,
+                              enzo_float a_sweep);
+int hipPPMComovingExpansionTerms(hipPPMData *Data, const hipPPMParameter *Para, enzo_float coef,
+                                 enzo_float Coefficient);
+
 /* Timing of the fused sweep kernel: with profiling enabled every sweep
    kernel launch is bracketed by HIP events on the handle's stream;
    hipPPMProfileRead synchronises, returns the summed kernel time (ms) and the

@@ -50,7 +50,15 @@ def lib(kind="port"):
     f = getattr(L, p + "compute_time_step")
     f.restype = C.c_int
     f.argtypes = [C.POINTER(abi.EnzoPPMGrid), C.POINTER(abi.EnzoPPMGlobals), C.POINTER(abi.PF),
-                  C.c_double, C.POINTER(C.c_double)]
+                  C.c_double, C.c_double, C.POINTER(C.c_double)]
+    f = getattr(L, p + "compute_pressure_at")
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(abi.EnzoPPMGrid), C.POINTER(abi.PF), C.POINTER(abi.EnzoPPMGlobals),
+                  C.c_double, C.c_double, abi.PF]
+    f = getattr(L, p + "comoving_expansion_terms")
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(abi.EnzoPPMGrid), C.POINTER(abi.PF), C.POINTER(abi.EnzoPPMGlobals),
+                  C.c_double, C.c_double]
     _libs[kind] = L
     return L
 
@@ -83,14 +91,54 @@ def compute_pressure(kind, grid, glob, min_support=0.0):
     return p
 
 
-def compute_time_step(kind, grid, glob, cfl):
+def compute_time_step(kind, grid, glob, cfl, aye=1.0, cellwidth=None):
+    """Courant step (calc_dt x CFL) with expansion factor `aye`; `cellwidth`
+    (3 arrays) replaces the grid's widths when given."""
     L = lib(kind)
     gs, _ = make_globals(**glob)
     cg, keep = grid.ctypes_grid()
-    cw = grid.ctypes_cellwidth()
+    if cellwidth is None:
+        cw = grid.ctypes_cellwidth()
+    else:
+        keep_cw = [abi.dptr(c) for c in cellwidth]  # noqa: F841
+        cw = (abi.PF * 3)(*keep_cw)
     dt = C.c_double(0)
-    st = getattr(L, PREFIX[kind] + "compute_time_step")(C.byref(cg), C.byref(gs), cw, cfl,
+    st = getattr(L, PREFIX[kind] + "compute_time_step")(C.byref(cg), C.byref(gs), cw, cfl, aye,
                                                        C.byref(dt))
     if st != abi.SUCCESS:
         raise RuntimeError(f"compute_time_step failed {st}")
     return dt.value
+
+
+def _old_fields(grid, old):
+    arr = (abi.PF * max(1, len(old.fields)))()
+    for k, f in enumerate(old.fields):
+        arr[k] = abi.dptr(f)
+    return arr
+
+
+def compute_pressure_at(kind, grid, old, glob, coef, min_support=0.0):
+    """Pressure of coef * grid + (1 - coef) * old (Grid_ComputePressure.C's
+    interpolated branch)."""
+    import numpy as np
+    L = lib(kind)
+    gs, _ = make_globals(**glob)
+    cg, keep = grid.ctypes_grid()
+    p = np.zeros(grid.shape)
+    getattr(L, PREFIX[kind] + "compute_pressure_at")(C.byref(cg), _old_fields(grid, old),
+                                                    C.byref(gs), coef, min_support, abi.dptr(p))
+    return p
+
+
+def comoving_expansion_terms(kind, grid, old, glob, coef, coefficient):
+    """grid::ComovingExpansionTerms on `grid` (in place): pressure at weight
+    `coef` between `old` (the state before the step; None: no old state) and
+    `grid`, then expand_terms with Coefficient = dt * (da/dt) / a."""
+    L = lib(kind)
+    gs, _ = make_globals(**glob)
+    cg, keep = grid.ctypes_grid()
+    ob = _old_fields(grid, old) if old is not None else None
+    st = getattr(L, PREFIX[kind] + "comoving_expansion_terms")(C.byref(cg), ob, C.byref(gs), coef,
+                                                               coefficient)
+    if st != abi.SUCCESS:
+        raise RuntimeError(f"comoving_expansion_terms failed {st}")

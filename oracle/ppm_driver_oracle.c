@@ -108,6 +108,10 @@ void KER(calc_dt)(const I *, const I *, const I *, const I *, const I *, const I
                   const R *, const I *, const R *, const R *, const R *, const R *,
                   const R *, const R *, R *, R *);
 
+void KER(expand_terms)(const I *, const I *, const I *, const R *, const I *, const R *, R *,
+                       const R *, R *, R *, R *, R *, R *, const R *, const R *, const R *,
+                       const R *, const R *, const R *, const I *, R *, const R *);
+
 /* ---------------------------------------------------------------- */
 /* grid::ComputePressure, time == Time branch (Grid_ComputePressure.C:96-166),
    EquationOfState 0, EOSType 0, no MHD / cosmic rays. */
@@ -137,11 +141,78 @@ int PFX(compute_pressure)(const enzo_ppm_grid *g, const enzo_ppm_globals *gl,
   return ENZO_PPM_SUCCESS;
 }
 
+/* grid::ComputePressure at an interpolated time (Grid_ComputePressure.C:
+   56-61 and the general branch :170-240): every field the pressure reads
+   is coef * BaryonField + (1 - coef) * OldBaryonField, coef being
+   (time - OldTime) / (Time - OldTime).  EquationOfState 0, EOSType 0. */
+int PFX(compute_pressure_at)(const enzo_ppm_grid *g, R *const *OldBaryonField,
+                             const enzo_ppm_globals *gl, R coef,
+                             R MinimumSupportEnergyCoefficient, R *pressure) {
+  I size = 1;
+  for (I d = 0; d < g->GridRank; d++) size *= g->GridDimension[d];
+  R **bf = g->BaryonField;
+  R *const *ob = OldBaryonField;
+  const R coefold = 1 - coef;
+  const R OneHalf = 0.5;
+  for (I i = 0; i < size; i++) {
+    R total_energy = coef * bf[g->TENum][i] + coefold * ob[g->TENum][i];
+    R density = coef * bf[g->DensNum][i] + coefold * ob[g->DensNum][i];
+    R velocity1 = coef * bf[g->Vel1Num][i] + coefold * ob[g->Vel1Num][i];
+    R velocity2 = 0, velocity3 = 0;
+    if (g->GridRank > 1) velocity2 = coef * bf[g->Vel2Num][i] + coefold * ob[g->Vel2Num][i];
+    if (g->GridRank > 2) velocity3 = coef * bf[g->Vel3Num][i] + coefold * ob[g->Vel3Num][i];
+    R gas_energy;
+    if (gl->DualEnergyFormalism == 0) {
+      R kineticE = OneHalf * (velocity1 * velocity1 + velocity2 * velocity2 +
+                              velocity3 * velocity3);
+      gas_energy = total_energy - kineticE;
+    } else {
+      gas_energy = coef * bf[g->GENum][i] + coefold * ob[g->GENum][i];
+    }
+    pressure[i] = (gl->Gamma - 1.0) * density * gas_energy;
+    if (pressure[i] < gl->tiny_number) pressure[i] = gl->tiny_number;
+    R min_pressure = MinimumSupportEnergyCoefficient * (gl->Gamma - 1.0) * density * density;
+    if (pressure[i] < min_pressure) pressure[i] = min_pressure;
+  }
+  return ENZO_PPM_SUCCESS;
+}
+
+/* grid::ComovingExpansionTerms (Grid_ComovingExpansionTerms.C:55-130):
+   the pressure at PressureTime -- the interpolated form with weight `coef`
+   when an old state exists (OldBaryonField non-NULL and coef != 1: the
+   reference's time != Time), else the time == Time form -- then
+   expand_terms over every cell with Coefficient = dtFixed * dadt / a,
+   imethod = HydroMethod (0, PPM direct Euler), no cosmic rays. */
+int PFX(comoving_expansion_terms)(const enzo_ppm_grid *g, R *const *OldBaryonField,
+                                  const enzo_ppm_globals *gl, R coef, R Coefficient) {
+  I size = 1;
+  for (I d = 0; d < g->GridRank; d++) size *= g->GridDimension[d];
+  R *p = (R *)malloc(sizeof(R) * (size_t)size);
+  if (!p) return ENZO_PPM_ERR_ARGUMENT;
+  if (OldBaryonField && coef != 1.0)
+    PFX(compute_pressure_at)(g, OldBaryonField, gl, coef, 0.0, p);
+  else
+    PFX(compute_pressure)(g, gl, 0.0, p);
+  R **bf = g->BaryonField;
+  R *d = bf[g->DensNum];
+  R *ge = gl->DualEnergyFormalism ? bf[g->GENum] : d;
+  R *v = g->GridRank > 1 ? bf[g->Vel2Num] : bf[g->Vel1Num];
+  R *w = g->GridRank > 2 ? bf[g->Vel3Num] : bf[g->Vel1Num];
+  I rank = g->GridRank, idual = gl->DualEnergyFormalism, imethod = 0, icr = 0;
+  R ecr = 0, ecrold = 0;
+  KER(expand_terms)(&rank, &size, &idual, &Coefficient, &imethod, &gl->Gamma, p, d,
+                    bf[g->TENum], ge, bf[g->Vel1Num], v, w, d, d, d, d, d, d, &icr, &ecr,
+                    &ecrold);
+  free(p);
+  return ENZO_PPM_SUCCESS;
+}
+
 /* calc_dt call of grid::ComputeTimeStep (Grid_ComputeTimeStep.C:151-174,209):
-   dtBaryons = CourantSafetyNumber * calc_dt(rank, ..., aye = 1).
-   CellWidth arrays are the grid's (not a-scaled) widths. */
+   dtBaryons = CourantSafetyNumber * calc_dt(rank, ..., aye).  CellWidth
+   arrays are the grid's (not a-scaled) widths; aye is a(Time) in comoving
+   coordinates (:103-106), else 1. */
 int PFX(compute_time_step)(const enzo_ppm_grid *g, const enzo_ppm_globals *gl,
-                           R *CellWidth[3], R CourantSafetyNumber, R *dt_out) {
+                           R *CellWidth[3], R CourantSafetyNumber, R aye, R *dt_out) {
   I size = 1;
   for (I d = 0; d < g->GridRank; d++) size *= g->GridDimension[d];
   R *p = (R *)malloc(sizeof(R) * (size_t)size);
@@ -151,7 +222,7 @@ int PFX(compute_time_step)(const enzo_ppm_grid *g, const enzo_ppm_globals *gl,
   for (I d = 0; d < g->GridRank; d++) {
     dims[d] = g->GridDimension[d]; st[d] = g->GridStartIndex[d]; en[d] = g->GridEndIndex[d];
   }
-  R one = 1.0, zero = 0.0, C2 = 0.0, dt = 0, dtv = 0;
+  R zero = 0.0, C2 = 0.0, dt = 0, dtv = 0;
   I ihydro = 0; /* PPM_DirectEuler */
   R unit = 1.0;
   R *dx = CellWidth[0], *dy = g->GridRank > 1 ? CellWidth[1] : &unit,
@@ -162,7 +233,7 @@ int PFX(compute_time_step)(const enzo_ppm_grid *g, const enzo_ppm_globals *gl,
   ERRCODE = 0;
   KER(calc_dt)(&g->GridRank, &dims[0], &dims[1], &dims[2], &st[0], &en[0], &st[1], &en[1],
                &st[2], &en[2], &ihydro, &C2, dx, dy, dz, &zero, &zero, &zero, &gl->Gamma,
-               &gl->PressureFree, &one, g->BaryonField[g->DensNum], p, bf0, v2, v3, &dt, &dtv);
+               &gl->PressureFree, &aye, g->BaryonField[g->DensNum], p, bf0, v2, v3, &dt, &dtv);
   free(p);
   *dt_out = dt * CourantSafetyNumber;
   return ERRCODE ? ENZO_PPM_ERR_NOT_SUPPORTED : ENZO_PPM_SUCCESS;

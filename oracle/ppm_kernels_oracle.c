@@ -1106,3 +1106,45 @@ void orc_calc_dt(const I *prank, const I *pidim, const I *pjdim, const I *pkdim,
 #undef D3
   *pdt = dt;
 }
+
+/* ------------------------------------------------------------------ */
+/* expand_terms.F:5-170 -- comoving expansion terms over isize cells,
+ * ENERGY_METHOD3 and VELOCITY_METHOD3 (the #defines at :59 and :137);
+ * p is the time-centred pressure, unchanged by these methods.  The old
+ * fields are not read by these methods; they are kept in the argument list
+ * as the reference has them. */
+void orc_expand_terms(const I *prank, const I *pisize, const I *pidual, const R *pcoef,
+                      const I *pimethod, const R *pgamma, R *p, const R *d, R *e, R *ge,
+                      R *u, R *v, R *w, const R *dold, const R *eold, const R *geold,
+                      const R *uold, const R *vold, const R *wold, const I *picr, R *ecr,
+                      const R *ecrold) {
+  const I rank = *prank, isize = *pisize, idual = *pidual, imethod = *pimethod, icr = *picr;
+  const R coef = *pcoef, gamma = *pgamma;
+  I i;
+  (void)dold; (void)eold; (void)geold; (void)uold; (void)vold; (void)wold; (void)ecrold;
+  /* gas energy (:66-83) */
+  if (idual == 1)
+    for (i = 0; i < isize; i++) ge[i] = ge[i] * (1.0 - coef) / (1.0 + coef);
+  /* cosmic-ray energy (:88-92) */
+  if (icr > 0)
+    for (i = 0; i < isize; i++) ecr[i] = ecr[i] * (2.0 - coef) / (2.0 + coef);
+  /* total energy (:93-130) */
+  if (imethod == 2) {
+    if (icr > 0)
+      for (i = 0; i < isize; i++) e[i] = e[i] * (1.0 - coef) / (1.0 + coef);
+    else
+      for (i = 0; i < isize; i++)
+        e[i] = fmx(e[i] - coef * 6.0 * p[i] / (d[i] + dold[i]), 0.5 * e[i]);
+  } else {
+    for (i = 0; i < isize; i++) {
+      e[i] = e[i] * (1.0 - coef) / (1.0 + coef);
+      e[i] = fmx(e[i] - coef * (3.0 - 2.0 / (gamma - 1.0)) * p[i] / d[i], 0.5 * e[i]);
+    }
+  }
+  /* velocities, semi-implicit (:158-166) */
+  for (i = 0; i < isize; i++) {
+    u[i] = u[i] * (1.0 - 0.5 * coef) / (1.0 + 0.5 * coef);
+    if (rank > 1) v[i] = v[i] * (1.0 - 0.5 * coef) / (1.0 + 0.5 * coef);
+    if (rank > 2) w[i] = w[i] * (1.0 - 0.5 * coef) / (1.0 + 0.5 * coef);
+  }
+}

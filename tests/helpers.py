@@ -157,3 +157,32 @@ def level_grids(kind, n=4, seed=7):
             out.append(g)
         return out, glob
     raise ValueError(kind)
+
+
+def oracle_comoving_steps(kind, g, glob, model, t, nsteps, cfl, cycle0=0, nthreads=8):
+    """`nsteps` comoving steps on the oracle in Enzo's order: periodic fill,
+    Courant step with a(t) on the unscaled widths, SolvePPM_DE with the
+    widths scaled by a(t + dt/2), then the expansion terms with the pressure
+    at the step's mid-time.  Returns (t, [dt per step])."""
+    dts = []
+    for n in range(nsteps):
+        a_now, _ = model(t)
+        g.fill_periodic()
+        dt = oracle.compute_time_step(kind, g, glob, cfl, aye=a_now)
+        a_half, _ = model(t + 0.5 * dt)
+        old = g.copy()
+        cw = g.cellwidth
+        g.cellwidth = [a_half * c if d < g.rank else c for d, c in enumerate(cw)]
+        g.dt = dt
+        try:
+            st, _ = oracle.solve_ppm_de(kind, g, glob, cycle=cycle0 + n, nthreads=nthreads)
+        finally:
+            g.cellwidth = cw
+        assert st == 1, f"oracle {kind} status {st}"
+        t_new = t + dt
+        a_mid, dadt = model(0.5 * (t_new + t))
+        coef = (0.5 * (t_new + t) - t) / (t_new - t)
+        oracle.comoving_expansion_terms(kind, g, old, glob, coef, dt * dadt / a_mid)
+        t = t_new
+        dts.append(dt)
+    return t, dts

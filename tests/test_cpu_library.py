@@ -105,6 +105,31 @@ def test_device_resident_steps(cpulib):
     assert all(r[1] for r in field_report(g, a))
 
 
+@pytest.mark.parametrize("dual", [1, 0])
+def test_comoving_steps(cpulib, dual):
+    """Comoving steps on the device state (Courant with a(t), sweeps on
+    a(t+dt/2)-scaled widths, expansion terms with the time-centred pressure)
+    against the oracle, bitwise, with gravity and 6 species."""
+    from enzo_ppm import comoving
+    from helpers import oracle_comoving_steps
+    g, glob, _ = problems.cosmological(12, seed=4)
+    glob = dict(glob, DualEnergyFormalism=dual)
+    model = comoving.EdS(0.1, 1.0)
+    ref = g.copy()
+    _, dts = oracle_comoving_steps("port", ref, glob, model, 1.0, 3, 0.4)
+    dg = device.DeviceGrid(g, glob, lib=cpulib)
+    try:
+        dg.set_comoving(True)
+        t = 1.0
+        for c in range(3):
+            t, dt = comoving.device_step(dg, model, t, c, 0.4)
+            assert dt == dts[c]
+        assert dg.download() == 1
+    finally:
+        dg.close()
+    assert all(r[1] for r in field_report(g, ref))
+
+
 # ---- batched AMR level (hipPPMLevel*) --------------------------------------
 from enzo_ppm.level import Level  # noqa: E402
 from helpers import level_grids, level_reference  # noqa: E402
