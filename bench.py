@@ -48,6 +48,9 @@ def parse():
                     help="training passes of the learned start (0: skip that part)")
     ap.add_argument("--learn-records", type=int, default=10_000_000,
                     help="harvested interface states to train on")
+    ap.add_argument("--cosmo-n", type=int, default=512,
+                    help="active cells per axis of the comoving workload (0: skip it)")
+    ap.add_argument("--cosmo-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (1 GiB read + 1 GiB write, "
                          "3 times) to scale HBM byte counters")
@@ -253,6 +256,82 @@ def learned_bench(lib, n, steps, warmup, epochs, cap):
             "held_out_newton": stats, "grid_step_learned": t}, w
 
 
+def cosmo_bench(lib, n, steps, weights, cfl=0.4):
+    """The cosmological workload: one n^3 comoving grid (lognormal density
+    with sigma_lnrho 2, Mach-100 k^-2 velocities, p = rho T with T = 1 code
+    unit, static seeded acceleration, 6 species; Einstein-de Sitter
+    expansion from a = 0.1), `steps` comoving steps each (periodic fill,
+    Courant with a(t), a(t+dt/2)-scaled sweeps, expansion terms): first the
+    batched form (the grid as a level: graph replay, with the learned start
+    when `weights` is given), then the plain per-grid form.  Times exclude
+    one warm-up step each."""
+    from enzo_ppm import comoving, device, problems
+    from enzo_ppm.level import Level
+    say = lambda m: print(f"bench: cosmological {n}^3: {m}", file=sys.stderr, flush=True)  # noqa: E731
+    t0 = time.perf_counter()
+    g, glob, _ = problems.cosmological(n, seed=5, log=say)
+    gen_s = time.perf_counter() - t0
+    model = comoving.EdS(0.1, 1.0)
+    cells = n ** 3
+    out = {"workload": f"{n}^3 comoving grid, a = 0.1 (EdS), sigma_lnrho 2, Mach 100, "
+                       "static acceleration, 6 species, dual energy, periodic",
+           "steps": steps, "generation_seconds": gen_s}
+    g0 = g.copy()
+    say("batched (level) form")
+    lv = Level([g], glob, lib=lib)
+    try:
+        lv.set_comoving(True)
+        if weights is not None:
+            lv.set_learned(weights)
+        t, _ = comoving.level_step(lv, model, 1.0, 0, cfl)
+        if lv.status()[0] != 1:
+            raise RuntimeError("level step failed")
+        t1 = time.perf_counter()
+        for c in range(1, 1 + steps):
+            t, dt = comoving.level_step(lv, model, t, c, cfl)
+        st, _, nfb = lv.status()
+        t2 = time.perf_counter()
+        if st != 1:
+            raise RuntimeError(f"level step status {st}")
+        lv.download()
+    finally:
+        lv.close()
+    mass_b = float(g.fields[0][g.active_slice()].sum())
+    out["batched_learned" if weights is not None else "batched"] = {
+        "value": cells * steps / (t2 - t1), "ms_per_step": (t2 - t1) * 1e3 / steps,
+        "fallback_cells": int(nfb), "a_end": model(t)[0]}
+    del lv
+    say("plain per-grid form")
+    g = g0
+    dg = device.DeviceGrid(g, glob, lib=lib)
+    try:
+        dg.set_comoving(True)
+        t, _ = comoving.device_step(dg, model, 1.0, 0, cfl)
+        dg.synchronize()
+        t1 = time.perf_counter()
+        for c in range(1, 1 + steps):
+            t, dt = comoving.device_step(dg, model, t, c, cfl)
+        dg.synchronize()
+        t2 = time.perf_counter()
+        if dg.download() != 1:
+            raise RuntimeError("per-grid comoving step failed")
+    finally:
+        dg.close()
+    mass_p = float(g.fields[0][g.active_slice()].sum())
+    out["plain_per_grid"] = {"value": cells * steps / (t2 - t1),
+                             "ms_per_step": (t2 - t1) * 1e3 / steps}
+    key = "batched_learned" if weights is not None else "batched"
+    out["speedup"] = out[key]["value"] / out["plain_per_grid"]["value"]
+    out["mass_relative_difference"] = abs(mass_b - mass_p) / abs(mass_p)
+    # algorithmic bytes per cell-update: 8 x [3 (R + W) + P] with R = 7 + C + G,
+    # W = 6 + C, P = 6 (C = 6 species, G = 1)
+    B = 8 * (3 * ((7 + 6 + 1) + (6 + 6)) + 6)
+    out["roofline"] = {"bytes_per_cell_update": B,
+                       "achieved_GBs": out[key]["value"] * B / 1e9,
+                       "frac": out[key]["value"] * B / 1e9 / PEAK_HBM_GBS}
+    return out
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -316,6 +395,11 @@ def main():
             out["level"] = level_bench(lib, a.level_grids, a.level_steps, weights=weights)
         except Exception as e:  # reported, not fatal
             out["level"] = {"error": str(e)}
+    if a.cosmo_n > 0:
+        try:
+            out["cosmological"] = cosmo_bench(lib, a.cosmo_n, a.cosmo_steps, weights)
+        except Exception as e:  # reported, not fatal
+            out["cosmological"] = {"error": str(e)}
     if not a.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         try:

@@ -100,36 +100,9 @@ __global__ void zero_u64_kernel(unsigned long long *p, int n) {
 // ------------------------------------------------------------------------
 // Periodic ghost fill along one axis for every slot (the periodic case of
 // the external boundary).  Thread per (slot, ghost cell).
-struct BCParams {
-  int axis, ghost, nslot;
-  int GD[3], act[3];
-  long long cells;
-  double *S;
-};
-
 __global__ void periodic_axis_kernel(BCParams B) {
-  const int a = B.axis;
-  const int b = (a + 1) % 3, c = (a + 2) % 3;
-  const long long per_slot = (long long)2 * B.ghost * B.GD[b] * B.GD[c];
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= per_slot * B.nslot) return;
-  const int slot = (int)(gid / per_slot);
-  long long r = gid % per_slot;
-  const int gl = (int)(r % (2 * B.ghost));
-  r /= 2 * B.ghost;
-  const int cb = (int)(r % B.GD[b]);
-  const int cc = (int)(r / B.GD[b]);
-  int dst = gl < B.ghost ? gl : B.ghost + B.act[a] + (gl - B.ghost);
-  int src = gl < B.ghost ? gl + B.act[a] : B.ghost + (gl - B.ghost);
-  int x[3];
-  x[b] = cb;
-  x[c] = cc;
-  x[a] = dst;
-  long long gd = ((long long)x[2] * B.GD[1] + x[1]) * B.GD[0] + x[0];
-  x[a] = src;
-  long long gs = ((long long)x[2] * B.GD[1] + x[1]) * B.GD[0] + x[0];
-  double *S = B.S + (long long)slot * B.cells;
-  S[gd] = S[gs];
+  if (gid < periodic_items(B)) periodic_item(B, gid);
 }
 
 }  // namespace

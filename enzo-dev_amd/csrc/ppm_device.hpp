@@ -49,7 +49,7 @@ using FbCfg = Cfg<1, kFbTS>;
 // (the grid of a batched level, 0 otherwise).
 __device__ inline void flag_group(const SweepParams &P, int grp) {
   if (atomicExch(&P.fb_flag[grp], 1) == 0)
-    P.fb_list[atomicAdd(P.fb_count, 1u)] = P.fb_tag | (unsigned)grp;
+    P.fb_list[atomicAdd(P.fb_count, 1u)] = P.fb_tag + (unsigned)grp;
 }
 
 // Every pencil of group `grp` (P.tp pencils along t1) redone whole with the
@@ -165,6 +165,44 @@ __device__ inline void block_min_dt(double local, unsigned long long *dtmin) {
 }
 
 // ------------------------------------------------------------------------
+// Periodic ghost fill along one axis (the periodic case of
+// ExternalBoundary::SetExternalBoundary): item gid copies one ghost cell of
+// one slot from the opposite side of the active region.
+struct BCParams {
+  int axis, ghost, nslot;
+  int GD[3], act[3];
+  long long cells;
+  double *S;
+};
+
+__device__ inline long long periodic_items(const BCParams &B) {
+  return (long long)2 * B.ghost * B.GD[(B.axis + 1) % 3] * B.GD[(B.axis + 2) % 3] * B.nslot;
+}
+
+__device__ inline void periodic_item(const BCParams &B, long long gid) {
+  const int a = B.axis;
+  const int b = (a + 1) % 3, c = (a + 2) % 3;
+  const long long per_slot = (long long)2 * B.ghost * B.GD[b] * B.GD[c];
+  const int slot = (int)(gid / per_slot);
+  long long r = gid % per_slot;
+  const int gl = (int)(r % (2 * B.ghost));
+  r /= 2 * B.ghost;
+  const int cb = (int)(r % B.GD[b]);
+  const int cc = (int)(r / B.GD[b]);
+  int dst = gl < B.ghost ? gl : B.ghost + B.act[a] + (gl - B.ghost);
+  int src = gl < B.ghost ? gl + B.act[a] : B.ghost + (gl - B.ghost);
+  int x[3];
+  x[b] = cb;
+  x[c] = cc;
+  x[a] = dst;
+  long long gd = ((long long)x[2] * B.GD[1] + x[1]) * B.GD[0] + x[0];
+  x[a] = src;
+  long long gs = ((long long)x[2] * B.GD[1] + x[1]) * B.GD[0] + x[0];
+  double *S = B.S + (long long)slot * B.cells;
+  S[gd] = S[gs];
+}
+
+// ------------------------------------------------------------------------
 // Comoving expansion terms of one cell (Grid_ComovingExpansionTerms.C:55-130
 // with expand_terms.F, ENERGY_METHOD3 / VELOCITY_METHOD3, HydroMethod 0):
 // the pressure at the interpolation weight `coef` between the state kept at
@@ -177,8 +215,8 @@ struct ExpandParams {
   long long cells;
   int rank, idual, interp;
   double gamma, tiny_number, coef, Coefficient;
-  double *A;          // current state (slots as in the sweeps)
-  const double *old;  // state at the step's start, same slot layout (interp only)
+  double *A;    // current state (slots as in the sweeps)
+  double *old;  // state at the step's start, slots S_D..S_GE (interp only)
 };
 
 __device__ inline void expand_cell(const ExpandParams &X, long long g) {

@@ -59,19 +59,23 @@ void level_sweep_kernel(const SweepParams *params, const Work *work) {
   }
 }
 
-// fallback over the level's list: entry = grid << 16 | group
+// fallback over the level's list: entries are level-wide group indices
+// (the grid's first group + the group), grp_grid maps them to their grid
 template <int LEARN>
 __global__ __launch_bounds__(kFbTS) void level_fallback_kernel(const SweepParams *params,
                                                               const unsigned *list,
+                                                              const int *grp_grid,
+                                                              const unsigned *grp_off,
                                                               unsigned *count, unsigned *done) {
   __shared__ double sm[NARR_FB * FbCfg::LW];
   DevExec ex;
   const unsigned n = *count;
   for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
     const unsigned e = list[k];
-    const SweepParams &P = params[e >> 16];
-    if (P.uni[0]) fallback_group<1, LEARN>(P, (int)(e & 0xffffu), sm, ex);
-    else fallback_group<0, LEARN>(P, (int)(e & 0xffffu), sm, ex);
+    const int g = grp_grid[e];
+    const SweepParams &P = params[g];
+    if (P.uni[0]) fallback_group<1, LEARN>(P, (int)(e - grp_off[g]), sm, ex);
+    else fallback_group<0, LEARN>(P, (int)(e - grp_off[g]), sm, ex);
   }
   fallback_launch_done(count, done);
 }
@@ -94,6 +98,36 @@ __global__ __launch_bounds__(256) void level_pressure_kernel(const PresParams *Q
   for (long long g = (long long)w.tile + threadIdx.x; g < end; g += blockDim.x)
     local = fmn(local, pressure_dt_cell(q, g));
   if (q.want_dt) block_min_dt(local, q.dtmin);
+}
+
+// comoving: the state at the step's start (slots S_D..S_GE) kept per chunk
+__global__ __launch_bounds__(256) void level_save_old_kernel(const ExpandParams *X,
+                                                             const Work *work) {
+  const Work w = work[blockIdx.x];
+  const ExpandParams &x = X[w.grid];
+  const long long lim = (long long)w.tile + kPresChunk;
+  const long long end = x.cells < lim ? x.cells : lim;
+  for (int s = 0; s <= S_GE; s++)
+    for (long long g = (long long)w.tile + threadIdx.x; g < end; g += blockDim.x)
+      x.old[s * x.cells + g] = x.A[s * x.cells + g];
+}
+
+// comoving expansion terms per chunk
+__global__ __launch_bounds__(256) void level_expand_kernel(const ExpandParams *X, const Work *work) {
+  const Work w = work[blockIdx.x];
+  const ExpandParams &x = X[w.grid];
+  const long long lim = (long long)w.tile + kPresChunk;
+  const long long end = x.cells < lim ? x.cells : lim;
+  for (long long g = (long long)w.tile + threadIdx.x; g < end; g += blockDim.x) expand_cell(x, g);
+}
+
+// periodic ghost fill of every grid along one axis: one workgroup row per grid
+__global__ void level_periodic_kernel(const BCParams *B) {
+  const BCParams &b = B[blockIdx.y];
+  const long long n = periodic_items(b);
+  for (long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x; gid < n;
+       gid += (long long)gridDim.x * blockDim.x)
+    periodic_item(b, gid);
 }
 
 __global__ void level_dt_init_kernel(unsigned long long *dtmin) {
@@ -170,6 +204,8 @@ struct hipPPMLevel {
   unsigned long long *dtmin = nullptr, *counters = nullptr;
   int *fb_flag = nullptr;
   unsigned *fb_list = nullptr, *fb_state = nullptr;
+  int *grp_grid = nullptr;          // level-wide group -> grid
+  unsigned *dgrp_off = nullptr;     // grid -> its first level-wide group
   SweepParams *dparams = nullptr;   // [parity 2][axis 3][grid]
   FlatParams *dflat = nullptr;      // [parity 2][axis 3][grid]
   PresParams *dpres = nullptr;      // [parity 2][grid]
@@ -193,6 +229,15 @@ struct hipPPMLevel {
   long long fallback_total = 0;
   float *learn = nullptr;           // learned two-shock start, when `learned`
   bool learned = false;
+  // comoving coordinates: sweeps on widths a_sweep * CellWidth, Courant pass
+  // on CellWidth (cws_dt) with aye, the state at each step's start in `old`
+  bool comoving = false, old_valid = false;
+  double a_sweep = 1.0, aye = 1.0;
+  std::vector<double> hcw;          // unscaled widths, cw_off layout
+  double *cws_dt = nullptr;
+  double *old = nullptr;            // (S_GE + 1) * cells per grid at ooff
+  std::vector<long long> ooff;
+  ExpandParams *dexp = nullptr;     // [parity 2][grid]
 };
 
 namespace {
@@ -205,8 +250,10 @@ void drop_graphs(hipPPMLevel *L) {
 void free_level(hipPPMLevel *L) {
   drop_graphs(L);
   void *ptrs[] = {L->A, L->B, L->pres, L->accel, L->tabs, L->cws, L->flat, L->qc, L->segs, L->dt,
-                  L->dtmin, L->counters, L->fb_flag, L->fb_list, L->fb_state, L->dparams,
-                  L->dflat, L->dpres, L->dwork, L->dpwork, L->learn};
+                  L->dtmin, L->counters, L->fb_flag, L->fb_list, L->fb_state, L->grp_grid,
+                  L->dgrp_off, L->dparams,
+                  L->dflat, L->dpres, L->dwork, L->dpwork, L->learn, L->cws_dt, L->old,
+                  L->dexp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (L->own_stream && L->stream) (void)hipStreamDestroy(L->stream);
@@ -240,6 +287,85 @@ void orient(int dim, bool diffusion, int &ax1, int &ax2) {
   if (dim == 0) { ax1 = 1; ax2 = 2; }
   else if (dim == 1) { ax1 = 0; ax2 = 2; }
   else { ax1 = 0; ax2 = 1; }
+}
+
+// sweep widths (a_sweep * CellWidth when comoving, the product as
+// Grid_SolveHydroEquations.C:386-408 takes it), their coefficient tables and
+// uniform-axis constants, uploaded; `uni_changed` tells whether an axis
+// changed between uniform and not (the launches depend on that)
+int upload_geometry(hipPPMLevel *L, bool *uni_changed = nullptr) {
+  std::vector<double> htabs, hcw(L->hcw.size());
+  std::vector<int> uni(L->uni.size());
+  std::vector<double> kt(L->kt.size(), 0.0);
+  for (int g = 0; g < L->ngrids; g++)
+    for (int d = 0; d < 3; d++) {
+      const int N = L->GD[3 * g + d];
+      const long long o = L->cw_off[3 * g + d];
+      std::vector<double> dx(N), tab;
+      for (int i = 0; i < N; i++)
+        dx[i] = L->comoving && d < L->rank ? L->a_sweep * L->hcw[o + i] : L->hcw[o + i];
+      build_tables(dx.data(), N, tab);
+      uni[3 * g + d] = uniform_axis(dx.data(), N, tab, &kt[((size_t)3 * g + d) * NTAB]) ? 1 : 0;
+      htabs.insert(htabs.end(), tab.begin(), tab.end());
+      std::copy(dx.begin(), dx.end(), hcw.begin() + o);
+    }
+  if (uni_changed) *uni_changed = uni != L->uni;
+  L->uni = uni;
+  L->kt = kt;
+  LCHK(hipMemcpyAsync(L->tabs, htabs.data(), sizeof(double) * htabs.size(), hipMemcpyHostToDevice,
+                      L->stream));
+  LCHK(hipMemcpyAsync(L->cws, hcw.data(), sizeof(double) * hcw.size(), hipMemcpyHostToDevice,
+                      L->stream));
+  LCHK(hipStreamSynchronize(L->stream));
+  return ENZO_PPM_SUCCESS;
+}
+
+// pressure/Courant and expansion parameters of every grid, both parities
+int upload_params(hipPPMLevel *L, double coef = 1.0, double Coefficient = 0.0) {
+  const int G = L->ngrids;
+  std::vector<PresParams> hp((size_t)2 * G);
+  std::vector<ExpandParams> hx((size_t)2 * G);
+  for (int par = 0; par < 2; par++)
+    for (int g = 0; g < G; g++) {
+      PresParams &Q = hp[(size_t)par * G + g];
+      memset(&Q, 0, sizeof(Q));
+      const hipPPMParameter &P = L->para[g];
+      Q.rank = L->rank;
+      Q.idual = (int)P.DualEnergyFormalism;
+      for (int d = 0; d < 3; d++) {
+        Q.GD[d] = L->GD[3 * g + d];
+        Q.st[d] = (int)P.GridStartIndex[d];
+        Q.en[d] = (int)P.GridEndIndex[d];
+        Q.cw[d] = (L->comoving ? L->cws_dt : L->cws) + L->cw_off[3 * g + d];
+      }
+      Q.cells = L->cells[g];
+      Q.A = (par == 0 ? L->A : L->B) + L->off[g];
+      Q.pres = L->pres + L->poff[g];
+      Q.gamma = P.Gamma;
+      Q.tiny_number = P.MinimumPressure;
+      Q.min_support = 0.0;
+      Q.aye = L->comoving ? L->aye : 1.0;
+      Q.want_dt = 1;
+      Q.dtmin = L->dtmin;
+      ExpandParams &X = hx[(size_t)par * G + g];
+      memset(&X, 0, sizeof(X));
+      X.cells = L->cells[g];
+      X.rank = L->rank;
+      X.idual = (int)P.DualEnergyFormalism;
+      X.interp = L->old_valid && coef != 1.0 ? 1 : 0;
+      X.gamma = P.Gamma;
+      X.tiny_number = P.MinimumPressure;
+      X.coef = coef;
+      X.Coefficient = Coefficient;
+      X.A = (par == 0 ? L->A : L->B) + L->off[g];
+      X.old = L->old ? L->old + L->ooff[g] : nullptr;
+    }
+  LCHK(hipMemcpyAsync(L->dpres, hp.data(), sizeof(PresParams) * hp.size(), hipMemcpyHostToDevice,
+                      L->stream));
+  LCHK(hipMemcpyAsync(L->dexp, hx.data(), sizeof(ExpandParams) * hx.size(),
+                      hipMemcpyHostToDevice, L->stream));
+  LCHK(hipStreamSynchronize(L->stream));
+  return ENZO_PPM_SUCCESS;
 }
 
 // SweepParams, FlatParams and work lists for the current shape choice
@@ -336,7 +462,7 @@ int build_plan(hipPPMLevel *L) {
         S.fb_list = L->fb_list;
         S.fb_count = L->fb_state;
         S.fb_done = L->fb_state + 1;
-        S.fb_tag = (unsigned)g << 16;
+        S.fb_tag = (unsigned)L->grp_off[g];
         S.fb_ok = pencil_fits<kFbTS>(S) ? 1 : 0;
         if ((long long)S.nblk_t1 * S.n_t2 > 65535) return ENZO_PPM_ERR_ARGUMENT;
         FlatParams &F = hflat[((size_t)par * 3 + d) * G + g];
@@ -449,10 +575,10 @@ int launch_level_sweep(hipPPMLevel *L, int d) {
     }
     if (L->learned)
       hipLaunchKernelGGL(level_fallback_kernel<1>, dim3(kFbBlocks), dim3(kFbTS), 0, L->stream, P,
-                         L->fb_list, L->fb_state, L->fb_state + 1);
+                         L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
     else
       hipLaunchKernelGGL(level_fallback_kernel<0>, dim3(kFbBlocks), dim3(kFbTS), 0, L->stream, P,
-                         L->fb_list, L->fb_state, L->fb_state + 1);
+                         L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
     LCHK(hipGetLastError());
   }
   L->parity ^= 1;
@@ -462,6 +588,11 @@ int launch_level_sweep(hipPPMLevel *L, int d) {
 // one level step: pressure (+ Courant dt) of every grid, then the sweeps in
 // the rotated order of CycleNumber
 int enqueue_step(hipPPMLevel *L, int cycle, int want_dt, double cfl) {
+  if (L->comoving) {
+    hipLaunchKernelGGL(level_save_old_kernel, dim3((unsigned)L->npwork), dim3(256), 0, L->stream,
+                       L->dexp + (size_t)L->parity * L->ngrids, L->dpwork);
+    LCHK(hipGetLastError());
+  }
   if (want_dt)
     hipLaunchKernelGGL(level_dt_init_kernel, dim3(1), dim3(64), 0, L->stream, L->dtmin);
   hipLaunchKernelGGL(level_pressure_kernel, dim3((unsigned)L->npwork), dim3(256), 0, L->stream,
@@ -530,7 +661,6 @@ int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMP
   L->seg_off.assign((size_t)G * 3 * kShapes, 0);
   L->seg_n.assign((size_t)G * 3 * kShapes, 0);
   long long tot = 0, ptot = 0, ttot = 0, wtot = 0, ftot = 0, qtot = 0;
-  std::vector<double> htabs, hcw;
   for (int g = 0; g < G; g++) {
     const hipPPMParameter &P = Para[g];
     long long c = 1;
@@ -547,16 +677,11 @@ int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMP
     int nmax = 1;
     for (int d = 0; d < 3; d++) {
       const int N = L->GD[3 * g + d];
-      std::vector<double> dx(N), tab;
-      for (int i = 0; i < N; i++)
-        dx[i] = P.CellWidth[d] ? P.CellWidth[d][i] : (d < P.GridRank ? P.dx[d] : 1.0);
-      build_tables(dx.data(), N, tab);
-      L->uni[3 * g + d] = uniform_axis(dx.data(), N, tab, &L->kt[((size_t)3 * g + d) * NTAB]) ? 1 : 0;
       L->tab_off[3 * g + d] = ttot;
-      htabs.insert(htabs.end(), tab.begin(), tab.end());
-      ttot += (long long)tab.size();
+      ttot += (long long)NTAB * N;
       L->cw_off[3 * g + d] = wtot;
-      hcw.insert(hcw.end(), dx.begin(), dx.end());
+      for (int i = 0; i < N; i++)
+        L->hcw.push_back(P.CellWidth[d] ? P.CellWidth[d][i] : (d < P.GridRank ? P.dx[d] : 1.0));
       wtot += N;
       fcap = std::max(fcap, (long long)N * L->GD[3 * g + (d + 2) % 3]);
       nmax = std::max(nmax, N);
@@ -587,12 +712,20 @@ int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMP
       }
     }
   long long ngroups = 0;
+  std::vector<int> hgrp_grid;
+  std::vector<unsigned> hgrp_off(G);
   for (int g = 0; g < G; g++) {
     long long m = 1;
     for (int a = 0; a < 3; a++)
       for (int b = 0; b < 3; b++)
         if (a != b) m = std::max(m, (long long)((L->GD[3 * g + a] + 3) / 4) * L->GD[3 * g + b]);
+    hgrp_off[g] = (unsigned)ngroups;
+    hgrp_grid.insert(hgrp_grid.end(), (size_t)m, g);
     ngroups += m;
+  }
+  if (ngroups >= (1ll << 32)) {
+    delete L;
+    return ENZO_PPM_ERR_ARGUMENT;
   }
   L->ngroups = ngroups;
   // pressure chunks
@@ -612,6 +745,7 @@ int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMP
       dev_alloc(&L->segs, L->hsegs.size()) || dev_alloc(&L->dt, 1) || dev_alloc(&L->dtmin, 1) ||
       dev_alloc(&L->counters, 4) || dev_alloc(&L->fb_flag, ngroups) ||
       dev_alloc(&L->fb_list, ngroups) || dev_alloc(&L->fb_state, 2) ||
+      dev_alloc(&L->grp_grid, ngroups) || dev_alloc(&L->dgrp_off, G) ||
       dev_alloc(&L->dpres, (size_t)2 * G) || dev_alloc(&L->dpwork, pw.size()))
     return fail(ENZO_PPM_ERR_HIP);
   if (stream) {
@@ -620,43 +754,19 @@ int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMP
     if (hipStreamCreate(&L->stream) != hipSuccess) return fail(ENZO_PPM_ERR_HIP);
     L->own_stream = true;
   }
-  if (hipMemcpy(L->tabs, htabs.data(), sizeof(double) * htabs.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(L->cws, hcw.data(), sizeof(double) * hcw.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      (!L->hsegs.empty() && hipMemcpy(L->segs, L->hsegs.data(), sizeof(int) * L->hsegs.size(),
+  if (upload_geometry(L) != ENZO_PPM_SUCCESS) return fail(ENZO_PPM_ERR_HIP);
+  if ((!L->hsegs.empty() && hipMemcpy(L->segs, L->hsegs.data(), sizeof(int) * L->hsegs.size(),
                                       hipMemcpyHostToDevice) != hipSuccess) ||
       hipMemcpy(L->dpwork, pw.data(), sizeof(Work) * pw.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(L->fb_flag, 0, sizeof(int) * ngroups) != hipSuccess ||
+      hipMemcpy(L->grp_grid, hgrp_grid.data(), sizeof(int) * ngroups, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(L->dgrp_off, hgrp_off.data(), sizeof(unsigned) * G, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(L->fb_state, 0, 2 * sizeof(unsigned)) != hipSuccess ||
       hipMemset(L->counters, 0, 4 * sizeof(unsigned long long)) != hipSuccess)
     return fail(ENZO_PPM_ERR_HIP);
   if (L->gravity && hipMemset(L->accel, 0, sizeof(double) * 3 * ptot) != hipSuccess)
     return fail(ENZO_PPM_ERR_HIP);
-  // pressure parameters of both parities
-  std::vector<PresParams> hp((size_t)2 * G);
-  for (int par = 0; par < 2; par++)
-    for (int g = 0; g < G; g++) {
-      PresParams &Q = hp[(size_t)par * G + g];
-      memset(&Q, 0, sizeof(Q));
-      const hipPPMParameter &P = Para[g];
-      Q.rank = L->rank;
-      Q.idual = (int)P.DualEnergyFormalism;
-      for (int d = 0; d < 3; d++) {
-        Q.GD[d] = L->GD[3 * g + d];
-        Q.st[d] = (int)P.GridStartIndex[d];
-        Q.en[d] = (int)P.GridEndIndex[d];
-        Q.cw[d] = L->cws + L->cw_off[3 * g + d];
-      }
-      Q.cells = L->cells[g];
-      Q.A = (par == 0 ? L->A : L->B) + L->off[g];
-      Q.pres = L->pres + L->poff[g];
-      Q.gamma = P.Gamma;
-      Q.tiny_number = P.MinimumPressure;
-      Q.min_support = 0.0;
-      Q.aye = 1.0;
-      Q.want_dt = 1;
-      Q.dtmin = L->dtmin;
-    }
-  if (hipMemcpy(L->dpres, hp.data(), sizeof(PresParams) * hp.size(), hipMemcpyHostToDevice) != hipSuccess)
+  if (dev_alloc(&L->dexp, (size_t)2 * G) || upload_params(L) != ENZO_PPM_SUCCESS)
     return fail(ENZO_PPM_ERR_HIP);
   st = build_plan(L);
   if (st != ENZO_PPM_SUCCESS) return fail(st);
@@ -721,7 +831,10 @@ int hipPPMLevelStep(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dtFixed, en
   if (!L || CycleNumber < 0) return ENZO_PPM_ERR_ARGUMENT;
   const int want_dt = dtFixed > 0.0 ? 0 : 1;
   if (!want_dt) LCHK(hipMemcpyAsync(L->dt, &dtFixed, sizeof(double), hipMemcpyHostToDevice, L->stream));
-  if (!L->use_graph) return enqueue_step(L, (int)(CycleNumber % L->rank), want_dt, cfl);
+  if (!L->use_graph) {
+    L->old_valid = L->comoving;
+    return enqueue_step(L, (int)(CycleNumber % L->rank), want_dt, cfl);
+  }
   // the launches of a step depend on the rotation, the buffer parity and the
   // dt mode
   const int rot = (int)(CycleNumber % L->rank);
@@ -754,6 +867,7 @@ int hipPPMLevelStep(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dtFixed, en
   }
   LCHK(hipGraphLaunch(it->second, L->stream));
   L->parity ^= (L->rank & 1);  // one swap per sweep
+  L->old_valid = L->comoving;
   return ENZO_PPM_SUCCESS;
 }
 
@@ -858,6 +972,128 @@ int hipPPMLevelTune(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dt, enzo_in
   (void)hipFree(save);
   if (shape_by_length)
     for (int c = 0; c < 4; c++) shape_by_length[c] = choice[c];
+  return st;
+}
+
+int hipPPMLevelSetComoving(hipPPMLevel *L, enzo_int enable) {
+  if (!L) return ENZO_PPM_ERR_ARGUMENT;
+  LCHK(hipStreamSynchronize(L->stream));
+  if (enable) {
+    if (!L->cws_dt) {
+      if (dev_alloc(&L->cws_dt, L->hcw.size())) return ENZO_PPM_ERR_HIP;
+      LCHK(hipMemcpy(L->cws_dt, L->hcw.data(), sizeof(double) * L->hcw.size(),
+                     hipMemcpyHostToDevice));
+    }
+    if (!L->old) {
+      long long tot = 0;
+      L->ooff.assign(L->ngrids, 0);
+      for (int g = 0; g < L->ngrids; g++) {
+        L->ooff[g] = tot;
+        tot += (long long)(S_GE + 1) * L->cells[g];
+      }
+      if (dev_alloc(&L->old, (size_t)tot)) return ENZO_PPM_ERR_HIP;
+    }
+  }
+  L->comoving = enable != 0;
+  L->old_valid = false;
+  if (!L->comoving) L->a_sweep = L->aye = 1.0;
+  int st = upload_geometry(L);
+  if (st == ENZO_PPM_SUCCESS) st = upload_params(L);
+  if (st == ENZO_PPM_SUCCESS) st = build_plan(L);  // graphs gain or lose the state copy
+  return st;
+}
+
+int hipPPMLevelSetExpansionFactors(hipPPMLevel *L, enzo_float a_courant, enzo_float a_sweep) {
+  if (!L || !L->comoving || !(a_courant > 0.0) || !(a_sweep > 0.0)) return ENZO_PPM_ERR_ARGUMENT;
+  LCHK(hipStreamSynchronize(L->stream));
+  L->aye = a_courant;
+  int st = upload_params(L);
+  if (st != ENZO_PPM_SUCCESS || a_sweep == L->a_sweep) return st;
+  L->a_sweep = a_sweep;
+  bool uni_changed = false;
+  st = upload_geometry(L, &uni_changed);
+  if (st != ENZO_PPM_SUCCESS) return st;
+  if (uni_changed) return build_plan(L);
+  // same launches: only the uniform-axis constants in the device-resident
+  // sweep parameters change, which captured graphs read at replay
+  const int G = L->ngrids;
+  for (int par = 0; par < 2; par++)
+    for (int d = 0; d < L->rank; d++)
+      for (int g = 0; g < G; g++) {
+        SweepParams &S = L->hparams[((size_t)par * 3 + d) * G + g];
+        const int rax = (d + 1) % 3, oax = (d + 2) % 3;
+        for (int t = 0; t < NTAB; t++) {
+          S.kt[t] = L->kt[((size_t)3 * g + d) * NTAB + t];
+          S.kt_row[t] = L->kt[((size_t)3 * g + rax) * NTAB + t];
+          S.kt_out[t] = L->kt[((size_t)3 * g + oax) * NTAB + t];
+        }
+      }
+  LCHK(hipMemcpy(L->dparams, L->hparams.data(), sizeof(SweepParams) * L->hparams.size(),
+                 hipMemcpyHostToDevice));
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelTimeStep(hipPPMLevel *L, enzo_float cfl, enzo_float *dt_host) {
+  if (!L || !dt_host) return ENZO_PPM_ERR_ARGUMENT;
+  hipLaunchKernelGGL(level_dt_init_kernel, dim3(1), dim3(64), 0, L->stream, L->dtmin);
+  hipLaunchKernelGGL(level_pressure_kernel, dim3((unsigned)L->npwork), dim3(256), 0, L->stream,
+                     L->dpres + (size_t)L->parity * L->ngrids, L->dpwork);
+  LCHK(hipGetLastError());
+  hipLaunchKernelGGL(level_dt_final_kernel, dim3(1), dim3(64), 0, L->stream, L->dtmin, cfl, L->dt);
+  LCHK(hipGetLastError());
+  LCHK(hipMemcpyAsync(dt_host, L->dt, sizeof(double), hipMemcpyDeviceToHost, L->stream));
+  LCHK(hipStreamSynchronize(L->stream));
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelComovingExpansionTerms(hipPPMLevel *L, enzo_float coef, enzo_float Coefficient) {
+  if (!L || !L->comoving) return ENZO_PPM_ERR_ARGUMENT;
+  LCHK(hipStreamSynchronize(L->stream));
+  int st = upload_params(L, coef, Coefficient);
+  if (st != ENZO_PPM_SUCCESS) return st;
+  hipLaunchKernelGGL(level_expand_kernel, dim3((unsigned)L->npwork), dim3(256), 0, L->stream,
+                     L->dexp + (size_t)L->parity * L->ngrids, L->dpwork);
+  LCHK(hipGetLastError());
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelPeriodicBoundary(hipPPMLevel *L) {
+  if (!L) return ENZO_PPM_ERR_ARGUMENT;
+  const int G = L->ngrids;
+  std::vector<BCParams> hb((size_t)3 * G);
+  long long nmax = 1;
+  for (int a = 0; a < L->rank; a++)
+    for (int g = 0; g < G; g++) {
+      const hipPPMParameter &P = L->para[g];
+      BCParams &B = hb[(size_t)a * G + g];
+      B.axis = a;
+      B.ghost = (int)P.GridStartIndex[a];
+      B.nslot = L->nslot;
+      for (int d = 0; d < 3; d++) {
+        B.GD[d] = L->GD[3 * g + d];
+        B.act[d] = (int)(P.GridEndIndex[d] - P.GridStartIndex[d] + 1);
+      }
+      if (P.GridDimension[a] - P.GridEndIndex[a] - 1 != B.ghost) return ENZO_PPM_ERR_ARGUMENT;
+      B.cells = L->cells[g];
+      B.S = (L->parity == 0 ? L->A : L->B) + L->off[g];
+      nmax = std::max(nmax, (long long)2 * B.ghost * B.GD[(a + 1) % 3] * B.GD[(a + 2) % 3] *
+                                B.nslot);
+    }
+  BCParams *db = nullptr;
+  if (dev_alloc(&db, hb.size())) return ENZO_PPM_ERR_HIP;
+  int st = ENZO_PPM_SUCCESS;
+  if (hipMemcpyAsync(db, hb.data(), sizeof(BCParams) * hb.size(), hipMemcpyHostToDevice,
+                     L->stream) != hipSuccess)
+    st = ENZO_PPM_ERR_HIP;
+  const unsigned bx = (unsigned)std::min<long long>((nmax + 255) / 256, 65535);
+  for (int a = 0; a < L->rank && st == ENZO_PPM_SUCCESS; a++) {
+    hipLaunchKernelGGL(level_periodic_kernel, dim3(bx, (unsigned)G), dim3(256), 0, L->stream,
+                       db + (size_t)a * G);
+    if (hipGetLastError() != hipSuccess) st = ENZO_PPM_ERR_HIP;
+  }
+  if (hipStreamSynchronize(L->stream) != hipSuccess) st = ENZO_PPM_ERR_HIP;
+  (void)hipFree(db);
+  if (st != ENZO_PPM_SUCCESS) snprintf(g_level_error, sizeof(g_level_error), "periodic fill failed");
   return st;
 }
 

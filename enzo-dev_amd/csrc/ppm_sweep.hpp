@@ -119,7 +119,7 @@ struct SweepParams {
   // and the count of finished fallback workgroups (which clear both)
   int *fb_flag;
   unsigned *fb_list, *fb_count, *fb_done;
-  unsigned fb_tag;  // or-ed into list entries (grid << 16 in a batched level)
+  unsigned fb_tag;  // added to list entries (the grid's first group in a batched level)
   int fb_ok;        // a one-pencil workgroup holds the whole sweep axis
   int tp;           // pencils per group (the sweep tile's TP)
   // learned two-shock start: network parameters (ppm_learn.hpp), read by

@@ -179,6 +179,11 @@ PROTOTYPES = {
     "hipPPMLevelCommitShapes": (C.c_int, [C.c_void_p]),
     "hipPPMLevelTune": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_int64, PI]),
     "hipPPMLevelSetLearnedGuess": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hipPPMLevelSetComoving": (C.c_int, [C.c_void_p, C.c_int64]),
+    "hipPPMLevelSetExpansionFactors": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "hipPPMLevelTimeStep": (C.c_int, [C.c_void_p, C.c_double, PF]),
+    "hipPPMLevelComovingExpansionTerms": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "hipPPMLevelPeriodicBoundary": (C.c_int, [C.c_void_p]),
     "hipPPMLevelDestroy": (C.c_int, [C.c_void_p]),
     "hipPPMSetLearnedGuess": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hipPPMSetComoving": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_int64]),

@@ -44,3 +44,21 @@ def device_step(dg, model, t, cycle, cfl, boundary=True):
     coef = (0.5 * (t_new + t) - t) / (t_new - t)
     dg.comoving_expansion_terms(coef, dt * dadt / a_mid)
     return t_new, dt
+
+
+def level_step(lv, model, t, cycle, cfl, boundary=True):
+    """One comoving step of a Level with comoving handling on (every grid
+    takes the level's Courant step).  Returns (new time, dt)."""
+    a_now, _ = model(t)
+    lv.set_expansion_factors(a_now, lv.a_sweep)
+    if boundary:
+        lv.periodic_boundary()
+    dt = lv.time_step(cfl)
+    a_half, _ = model(t + 0.5 * dt)
+    lv.set_expansion_factors(a_now, a_half)
+    lv.step(cycle, dt, cfl)
+    t_new = t + dt
+    a_mid, dadt = model(0.5 * (t_new + t))
+    coef = (0.5 * (t_new + t) - t) / (t_new - t)
+    lv.comoving_expansion_terms(coef, dt * dadt / a_mid)
+    return t_new, dt

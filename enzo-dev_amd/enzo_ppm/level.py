@@ -74,6 +74,28 @@ class Level:
         self._check(self.lib.hipPPMLevelTune(self.handle, cycle, dt, repeats, out), "LevelTune")
         return list(out)
 
+    def set_comoving(self, enable=True):
+        self._check(self.lib.hipPPMLevelSetComoving(self.handle, int(enable)), "LevelSetComoving")
+        self.a_sweep = 1.0
+
+    def set_expansion_factors(self, a_courant, a_sweep):
+        self._check(self.lib.hipPPMLevelSetExpansionFactors(self.handle, a_courant, a_sweep),
+                    "LevelSetExpansionFactors")
+        self.a_sweep = a_sweep
+
+    def time_step(self, cfl):
+        """The level's Courant step (device), returned to the host."""
+        dt = C.c_double(0.0)
+        self._check(self.lib.hipPPMLevelTimeStep(self.handle, cfl, C.byref(dt)), "LevelTimeStep")
+        return dt.value
+
+    def comoving_expansion_terms(self, coef, coefficient):
+        self._check(self.lib.hipPPMLevelComovingExpansionTerms(self.handle, coef, coefficient),
+                    "LevelComovingExpansionTerms")
+
+    def periodic_boundary(self):
+        self._check(self.lib.hipPPMLevelPeriodicBoundary(self.handle), "LevelPeriodicBoundary")
+
     def set_learned(self, weights):
         """Learned two-shock start for every grid (None: reference start)."""
         if weights is None:

@@ -40,11 +40,19 @@ def sod_x(n=64, rank=3, gamma=GAMMA, dual_energy=True, flags=(1, 1, 1), cfl=0.4)
     return g, glob, dict(cfl=cfl, bc="periodic")
 
 
-def gaussian_random_field(n, kmin, kmax, slope, rng, ncomp=1):
+def gaussian_random_field(n, kmin, kmax, slope, rng, ncomp=1, sparse=False):
     """Real periodic GRF on an n^3 grid with power P(k) ~ k^slope on integer
-    shells kmin <= |k| <= kmax (zero mean, unit variance per component)."""
+    shells kmin <= |k| <= kmax (zero mean, unit variance per component).
+    sparse: draw random numbers only for the modes inside the shells (a
+    different seeded field, much cheaper at large n)."""
     kx = np.fft.fftfreq(n, d=1.0 / n)
-    kz, ky, kxx = np.meshgrid(kx, kx, np.fft.rfftfreq(n, d=1.0 / n), indexing="ij")
+    if sparse:
+        kc = int(kmax) + 1
+        lo = kx[np.abs(kx) <= kc]
+        kr_x = np.fft.rfftfreq(n, d=1.0 / n)
+        kz, ky, kxx = np.meshgrid(lo, lo, kr_x[kr_x <= kc], indexing="ij")
+    else:
+        kz, ky, kxx = np.meshgrid(kx, kx, np.fft.rfftfreq(n, d=1.0 / n), indexing="ij")
     k = np.sqrt(kxx * kxx + ky * ky + kz * kz)
     kr = np.rint(k)
     mask = (kr >= kmin) & (kr <= kmax)
@@ -53,7 +61,15 @@ def gaussian_random_field(n, kmin, kmax, slope, rng, ncomp=1):
     out = []
     for _ in range(ncomp):
         ph = rng.standard_normal(k.shape) + 1j * rng.standard_normal(k.shape)
-        f = np.fft.irfftn(ph * amp, s=(n, n, n), axes=(0, 1, 2))
+        if sparse:
+            full = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
+            iz = (kz % n).astype(np.intp)
+            iy = (ky % n).astype(np.intp)
+            full[iz, iy, kxx.astype(np.intp)] = ph * amp
+            f = np.fft.irfftn(full, s=(n, n, n), axes=(0, 1, 2))
+            del full
+        else:
+            f = np.fft.irfftn(ph * amp, s=(n, n, n), axes=(0, 1, 2))
         f -= f.mean()
         f /= f.std()
         out.append(f)
@@ -104,13 +120,13 @@ def cosmological(n=512, seed=5, sigma_lnrho=2.0, mach=100.0, T=1.0, ncolours=6, 
     g = Grid([n] * 3, ncolours=ncolours, dual_energy=True, gravity=True, dx=1.0 / n)
     a = g.active_slice()
     say("density")
-    lnrho = gaussian_random_field(n, 1, 16, -2.0, rng)[0] * sigma_lnrho
+    lnrho = gaussian_random_field(n, 1, 16, -2.0, rng, sparse=True)[0] * sigma_lnrho
     rho = np.exp(lnrho)
     del lnrho
     rho /= rho.mean()
     cs = np.sqrt(gamma * T)
     say("velocity")
-    vel = gaussian_random_field(n, 1, 16, -2.0, rng, ncomp=3)
+    vel = gaussian_random_field(n, 1, 16, -2.0, rng, ncomp=3, sparse=True)
     vrms = np.sqrt(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2))
     for v in vel:
         v *= mach * cs / vrms
@@ -121,7 +137,7 @@ def cosmological(n=512, seed=5, sigma_lnrho=2.0, mach=100.0, T=1.0, ncolours=6, 
     for c in g.colnum:
         g.fields[c][a] = rng.uniform(0.0, 1.0, size=(n, n, n))
     say("acceleration")
-    acc = gaussian_random_field(n, 1, 16, -2.0, rng, ncomp=3)
+    acc = gaussian_random_field(n, 1, 16, -2.0, rng, ncomp=3, sparse=True)
     arms = np.sqrt(np.mean(acc[0] ** 2 + acc[1] ** 2 + acc[2] ** 2))
     for d in range(3):
         g.accel[d][a] = acc[d] * (g_frac * (mach * cs) ** 2 / arms)

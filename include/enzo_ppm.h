@@ -395,6 +395,22 @@ int hipPPMLevelCommitShapes(hipPPMLevel *Level);
 int hipPPMLevelTune(hipPPMLevel *Level, enzo_int CycleNumber, enzo_float dt, enzo_int repeats,
                     enzo_int *shape_by_length);
 
+/* Comoving coordinates for the level, as hipPPMSetComoving /
+   hipPPMSetExpansionFactors / hipPPMComovingExpansionTerms for one grid:
+   with comoving on, every hipPPMLevelStep first keeps the state of each
+   grid.  hipPPMLevelTimeStep computes the level's Courant step (pressure
+   and calc_dt with a_courant) and returns it, so that the caller can take
+   a(t + dt/2) before hipPPMLevelStep(..., dt, ...). */
+int hipPPMLevelSetComoving(hipPPMLevel *Level, enzo_int enable);
+int hipPPMLevelSetExpansionFactors(hipPPMLevel *Level, enzo_float a_courant, enzo_float a_sweep);
+int hipPPMLevelTimeStep(hipPPMLevel *Level, enzo_float CourantSafetyNumber, enzo_float *dt);
+int hipPPMLevelComovingExpansionTerms(hipPPMLevel *Level, enzo_float coef, enzo_float Coefficient);
+
+/* Periodic ghost fill of every grid of the level (for grids that span the
+   domain, as the root grid does; the periodic case of
+   ExternalBoundary::SetExternalBoundary). */
+int hipPPMLevelPeriodicBoundary(hipPPMLevel *Level);
+
 /* The learned two-shock start for every grid of the level (see
    hipPPMSetLearnedGuess; NULL switches back to the reference start). */
 int hipPPMLevelSetLearnedGuess(hipPPMLevel *Level, const float *weights);

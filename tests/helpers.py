@@ -186,3 +186,35 @@ def oracle_comoving_steps(kind, g, glob, model, t, nsteps, cfl, cycle0=0, nthrea
         t = t_new
         dts.append(dt)
     return t, dts
+
+
+def oracle_comoving_level(grids, glob, model, t, nsteps, cfl, boundary=True):
+    """The oracle's per-grid loop for a comoving level: periodic fill of
+    every grid (when `boundary`), the level's Courant step (minimum over the
+    grids, with a(t)), then per grid SolvePPM_DE on a(t+dt/2)-scaled widths
+    and the expansion terms.  Returns (t, [dt per step])."""
+    dts = []
+    for n in range(nsteps):
+        a_now, _ = model(t)
+        if boundary:
+            for g in grids:
+                g.fill_periodic()
+        dt = min(oracle.compute_time_step("port", g, glob, cfl, aye=a_now) for g in grids)
+        a_half, _ = model(t + 0.5 * dt)
+        t_new = t + dt
+        a_mid, dadt = model(0.5 * (t_new + t))
+        coef = (0.5 * (t_new + t) - t) / (t_new - t)
+        for g in grids:
+            old = g.copy()
+            cw = g.cellwidth
+            g.cellwidth = [a_half * c if d < g.rank else c for d, c in enumerate(cw)]
+            g.dt = dt
+            try:
+                st, _ = oracle.solve_ppm_de("port", g, glob, cycle=n)
+            finally:
+                g.cellwidth = cw
+            assert st == 1
+            oracle.comoving_expansion_terms("port", g, old, glob, coef, dt * dadt / a_mid)
+        t = t_new
+        dts.append(dt)
+    return t, dts

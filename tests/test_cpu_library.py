@@ -219,3 +219,31 @@ def test_level_tune_restores_state(cpulib):
     level_reference(ref, glob, 1, dt=dt)
     for a, b in zip(lv.grids, ref):
         assert all(r[1] for r in field_report(a, b))
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_level_comoving(cpulib, graph):
+    """A comoving level (two cosmological grids, periodic fill on the device,
+    level Courant step with a(t), a(t+dt/2)-scaled widths, expansion terms)
+    against the oracle's per-grid loop, bitwise."""
+    from enzo_ppm import comoving
+    from helpers import oracle_comoving_level
+    grids = [problems.cosmological(n, seed=n)[0] for n in (12, 10)]
+    glob = problems.cosmological(4, seed=0)[1]
+    model = comoving.EdS(0.1, 1.0)
+    ref = [g.copy() for g in grids]
+    _, dts = oracle_comoving_level(ref, glob, model, 1.0, 3, 0.4)
+    lv = Level([g.copy() for g in grids], glob, lib=cpulib)
+    try:
+        lv.use_graph(graph)
+        lv.set_comoving(True)
+        t = 1.0
+        for c in range(3):
+            t, dt = comoving.level_step(lv, model, t, c, 0.4)
+            assert dt == dts[c]
+            assert lv.status()[0] == 1
+        lv.download()
+    finally:
+        lv.close()
+    for a, b in zip(lv.grids, ref):
+        assert all(r[1] for r in field_report(a, b))
