@@ -47,7 +47,7 @@ constexpr int kShapeTS[kShapes] = {16, 32, 64};
 
 template <int TP, int TS, int UNI, int LEARN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-void level_sweep_kernel(const SweepParams *params, const Work *work) {
+void level_sweep_kernel(const SweepParams *__restrict__ params, const Work *__restrict__ work) {
   __shared__ double sm[Cfg<TP, TS>::SMEM];
   DevExec ex;
   const Work w = work[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
@@ -62,7 +62,7 @@ void level_sweep_kernel(const SweepParams *params, const Work *work) {
 // fallback over the level's list: entries are level-wide group indices
 // (the grid's first group + the group), grp_grid maps them to their grid
 template <int LEARN>
-__global__ __launch_bounds__(kFbTS) void level_fallback_kernel(const SweepParams *params,
+__global__ __launch_bounds__(kFbTS) void level_fallback_kernel(const SweepParams *__restrict__ params,
                                                               const unsigned *list,
                                                               const int *grp_grid,
                                                               const unsigned *grp_off,
@@ -81,15 +81,18 @@ __global__ __launch_bounds__(kFbTS) void level_fallback_kernel(const SweepParams
 }
 
 // per-sweep preparation of every grid: one workgroup per grid
-__global__ void level_prep_kernel(const FlatParams *F) {
-  const FlatParams &f = F[blockIdx.x];
+__global__ void level_prep_kernel(const FlatParams *__restrict__ F) {
+  const FlatParams &f = F[blockIdx.y];
   const long long n = f.N + (f.iflatten ? (long long)(f.i2 - f.i1 + 3) * f.n_outer : 0);
-  for (long long gid = threadIdx.x; gid < n; gid += blockDim.x) prep_item(f, gid);
+  for (long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x; gid < n;
+       gid += (long long)gridDim.x * blockDim.x)
+    prep_item(f, gid);
 }
 
 // pressure + Courant over chunks of kPresChunk cells: work = (grid, first cell)
 constexpr int kPresChunk = 4096;
-__global__ __launch_bounds__(256) void level_pressure_kernel(const PresParams *Q, const Work *work) {
+__global__ __launch_bounds__(256) void level_pressure_kernel(const PresParams *__restrict__ Q,
+                                                             const Work *__restrict__ work) {
   const Work w = work[blockIdx.x];
   const PresParams &q = Q[w.grid];
   double local = 1.0e20;
@@ -101,8 +104,8 @@ __global__ __launch_bounds__(256) void level_pressure_kernel(const PresParams *Q
 }
 
 // comoving: the state at the step's start (slots S_D..S_GE) kept per chunk
-__global__ __launch_bounds__(256) void level_save_old_kernel(const ExpandParams *X,
-                                                             const Work *work) {
+__global__ __launch_bounds__(256) void level_save_old_kernel(const ExpandParams *__restrict__ X,
+                                                             const Work *__restrict__ work) {
   const Work w = work[blockIdx.x];
   const ExpandParams &x = X[w.grid];
   const long long lim = (long long)w.tile + kPresChunk;
@@ -113,7 +116,8 @@ __global__ __launch_bounds__(256) void level_save_old_kernel(const ExpandParams 
 }
 
 // comoving expansion terms per chunk
-__global__ __launch_bounds__(256) void level_expand_kernel(const ExpandParams *X, const Work *work) {
+__global__ __launch_bounds__(256) void level_expand_kernel(const ExpandParams *__restrict__ X,
+                                                           const Work *__restrict__ work) {
   const Work w = work[blockIdx.x];
   const ExpandParams &x = X[w.grid];
   const long long lim = (long long)w.tile + kPresChunk;
@@ -122,7 +126,7 @@ __global__ __launch_bounds__(256) void level_expand_kernel(const ExpandParams *X
 }
 
 // periodic ghost fill of every grid along one axis: one workgroup row per grid
-__global__ void level_periodic_kernel(const BCParams *B) {
+__global__ void level_periodic_kernel(const BCParams *__restrict__ B) {
   const BCParams &b = B[blockIdx.y];
   const long long n = periodic_items(b);
   for (long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x; gid < n;
@@ -219,6 +223,7 @@ struct hipPPMLevel {
   };
   std::vector<Launch> launches[3];
   std::vector<long long> launch_slices[3];  // index of the first launch of each slice
+  int prep_blocks[3] = {1, 1, 1};           // workgroups per grid of the prep launch
   std::vector<SweepParams> hparams;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -389,6 +394,7 @@ int build_plan(hipPPMLevel *L) {
   for (int d = 0; d < 3; d++) {
     L->launches[d].clear();
     L->launch_slices[d].clear();
+    L->prep_blocks[d] = 1;
   }
   for (int d = 0; d < L->rank; d++) {
     // parameters of every grid for axis d
@@ -464,7 +470,6 @@ int build_plan(hipPPMLevel *L) {
         S.fb_done = L->fb_state + 1;
         S.fb_tag = (unsigned)L->grp_off[g];
         S.fb_ok = pencil_fits<kFbTS>(S) ? 1 : 0;
-        if ((long long)S.nblk_t1 * S.n_t2 > 65535) return ENZO_PPM_ERR_ARGUMENT;
         FlatParams &F = hflat[((size_t)par * 3 + d) * G + g];
         memset(&F, 0, sizeof(F));
         F.dim = d;
@@ -482,6 +487,10 @@ int build_plan(hipPPMLevel *L) {
         F.dt = 0.0;
         F.dtp = L->dt;
         F.qc = L->qc + L->qc_off[g];
+        const long long items =
+            F.N + (F.iflatten ? (long long)(F.i2 - F.i1 + 3) * F.n_outer : 0);
+        L->prep_blocks[d] = (int)std::max<long long>(
+            L->prep_blocks[d], std::min<long long>((items + 255) / 256, 1024));
       }
       if (L->diffusion) max_slices = std::max(max_slices, gd[(d + 2) % 3]);
     }
@@ -544,7 +553,8 @@ int build_plan(hipPPMLevel *L) {
 int launch_level_sweep(hipPPMLevel *L, int d) {
   const int G = L->ngrids;
   const SweepParams *P = L->dparams + ((size_t)L->parity * 3 + d) * G;
-  hipLaunchKernelGGL(level_prep_kernel, dim3((unsigned)G), dim3(256), 0, L->stream,
+  hipLaunchKernelGGL(level_prep_kernel, dim3((unsigned)L->prep_blocks[d], (unsigned)G), dim3(256),
+                     0, L->stream,
                      L->dflat + ((size_t)L->parity * 3 + d) * G);
   LCHK(hipGetLastError());
   const auto &sl = L->launch_slices[d];
