@@ -40,6 +40,19 @@ struct Work {
   int grid, tile;
 };
 
+// sibling copy box: every slot of grid `dst` at [d0, d0 + n) takes grid
+// `src`'s values at [s0, s0 + n) (local, ghost-inclusive indices, x first)
+struct CopyBox {
+  int dst, src;
+  int d0[3], s0[3], n[3];
+};
+
+// where a grid's state lives in the arena
+struct GridGeom {
+  long long off, cells;
+  int GD[3];
+};
+
 // tile shapes of the level kernels (256 work-items each)
 constexpr int kShapes = 3;
 constexpr int kShapeTP[kShapes] = {16, 8, 4};
@@ -123,6 +136,29 @@ __global__ __launch_bounds__(256) void level_expand_kernel(const ExpandParams *_
   const long long lim = (long long)w.tile + kPresChunk;
   const long long end = x.cells < lim ? x.cells : lim;
   for (long long g = (long long)w.tile + threadIdx.x; g < end; g += blockDim.x) expand_cell(x, g);
+}
+
+// sibling copies (grid::CopyZonesFromGrid): one workgroup row per box
+__global__ void level_copy_kernel(const CopyBox *__restrict__ boxes,
+                                  const GridGeom *__restrict__ geom, double *S, int nslot) {
+  const CopyBox &b = boxes[blockIdx.y];
+  const GridGeom &gd = geom[b.dst], &gs = geom[b.src];
+  const long long per_slot = (long long)b.n[0] * b.n[1] * b.n[2];
+  const long long n = per_slot * nslot;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int slot = (int)(q / per_slot);
+    long long r = q % per_slot;
+    const int x = (int)(r % b.n[0]);
+    r /= b.n[0];
+    const int y = (int)(r % b.n[1]);
+    const int z = (int)(r / b.n[1]);
+    const long long id =
+        ((long long)(b.d0[2] + z) * gd.GD[1] + (b.d0[1] + y)) * gd.GD[0] + (b.d0[0] + x);
+    const long long is =
+        ((long long)(b.s0[2] + z) * gs.GD[1] + (b.s0[1] + y)) * gs.GD[0] + (b.s0[0] + x);
+    S[gd.off + (long long)slot * gd.cells + id] = S[gs.off + (long long)slot * gs.cells + is];
+  }
 }
 
 // periodic ghost fill of every grid along one axis: one workgroup row per grid
@@ -243,6 +279,10 @@ struct hipPPMLevel {
   double *old = nullptr;            // (S_GE + 1) * cells per grid at ooff
   std::vector<long long> ooff;
   ExpandParams *dexp = nullptr;     // [parity 2][grid]
+  // sibling copies (hipPPMLevelSetSiblings)
+  CopyBox *dboxes = nullptr;
+  GridGeom *dgeom = nullptr;
+  long long nboxes = 0, max_box = 0;
 };
 
 namespace {
@@ -258,7 +298,7 @@ void free_level(hipPPMLevel *L) {
                   L->dtmin, L->counters, L->fb_flag, L->fb_list, L->fb_state, L->grp_grid,
                   L->dgrp_off, L->dparams,
                   L->dflat, L->dpres, L->dwork, L->dpwork, L->learn, L->cws_dt, L->old,
-                  L->dexp};
+                  L->dexp, L->dboxes, L->dgeom};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (L->own_stream && L->stream) (void)hipStreamDestroy(L->stream);
@@ -1063,6 +1103,79 @@ int hipPPMLevelComovingExpansionTerms(hipPPMLevel *L, enzo_float coef, enzo_floa
   if (st != ENZO_PPM_SUCCESS) return st;
   hipLaunchKernelGGL(level_expand_kernel, dim3((unsigned)L->npwork), dim3(256), 0, L->stream,
                      L->dexp + (size_t)L->parity * L->ngrids, L->dpwork);
+  LCHK(hipGetLastError());
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelSetSiblings(hipPPMLevel *L, const enzo_long_int *GridGlobalStart,
+                           const enzo_long_int *DomainActive, enzo_int periodic) {
+  if (!L || !GridGlobalStart || !DomainActive) return ENZO_PPM_ERR_ARGUMENT;
+  const int G = L->ngrids, rank = L->rank;
+  std::vector<CopyBox> boxes;
+  long long max_box = 0;
+  const int sh_lo = periodic ? -1 : 0, sh_hi = periodic ? 1 : 0;
+  for (int a = 0; a < G; a++)
+    for (int b = 0; b < G; b++)
+      for (int sz = (rank > 2 ? sh_lo : 0); sz <= (rank > 2 ? sh_hi : 0); sz++)
+        for (int sy = (rank > 1 ? sh_lo : 0); sy <= (rank > 1 ? sh_hi : 0); sy++)
+          for (int sx = sh_lo; sx <= sh_hi; sx++) {
+            const int sft[3] = {sx, sy, sz};
+            if (a == b && sx == 0 && sy == 0 && sz == 0) continue;
+            CopyBox bx;
+            bx.dst = a;
+            bx.src = b;
+            bool ok = true;
+            for (int d = 0; d < 3 && ok; d++) {
+              if (d >= rank) {
+                bx.d0[d] = bx.s0[d] = 0;
+                bx.n[d] = 1;
+                continue;
+              }
+              const long long A0 = GridGlobalStart[3 * a + d] + sft[d] * DomainActive[d];
+              const long long A1 = A0 + L->GD[3 * a + d];
+              const long long B0 = GridGlobalStart[3 * b + d] + L->para[b].GridStartIndex[d];
+              const long long B1 = GridGlobalStart[3 * b + d] + L->para[b].GridEndIndex[d] + 1;
+              const long long lo = std::max(A0, B0), hi = std::min(A1, B1);
+              if (hi <= lo) {
+                ok = false;
+                break;
+              }
+              bx.d0[d] = (int)(lo - A0);
+              bx.s0[d] = (int)(lo - GridGlobalStart[3 * b + d]);
+              bx.n[d] = (int)(hi - lo);
+            }
+            if (!ok) continue;
+            boxes.push_back(bx);
+            max_box = std::max(max_box, (long long)bx.n[0] * bx.n[1] * bx.n[2]);
+          }
+  if (boxes.size() > 65535) return ENZO_PPM_ERR_NOT_SUPPORTED;
+  std::vector<GridGeom> geom(G);
+  for (int g = 0; g < G; g++) {
+    geom[g].off = L->off[g];
+    geom[g].cells = L->cells[g];
+    for (int d = 0; d < 3; d++) geom[g].GD[d] = L->GD[3 * g + d];
+  }
+  LCHK(hipStreamSynchronize(L->stream));
+  if (L->dboxes) (void)hipFree(L->dboxes);
+  L->dboxes = nullptr;
+  if (!L->dgeom && dev_alloc(&L->dgeom, G)) return ENZO_PPM_ERR_HIP;
+  if (dev_alloc(&L->dboxes, boxes.size())) return ENZO_PPM_ERR_HIP;
+  if (!boxes.empty())
+    LCHK(hipMemcpy(L->dboxes, boxes.data(), sizeof(CopyBox) * boxes.size(),
+                   hipMemcpyHostToDevice));
+  LCHK(hipMemcpy(L->dgeom, geom.data(), sizeof(GridGeom) * G, hipMemcpyHostToDevice));
+  L->nboxes = (long long)boxes.size();
+  L->max_box = max_box;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelCopyZones(hipPPMLevel *L) {
+  if (!L || !L->dgeom) return ENZO_PPM_ERR_ARGUMENT;
+  if (L->nboxes == 0) return ENZO_PPM_SUCCESS;
+  const unsigned bx =
+      (unsigned)std::min<long long>((L->max_box * L->nslot + 255) / 256, 1024);
+  hipLaunchKernelGGL(level_copy_kernel, dim3(bx, (unsigned)L->nboxes), dim3(256), 0, L->stream,
+                     L->dboxes, L->dgeom, L->parity == 0 ? L->A : L->B, L->nslot);
   LCHK(hipGetLastError());
   return ENZO_PPM_SUCCESS;
 }

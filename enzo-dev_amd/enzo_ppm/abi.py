@@ -184,6 +184,8 @@ PROTOTYPES = {
     "hipPPMLevelTimeStep": (C.c_int, [C.c_void_p, C.c_double, PF]),
     "hipPPMLevelComovingExpansionTerms": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "hipPPMLevelPeriodicBoundary": (C.c_int, [C.c_void_p]),
+    "hipPPMLevelSetSiblings": (C.c_int, [C.c_void_p, PI, PI, C.c_int64]),
+    "hipPPMLevelCopyZones": (C.c_int, [C.c_void_p]),
     "hipPPMLevelDestroy": (C.c_int, [C.c_void_p]),
     "hipPPMSetLearnedGuess": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hipPPMSetComoving": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_int64]),

@@ -96,6 +96,17 @@ class Level:
     def periodic_boundary(self):
         self._check(self.lib.hipPPMLevelPeriodicBoundary(self.handle), "LevelPeriodicBoundary")
 
+    def set_siblings(self, global_starts, domain_active, periodic=True):
+        """Sibling copy boxes from each grid's global start (index of its
+        first cell, ghosts included) and the domain's active cells."""
+        starts = abi.iarr([v for s in global_starts for v in s])
+        self._check(self.lib.hipPPMLevelSetSiblings(self.handle, starts, abi.iarr(domain_active, 3),
+                                                    int(periodic)), "LevelSetSiblings")
+
+    def copy_zones(self):
+        """Apply the sibling copies to the level's current state."""
+        self._check(self.lib.hipPPMLevelCopyZones(self.handle), "LevelCopyZones")
+
     def set_learned(self, weights):
         """Learned two-shock start for every grid (None: reference start)."""
         if weights is None:

@@ -411,6 +411,21 @@ int hipPPMLevelComovingExpansionTerms(hipPPMLevel *Level, enzo_float coef, enzo_
    ExternalBoundary::SetExternalBoundary). */
 int hipPPMLevelPeriodicBoundary(hipPPMLevel *Level);
 
+/* Sibling ghost copies (grid::CopyZonesFromGrid between the grids of one
+   level, as SetBoundaryConditions applies it; Grid_CopyZonesFromGrid.C:
+   43-230 without shearing).  hipPPMLevelSetSiblings records, from each
+   grid's GridGlobalStart (3 per grid: the global index of its first cell,
+   ghosts included, as SolvePPM_DE receives it) and the domain's active
+   cells per axis, every overlap of a grid's whole extent with another
+   grid's active region -- with the domain's periodic images when
+   `periodic` (a single grid spanning the domain then gets exactly the
+   periodic fill).  hipPPMLevelCopyZones applies them to the current state,
+   every field the level holds (those the hydro step uses: no Vel3 in 2-D,
+   no Vel2/Vel3 in 1-D, no gas energy without dual energy), in one launch. */
+int hipPPMLevelSetSiblings(hipPPMLevel *Level, const enzo_long_int *GridGlobalStart,
+                           const enzo_long_int *DomainActive, enzo_int periodic);
+int hipPPMLevelCopyZones(hipPPMLevel *Level);
+
 /* The learned two-shock start for every grid of the level (see
    hipPPMSetLearnedGuess; NULL switches back to the reference start). */
 int hipPPMLevelSetLearnedGuess(hipPPMLevel *Level, const float *weights);

@@ -105,7 +105,10 @@ def learned_vs_oracle(lib, g, glob, w, steps, cfl=0.4):
 
 
 def l1_rel(a, b):
-    return float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+    """L1 relative difference; absolute where the reference field is zero."""
+    den = float(np.sum(np.abs(b)))
+    num = float(np.sum(np.abs(a - b)))
+    return num / den if den > 0 else num
 
 
 def check_learned_run(ref, learned, one_step):
