@@ -79,6 +79,18 @@ __global__ __launch_bounds__(256) void pressure_dt_kernel(PresParams Q) {
   if (Q.want_dt) block_min_dt(local, Q.dtmin);
 }
 
+__global__ void fluxcorr_kernel(FluxCorrParams F) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= F.nface) return;
+  const int i = (int)(p % F.Dim[0]);
+  const int j = (int)((p / F.Dim[0]) % F.Dim[1]);
+  const int k = (int)(p / ((long long)F.Dim[0] * F.Dim[1]));
+  const long long x =
+      ((long long)(F.Start[2] + k) * F.GD[1] + (F.Start[1] + j)) * F.GD[0] + (F.Start[0] + i);
+  fluxcorr_side(F, x, 0, p);
+  fluxcorr_side(F, x + F.Offset, 1, p);
+}
+
 __global__ void expand_kernel(ExpandParams X) {
   for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < X.cells;
        g += (long long)gridDim.x * blockDim.x)
@@ -951,6 +963,144 @@ int hipPPMComovingExpansionTerms(hipPPMData *D, const hipPPMParameter *P, enzo_f
   hipLaunchKernelGGL(expand_kernel, dim3((unsigned)nb), dim3(256), 0, D->stream, X);
   HIPCHK(hipGetLastError());
   return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
+                                  const enzo_fluxes *InitialFluxes,
+                                  const enzo_fluxes *RefinedFluxes,
+                                  enzo_fluxes *BoundaryFluxesThisTimeStep,
+                                  enzo_int CorrectParentBoundaryFlux, const enzo_int *ColourKind) {
+  if (!D || !P || !InitialFluxes || !RefinedFluxes) return ENZO_PPM_ERR_ARGUMENT;
+  if (CorrectParentBoundaryFlux && !BoundaryFluxesThisTimeStep) return ENZO_PPM_ERR_ARGUMENT;
+  if (P->GridRank != 3) return ENZO_PPM_ERR_NOT_SUPPORTED;  // the device holds Vel3 in 3-D only
+  const int ncol = (int)P->NumberOfColours;
+  for (int c = 0; c < ncol; c++)
+    if (!ColourKind || ColourKind[c] < 0 || ColourKind[c] > 2) return ENZO_PPM_ERR_ARGUMENT;
+  FluxCorrParams F;
+  memset(&F, 0, sizeof(F));
+  F.A = D->A;
+  F.cells = D->cells;
+  for (int d = 0; d < 3; d++) F.GD[d] = D->GD[d];
+  F.idual = (int)P->DualEnergyFormalism;
+  F.rank = (int)P->GridRank;
+  F.tiny_number = P->MinimumPressure;
+  // corrected list: density, total energy, gas energy, the velocities in
+  // field-number order (Grid_CorrectForRefinedFluxes.C:301-356)
+  long long lf[6];
+  int nl = 0;
+  lf[nl] = P->DensNum; F.list[nl++] = S_D;
+  lf[nl] = P->TENum; F.list[nl++] = S_E;
+  if (F.idual) { lf[nl] = P->GENum; F.list[nl++] = S_GE; }
+  F.nchecked = nl;
+  {
+    long long vn[3] = {P->Vel1Num, P->Vel2Num, P->Vel3Num};
+    int vs[3] = {S_VX, S_VY, S_VZ};
+    for (int a = 0; a < 3; a++)
+      for (int b = a + 1; b < 3; b++)
+        if (vn[b] < vn[a]) { std::swap(vn[a], vn[b]); std::swap(vs[a], vs[b]); }
+    for (int a = 0; a < 3; a++) { lf[nl] = vn[a]; F.list[nl++] = vs[a]; }
+  }
+  F.nl = nl;
+  F.cons[F.ncons++] = S_E;
+  if (F.idual) F.cons[F.ncons++] = S_GE;
+  F.cons[F.ncons++] = S_VX;
+  F.cons[F.ncons++] = S_VY;
+  F.cons[F.ncons++] = S_VZ;
+  for (int c = 0; c < ncol; c++) {
+    if (ColourKind[c] == 0) F.spec[F.nspec++] = S_COL0 + c;
+    else if (ColourKind[c] == 2) F.cons[F.ncons++] = S_COL0 + c;
+  }
+  const enzo_fluxes *In = InitialFluxes, *Rf = RefinedFluxes;
+  std::vector<double> stage;
+  double *dflux = nullptr;
+  size_t cap = 0;
+  int st = ENZO_PPM_SUCCESS;
+  for (int dim = 0; dim < 3 && st == ENZO_PPM_SUCCESS; dim++) {
+    if (D->GD[dim] <= 1) continue;
+    for (int j = 0; j < 3; j++)
+      if (In->LeftFluxStartGlobalIndex[dim][j] != Rf->LeftFluxStartGlobalIndex[dim][j] ||
+          In->LeftFluxEndGlobalIndex[dim][j] != Rf->LeftFluxEndGlobalIndex[dim][j])
+        st = ENZO_PPM_ERR_ARGUMENT;
+    long long Start[3], End[3];
+    for (int i = 0; i < 3 && st == ENZO_PPM_SUCCESS; i++) {
+      Start[i] = std::max(In->LeftFluxStartGlobalIndex[dim][i], Rf->LeftFluxStartGlobalIndex[dim][i]) -
+                 P->GridGlobalStart[i];
+      End[i] = std::min(In->LeftFluxEndGlobalIndex[dim][i], Rf->LeftFluxEndGlobalIndex[dim][i]) -
+               P->GridGlobalStart[i];
+      if (Start[i] < 0 || End[i] > D->GD[i]) st = ENZO_PPM_ERR_ARGUMENT;
+    }
+    if (st != ENZO_PPM_SUCCESS) break;
+    Start[dim] = std::max(Start[dim] - 1, 0ll);
+    End[dim] = Start[dim];
+    long long Dim[3];
+    for (int i = 0; i < 3; i++) Dim[i] = End[i] - Start[i] + 1;
+    long long Offset = In->RightFluxStartGlobalIndex[dim][dim] - In->LeftFluxStartGlobalIndex[dim][dim] + 2;
+    Offset = std::min(Offset, (long long)D->GD[dim] - 1);
+    int corrL = 1, corrR = 1, bndL = 0, bndR = 0;
+    if (Start[dim] == P->GridStartIndex[dim] - 1) { bndL = 1; corrL = 0; }
+    if (Start[dim] + Offset == P->GridEndIndex[dim] + 1) { bndR = 1; corrR = 0; }
+    // the face cells must lie in the grid (both sides)
+    for (int i = 0; i < 3; i++)
+      if (Start[i] < 0 || End[i] >= D->GD[i]) st = ENZO_PPM_ERR_ARGUMENT;
+    if (Start[dim] + Offset >= D->GD[dim]) st = ENZO_PPM_ERR_ARGUMENT;
+    if (st != ENZO_PPM_SUCCESS) break;
+    const long long nface = Dim[0] * Dim[1] * Dim[2];
+    // the grid's own boundary fluxes take the refined ones (the reference
+    // assigns them before any undo, which only happens on corrected sides)
+    if (CorrectParentBoundaryFlux && (bndL || bndR)) {
+      long long GFS[3], GFD[3];
+      for (int i = 0; i < 3; i++) {
+        GFS[i] = P->GridStartIndex[i];
+        GFD[i] = P->GridEndIndex[i] - P->GridStartIndex[i] + 1;
+      }
+      GFS[dim] = Start[dim];
+      GFD[dim] = 1;
+      for (int l = 0; l < nl; l++)
+        for (long long k = Start[2]; k <= End[2]; k++)
+          for (long long j = Start[1]; j <= End[1]; j++)
+            for (long long i = Start[0]; i <= End[0]; i++) {
+              const long long fi = ((k - Start[2]) * Dim[1] + (j - Start[1])) * Dim[0] + (i - Start[0]);
+              const long long gfi = (i - GFS[0]) + (j - GFS[1]) * GFD[0] + (k - GFS[2]) * GFD[1] * GFD[0];
+              if (bndL) BoundaryFluxesThisTimeStep->LeftFluxes[lf[l]][dim][gfi] = Rf->LeftFluxes[lf[l]][dim][fi];
+              if (bndR) BoundaryFluxesThisTimeStep->RightFluxes[lf[l]][dim][gfi] = Rf->RightFluxes[lf[l]][dim][fi];
+            }
+    }
+    for (int i = 0; i < dim; i++) Offset *= D->GD[i];
+    // stage [list field][side][initial, refined][face position]
+    stage.assign((size_t)nl * 4 * nface, 0.0);
+    for (int l = 0; l < nl; l++) {
+      const enzo_float *src[4] = {In->LeftFluxes[lf[l]][dim], Rf->LeftFluxes[lf[l]][dim],
+                                  In->RightFluxes[lf[l]][dim], Rf->RightFluxes[lf[l]][dim]};
+      for (int q = 0; q < 4; q++) {
+        if (!src[q]) { st = ENZO_PPM_ERR_ARGUMENT; break; }
+        std::copy(src[q], src[q] + nface, stage.begin() + ((size_t)l * 4 + q) * nface);
+      }
+    }
+    if (st != ENZO_PPM_SUCCESS) break;
+    if (stage.size() > cap) {
+      if (dflux) (void)hipFree(dflux);
+      dflux = nullptr;
+      cap = stage.size();
+      if (alloc_dev((void **)&dflux, sizeof(double) * cap)) { st = ENZO_PPM_ERR_HIP; break; }
+    }
+    if (hipMemcpyAsync(dflux, stage.data(), sizeof(double) * stage.size(), hipMemcpyHostToDevice,
+                       D->stream) != hipSuccess) { st = ENZO_PPM_ERR_HIP; break; }
+    for (int i = 0; i < 3; i++) {
+      F.Start[i] = (int)Start[i];
+      F.Dim[i] = (int)Dim[i];
+    }
+    F.Offset = Offset;
+    F.corr[0] = corrL;
+    F.corr[1] = corrR;
+    F.flux = dflux;
+    F.nface = nface;
+    hipLaunchKernelGGL(fluxcorr_kernel, dim3((unsigned)((nface + 255) / 256)), dim3(256), 0,
+                       D->stream, F);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(D->stream) != hipSuccess)
+      st = ENZO_PPM_ERR_HIP;
+  }
+  if (dflux) (void)hipFree(dflux);
+  return st;
 }
 
 int hipPPMPeriodicBoundary(hipPPMData *D, const hipPPMParameter *P) {

@@ -265,6 +265,80 @@ __device__ inline void expand_cell(const ExpandParams &X, long long g) {
 }
 
 // ------------------------------------------------------------------------
+// AMR flux correction of the cells just outside one face pair of a proper
+// subgrid (Grid_CorrectForRefinedFluxes.C:53-870 with SUBlingGrid FALSE,
+// Cartesian, PPM direct Euler, RadiativeCooling 0, FluxCorrection 1).  One
+// work-item per face position; it treats the left cell and then the right
+// cell, each in the reference's order: conserved form (energies, velocities
+// and kind-2 colours times density, species-like colours over density),
+// the corrections in list order (density, total energy, gas energy, the
+// velocities) with the undo of all of them when density or an energy would
+// not stay positive, back to specific form, dual-energy consistency, and
+// species times the new density.
+struct FluxCorrParams {
+  double *A;
+  long long cells;
+  int GD[3];
+  int Start[3], Dim[3];
+  long long Offset;           // right cell = left cell + Offset
+  int corr[2];                // correct the left / right cells
+  int nl, nchecked;           // corrected fields; the first nchecked must stay > 0
+  int list[6];                // their slots
+  int ncons, nspec;
+  int cons[6 + 45], spec[45]; // slots multiplied by / divided by density
+  int idual, rank;
+  double tiny_number;
+  const double *flux;         // [field in list][side][initial, refined][face position]
+  long long nface;
+};
+
+__device__ inline void fluxcorr_side(const FluxCorrParams &F, long long x, int side, long long fi) {
+  double *A = F.A;
+  const long long n = F.cells;
+  for (int c = 0; c < F.ncons; c++) A[F.cons[c] * n + x] *= A[S_D * n + x];
+  for (int c = 0; c < F.nspec; c++) A[F.spec[c] * n + x] /= A[S_D * n + x];
+  if (F.corr[side]) {
+    double ini[6], ref[6];
+    for (int l = 0; l < F.nl; l++) {
+      const double *q = F.flux + ((long long)(l * 2 + side) * 2) * F.nface;
+      ini[l] = q[fi];
+      ref[l] = q[F.nface + fi];
+    }
+    for (int l = 0; l < F.nl; l++) {
+      double &v = A[F.list[l] * n + x];
+      if (side == 0) v += ini[l] - ref[l];
+      else v += -(ini[l] - ref[l]);
+      if (l < F.nchecked && v <= 0) {
+        for (int u = 0; u < F.nl; u++) {
+          if (u <= l) {
+            if (side == 0) A[F.list[u] * n + x] -= (ini[u] - ref[u]);
+            else A[F.list[u] * n + x] -= -(ini[u] - ref[u]);
+          }
+          ref[u] = ini[u];
+        }
+      }
+    }
+  }
+  for (int c = 0; c < F.ncons; c++) A[F.cons[c] * n + x] /= A[S_D * n + x];
+  if (F.idual) {
+    double ge = fmx(A[S_GE * n + x], F.tiny_number);
+    A[S_GE * n + x] = ge;
+    const double v1 = A[S_VX * n + x];
+    double te = ge + 0.5 * v1 * v1;
+    if (F.rank > 1) {
+      const double v2 = A[S_VY * n + x];
+      te += 0.5 * v2 * v2;
+    }
+    if (F.rank > 2) {
+      const double v3 = A[S_VZ * n + x];
+      te += 0.5 * v3 * v3;
+    }
+    A[S_E * n + x] = te;
+  }
+  for (int c = 0; c < F.nspec; c++) A[F.spec[c] * n + x] *= A[S_D * n + x];
+}
+
+// ------------------------------------------------------------------------
 // Per-sweep preparation item `gid`: qc(i) = dt/dx(i) for gid < N (used by
 // the fluxes, flux_twoshock.F:366, and the gas-energy source, :393), then the
 // flattening of line 1 of every slice, one (slice, cell in [i1-1, i2+1]) per

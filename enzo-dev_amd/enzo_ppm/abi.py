@@ -189,6 +189,9 @@ PROTOTYPES = {
     "hipPPMLevelDestroy": (C.c_int, [C.c_void_p]),
     "hipPPMSetLearnedGuess": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hipPPMSetComoving": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_int64]),
+    "hipPPMCorrectForRefinedFluxes": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter),
+                                                C.POINTER(EnzoFluxes), C.POINTER(EnzoFluxes),
+                                                C.POINTER(EnzoFluxes), C.c_int64, PI]),
     "hipPPMSetExpansionFactors": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_double,
                                             C.c_double]),
     "hipPPMComovingExpansionTerms": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter),

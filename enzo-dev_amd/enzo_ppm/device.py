@@ -39,13 +39,13 @@ class DeviceGrid:
     """Device-resident state of one grid for multi-step runs: upload once,
     then pressure + Courant pass and rotated sweeps stay on the GPU."""
 
-    def __init__(self, grid, glob, lib=None, stream=None):
+    def __init__(self, grid, glob, lib=None, stream=None, global_start=(0, 0, 0)):
         self.lib = L = lib or abi.load_library()
         self.grid = grid
         self.glob = dict(glob)
         _, gd = make_globals(**glob)
         self.gdict = gd
-        self.para = grid.hip_parameter(L, gd)
+        self.para = grid.hip_parameter(L, gd, global_start=global_start)
         self.handle = C.c_void_p(None)
         _check(L.hipPPMInitData(C.byref(self.handle), C.byref(self.para), stream), "InitData")
         self._keep = None
@@ -101,6 +101,16 @@ class DeviceGrid:
 
     def fallback_count(self):
         return self.lib.hipPPMFallbackCount(self.handle)
+
+    def correct_for_refined_fluxes(self, initial, refined, boundary=None, colour_kind=(),
+                                   correct_parent_boundary_flux=0):
+        """Flux correction of the device state for one proper subgrid
+        (hipPPMCorrectForRefinedFluxes); structs are abi.EnzoFluxes."""
+        kinds = abi.iarr(list(colour_kind), max(1, len(colour_kind)))
+        _check(self.lib.hipPPMCorrectForRefinedFluxes(
+            self.handle, C.byref(self.para), C.byref(initial), C.byref(refined),
+            C.byref(boundary) if boundary is not None else None,
+            int(correct_parent_boundary_flux), kinds), "CorrectForRefinedFluxes")
 
     def set_comoving(self, enable=True):
         """Comoving handling on the device state (hipPPMSetComoving)."""

@@ -240,6 +240,28 @@ int hipPPMStep(hipPPMData *Data, const hipPPMParameter *Para, enzo_int CycleNumb
    (the periodic case of ExternalBoundary::SetExternalBoundary). */
 int hipPPMPeriodicBoundary(hipPPMData *Data, const hipPPMParameter *Para);
 
+/* ---- AMR flux correction ------------------------------------------------
+   grid::CorrectForRefinedFluxes (Grid_CorrectForRefinedFluxes.C:53-870) on
+   the device-resident state, for a proper subgrid (its SUBlingGrid FALSE
+   branch), Cartesian coordinates, PPM direct Euler, RadiativeCooling 0 and
+   FluxCorrection 1, in 3-D: InitialFluxes is the subgrid's SubgridFluxes
+   entry of this grid's hydro step, RefinedFluxes the subgrid's boundary
+   fluxes projected to this grid (grid::GetProjectedBoundaryFluxes); the
+   cells just outside the subgrid's faces take the difference, in conserved
+   form, with the reference's undo when density or an energy would not stay
+   positive; on faces that lie on this grid's own boundary the cells are left
+   alone and, with CorrectParentBoundaryFlux, BoundaryFluxesThisTimeStep
+   takes the refined fluxes.  ColourKind[NumberOfColours]: 0 species-like
+   (FieldType ElectronDensity..ExtraType1, MetalSNIa/II: kept as fractions
+   of the density), 1 passive (untouched), 2 conserved (multiplied by the
+   density around the correction, not corrected).  RefinedFluxes is read
+   only (the reference resets entries it undoes, then frees them). */
+int hipPPMCorrectForRefinedFluxes(hipPPMData *Data, const hipPPMParameter *Para,
+                                  const enzo_fluxes *InitialFluxes,
+                                  const enzo_fluxes *RefinedFluxes,
+                                  enzo_fluxes *BoundaryFluxesThisTimeStep,
+                                  enzo_int CorrectParentBoundaryFlux, const enzo_int *ColourKind);
+
 /* ---- comoving coordinates ----------------------------------------------
    The reference recasts the solver in comoving form by scaling the cell
    widths it hands to SolvePPM_DE by a(t + dt/2)

@@ -142,3 +142,40 @@ def comoving_expansion_terms(kind, grid, old, glob, coef, coefficient):
                                                                coefficient)
     if st != abi.SUCCESS:
         raise RuntimeError(f"comoving_expansion_terms failed {st}")
+
+
+def _fx_protos(L):
+    f = L.orc_project_fluxes
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(abi.EnzoFluxes), C.c_int64, C.c_int64, abi.PI,
+                  C.POINTER(abi.EnzoFluxes)]
+    f = L.orc_correct_for_refined_fluxes
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(abi.EnzoPPMGrid), abi.PI, C.POINTER(abi.EnzoFluxes),
+                  C.POINTER(abi.EnzoFluxes), C.POINTER(abi.EnzoFluxes), C.c_int64, C.c_int64,
+                  C.c_double, C.c_int64, abi.PI, abi.PI]
+    return L
+
+
+def project_fluxes(fine, rank, nfields, refinement, projected):
+    """grid::GetProjectedBoundaryFluxes: `fine` and `projected` are
+    enzo_fluxes structs (projected's arrays sized for the coarse faces)."""
+    L = _fx_protos(lib("port"))
+    st = L.orc_project_fluxes(C.byref(fine), rank, nfields, abi.iarr(refinement, 3),
+                              C.byref(projected))
+    if st != abi.SUCCESS:
+        raise RuntimeError(f"project_fluxes failed {st}")
+
+
+def correct_for_refined_fluxes(grid, global_start, initial, refined, boundary, glob,
+                               colkind, correct_parent_boundary_flux=0):
+    """grid::CorrectForRefinedFluxes for a proper subgrid (in place on
+    `grid` and `refined`/`boundary`).  Returns the status."""
+    L = _fx_protos(lib("port"))
+    _, g = make_globals(**glob)
+    cg, keep = grid.ctypes_grid()
+    return L.orc_correct_for_refined_fluxes(
+        C.byref(cg), abi.iarr(global_start, 3), C.byref(initial), C.byref(refined),
+        C.byref(boundary) if boundary is not None else None, correct_parent_boundary_flux,
+        g["DualEnergyFormalism"], g["tiny_number"], grid.ncolours,
+        abi.iarr(grid.colnum, max(1, grid.ncolours)), abi.iarr(colkind, max(1, len(colkind))))

@@ -323,3 +323,9 @@ def test_level_decomposed_box_equals_whole(cpulib):
         sl = h.active_slice()
         for x, y in zip(h.fields, whole.fields):
             assert bit_equal(x[sl], y[w])
+
+
+@pytest.mark.parametrize("name", ["interior", "undo", "on_left_x_edge", "on_right_z_edge"])
+def test_flux_correction_matches_oracle(cpulib, name):
+    import fluxcorr_checks as fc
+    assert fc.run_case(cpulib, name) > 0
