@@ -79,6 +79,34 @@ __global__ __launch_bounds__(256) void pressure_dt_kernel(PresParams Q) {
   if (Q.want_dt) block_min_dt(local, Q.dtmin);
 }
 
+// projection of one face's fine fluxes: item = (array, coarse position);
+// each sums its fine cells in the reference's loop order (k, j, i)
+struct ProjParams {
+  const double *fine;  // [array][fine position]
+  double *coarse;      // [array][coarse position]
+  long long nfine, ncoarse;
+  int narrays;
+  int Dims[3], PD[3], r[3];
+  double dArea;
+};
+
+__global__ void project_kernel(ProjParams Q) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long long)Q.narrays * Q.ncoarse) return;
+  const int a = (int)(q / Q.ncoarse);
+  const long long p = q % Q.ncoarse;
+  const int i1 = (int)(p % Q.PD[0]);
+  const int j1 = (int)((p / Q.PD[0]) % Q.PD[1]);
+  const int k1 = (int)(p / ((long long)Q.PD[0] * Q.PD[1]));
+  const double *f = Q.fine + (long long)a * Q.nfine;
+  double acc = 0.0;
+  for (int k = k1 * Q.r[2]; k < (k1 + 1) * Q.r[2] && k < Q.Dims[2]; k++)
+    for (int j = j1 * Q.r[1]; j < (j1 + 1) * Q.r[1] && j < Q.Dims[1]; j++)
+      for (int i = i1 * Q.r[0]; i < (i1 + 1) * Q.r[0] && i < Q.Dims[0]; i++)
+        acc += f[i + (long long)j * Q.Dims[0] + (long long)k * Q.Dims[0] * Q.Dims[1]] * Q.dArea;
+  Q.coarse[(long long)a * Q.ncoarse + p] = acc;
+}
+
 __global__ void fluxcorr_kernel(FluxCorrParams F) {
   const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= F.nface) return;
@@ -963,6 +991,80 @@ int hipPPMComovingExpansionTerms(hipPPMData *D, const hipPPMParameter *P, enzo_f
   hipLaunchKernelGGL(expand_kernel, dim3((unsigned)nb), dim3(256), 0, D->stream, X);
   HIPCHK(hipGetLastError());
   return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMProjectBoundaryFluxes(const enzo_fluxes *Fine, enzo_int rank, enzo_int nfields,
+                                const enzo_int *Refinement, enzo_fluxes *Projected) {
+  if (!Fine || !Projected || !Refinement || rank < 1 || rank > 3 || nfields < 1 ||
+      nfields > ENZO_PPM_MAX_NUMBER_OF_BARYON_FIELDS)
+    return ENZO_PPM_ERR_ARGUMENT;
+  for (int i = 0; i < 3; i++)
+    if (Refinement[i] < 1) return ENZO_PPM_ERR_ARGUMENT;
+  for (int dim = 0; dim < rank; dim++)
+    for (int i = 0; i < rank; i++) {
+      Projected->LeftFluxStartGlobalIndex[dim][i] = Fine->LeftFluxStartGlobalIndex[dim][i] / Refinement[i];
+      Projected->LeftFluxEndGlobalIndex[dim][i] = Fine->LeftFluxEndGlobalIndex[dim][i] / Refinement[i];
+      Projected->RightFluxStartGlobalIndex[dim][i] = Fine->RightFluxStartGlobalIndex[dim][i] / Refinement[i];
+      Projected->RightFluxEndGlobalIndex[dim][i] = Fine->RightFluxEndGlobalIndex[dim][i] / Refinement[i];
+    }
+  int st = ENZO_PPM_SUCCESS;
+  for (int dim = 0; dim < rank && st == ENZO_PPM_SUCCESS; dim++) {
+    ProjParams Q;
+    long long TotalRefinement = 1;
+    for (int i = 0; i < 3; i++) {
+      Q.Dims[i] = i < rank ? (int)(Fine->LeftFluxEndGlobalIndex[dim][i] -
+                                   Fine->LeftFluxStartGlobalIndex[dim][i] + 1) : 1;
+      Q.r[i] = (int)Refinement[i];
+      TotalRefinement *= Refinement[i];
+    }
+    for (int i = 0; i < 3; i++) {
+      Q.PD[i] = i != dim ? Q.Dims[i] / Q.r[i] : 1;
+      if (i != dim && i < rank && Q.Dims[i] % Q.r[i] != 0) st = ENZO_PPM_ERR_ARGUMENT;
+    }
+    if (st != ENZO_PPM_SUCCESS) break;
+    Q.nfine = (long long)Q.Dims[0] * Q.Dims[1] * Q.Dims[2];
+    Q.ncoarse = (long long)Q.PD[0] * Q.PD[1] * Q.PD[2];
+    Q.narrays = 2 * (int)nfields;
+    Q.dArea = 1.0 / double(TotalRefinement);
+    std::vector<double> fine((size_t)Q.narrays * Q.nfine), coarse((size_t)Q.narrays * Q.ncoarse);
+    for (int f = 0; f < nfields && st == ENZO_PPM_SUCCESS; f++) {
+      const enzo_float *L = Fine->LeftFluxes[f][dim], *R = Fine->RightFluxes[f][dim];
+      if (!L || !R || !Projected->LeftFluxes[f][dim] || !Projected->RightFluxes[f][dim]) {
+        st = ENZO_PPM_ERR_ARGUMENT;
+        break;
+      }
+      std::copy(L, L + Q.nfine, fine.begin() + (size_t)(2 * f) * Q.nfine);
+      std::copy(R, R + Q.nfine, fine.begin() + (size_t)(2 * f + 1) * Q.nfine);
+    }
+    if (st != ENZO_PPM_SUCCESS) break;
+    double *dfine = nullptr, *dcoarse = nullptr;
+    if (alloc_dev((void **)&dfine, sizeof(double) * fine.size()) ||
+        alloc_dev((void **)&dcoarse, sizeof(double) * coarse.size())) {
+      st = ENZO_PPM_ERR_HIP;
+    } else if (hipMemcpy(dfine, fine.data(), sizeof(double) * fine.size(), hipMemcpyHostToDevice) !=
+               hipSuccess) {
+      st = ENZO_PPM_ERR_HIP;
+    } else {
+      Q.fine = dfine;
+      Q.coarse = dcoarse;
+      const long long n = (long long)Q.narrays * Q.ncoarse;
+      hipLaunchKernelGGL(project_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, Q);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpy(coarse.data(), dcoarse, sizeof(double) * coarse.size(), hipMemcpyDeviceToHost) !=
+              hipSuccess)
+        st = ENZO_PPM_ERR_HIP;
+    }
+    if (dfine) (void)hipFree(dfine);
+    if (dcoarse) (void)hipFree(dcoarse);
+    if (st != ENZO_PPM_SUCCESS) break;
+    for (int f = 0; f < nfields; f++) {
+      std::copy(coarse.begin() + (size_t)(2 * f) * Q.ncoarse,
+                coarse.begin() + (size_t)(2 * f + 1) * Q.ncoarse, Projected->LeftFluxes[f][dim]);
+      std::copy(coarse.begin() + (size_t)(2 * f + 1) * Q.ncoarse,
+                coarse.begin() + (size_t)(2 * f + 2) * Q.ncoarse, Projected->RightFluxes[f][dim]);
+    }
+  }
+  return st;
 }
 
 int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,

@@ -189,6 +189,8 @@ PROTOTYPES = {
     "hipPPMLevelDestroy": (C.c_int, [C.c_void_p]),
     "hipPPMSetLearnedGuess": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hipPPMSetComoving": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_int64]),
+    "hipPPMProjectBoundaryFluxes": (C.c_int, [C.POINTER(EnzoFluxes), C.c_int64, C.c_int64, PI,
+                                              C.POINTER(EnzoFluxes)]),
     "hipPPMCorrectForRefinedFluxes": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter),
                                                 C.POINTER(EnzoFluxes), C.POINTER(EnzoFluxes),
                                                 C.POINTER(EnzoFluxes), C.c_int64, PI]),

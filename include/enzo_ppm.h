@@ -256,6 +256,16 @@ int hipPPMPeriodicBoundary(hipPPMData *Data, const hipPPMParameter *Para);
    of the density), 1 passive (untouched), 2 conserved (multiplied by the
    density around the correction, not corrected).  RefinedFluxes is read
    only (the reference resets entries it undoes, then frees them). */
+/* grid::GetProjectedBoundaryFluxes (Grid_GetProjectedBoundaryFluxes.C:
+   43-111): a subgrid's boundary fluxes (Fine, in its index space) summed
+   onto its parent's resolution -- each fine value times 1 / (product of
+   the refinement factors), in the reference's summation order -- into
+   Projected, whose arrays the caller sizes for the coarse faces; its index
+   ranges are set here.  Face extents must be multiples of the factors. */
+int hipPPMProjectBoundaryFluxes(const enzo_fluxes *Fine, enzo_int GridRank,
+                                enzo_int NumberOfBaryonFields, const enzo_int *RefinementFactors,
+                                enzo_fluxes *Projected);
+
 int hipPPMCorrectForRefinedFluxes(hipPPMData *Data, const hipPPMParameter *Para,
                                   const enzo_fluxes *InitialFluxes,
                                   const enzo_fluxes *RefinedFluxes,

@@ -62,3 +62,27 @@ def run_case(lib, name, kinds=(0, 1, 2)):
     assert np.array_equal(own.flat(), own_b.flat())
     changed = sum(int(np.count_nonzero(x != y)) for x, y in zip(a.fields, g.fields))
     return changed
+
+
+def run_projection(lib, r):
+    """hipPPMProjectBoundaryFluxes against the oracle, bitwise."""
+    parent, _, _ = problems.turbulent(16, seed=4, ncolours=1)
+    cbox = ((2, 4, 6), (5, 7, 9))
+    fine_box = (tuple(c * r for c in cbox[0]), tuple((c + 1) * r - 1 for c in cbox[1]))
+    fine = SubgridFluxes(parent, [fine_box])
+    a, b = SubgridFluxes(parent, [cbox]), SubgridFluxes(parent, [cbox])
+    rng = np.random.default_rng(r)
+    for (f, dim), (L, R) in fine.arrays[0].items():
+        L[:] = rng.normal(0, 1.0, L.shape)
+        R[:] = rng.normal(0, 1.0, R.shape)
+    oracle.project_fluxes(fine.structs[0], 3, parent.nfields, [r, r, r], b.structs[0])
+    import ctypes as C
+    from enzo_ppm import abi
+    st = lib.hipPPMProjectBoundaryFluxes(C.byref(fine.structs[0]), 3, parent.nfields,
+                                         abi.iarr([r, r, r]), C.byref(a.structs[0]))
+    assert st == 1
+    assert np.array_equal(a.flat(), b.flat())
+    for d in range(3):
+        for j in range(3):
+            assert a.structs[0].LeftFluxStartGlobalIndex[d][j] == \
+                b.structs[0].LeftFluxStartGlobalIndex[d][j]

@@ -329,3 +329,9 @@ def test_level_decomposed_box_equals_whole(cpulib):
 def test_flux_correction_matches_oracle(cpulib, name):
     import fluxcorr_checks as fc
     assert fc.run_case(cpulib, name) > 0
+
+
+@pytest.mark.parametrize("r", [2, 4])
+def test_flux_projection_matches_oracle(cpulib, r):
+    import fluxcorr_checks as fc
+    fc.run_projection(cpulib, r)

@@ -21,3 +21,8 @@ def lib():
 @pytest.mark.parametrize("name", sorted(fc.CASES))
 def test_flux_correction_matches_oracle(lib, name):
     assert fc.run_case(lib, name) > 0
+
+
+@pytest.mark.parametrize("r", [2, 4])
+def test_flux_projection_matches_oracle(lib, r):
+    fc.run_projection(lib, r)
