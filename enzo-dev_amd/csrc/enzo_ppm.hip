@@ -1070,10 +1070,12 @@ int hipPPMProjectBoundaryFluxes(const enzo_fluxes *Fine, enzo_int rank, enzo_int
 int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
                                   const enzo_fluxes *InitialFluxes,
                                   const enzo_fluxes *RefinedFluxes,
-                                  enzo_fluxes *BoundaryFluxesThisTimeStep,
+                                  enzo_fluxes *BoundaryFluxesThisTimeStep, enzo_int SUBlingGrid,
+                                  const enzo_int *GlobalDim, const enzo_int *PeriodicFace,
                                   enzo_int CorrectParentBoundaryFlux, const enzo_int *ColourKind) {
   if (!D || !P || !InitialFluxes || !RefinedFluxes) return ENZO_PPM_ERR_ARGUMENT;
   if (CorrectParentBoundaryFlux && !BoundaryFluxesThisTimeStep) return ENZO_PPM_ERR_ARGUMENT;
+  if (SUBlingGrid && (!GlobalDim || !PeriodicFace)) return ENZO_PPM_ERR_ARGUMENT;
   if (P->GridRank != 3) return ENZO_PPM_ERR_NOT_SUPPORTED;  // the device holds Vel3 in 3-D only
   const int ncol = (int)P->NumberOfColours;
   for (int c = 0; c < ncol; c++)
@@ -1119,16 +1121,45 @@ int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
   int st = ENZO_PPM_SUCCESS;
   for (int dim = 0; dim < 3 && st == ENZO_PPM_SUCCESS; dim++) {
     if (D->GD[dim] <= 1) continue;
-    for (int j = 0; j < 3; j++)
-      if (In->LeftFluxStartGlobalIndex[dim][j] != Rf->LeftFluxStartGlobalIndex[dim][j] ||
-          In->LeftFluxEndGlobalIndex[dim][j] != Rf->LeftFluxEndGlobalIndex[dim][j])
-        st = ENZO_PPM_ERR_ARGUMENT;
+    // Refined's index ranges of this direction, edited for a SUBling as the
+    // reference edits them in place (:147-222)
+    long long RLS[3], RLE[3], RRS[3];
+    for (int i = 0; i < 3; i++) {
+      RLS[i] = Rf->LeftFluxStartGlobalIndex[dim][i];
+      RLE[i] = Rf->LeftFluxEndGlobalIndex[dim][i];
+      RRS[i] = Rf->RightFluxStartGlobalIndex[dim][i];
+    }
+    long long RefinedDim[3] = {0, 0, 0}, RefinedOffset[3] = {0, 0, 0}, InitialOffset[3] = {0, 0, 0};
+    int corrL = 1, corrR = 1;
+    if (!SUBlingGrid) {
+      for (int j = 0; j < 3; j++)
+        if (In->LeftFluxStartGlobalIndex[dim][j] != RLS[j] || In->LeftFluxEndGlobalIndex[dim][j] != RLE[j])
+          st = ENZO_PPM_ERR_ARGUMENT;
+    } else {
+      const long long G = GlobalDim[dim];
+      for (int i = 0; i < 3; i++) RefinedDim[i] = RLE[i] - RLS[i] + 1;
+      if (RLS[dim] == 0 && PeriodicFace[2 * dim]) {
+        RLS[dim] = G - 1;
+        RLE[dim] = G - 1;
+      } else {
+        RLS[dim]--;
+        RLE[dim]--;
+      }
+      if (RRS[dim] == G - 1 && PeriodicFace[2 * dim + 1]) RRS[dim] = 0;
+      else RRS[dim]++;
+      corrR = In->RightFluxStartGlobalIndex[dim][dim] == RLS[dim];
+      corrL = In->LeftFluxStartGlobalIndex[dim][dim] == RRS[dim];
+      for (int i = 0; i < 3; i++)
+        RefinedOffset[i] = std::max(In->LeftFluxStartGlobalIndex[dim][i] - RLS[i], 0ll);
+      RLS[dim] = 0;
+      RefinedOffset[dim] = 0;
+    }
+    if (st != ENZO_PPM_SUCCESS) break;
+    if (!corrL && !corrR) continue;  // a SUBling that meets neither face
     long long Start[3], End[3];
     for (int i = 0; i < 3 && st == ENZO_PPM_SUCCESS; i++) {
-      Start[i] = std::max(In->LeftFluxStartGlobalIndex[dim][i], Rf->LeftFluxStartGlobalIndex[dim][i]) -
-                 P->GridGlobalStart[i];
-      End[i] = std::min(In->LeftFluxEndGlobalIndex[dim][i], Rf->LeftFluxEndGlobalIndex[dim][i]) -
-               P->GridGlobalStart[i];
+      Start[i] = std::max((long long)In->LeftFluxStartGlobalIndex[dim][i], RLS[i]) - P->GridGlobalStart[i];
+      End[i] = std::min((long long)In->LeftFluxEndGlobalIndex[dim][i], RLE[i]) - P->GridGlobalStart[i];
       if (Start[i] < 0 || End[i] > D->GD[i]) st = ENZO_PPM_ERR_ARGUMENT;
     }
     if (st != ENZO_PPM_SUCCESS) break;
@@ -1138,15 +1169,52 @@ int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
     for (int i = 0; i < 3; i++) Dim[i] = End[i] - Start[i] + 1;
     long long Offset = In->RightFluxStartGlobalIndex[dim][dim] - In->LeftFluxStartGlobalIndex[dim][dim] + 2;
     Offset = std::min(Offset, (long long)D->GD[dim] - 1);
-    int corrL = 1, corrR = 1, bndL = 0, bndR = 0;
+    if (SUBlingGrid) {
+      // this grid's own first / last active layer; Initial spans its whole
+      // face (:287-303)
+      Offset -= 2;
+      Start[dim]++;
+      End[dim]++;
+      for (int i = 0; i < 3; i++) {
+        if (i != dim) {
+          Dim[i] = P->GridEndIndex[i] - P->GridStartIndex[i] + 1;
+          InitialOffset[i] = std::max(RLS[i] - In->LeftFluxStartGlobalIndex[dim][i], 0ll);
+        } else {
+          Dim[i] = 1;
+          InitialOffset[i] = 0;
+        }
+      }
+    }
+    int bndL = 0, bndR = 0;
     if (Start[dim] == P->GridStartIndex[dim] - 1) { bndL = 1; corrL = 0; }
     if (Start[dim] + Offset == P->GridEndIndex[dim] + 1) { bndR = 1; corrR = 0; }
-    // the face cells must lie in the grid (both sides)
+    // the loop region, and the face cells within the grid (both sides)
+    long long Ext[3];
+    for (int i = 0; i < 3; i++) Ext[i] = End[i] - Start[i] + 1;
+    if (Ext[0] <= 0 || Ext[1] <= 0 || Ext[2] <= 0) continue;  // faces do not overlap
     for (int i = 0; i < 3; i++)
       if (Start[i] < 0 || End[i] >= D->GD[i]) st = ENZO_PPM_ERR_ARGUMENT;
-    if (Start[dim] + Offset >= D->GD[dim]) st = ENZO_PPM_ERR_ARGUMENT;
+    if (Start[dim] + Offset >= D->GD[dim] || Start[dim] + Offset < 0) st = ENZO_PPM_ERR_ARGUMENT;
+    // every flux index within the arrays: Initial of extent Dim (the
+    // grid's own face for a SUBling), Refined of extent RefinedDim
+    for (int i = 0; i < 3; i++) {
+      const long long inext = In->LeftFluxEndGlobalIndex[dim][i] - In->LeftFluxStartGlobalIndex[dim][i] + 1;
+      if (Dim[i] != inext || Ext[i] - 1 + InitialOffset[i] > Dim[i] - 1) st = ENZO_PPM_ERR_ARGUMENT;
+      if (SUBlingGrid && Ext[i] - 1 + RefinedOffset[i] > RefinedDim[i] - 1) st = ENZO_PPM_ERR_ARGUMENT;
+    }
     if (st != ENZO_PPM_SUCCESS) break;
-    const long long nface = Dim[0] * Dim[1] * Dim[2];
+    const long long nface = Ext[0] * Ext[1] * Ext[2];
+    // flux index of loop position (i, j, k) in Initial (and, for a proper
+    // subgrid, Refined) and, for a SUBling, in Refined (:455-470)
+    auto flux_index = [&](long long i, long long j, long long k) {
+      return ((k - Start[2] + InitialOffset[2]) * Dim[1] + (j - Start[1] + InitialOffset[1])) * Dim[0] +
+             (i - Start[0] + InitialOffset[0]);
+    };
+    auto refined_index = [&](long long i, long long j, long long k) {
+      if (!SUBlingGrid) return flux_index(i, j, k);
+      return ((k - Start[2] + RefinedOffset[2]) * RefinedDim[1] + (j - Start[1] + RefinedOffset[1])) *
+                 RefinedDim[0] + (i - Start[0] + RefinedOffset[0]);
+    };
     // the grid's own boundary fluxes take the refined ones (the reference
     // assigns them before any undo, which only happens on corrected sides)
     if (CorrectParentBoundaryFlux && (bndL || bndR)) {
@@ -1161,22 +1229,33 @@ int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
         for (long long k = Start[2]; k <= End[2]; k++)
           for (long long j = Start[1]; j <= End[1]; j++)
             for (long long i = Start[0]; i <= End[0]; i++) {
-              const long long fi = ((k - Start[2]) * Dim[1] + (j - Start[1])) * Dim[0] + (i - Start[0]);
+              const long long fi = flux_index(i, j, k);
               const long long gfi = (i - GFS[0]) + (j - GFS[1]) * GFD[0] + (k - GFS[2]) * GFD[1] * GFD[0];
               if (bndL) BoundaryFluxesThisTimeStep->LeftFluxes[lf[l]][dim][gfi] = Rf->LeftFluxes[lf[l]][dim][fi];
               if (bndR) BoundaryFluxesThisTimeStep->RightFluxes[lf[l]][dim][gfi] = Rf->RightFluxes[lf[l]][dim][fi];
             }
     }
     for (int i = 0; i < dim; i++) Offset *= D->GD[i];
-    // stage [list field][side][initial, refined][face position]
+    // stage [list field][side][initial, refined][loop position]: the left
+    // side pairs initial left with refined left (proper subgrid) or refined
+    // right (SUBling), the right side the other way round
     stage.assign((size_t)nl * 4 * nface, 0.0);
-    for (int l = 0; l < nl; l++) {
-      const enzo_float *src[4] = {In->LeftFluxes[lf[l]][dim], Rf->LeftFluxes[lf[l]][dim],
-                                  In->RightFluxes[lf[l]][dim], Rf->RightFluxes[lf[l]][dim]};
-      for (int q = 0; q < 4; q++) {
-        if (!src[q]) { st = ENZO_PPM_ERR_ARGUMENT; break; }
-        std::copy(src[q], src[q] + nface, stage.begin() + ((size_t)l * 4 + q) * nface);
-      }
+    for (int l = 0; l < nl && st == ENZO_PPM_SUCCESS; l++) {
+      const enzo_float *iL = In->LeftFluxes[lf[l]][dim], *iR = In->RightFluxes[lf[l]][dim];
+      const enzo_float *rL = Rf->LeftFluxes[lf[l]][dim], *rR = Rf->RightFluxes[lf[l]][dim];
+      if (!iL || !iR || !rL || !rR) { st = ENZO_PPM_ERR_ARGUMENT; break; }
+      const enzo_float *src[4] = {iL, SUBlingGrid ? rR : rL, iR, SUBlingGrid ? rL : rR};
+      double *out = stage.data() + (size_t)l * 4 * nface;
+      long long p = 0;
+      for (long long k = Start[2]; k <= End[2]; k++)
+        for (long long j = Start[1]; j <= End[1]; j++)
+          for (long long i = Start[0]; i <= End[0]; i++, p++) {
+            const long long fi = flux_index(i, j, k), rfi = refined_index(i, j, k);
+            out[p] = src[0][fi];
+            out[nface + p] = src[1][rfi];
+            out[2 * nface + p] = src[2][fi];
+            out[3 * nface + p] = src[3][rfi];
+          }
     }
     if (st != ENZO_PPM_SUCCESS) break;
     if (stage.size() > cap) {
@@ -1189,11 +1268,12 @@ int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
                        D->stream) != hipSuccess) { st = ENZO_PPM_ERR_HIP; break; }
     for (int i = 0; i < 3; i++) {
       F.Start[i] = (int)Start[i];
-      F.Dim[i] = (int)Dim[i];
+      F.Dim[i] = (int)Ext[i];
     }
     F.Offset = Offset;
     F.corr[0] = corrL;
     F.corr[1] = corrR;
+    F.subling = SUBlingGrid ? 1 : 0;
     F.flux = dflux;
     F.nface = nface;
     hipLaunchKernelGGL(fluxcorr_kernel, dim3((unsigned)((nface + 255) / 256)), dim3(256), 0,

@@ -266,8 +266,9 @@ __device__ inline void expand_cell(const ExpandParams &X, long long g) {
 
 // ------------------------------------------------------------------------
 // AMR flux correction of the cells just outside one face pair of a proper
-// subgrid (Grid_CorrectForRefinedFluxes.C:53-870 with SUBlingGrid FALSE,
-// Cartesian, PPM direct Euler, RadiativeCooling 0, FluxCorrection 1).  One
+// subgrid, or of this grid's own first / last active layer for a SUBling
+// (Grid_CorrectForRefinedFluxes.C:53-870, Cartesian, PPM direct Euler,
+// RadiativeCooling 0, FluxCorrection 1).  One
 // work-item per face position; it treats the left cell and then the right
 // cell, each in the reference's order: conserved form (energies, velocities
 // and kind-2 colours times density, species-like colours over density),
@@ -290,6 +291,8 @@ struct FluxCorrParams {
   double tiny_number;
   const double *flux;         // [field in list][side][initial, refined][face position]
   long long nface;
+  int subling;                // SUBlingGrid: the pairs are (initial left, refined right)
+                              // and (initial right, refined left), signs swapped
 };
 
 __device__ inline void fluxcorr_side(const FluxCorrParams &F, long long x, int side, long long fi) {
@@ -304,15 +307,25 @@ __device__ inline void fluxcorr_side(const FluxCorrParams &F, long long x, int s
       ini[l] = q[fi];
       ref[l] = q[F.nface + fi];
     }
+    // a proper subgrid adds initial - refined on the left and its negative
+    // on the right; a SUBling the other way round
+    // (Grid_CorrectForRefinedFluxes.C:491-544)
+    const bool neg = (side == 1) != (F.subling != 0);
+    // the reference's undo on a SUBling's right face subtracts the amount of
+    // the field found non-positive from every field corrected before it
+    // (:638-652)
+    const bool own_amount = F.subling != 0 && side == 1;
     for (int l = 0; l < F.nl; l++) {
       double &v = A[F.list[l] * n + x];
-      if (side == 0) v += ini[l] - ref[l];
-      else v += -(ini[l] - ref[l]);
+      if (neg) v += -(ini[l] - ref[l]);
+      else v += ini[l] - ref[l];
       if (l < F.nchecked && v <= 0) {
+        const double amt = ini[l] - ref[l];
         for (int u = 0; u < F.nl; u++) {
           if (u <= l) {
-            if (side == 0) A[F.list[u] * n + x] -= (ini[u] - ref[u]);
-            else A[F.list[u] * n + x] -= -(ini[u] - ref[u]);
+            if (own_amount) A[F.list[u] * n + x] -= amt;
+            else if (neg) A[F.list[u] * n + x] -= -(ini[u] - ref[u]);
+            else A[F.list[u] * n + x] -= (ini[u] - ref[u]);
           }
           ref[u] = ini[u];
         }

@@ -103,13 +103,19 @@ class DeviceGrid:
         return self.lib.hipPPMFallbackCount(self.handle)
 
     def correct_for_refined_fluxes(self, initial, refined, boundary=None, colour_kind=(),
-                                   correct_parent_boundary_flux=0):
-        """Flux correction of the device state for one proper subgrid
-        (hipPPMCorrectForRefinedFluxes); structs are abi.EnzoFluxes."""
+                                   correct_parent_boundary_flux=0, subling=0, global_dim=None,
+                                   periodic_face=None):
+        """Flux correction of the device state (hipPPMCorrectForRefinedFluxes)
+        for one proper subgrid, or, with `subling`, for a finer grid that is
+        not this grid's child but touches one of its faces (`global_dim`: the
+        level's domain size in cells per axis; `periodic_face`: 6 flags, left
+        and right face per axis).  Structs are abi.EnzoFluxes."""
         kinds = abi.iarr(list(colour_kind), max(1, len(colour_kind)))
         _check(self.lib.hipPPMCorrectForRefinedFluxes(
             self.handle, C.byref(self.para), C.byref(initial), C.byref(refined),
-            C.byref(boundary) if boundary is not None else None,
+            C.byref(boundary) if boundary is not None else None, int(subling),
+            abi.iarr(global_dim, 3) if global_dim is not None else None,
+            abi.iarr(periodic_face, 6) if periodic_face is not None else None,
             int(correct_parent_boundary_flux), kinds), "CorrectForRefinedFluxes")
 
     def set_comoving(self, enable=True):

@@ -242,20 +242,36 @@ int hipPPMPeriodicBoundary(hipPPMData *Data, const hipPPMParameter *Para);
 
 /* ---- AMR flux correction ------------------------------------------------
    grid::CorrectForRefinedFluxes (Grid_CorrectForRefinedFluxes.C:53-870) on
-   the device-resident state, for a proper subgrid (its SUBlingGrid FALSE
-   branch), Cartesian coordinates, PPM direct Euler, RadiativeCooling 0 and
-   FluxCorrection 1, in 3-D: InitialFluxes is the subgrid's SubgridFluxes
-   entry of this grid's hydro step, RefinedFluxes the subgrid's boundary
-   fluxes projected to this grid (grid::GetProjectedBoundaryFluxes); the
-   cells just outside the subgrid's faces take the difference, in conserved
-   form, with the reference's undo when density or an energy would not stay
-   positive; on faces that lie on this grid's own boundary the cells are left
-   alone and, with CorrectParentBoundaryFlux, BoundaryFluxesThisTimeStep
-   takes the refined fluxes.  ColourKind[NumberOfColours]: 0 species-like
-   (FieldType ElectronDensity..ExtraType1, MetalSNIa/II: kept as fractions
-   of the density), 1 passive (untouched), 2 conserved (multiplied by the
-   density around the correction, not corrected).  RefinedFluxes is read
-   only (the reference resets entries it undoes, then frees them). */
+   the device-resident state, Cartesian coordinates, PPM direct Euler,
+   RadiativeCooling 0 and FluxCorrection 1, in 3-D, for the two calls of
+   UpdateFromFinerGrids.C:
+   - a proper subgrid (SUBlingGrid 0, :155-178): InitialFluxes is the
+     subgrid's SubgridFluxes entry of this grid's hydro step, RefinedFluxes
+     the subgrid's boundary fluxes projected to this grid
+     (grid::GetProjectedBoundaryFluxes); the cells just outside the
+     subgrid's faces take the difference; on faces that lie on this grid's
+     own boundary the cells are left alone and, with
+     CorrectParentBoundaryFlux, BoundaryFluxesThisTimeStep takes the
+     refined fluxes;
+   - a SUBling (SUBlingGrid 1, :180-198): a finer grid that is not this
+     grid's child but touches one of its faces; InitialFluxes is this
+     grid's own boundary fluxes, RefinedFluxes the SUBling's fluxes
+     projected to this level; this grid's first / last active layer along
+     the touching face takes the difference.  GlobalDim[3] is the level's
+     domain size in cells and PeriodicFace[6] flags (left, right per axis)
+     the periodic domain faces, across which the faces also meet; both may
+     be NULL for a proper subgrid.  The reference's undo on a SUBling's
+     right-hand face subtracts the amount of the field found non-positive
+     from every field corrected before it; this does the same.
+   Corrections are made in conserved form, with the reference's undo when
+   density or an energy would not stay positive.  ColourKind
+   [NumberOfColours]: 0 species-like (FieldType ElectronDensity..ExtraType1,
+   MetalSNIa/II: kept as fractions of the density), 1 passive (untouched),
+   2 conserved (multiplied by the density around the correction, not
+   corrected).  RefinedFluxes is read only (the reference edits its index
+   ranges for a SUBling and resets entries it undoes, then frees it).
+   ENZO_PPM_ERR_ARGUMENT where the reference fails or a flux index would
+   leave the arrays. */
 /* grid::GetProjectedBoundaryFluxes (Grid_GetProjectedBoundaryFluxes.C:
    43-111): a subgrid's boundary fluxes (Fine, in its index space) summed
    onto its parent's resolution -- each fine value times 1 / (product of
@@ -269,7 +285,8 @@ int hipPPMProjectBoundaryFluxes(const enzo_fluxes *Fine, enzo_int GridRank,
 int hipPPMCorrectForRefinedFluxes(hipPPMData *Data, const hipPPMParameter *Para,
                                   const enzo_fluxes *InitialFluxes,
                                   const enzo_fluxes *RefinedFluxes,
-                                  enzo_fluxes *BoundaryFluxesThisTimeStep,
+                                  enzo_fluxes *BoundaryFluxesThisTimeStep, enzo_int SUBlingGrid,
+                                  const enzo_int *GlobalDim, const enzo_int *PeriodicFace,
                                   enzo_int CorrectParentBoundaryFlux, const enzo_int *ColourKind);
 
 /* ---- comoving coordinates ----------------------------------------------

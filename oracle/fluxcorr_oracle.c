@@ -8,8 +8,11 @@
  *     resolution, each fine value times 1 / (product of the refinement
  *     factors), in the fine loop order (k, j, i);
  *   - grid::CorrectForRefinedFluxes (Grid_CorrectForRefinedFluxes.C:53-870)
- *     for a proper subgrid (SUBlingGrid == FALSE), Cartesian coordinates,
- *     HydroMethod PPM_DirectEuler, RadiativeCooling 0, FluxCorrection 1.
+ *     for a proper subgrid (SUBlingGrid FALSE, UpdateFromFinerGrids.C:
+ *     155-178) and for a SUBling (SUBlingGrid TRUE, UpdateFromFinerGrids.C:
+ *     180-198: a finer grid that is not this grid's child but touches one
+ *     of its faces), Cartesian coordinates, HydroMethod PPM_DirectEuler,
+ *     RadiativeCooling 0, FluxCorrection 1.
  *
  * Fields: density, total energy, gas energy (with dual energy) and the three
  * velocities play their roles; each colour field has a kind:
@@ -24,7 +27,9 @@
  * The reference routine is C++ inside the grid class and cannot be built on
  * its own; what pins this restatement is listed in
  * tests/test_fluxcorr_oracle.py (identity, conservation and fraction
- * invariants, the undo rule, the projection's sums).
+ * invariants, the undo rule, the projection's sums, and a SUBling's
+ * correction equal to the proper-subgrid correction of one grid holding
+ * both neighbours, across a periodic wrap too).
  */
 #include <math.h>
 #include <stdint.h>
@@ -87,15 +92,24 @@ int orc_project_fluxes(const enzo_fluxes *Fine, I rank, I nfields, const I *Refi
 }
 
 /* Grid_CorrectForRefinedFluxes.C:53-870 (see the file comment for the
- * supported case).  GridGlobalStart: global index of the grid's first cell,
+ * supported cases).  GridGlobalStart: global index of the grid's first cell,
  * ghosts included (nlongint((CellLeftEdge[i][0] - DomainLeftEdge[i]) /
  * CellWidth[i][0])).  Boundary may be NULL (then CorrectParentBoundaryFlux
- * must be 0).  Returns ENZO_PPM_ERR_ARGUMENT where the reference fails. */
+ * must be 0).  SUBlingGrid: Initial is this grid's own boundary fluxes and
+ * Refined a SUBling's projected fluxes; GlobalDim[dim] is the level's
+ * domain size in cells (nlongint((DomainRightEdge - DomainLeftEdge) /
+ * CellWidth)) and PeriodicFace[2 dim + side] is 1 where the domain's
+ * left/right face along dim is periodic (:147-216).  The reference edits
+ * Refined's index ranges for a SUBling in place; here they are edited in a
+ * copy.  Refined's flux values are reset where corrections are undone, as
+ * in the reference.  Returns ENZO_PPM_ERR_ARGUMENT where the reference
+ * fails. */
 int orc_correct_for_refined_fluxes(const enzo_ppm_grid *g, const I *GridGlobalStart,
                                    const enzo_fluxes *Initial, enzo_fluxes *Refined,
                                    enzo_fluxes *Boundary, I CorrectParentBoundaryFlux,
                                    I DualEnergyFormalism, R tiny_number, I ncolours,
-                                   const I *colnum, const I *colkind) {
+                                   const I *colnum, const I *colkind, I SUBlingGrid,
+                                   const I *GlobalDim, const I *PeriodicFace) {
   R **BF = g->BaryonField;
   const I rank = g->GridRank;
   const I *GD = g->GridDimension;
@@ -125,18 +139,52 @@ int orc_correct_for_refined_fluxes(const enzo_ppm_grid *g, const I *GridGlobalSt
     else if (colkind[c] == 2) cons[ncons++] = colnum[c];
   }
   const I D = g->DensNum;
+  if (SUBlingGrid && (!GlobalDim || !PeriodicFace)) return ENZO_PPM_ERR_ARGUMENT;
   for (I dim = 0; dim < rank; dim++) {
     if (GD[dim] <= 1) continue;
-    for (I j = 0; j < rank; j++)
-      if (Initial->LeftFluxStartGlobalIndex[dim][j] != Refined->LeftFluxStartGlobalIndex[dim][j] ||
-          Initial->LeftFluxEndGlobalIndex[dim][j] != Refined->LeftFluxEndGlobalIndex[dim][j])
-        return ENZO_PPM_ERR_ARGUMENT;
+    /* Refined's index ranges of this direction (edited for a SUBling) */
+    I RLS[3], RLE[3], RRS[3];
+    for (I i = 0; i < 3; i++) {
+      RLS[i] = Refined->LeftFluxStartGlobalIndex[dim][i];
+      RLE[i] = Refined->LeftFluxEndGlobalIndex[dim][i];
+      RRS[i] = Refined->RightFluxStartGlobalIndex[dim][i];
+    }
+    I RefinedDim[3] = {0, 0, 0}, RefinedOffset[3] = {0, 0, 0}, InitialOffset[3] = {0, 0, 0};
+    I CorrectLeftBaryonField = 1, CorrectRightBaryonField = 1;
+    if (!SUBlingGrid) {
+      for (I j = 0; j < rank; j++)
+        if (Initial->LeftFluxStartGlobalIndex[dim][j] != RLS[j] ||
+            Initial->LeftFluxEndGlobalIndex[dim][j] != RLE[j])
+          return ENZO_PPM_ERR_ARGUMENT;
+    } else {
+      /* :147-222: the SUBling's faces moved one cell outwards (wrapped on a
+       * periodic domain face); the face that meets this grid's own face is
+       * the one corrected */
+      const I G = GlobalDim[dim];
+      for (I i = 0; i < 3; i++) RefinedDim[i] = RLE[i] - RLS[i] + 1;
+      if (RLS[dim] == 0 && PeriodicFace[2 * dim]) {
+        RLS[dim] = G - 1;
+        RLE[dim] = G - 1;
+      } else {
+        RLS[dim]--;
+        RLE[dim]--;
+      }
+      if (RRS[dim] == G - 1 && PeriodicFace[2 * dim + 1])
+        RRS[dim] = 0;
+      else
+        RRS[dim]++;
+      CorrectRightBaryonField = Initial->RightFluxStartGlobalIndex[dim][dim] == RLS[dim];
+      CorrectLeftBaryonField = Initial->LeftFluxStartGlobalIndex[dim][dim] == RRS[dim];
+      for (I i = 0; i < 3; i++)
+        RefinedOffset[i] = imax(Initial->LeftFluxStartGlobalIndex[dim][i] - RLS[i], 0);
+      RLS[dim] = 0;
+      RefinedOffset[dim] = 0;
+    }
+    if (!CorrectLeftBaryonField && !CorrectRightBaryonField) continue;
     I Start[3] = {0, 0, 0}, End[3] = {0, 0, 0}, Dim[3];
     for (I i = 0; i < rank; i++) {
-      Start[i] = imax(Initial->LeftFluxStartGlobalIndex[dim][i],
-                      Refined->LeftFluxStartGlobalIndex[dim][i]) - GridGlobalStart[i];
-      End[i] = imin(Initial->LeftFluxEndGlobalIndex[dim][i], Refined->LeftFluxEndGlobalIndex[dim][i]) -
-               GridGlobalStart[i];
+      Start[i] = imax(Initial->LeftFluxStartGlobalIndex[dim][i], RLS[i]) - GridGlobalStart[i];
+      End[i] = imin(Initial->LeftFluxEndGlobalIndex[dim][i], RLE[i]) - GridGlobalStart[i];
       if (Start[i] < 0 || End[i] > GD[i]) return ENZO_PPM_ERR_ARGUMENT;
     }
     Start[dim] = imax(Start[dim] - 1, 0);
@@ -145,7 +193,22 @@ int orc_correct_for_refined_fluxes(const enzo_ppm_grid *g, const I *GridGlobalSt
     I Offset = Initial->RightFluxStartGlobalIndex[dim][dim] -
                Initial->LeftFluxStartGlobalIndex[dim][dim] + 2;
     Offset = imin(Offset, GD[dim] - 1);
-    I CorrectLeftBaryonField = 1, CorrectRightBaryonField = 1;
+    if (SUBlingGrid) {
+      /* :287-303: the corrected cells are this grid's own first / last
+       * active layer, and Initial spans the grid's whole face */
+      Offset -= 2;
+      Start[dim]++;
+      End[dim]++;
+      for (I i = 0; i < rank; i++) {
+        if (i != dim) {
+          Dim[i] = g->GridEndIndex[i] - g->GridStartIndex[i] + 1;
+          InitialOffset[i] = imax(RLS[i] - Initial->LeftFluxStartGlobalIndex[dim][i], 0);
+        } else {
+          Dim[i] = 1;
+          InitialOffset[i] = 0;
+        }
+      }
+    }
     I CorrectLeftBoundaryFlux = 0, CorrectRightBoundaryFlux = 0;
     if (Start[dim] == g->GridStartIndex[dim] - 1) {
       CorrectLeftBoundaryFlux = 1;
@@ -185,32 +248,55 @@ int orc_correct_for_refined_fluxes(const enzo_ppm_grid *g, const I *GridGlobalSt
         for (I j = Start[1]; j <= End[1]; j++)
           for (I i = Start[0]; i <= End[0]; i++) {
             const I x = (k * GD[1] + j) * GD[0] + i;
-            const I fi = ((k - Start[2]) * Dim[1] + (j - Start[1])) * Dim[0] + (i - Start[0]);
+            const I fi = ((k - Start[2] + InitialOffset[2]) * Dim[1] + (j - Start[1] + InitialOffset[1])) *
+                             Dim[0] + (i - Start[0] + InitialOffset[0]);
+            const I rfi = SUBlingGrid ? ((k - Start[2] + RefinedOffset[2]) * RefinedDim[1] +
+                                         (j - Start[1] + RefinedOffset[1])) * RefinedDim[0] +
+                                            (i - Start[0] + RefinedOffset[0])
+                                      : fi;
             const I gfi = (i - GFS[0]) + (j - GFS[1]) * GFD[0] + (k - GFS[2]) * GFD[1] * GFD[0];
+            R *IL = Initial->LeftFluxes[f][dim], *IR = Initial->RightFluxes[f][dim];
+            R *RL = Refined->LeftFluxes[f][dim], *RR = Refined->RightFluxes[f][dim];
             if (CorrectLeftBoundaryFlux && CorrectParentBoundaryFlux)
-              Boundary->LeftFluxes[f][dim][gfi] = Refined->LeftFluxes[f][dim][fi];
-            if (CorrectLeftBaryonField)
-              BF[f][x] += Initial->LeftFluxes[f][dim][fi] - Refined->LeftFluxes[f][dim][fi];
+              Boundary->LeftFluxes[f][dim][gfi] = RL[fi];
+            if (CorrectLeftBaryonField) {
+              if (!SUBlingGrid) BF[f][x] += IL[fi] - RL[fi];
+              else BF[f][x] += -(IL[fi] - RR[rfi]);
+            }
             if (CorrectRightBoundaryFlux && CorrectParentBoundaryFlux)
-              Boundary->RightFluxes[f][dim][gfi] = Refined->RightFluxes[f][dim][fi];
-            if (CorrectRightBaryonField)
-              BF[f][x + Offset] +=
-                  -(Initial->RightFluxes[f][dim][fi] - Refined->RightFluxes[f][dim][fi]);
+              Boundary->RightFluxes[f][dim][gfi] = RR[fi];
+            if (CorrectRightBaryonField) {
+              if (!SUBlingGrid) BF[f][x + Offset] += -(IR[fi] - RR[fi]);
+              else BF[f][x + Offset] += IR[fi] - RL[rfi];
+            }
             if (!checked) continue;
             if (CorrectLeftBaryonField && BF[f][x] <= 0)
               for (I u = 0; u < nl; u++) {
                 const I ff = list[u];
-                if (u <= li)
-                  BF[ff][x] -= (Initial->LeftFluxes[ff][dim][fi] - Refined->LeftFluxes[ff][dim][fi]);
-                Refined->LeftFluxes[ff][dim][fi] = Initial->LeftFluxes[ff][dim][fi];
+                R *uIL = Initial->LeftFluxes[ff][dim];
+                R *uRL = Refined->LeftFluxes[ff][dim], *uRR = Refined->RightFluxes[ff][dim];
+                if (!SUBlingGrid) {
+                  if (u <= li) BF[ff][x] -= (uIL[fi] - uRL[fi]);
+                  uRL[fi] = uIL[fi];
+                } else {
+                  if (u <= li) BF[ff][x] -= -(uIL[fi] - uRR[rfi]);
+                  uRR[rfi] = uIL[fi];
+                }
               }
             if (CorrectRightBaryonField && BF[f][x + Offset] <= 0)
               for (I u = 0; u < nl; u++) {
                 const I ff = list[u];
-                if (u <= li)
-                  BF[ff][x + Offset] -=
-                      -(Initial->RightFluxes[ff][dim][fi] - Refined->RightFluxes[ff][dim][fi]);
-                Refined->RightFluxes[ff][dim][fi] = Initial->RightFluxes[ff][dim][fi];
+                R *uIR = Initial->RightFluxes[ff][dim];
+                R *uRL = Refined->LeftFluxes[ff][dim], *uRR = Refined->RightFluxes[ff][dim];
+                if (!SUBlingGrid) {
+                  if (u <= li) BF[ff][x + Offset] -= -(uIR[fi] - uRR[fi]);
+                  uRR[fi] = uIR[fi];
+                } else {
+                  /* :638-652 subtract the amount of the field being checked
+                   * (f), not of ff, from every field already corrected */
+                  if (u <= li) BF[ff][x + Offset] -= (IR[fi] - RL[rfi]);
+                  uRL[rfi] = uIR[fi];
+                }
               }
           }
     }

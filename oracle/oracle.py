@@ -153,7 +153,7 @@ def _fx_protos(L):
     f.restype = C.c_int
     f.argtypes = [C.POINTER(abi.EnzoPPMGrid), abi.PI, C.POINTER(abi.EnzoFluxes),
                   C.POINTER(abi.EnzoFluxes), C.POINTER(abi.EnzoFluxes), C.c_int64, C.c_int64,
-                  C.c_double, C.c_int64, abi.PI, abi.PI]
+                  C.c_double, C.c_int64, abi.PI, abi.PI, C.c_int64, abi.PI, abi.PI]
     return L
 
 
@@ -168,9 +168,12 @@ def project_fluxes(fine, rank, nfields, refinement, projected):
 
 
 def correct_for_refined_fluxes(grid, global_start, initial, refined, boundary, glob,
-                               colkind, correct_parent_boundary_flux=0):
-    """grid::CorrectForRefinedFluxes for a proper subgrid (in place on
-    `grid` and `refined`/`boundary`).  Returns the status."""
+                               colkind, correct_parent_boundary_flux=0, subling=0,
+                               global_dim=None, periodic_face=None):
+    """grid::CorrectForRefinedFluxes (in place on `grid` and
+    `refined`/`boundary`) for a proper subgrid, or for a SUBling with the
+    level's domain size `global_dim` (3) and `periodic_face` (6: left/right
+    per axis).  Returns the status."""
     L = _fx_protos(lib("port"))
     _, g = make_globals(**glob)
     cg, keep = grid.ctypes_grid()
@@ -178,4 +181,6 @@ def correct_for_refined_fluxes(grid, global_start, initial, refined, boundary, g
         C.byref(cg), abi.iarr(global_start, 3), C.byref(initial), C.byref(refined),
         C.byref(boundary) if boundary is not None else None, correct_parent_boundary_flux,
         g["DualEnergyFormalism"], g["tiny_number"], grid.ncolours,
-        abi.iarr(grid.colnum, max(1, grid.ncolours)), abi.iarr(colkind, max(1, len(colkind))))
+        abi.iarr(grid.colnum, max(1, grid.ncolours)), abi.iarr(colkind, max(1, len(colkind))),
+        int(subling), abi.iarr(global_dim, 3) if global_dim is not None else None,
+        abi.iarr(periodic_face, 6) if periodic_face is not None else None)

@@ -335,3 +335,10 @@ def test_flux_correction_matches_oracle(cpulib, name):
 def test_flux_projection_matches_oracle(cpulib, r):
     import fluxcorr_checks as fc
     fc.run_projection(cpulib, r)
+
+
+@pytest.mark.parametrize("name", ["left", "right", "left_undo_density", "right_undo_energy",
+                                  "left_periodic", "right_periodic"])
+def test_subling_flux_correction_matches_oracle(cpulib, name):
+    import fluxcorr_checks as fc
+    assert fc.run_subling_case(cpulib, name) > 0

@@ -7,7 +7,14 @@ momentum and energy into the face cells; species keep their fractions;
 total and gas energy are consistent afterwards; a correction that would
 empty a cell is undone for every corrected field; faces on the grid's own
 boundary leave the cells alone and replace the boundary fluxes; and the
-projection conserves the fine fluxes and averages a uniform one."""
+projection conserves the fine fluxes and averages a uniform one.  For a
+SUBling (SUBlingGrid TRUE): the correction of a grid at the face a finer
+grid of its neighbour touches equals, bit for bit, the proper-subgrid
+correction of one grid holding both neighbours on the cells next to that
+face (undo included), a periodic wrap of the same placement gives the same
+result, grids that do not touch change nothing, and the reference's undo on
+a SUBling's right-hand face subtracts the amount of the field it found
+non-positive from the fields corrected before it."""
 import numpy as np
 import pytest
 
@@ -177,3 +184,108 @@ def test_projection_averages_and_conserves(r):
         assert np.isclose(L.sum() * r ** 3, fL.sum(), rtol=1e-12, atol=1e-12)
     P = proj.structs[0]
     assert [P.LeftFluxStartGlobalIndex[0][d] for d in range(3)] == list(cbox[0])
+
+
+# ---------------------------------------------------------------------------
+# SUBlings (SUBlingGrid TRUE): the correction of a grid at a face that a
+# finer grid of its NEIGHBOUR touches must equal, cell for cell, the
+# proper-subgrid correction of one grid holding both neighbours, on the
+# cells next to that face; a periodic wrap must not change it.
+import fluxcorr_checks as fc  # noqa: E402
+
+
+def proper_joined(AB, glob, side, ini, ref):
+    a = AB.copy()
+    r = fc.copy_fluxes(ref, AB, fc.GBOX[side])
+    assert oracle.correct_for_refined_fluxes(a, GS, ini.structs[0], r.structs[0], None, glob,
+                                             list(fc.KINDS)) == 1
+    return a, r
+
+
+def layers(side):
+    """(x of the corrected layer in the target, in AB; x of the target's
+    other face layer), local indices; y, z window of G's face."""
+    yz = (slice(4 + 3, 11 + 3), slice(6 + 3, 12 + 3))
+    return (3, 19, 18, yz) if side == "left" else (18, 18, 3, yz)
+
+
+@pytest.mark.parametrize("side,empty", [("left", None), ("right", None), ("left", 0),
+                                        ("right", 0), ("left", 1)])
+def test_subling_equals_proper_correction_of_joined_grid(side, empty):
+    AB, glob, ini, ref = fc.subling_setup(side, seed=5, empty=empty)
+    a, ra = proper_joined(AB, glob, side, ini, ref)
+    T, gs, own, refT, gdim, per, gbox = fc.subling_target(AB, side, ini, ref)
+    t, rt = fc.oracle_subling(T, gs, own, refT, gdim, per, gbox, glob)
+    xt, xab, xo, (ys, zs) = layers(side)
+    for f in range(AB.nfields):
+        assert np.array_equal(t.fields[f][zs, ys, xt], a.fields[f][zs, ys, xab]), f
+        if empty is None:
+            assert not np.array_equal(t.fields[f][zs, ys, xt], T.fields[f][zs, ys, xt]) or \
+                f in (AB.colnum[1],), f
+        # the other face layer of the target only makes the conserved-form
+        # round trip; every other cell is untouched
+        rel = np.abs(t.fields[f][zs, ys, xo] - T.fields[f][zs, ys, xo]) / \
+            np.maximum(np.abs(T.fields[f][zs, ys, xo]), 1e-300)
+        assert rel.max() <= 1e-15, (f, rel.max())
+        m = np.ones(T.shape, bool)
+        m[zs, ys, xt] = False
+        m[zs, ys, xo] = False
+        assert np.array_equal(t.fields[f][m], T.fields[f][m]), f
+    # where the correction was undone, the refined fluxes were reset alike
+    k = 1 if side == "left" else 0
+    for f in range(AB.nfields):
+        assert np.array_equal(rt.arrays[0][(f, 0)][k], ra.arrays[0][(f, 0)][k]), f
+    if empty is not None:   # the undo happened: the emptying flux was reset
+        assert rt.arrays[0][(empty, 0)][k][0] == ini.arrays[0][(empty, 0)][k][0]
+        assert refT.arrays[0][(empty, 0)][k][0] != ini.arrays[0][(empty, 0)][k][0]
+
+
+AB_TE = 1      # field number of the total energy in these grids
+
+
+def test_subling_right_face_energy_undo_keeps_reference_amount():
+    """On a SUBling's right-hand face the reference's undo subtracts the
+    amount of the field found non-positive from every field corrected before
+    it (Grid_CorrectForRefinedFluxes.C:638-652): when the total energy is
+    undone, the density loses the energy's amount, not its own."""
+    side = "right"
+    AB, glob, ini, ref = fc.subling_setup(side, seed=5, empty=AB_TE)
+    T, gs, own, refT, gdim, per, gbox = fc.subling_target(AB, side, ini, ref)
+    t, _ = fc.oracle_subling(T, gs, own, refT, gdim, per, gbox, glob)
+    a, _ = proper_joined(AB, glob, side, ini, ref)
+    cell = (6 + 3, 4 + 3, 18)
+    D, E = AB.DensNum, AB.TENum
+    IR = {f: own.arrays[0][(f, 0)][1].reshape(16, 16)[6, 4] for f in (D, E)}
+    RL = {f: refT.arrays[0][(f, 0)][0][0] for f in (D, E)}
+    aD, aE = IR[D] - RL[D], IR[E] - RL[E]
+    r0 = T.fields[D][cell]
+    assert T.fields[E][cell] * r0 + aE <= 0          # the energy correction was undone
+    assert t.fields[D][cell] == (r0 + aD) - aE
+    assert a.fields[D][cell] == (r0 + aD) - aD       # a proper subgrid's undo
+    assert t.fields[D][cell] != a.fields[D][cell]
+
+
+@pytest.mark.parametrize("side", ["left", "right"])
+def test_subling_across_periodic_boundary(side):
+    AB, glob, ini, ref = fc.subling_setup(side, seed=7, empty=None)
+    T, gs, own, refT, gdim, per, gbox = fc.subling_target(AB, side, ini, ref, wrap=False)
+    t0, _ = fc.oracle_subling(T, gs, own, refT, gdim, per, gbox, glob)
+    T, gs, own, refT, gdim, per, gbox = fc.subling_target(AB, side, ini, ref, wrap=True)
+    t1, _ = fc.oracle_subling(T, gs, own, refT, gdim, per, gbox, glob)
+    for f in range(AB.nfields):
+        assert np.array_equal(t0.fields[f], t1.fields[f]), f
+    assert any(not np.array_equal(x, y) for x, y in zip(t1.fields, T.fields))
+    # the same placement on a non-periodic domain: the grids do not touch
+    t2, _ = fc.oracle_subling(T, gs, own, refT, gdim, [0] * 6, gbox, glob)
+    for f in range(AB.nfields):
+        assert np.array_equal(t2.fields[f], T.fields[f]), f
+
+
+def test_subling_not_touching_changes_nothing():
+    AB, glob, ini, ref = fc.subling_setup("left", seed=2)
+    T, gs, own, refT, gdim, per, _ = fc.subling_target(AB, "left", ini, ref)
+    far = ((3, 4, 6), (8, 10, 11))            # G two cells away from B's face
+    refF = fc.copy_fluxes(refT, T, far)
+    t, _ = fc.oracle_subling(T, gs, own, refF, gdim, per, far, glob)
+    for f in range(AB.nfields):
+        assert np.array_equal(t.fields[f], T.fields[f]), f

@@ -1,7 +1,9 @@
 """AMR flux correction on the GPU (hipPPMCorrectForRefinedFluxes) against the
 oracle's restatement of grid::CorrectForRefinedFluxes, every field bitwise,
 plus the grid's own boundary fluxes: an interior subgrid, one whose
-corrections empty cells (undone), and subgrids on the grid's own faces."""
+corrections empty cells (undone), and subgrids on the grid's own faces; and
+SUBlings (finer grids of a neighbour touching this grid's faces) on either
+side, with undone corrections and across a periodic domain boundary."""
 import pytest
 
 import fluxcorr_checks as fc
@@ -26,3 +28,8 @@ def test_flux_correction_matches_oracle(lib, name):
 @pytest.mark.parametrize("r", [2, 4])
 def test_flux_projection_matches_oracle(lib, r):
     fc.run_projection(lib, r)
+
+
+@pytest.mark.parametrize("name", sorted(fc.SUBLING_CASES))
+def test_subling_flux_correction_matches_oracle(lib, name):
+    assert fc.run_subling_case(lib, name) > 0
