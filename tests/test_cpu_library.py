@@ -56,6 +56,11 @@ CASES = {
     "gravity": lambda: problems.random_smooth([10, 12, 8], seed=2, ncolours=1, gravity=True),
     "sod_2d": lambda: problems.sod_x(20, rank=2, gamma=1.4, flags=(1, 1, 1)),
     "turbulent": lambda: problems.turbulent(12, seed=3),
+    # the longest pencil the reference allows (MAX_ANY_SINGLE_DIRECTION 1031
+    # cells, ghosts included), one active cell, ragged extents
+    "longest_y": lambda: problems.random_smooth([3, 1025, 2], seed=11, ncolours=1),
+    "one_cell": lambda: problems.random_smooth([1, 1, 1], seed=11, ncolours=1),
+    "ragged": lambda: problems.random_smooth([5, 33, 2], seed=11, ncolours=1),
 }
 
 

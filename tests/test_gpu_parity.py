@@ -53,6 +53,15 @@ CASES = {
     "stretched_grid": lambda: problems.random_smooth([12, 10, 11], seed=7, ncolours=1, stretch=0.2),
     "stretched_diffusion": lambda: problems.random_smooth([12, 10, 11], seed=7, ncolours=1,
                                                           stretch=0.2, flags=(1, 1, 1)),
+    # the reference's longest pencil (fortran.def: MAX_ANY_SINGLE_DIRECTION
+    # 1031 cells, ghosts included) along each axis, one active cell, and
+    # ragged extents
+    "longest_x": lambda: problems.random_smooth([1025, 3, 2], seed=11, ncolours=1),
+    "longest_y": lambda: problems.random_smooth([3, 1025, 2], seed=11, ncolours=1),
+    "longest_z": lambda: problems.random_smooth([2, 3, 1025], seed=11, ncolours=1),
+    "one_cell": lambda: problems.random_smooth([1, 1, 1], seed=11, ncolours=1),
+    "ragged": lambda: problems.random_smooth([5, 33, 2], seed=11, ncolours=1),
+    "one_by_long": lambda: problems.random_smooth([1, 200, 1], seed=11, ncolours=1),
 }
 
 
