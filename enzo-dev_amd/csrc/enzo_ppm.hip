@@ -145,6 +145,11 @@ __global__ void periodic_axis_kernel(BCParams B) {
   if (gid < periodic_items(B)) periodic_item(B, gid);
 }
 
+__global__ void ext_bc_kernel(ExtBCParams B) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid < ext_bc_items(B)) ext_bc_item(B, gid);
+}
+
 }  // namespace
 
 // ==========================================================================
@@ -1304,6 +1309,66 @@ int hipPPMPeriodicBoundary(hipPPMData *D, const hipPPMParameter *P) {
                        D->stream, B);
     HIPCHK(hipGetLastError());
   }
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMExternalBoundary(hipPPMData *D, const hipPPMParameter *P, const enzo_int *BoundaryType,
+                           const enzo_int *GridOffset, const enzo_int *BoundaryDimension,
+                           enzo_int NumberOfBaryonFields, const enzo_float *InflowValue) {
+  if (!D || !P || !BoundaryType || !GridOffset || !BoundaryDimension) return ENZO_PPM_ERR_ARGUMENT;
+  std::vector<long long> f;
+  slot_fields(P, f);
+  // check every face first: nothing is written when a face is refused
+  for (int pass = 0; pass < 2; pass++)
+    for (int a = 0; a < P->GridRank; a++) {
+      if (BoundaryDimension[a] <= 1) continue;
+      const int S = (int)P->GridStartIndex[a], E = (int)P->GridEndIndex[a], n = E - S + 1;
+      for (int face = 0; face < 2; face++) {
+        const bool touches = face == 0 ? GridOffset[a] == 0
+                                       : GridOffset[a] + D->GD[a] == BoundaryDimension[a];
+        if (!touches) continue;
+        const int type = (int)BoundaryType[2 * a + face];
+        if (type < BK_REFLECTING || type > BK_SHEARING) return ENZO_PPM_ERR_NOT_SUPPORTED;
+        const int layers = face == 0 ? S : D->GD[a] - E - 1;
+        if (type == BK_PERIODIC || layers <= 0) continue;  // periodic: CopyZonesFromGrid's part
+        if (pass == 0) {
+          // every source cell inside the grid; shearing reads no ghost of
+          // its own face (the reference's in-order loop would see it)
+          if (type == BK_REFLECTING && (face == 0 ? 2 * S - 1 >= D->GD[a] : E - (layers - 1) < 0))
+            return ENZO_PPM_ERR_ARGUMENT;
+          if (type == BK_SHEARING && n < layers) return ENZO_PPM_ERR_NOT_SUPPORTED;
+          if (type == BK_INFLOW) {
+            if (!InflowValue) return ENZO_PPM_ERR_ARGUMENT;
+            for (int s = 0; s < D->nslot; s++)
+              if (f[s] >= NumberOfBaryonFields) return ENZO_PPM_ERR_ARGUMENT;
+          }
+          continue;
+        }
+        ExtBCParams B;
+        memset(&B, 0, sizeof(B));
+        B.axis = a;
+        B.face = face;
+        B.type = type;
+        B.layers = layers;
+        B.S = S;
+        B.E = E;
+        B.n = n;
+        for (int d = 0; d < 3; d++) B.GD[d] = D->GD[d];
+        B.nslot = D->nslot;
+        B.vslot = S_VX + a;
+        B.cells = D->cells;
+        B.A = D->A;
+        if (type == BK_INFLOW)
+          for (int s = 0; s < D->nslot; s++)
+            B.inflow[s] = f[s] >= 0 ? InflowValue[f[s] * 6 + 2 * a + face] : 0.0;
+        const long long items = (long long)layers * D->GD[(a + 1) % 3] * D->GD[(a + 2) % 3] * D->nslot;
+        // faces in the reference's order (x left, x right, y left, ...):
+        // one launch each on the stream
+        hipLaunchKernelGGL(ext_bc_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0,
+                           D->stream, B);
+        HIPCHK(hipGetLastError());
+      }
+    }
   return ENZO_PPM_SUCCESS;
 }
 

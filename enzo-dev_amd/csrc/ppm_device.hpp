@@ -203,6 +203,60 @@ __device__ inline void periodic_item(const BCParams &B, long long gid) {
 }
 
 // ------------------------------------------------------------------------
+// External boundary of one domain face (ExternalBoundary::SetExternalBoundary,
+// ExternalBoundary_SetExternalBoundaryIO.C:50-1010 without OOC_BOUNDARY):
+// ghost layer i of the face takes, per its boundary type, the mirrored
+// active cell (reflecting; the normal velocity's sign flipped), the edge
+// cell (outflow), the face's value (inflow) or the cell one active extent
+// inwards (shearing).  One work-item per (slot, ghost cell of the face).
+enum BoundaryKind { BK_REFLECTING = 0, BK_OUTFLOW = 1, BK_INFLOW = 2, BK_PERIODIC = 3, BK_SHEARING = 4 };
+struct ExtBCParams {
+  int axis, face, type, layers;  // face 0 left, 1 right; ghost layers of the face
+  int S, E, n;                   // GridStartIndex, GridEndIndex, active cells along axis
+  int GD[3];
+  int nslot, vslot;              // slots; the normal velocity's slot
+  long long cells;
+  double *A;
+  double inflow[S_COL0 + ENZO_PPM_MAX_COLOR];  // per slot (inflow faces)
+};
+
+__device__ inline long long ext_bc_items(const ExtBCParams &B) {
+  return (long long)B.layers * B.GD[(B.axis + 1) % 3] * B.GD[(B.axis + 2) % 3] * B.nslot;
+}
+
+__device__ inline void ext_bc_item(const ExtBCParams &B, long long gid) {
+  const int a = B.axis, b = (a + 1) % 3, c = (a + 2) % 3;
+  const long long per_slot = (long long)B.layers * B.GD[b] * B.GD[c];
+  const int slot = (int)(gid / per_slot);
+  long long r = gid % per_slot;
+  const int i = (int)(r % B.layers);
+  r /= B.layers;
+  int x[3];
+  x[b] = (int)(r % B.GD[b]);
+  x[c] = (int)(r / B.GD[b]);
+  int dst, src;
+  if (B.face == 0) {
+    dst = i;
+    src = B.type == BK_REFLECTING ? 2 * B.S - 1 - i : B.type == BK_OUTFLOW ? B.S : i + B.n;
+  } else {
+    dst = B.E + 1 + i;
+    src = B.type == BK_REFLECTING ? B.E - i : B.type == BK_OUTFLOW ? B.E : dst - B.n;
+  }
+  double *F = B.A + (long long)slot * B.cells;
+  x[a] = dst;
+  const long long gd = ((long long)x[2] * B.GD[1] + x[1]) * B.GD[0] + x[0];
+  double v;
+  if (B.type == BK_INFLOW) {
+    v = B.inflow[slot];
+  } else {
+    x[a] = src;
+    v = F[((long long)x[2] * B.GD[1] + x[1]) * B.GD[0] + x[0]];
+    if (B.type == BK_REFLECTING) v = (slot == B.vslot ? -1.0 : 1.0) * v;
+  }
+  F[gd] = v;
+}
+
+// ------------------------------------------------------------------------
 // Comoving expansion terms of one cell (Grid_ComovingExpansionTerms.C:55-130
 // with expand_terms.F, ENERGY_METHOD3 / VELOCITY_METHOD3, HydroMethod 0):
 // the pressure at the interpolation weight `coef` between the state kept at

@@ -191,6 +191,8 @@ PROTOTYPES = {
     "hipPPMSetComoving": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_int64]),
     "hipPPMProjectBoundaryFluxes": (C.c_int, [C.POINTER(EnzoFluxes), C.c_int64, C.c_int64, PI,
                                               C.POINTER(EnzoFluxes)]),
+    "hipPPMExternalBoundary": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), PI, PI, PI,
+                                         C.c_int64, PF]),
     "hipPPMCorrectForRefinedFluxes": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter),
                                                 C.POINTER(EnzoFluxes), C.POINTER(EnzoFluxes),
                                                 C.POINTER(EnzoFluxes), C.c_int64, PI, PI,

@@ -87,6 +87,22 @@ class DeviceGrid:
     def periodic_boundary(self):
         _check(self.lib.hipPPMPeriodicBoundary(self.handle, C.byref(self.para)), "Boundary")
 
+    def external_boundary(self, types, grid_offset=(0, 0, 0), boundary_dims=None, inflow=None):
+        """Domain-face ghost fill (hipPPMExternalBoundary): types[2 dim +
+        face] in the reference's numbering (reflecting 0, outflow 1, inflow 2,
+        periodic 3, shearing 4); boundary_dims defaults to this grid's
+        dimensions (the grid spans the domain); inflow[field][2 dim + face]."""
+        bd = list(self.grid.dims) if boundary_dims is None else list(boundary_dims)
+        nf = self.grid.nfields
+        vals = None
+        if inflow is not None:
+            flat = np.ascontiguousarray(np.asarray(inflow, dtype=np.float64).reshape(nf, 6))
+            self._inflow = flat
+            vals = flat.ctypes.data_as(abi.PF)
+        _check(self.lib.hipPPMExternalBoundary(self.handle, C.byref(self.para), abi.iarr(types, 6),
+                                               abi.iarr(grid_offset, 3), abi.iarr(bd, 3), nf, vals),
+               "ExternalBoundary")
+
     def synchronize(self):
         _check(self.lib.hipPPMSynchronize(self.handle), "Synchronize")
 

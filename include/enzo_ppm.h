@@ -236,9 +236,38 @@ int hipPPMPressureAndTimeStep(hipPPMData *Data, const hipPPMParameter *Para,
 int hipPPMStep(hipPPMData *Data, const hipPPMParameter *Para, enzo_int CycleNumber,
                enzo_float dtFixed);
 
-/* Periodic ghost-zone fill of every baryon field on the device
-   (the periodic case of ExternalBoundary::SetExternalBoundary). */
+/* Periodic ghost-zone fill of every baryon field on the device: what
+   SetBoundaryConditions gives a grid that spans a periodic domain, through
+   CopyZonesFromGrid from its own periodic images (CheckForOverlap's
+   EdgeOffsets; ExternalBoundary::SetExternalBoundary leaves periodic faces
+   alone in the reference build, which does not define USE_PERIODIC). */
 int hipPPMPeriodicBoundary(hipPPMData *Data, const hipPPMParameter *Para);
+
+/* grid::SetExternalBoundaryValues (Grid_SetExternalBoundaryValues.C:91-117
+   -> ExternalBoundary::SetExternalBoundary,
+   ExternalBoundary_SetExternalBoundaryIO.C:50-1010, built without
+   OOC_BOUNDARY) on the device state, with one boundary type per domain face
+   (as ExternalBoundary::InitializeExternalBoundaryFace sets them):
+   BoundaryType[2 dim + face] (face 0 left, 1 right) in the reference's
+   numbering -- reflecting 0 (mirror of the active cells, the normal
+   velocity Velocity1/2/3 negated), outflow 1 (the edge cell), inflow 2
+   (InflowValue[field * 6 + 2 dim + face], field = BaryonField number,
+   NumberOfBaryonFields rows; NULL without inflow faces), periodic 3 (left
+   alone: see hipPPMPeriodicBoundary), shearing 4 (the cell one active extent
+   inwards).  GridOffset[dim] = nint((GridLeftEdge - DomainLeftEdge) /
+   CellWidth) and BoundaryDimension[dim] (the domain's cells, ghosts
+   included) decide which faces lie on the domain's edge: the left one when
+   GridOffset is 0, the right one when GridOffset + GridDimension equals
+   BoundaryDimension.  Faces in the reference's order (x left, x right, y
+   left, ..., z right), each over the grid's whole transverse extent, so
+   corners take the later axis' values.  Every field held on the device
+   (Vel3 is not held in 2-D).  Other types (hydrostatic, ...), or a shearing
+   face whose grid has fewer active cells than ghost layers:
+   ENZO_PPM_ERR_NOT_SUPPORTED, with nothing written. */
+int hipPPMExternalBoundary(hipPPMData *Data, const hipPPMParameter *Para,
+                           const enzo_int *BoundaryType, const enzo_int *GridOffset,
+                           const enzo_int *BoundaryDimension, enzo_int NumberOfBaryonFields,
+                           const enzo_float *InflowValue);
 
 /* ---- AMR flux correction ------------------------------------------------
    grid::CorrectForRefinedFluxes (Grid_CorrectForRefinedFluxes.C:53-870) on

@@ -24,6 +24,18 @@ domain it equals the periodic external boundary fill
 (ExternalBoundary::SetExternalBoundary, periodic case), and for a domain cut
 into sibling grids every grid equals, ghosts included, the matching window
 of the undivided grid after its periodic fill.
+
+Also a restatement of ExternalBoundary::SetExternalBoundary
+(ExternalBoundary_SetExternalBoundaryIO.C:50-1010, the branch built without
+OOC_BOUNDARY) as grid::SetExternalBoundaryValues calls it for every baryon
+field (Grid_SetExternalBoundaryValues.C:91-117), with one boundary type per
+domain face (as ExternalBoundary::InitializeExternalBoundaryFace sets
+them): `external_boundary`.  Pinned by tests/test_boundary_oracle.py: a
+reflecting face equals the periodic fill of the grid joined to its mirror
+image (normal velocity negated); outflow ghosts repeat the edge cell and the
+fill is idempotent; inflow ghosts take the face's values; axes are filled in
+order x, y, z (corners from the later axis); faces the grid does not share
+with the domain, and periodic faces, are left alone.
 """
 import itertools
 
@@ -112,3 +124,61 @@ def split_domain(g, parts):
                 out.append(h)
                 starts.append([lo[d] - gh if d < g.rank else 0 for d in range(3)])
     return out, starts
+
+
+# boundary types of the reference (typedefs.h: reflecting 0, outflow 1,
+# inflow 2, periodic 3, shearing 4)
+REFLECTING, OUTFLOW, INFLOW, PERIODIC, SHEARING = 0, 1, 2, 3, 4
+
+
+def external_boundary(grid, types, grid_offset, boundary_dims, inflow=None, fields=None):
+    """Ghost fill of the domain faces a grid touches, in place.
+
+    types[2 * dim + face] (face 0 left, 1 right): the face's boundary type;
+    grid_offset: nint((GridLeftEdge - DomainLeftEdge) / CellWidth) per axis;
+    boundary_dims: the domain's cells per axis, ghosts included
+    (BoundaryDimension); inflow[field][2 * dim + face]: inflow values.
+    A left face is filled when grid_offset[dim] == 0, a right one when
+    grid_offset[dim] + GridDimension[dim] == boundary_dims[dim]; the normal
+    velocity (Velocity1/2/3 for x/y/z faces) changes sign at a reflecting
+    face; periodic faces are left alone (the reference build leaves
+    USE_PERIODIC undefined: periodic images come from CopyZonesFromGrid).
+    `fields`: field numbers to fill (default all)."""
+    vel = [grid.Vel1Num, grid.Vel2Num, grid.Vel3Num]
+    fields = range(grid.nfields) if fields is None else fields
+    S, E, GD = grid.start, grid.end, grid.dims
+    for dim in range(grid.rank):
+        if boundary_dims[dim] <= 1:
+            continue
+        for f in fields:
+            a = np.moveaxis(grid.fields[f], 2 - dim, 0)   # sweep axis first
+            sign = -1.0 if f == vel[dim] else 1.0
+            n = E[dim] - S[dim] + 1
+            if grid_offset[dim] == 0:            # left face (:145-260)
+                t = types[2 * dim]
+                for i in range(S[dim]):
+                    if t == REFLECTING:
+                        a[i] = sign * a[2 * S[dim] - 1 - i]
+                    elif t == OUTFLOW:
+                        a[i] = a[S[dim]]
+                    elif t == INFLOW:
+                        a[i] = inflow[f][2 * dim]
+                    elif t == SHEARING:
+                        a[i] = a[i + n]
+                    elif t != PERIODIC:
+                        raise ValueError(f"boundary type {t} not restated")
+            if grid_offset[dim] + GD[dim] == boundary_dims[dim]:   # right face
+                t = types[2 * dim + 1]
+                for i in range(GD[dim] - E[dim] - 1):
+                    x = E[dim] + 1 + i
+                    if t == REFLECTING:
+                        a[x] = sign * a[x - (2 * i + 1)]
+                    elif t == OUTFLOW:
+                        a[x] = a[x - 1 - i]
+                    elif t == INFLOW:
+                        a[x] = inflow[f][2 * dim + 1]
+                    elif t == SHEARING:
+                        a[x] = a[x - n]
+                    elif t != PERIODIC:
+                        raise ValueError(f"boundary type {t} not restated")
+    return grid

@@ -347,3 +347,36 @@ def test_flux_projection_matches_oracle(cpulib, r):
 def test_subling_flux_correction_matches_oracle(cpulib, name):
     import fluxcorr_checks as fc
     assert fc.run_subling_case(cpulib, name) > 0
+
+
+@pytest.mark.parametrize("name", ["reflecting", "outflow", "mixed_inflow", "shearing_periodic",
+                                  "inside_domain", "one_active_cell", "rank2"])
+def test_external_boundary_matches_oracle(cpulib, name):
+    import boundary_checks as bc
+    assert bc.run_external_case(cpulib, name) > 0
+
+
+def test_external_boundary_refusals_write_nothing(cpulib):
+    """Face types outside the restated ones (5, the hydrostatic type) and a
+    shearing face on a grid with fewer active cells than ghost layers are
+    refused before any face is filled."""
+    import ctypes as C
+    from enzo_ppm import abi
+    g, glob, _ = problems.random_smooth([2, 5, 4], seed=1, ncolours=1)
+    a = g.copy()
+    dg = device.DeviceGrid(a, glob, lib=cpulib)
+    try:
+        for types in ([1, 1, 5, 1, 1, 1], [4, 4, 1, 1, 1, 1]):
+            st = cpulib.hipPPMExternalBoundary(dg.handle, C.byref(dg.para), abi.iarr(types, 6),
+                                               abi.iarr([0, 0, 0], 3), abi.iarr(g.dims, 3),
+                                               g.nfields, None)
+            assert st == abi.ERR_NOT_SUPPORTED
+        st = cpulib.hipPPMExternalBoundary(dg.handle, C.byref(dg.para), abi.iarr([2] * 6, 6),
+                                           abi.iarr([0, 0, 0], 3), abi.iarr(g.dims, 3),
+                                           g.nfields, None)
+        assert st == abi.ERR_ARGUMENT          # inflow faces need their values
+        assert dg.download() == 1
+    finally:
+        dg.close()
+    for x, y in zip(a.fields, g.fields):
+        assert bit_equal(x, y)
