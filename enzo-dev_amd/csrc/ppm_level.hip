@@ -170,6 +170,15 @@ __global__ void level_periodic_kernel(const BCParams *__restrict__ B) {
     periodic_item(b, gid);
 }
 
+// one domain face of every grid in the list (blockIdx.y)
+__global__ void level_ext_bc_kernel(const ExtBCParams *__restrict__ B) {
+  const ExtBCParams &b = B[blockIdx.y];
+  const long long n = ext_bc_items(b);
+  for (long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x; gid < n;
+       gid += (long long)gridDim.x * blockDim.x)
+    ext_bc_item(b, gid);
+}
+
 __global__ void level_dt_init_kernel(unsigned long long *dtmin) {
   if (threadIdx.x == 0) *dtmin = (unsigned long long)__double_as_longlong(1.0e20);
 }
@@ -1217,6 +1226,97 @@ int hipPPMLevelPeriodicBoundary(hipPPMLevel *L) {
   if (hipStreamSynchronize(L->stream) != hipSuccess) st = ENZO_PPM_ERR_HIP;
   (void)hipFree(db);
   if (st != ENZO_PPM_SUCCESS) snprintf(g_level_error, sizeof(g_level_error), "periodic fill failed");
+  return st;
+}
+
+int hipPPMLevelExternalBoundary(hipPPMLevel *L, const enzo_int *BoundaryType,
+                                const enzo_int *GridOffset, const enzo_int *BoundaryDimension,
+                                enzo_int NumberOfBaryonFields, const enzo_float *InflowValue) {
+  if (!L || !BoundaryType || !GridOffset || !BoundaryDimension) return ENZO_PPM_ERR_ARGUMENT;
+  const int G = L->ngrids;
+  // per face in the reference's order (x left, x right, y left, ...): the
+  // grids that have it on the domain's edge; everything checked first
+  std::vector<std::vector<ExtBCParams>> faces(6);
+  for (int a = 0; a < L->rank; a++) {
+    if (BoundaryDimension[a] <= 1) continue;
+    for (int face = 0; face < 2; face++) {
+      const int type = (int)BoundaryType[2 * a + face];
+      if (type < BK_REFLECTING || type > BK_SHEARING) return ENZO_PPM_ERR_NOT_SUPPORTED;
+      if (type == BK_PERIODIC) continue;
+      for (int g = 0; g < G; g++) {
+        const hipPPMParameter &P = L->para[g];
+        const int *GD = &L->GD[3 * g];
+        const bool touches = face == 0 ? GridOffset[3 * g + a] == 0
+                                       : GridOffset[3 * g + a] + GD[a] == BoundaryDimension[a];
+        const int S = (int)P.GridStartIndex[a], E = (int)P.GridEndIndex[a], n = E - S + 1;
+        const int layers = face == 0 ? S : GD[a] - E - 1;
+        if (!touches || layers <= 0) continue;
+        if (type == BK_REFLECTING && (face == 0 ? 2 * S - 1 >= GD[a] : E - (layers - 1) < 0))
+          return ENZO_PPM_ERR_ARGUMENT;
+        if (type == BK_SHEARING && n < layers) return ENZO_PPM_ERR_NOT_SUPPORTED;
+        ExtBCParams B;
+        memset(&B, 0, sizeof(B));
+        B.axis = a;
+        B.face = face;
+        B.type = type;
+        B.layers = layers;
+        B.S = S;
+        B.E = E;
+        B.n = n;
+        for (int d = 0; d < 3; d++) B.GD[d] = GD[d];
+        B.nslot = L->nslot;
+        B.vslot = S_VX + a;
+        B.cells = L->cells[g];
+        B.A = (L->parity == 0 ? L->A : L->B) + L->off[g];
+        if (type == BK_INFLOW) {
+          if (!InflowValue) return ENZO_PPM_ERR_ARGUMENT;
+          long long fnum[S_COL0 + ENZO_PPM_MAX_COLOR];
+          for (int s = 0; s < L->nslot; s++) fnum[s] = -1;
+          fnum[S_D] = P.DensNum;
+          fnum[S_E] = P.TENum;
+          fnum[S_VX] = P.Vel1Num;
+          if (P.GridRank > 1) fnum[S_VY] = P.Vel2Num;
+          if (P.GridRank > 2) fnum[S_VZ] = P.Vel3Num;
+          if (P.DualEnergyFormalism) fnum[S_GE] = P.GENum;
+          for (int c = 0; c < L->ncol; c++) fnum[S_COL0 + c] = P.colnum[c];
+          for (int s = 0; s < L->nslot; s++) {
+            if (fnum[s] >= NumberOfBaryonFields) return ENZO_PPM_ERR_ARGUMENT;
+            B.inflow[s] = fnum[s] >= 0 ? InflowValue[fnum[s] * 6 + 2 * a + face] : 0.0;
+          }
+        }
+        faces[2 * a + face].push_back(B);
+      }
+    }
+  }
+  size_t tot = 0;
+  for (auto &v : faces) tot += v.size();
+  if (tot == 0) return ENZO_PPM_SUCCESS;
+  std::vector<ExtBCParams> hb;
+  hb.reserve(tot);
+  for (auto &v : faces) hb.insert(hb.end(), v.begin(), v.end());
+  ExtBCParams *db = nullptr;
+  if (dev_alloc(&db, hb.size())) return ENZO_PPM_ERR_HIP;
+  int st = ENZO_PPM_SUCCESS;
+  if (hipMemcpyAsync(db, hb.data(), sizeof(ExtBCParams) * hb.size(), hipMemcpyHostToDevice,
+                     L->stream) != hipSuccess)
+    st = ENZO_PPM_ERR_HIP;
+  size_t first = 0;
+  for (int f = 0; f < 6 && st == ENZO_PPM_SUCCESS; f++) {
+    const size_t cnt = faces[f].size();
+    if (cnt == 0) continue;
+    long long nmax = 1;
+    for (const auto &B : faces[f])
+      nmax = std::max(nmax, (long long)B.layers * B.GD[(B.axis + 1) % 3] * B.GD[(B.axis + 2) % 3] *
+                                B.nslot);
+    const unsigned bx = (unsigned)std::min<long long>((nmax + 255) / 256, 65535);
+    hipLaunchKernelGGL(level_ext_bc_kernel, dim3(bx, (unsigned)cnt), dim3(256), 0, L->stream,
+                       db + first);
+    if (hipGetLastError() != hipSuccess) st = ENZO_PPM_ERR_HIP;
+    first += cnt;
+  }
+  if (hipStreamSynchronize(L->stream) != hipSuccess) st = ENZO_PPM_ERR_HIP;
+  (void)hipFree(db);
+  if (st != ENZO_PPM_SUCCESS) snprintf(g_level_error, sizeof(g_level_error), "external boundary failed");
   return st;
 }
 

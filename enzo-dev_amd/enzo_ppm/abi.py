@@ -193,6 +193,7 @@ PROTOTYPES = {
                                               C.POINTER(EnzoFluxes)]),
     "hipPPMExternalBoundary": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), PI, PI, PI,
                                          C.c_int64, PF]),
+    "hipPPMLevelExternalBoundary": (C.c_int, [C.c_void_p, PI, PI, PI, C.c_int64, PF]),
     "hipPPMCorrectForRefinedFluxes": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter),
                                                 C.POINTER(EnzoFluxes), C.POINTER(EnzoFluxes),
                                                 C.POINTER(EnzoFluxes), C.c_int64, PI, PI,

@@ -96,6 +96,21 @@ class Level:
     def periodic_boundary(self):
         self._check(self.lib.hipPPMLevelPeriodicBoundary(self.handle), "LevelPeriodicBoundary")
 
+    def external_boundary(self, types, grid_offsets, boundary_dims, inflow=None):
+        """Domain-face ghost fill of every grid (hipPPMLevelExternalBoundary):
+        types[2 dim + face] as in DeviceGrid.external_boundary, grid_offsets
+        3 per grid (active cells from the domain's left edge),
+        boundary_dims the domain's cells with ghosts, inflow[field][6]."""
+        nf = self.grids[0].nfields
+        vals = None
+        if inflow is not None:
+            self._inflow = np.ascontiguousarray(np.asarray(inflow, dtype=np.float64).reshape(nf, 6))
+            vals = self._inflow.ctypes.data_as(abi.PF)
+        offs = abi.iarr([v for o in grid_offsets for v in o])
+        self._check(self.lib.hipPPMLevelExternalBoundary(self.handle, abi.iarr(types, 6), offs,
+                                                         abi.iarr(boundary_dims, 3), nf, vals),
+                    "LevelExternalBoundary")
+
     def set_siblings(self, global_starts, domain_active, periodic=True):
         """Sibling copy boxes from each grid's global start (index of its
         first cell, ghosts included) and the domain's active cells."""

@@ -484,10 +484,19 @@ int hipPPMLevelSetExpansionFactors(hipPPMLevel *Level, enzo_float a_courant, enz
 int hipPPMLevelTimeStep(hipPPMLevel *Level, enzo_float CourantSafetyNumber, enzo_float *dt);
 int hipPPMLevelComovingExpansionTerms(hipPPMLevel *Level, enzo_float coef, enzo_float Coefficient);
 
-/* Periodic ghost fill of every grid of the level (for grids that span the
-   domain, as the root grid does; the periodic case of
-   ExternalBoundary::SetExternalBoundary). */
+/* Periodic ghost fill of every grid of the level (for grids that span a
+   periodic domain alone, as a root grid does: the copies from their own
+   periodic images that SetBoundaryConditions makes, see
+   hipPPMPeriodicBoundary). */
 int hipPPMLevelPeriodicBoundary(hipPPMLevel *Level);
+
+/* hipPPMExternalBoundary for every grid of the level
+   (SetBoundaryConditions.C:112-113 on level 0): GridOffset holds 3 entries
+   per grid; a grid's faces on the domain's edge are filled, in the
+   reference's face order. */
+int hipPPMLevelExternalBoundary(hipPPMLevel *Level, const enzo_int *BoundaryType,
+                                const enzo_int *GridOffset, const enzo_int *BoundaryDimension,
+                                enzo_int NumberOfBaryonFields, const enzo_float *InflowValue);
 
 /* Sibling ghost copies (grid::CopyZonesFromGrid between the grids of one
    level, as SetBoundaryConditions applies it; Grid_CopyZonesFromGrid.C:

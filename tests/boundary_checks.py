@@ -56,3 +56,35 @@ def run_external_case(lib, name):
     rep = field_report(a, b)
     assert all(r[1] for r in rep), rep
     return sum(int(np.count_nonzero(x != y)) for x, y in zip(a.fields, g.fields))
+
+
+def run_level_external(lib, types, parts=(2, 1, 2)):
+    """hipPPMLevelExternalBoundary on a 12^3 domain cut into siblings (junk
+    in every ghost cell) against the oracle's fill of each grid at its
+    offset in the domain, every field bitwise."""
+    from enzo_ppm.level import Level
+    whole, glob, _ = problems.turbulent(12, seed=8, ncolours=2)
+    pieces, starts = bo.split_domain(whole, parts)
+    rng = np.random.default_rng(5)
+    for h in pieces:
+        sl = h.active_slice()
+        for f in h.fields:
+            keep = f[sl].copy()
+            f[...] = rng.normal(size=f.shape)
+            f[sl] = keep
+    gh = whole.ghost
+    offs = [[s[d] + gh for d in range(3)] for s in starts]
+    bd = [whole.active[d] + 2 * gh for d in range(3)]
+    inflow = [[10.0 * f + k for k in range(6)] for f in range(whole.nfields)]
+    ref = [h.copy() for h in pieces]
+    for h, o in zip(ref, offs):
+        bo.external_boundary(h, types, o, bd, inflow=inflow)
+    lv = Level([h.copy() for h in pieces], glob, lib=lib)
+    try:
+        lv.external_boundary(types, offs, bd, inflow=inflow)
+        lv.download()
+    finally:
+        lv.close()
+    for a, b in zip(lv.grids, ref):
+        rep = field_report(a, b)
+        assert all(r[1] for r in rep), rep

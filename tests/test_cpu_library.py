@@ -380,3 +380,9 @@ def test_external_boundary_refusals_write_nothing(cpulib):
         dg.close()
     for x, y in zip(a.fields, g.fields):
         assert bit_equal(x, y)
+
+
+@pytest.mark.parametrize("types", [[0, 1, 3, 3, 2, 0], [1, 1, 0, 0, 4, 4]])
+def test_level_external_boundary_matches_oracle(cpulib, types):
+    import boundary_checks as bc
+    bc.run_level_external(cpulib, types)

@@ -2,7 +2,8 @@
 oracle's restatement of ExternalBoundary::SetExternalBoundary, every field
 bitwise: reflecting, outflow, inflow, shearing and periodic faces, a grid
 inside the domain (only its faces on the domain's edge are filled), one
-active cell between reflecting faces, and a 2-D grid."""
+active cell between reflecting faces, and a 2-D grid; and the same for every
+grid of a level of siblings (hipPPMLevelExternalBoundary)."""
 import pytest
 
 import boundary_checks as bc
@@ -22,3 +23,8 @@ def lib():
 @pytest.mark.parametrize("name", sorted(bc.EXT_CASES))
 def test_external_boundary_matches_oracle(lib, name):
     assert bc.run_external_case(lib, name) > 0
+
+
+@pytest.mark.parametrize("types", [[0, 1, 3, 3, 2, 0], [1, 1, 0, 0, 4, 4]])
+def test_level_external_boundary_matches_oracle(lib, types):
+    bc.run_level_external(lib, types)
