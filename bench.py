@@ -213,6 +213,40 @@ def time_grid(lib, n, steps, warmup, weights=None):
                 sweep_ms_avg=sweep_ms / max(1, launches), fallback_cells=int(fb), grid=g)
 
 
+def host_buffer_rate(lib, n, calls=3):
+    """The drop-in call with host arrays (hipPPMSolvePPM_DE, as
+    grid::SolvePPM_DE would make it): upload of every field, pressure, the
+    three sweeps and download, on a handle kept across calls; one untimed
+    call, then `calls` timed.  PCIe-inclusive: not the metric."""
+    import ctypes as C
+    from enzo_ppm import device, problems
+    g, glob, run = problems.turbulent(n, seed=2024)
+    h = C.c_void_p(None)
+    try:
+        dg = device.DeviceGrid(g.copy(), glob, lib=lib)
+        try:
+            dg.periodic_boundary()
+            g.dt = dg.pressure_and_dt(run["cfl"])
+        finally:
+            dg.close()
+        elapsed = 0.0
+        for c in range(1 + calls):
+            g.fill_periodic()                 # the caller's boundary fill, untimed
+            t0 = time.perf_counter()
+            st = device.solve_ppm_de(g, glob, cycle=c, handle=h, lib=lib)
+            t1 = time.perf_counter()
+            if st != 1:
+                raise RuntimeError(f"SolvePPM_DE status {st}")
+            if c > 0:
+                elapsed += t1 - t0
+    finally:
+        if h.value:
+            lib.hipPPMDestroy(h)
+    return {"value": n ** 3 * calls / elapsed, "unit": "cell-updates/s",
+            "ms_per_call": elapsed * 1e3 / calls,
+            "what": "hipPPMSolvePPM_DE with host arrays: upload, pressure, 3 sweeps, download"}
+
+
 def learned_bench(lib, n, steps, warmup, epochs, cap):
     """The learned two-shock start end to end: harvest of plain-solver
     interface states from the 64^3 Sod tube (one step, every interface) and
@@ -384,6 +418,10 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_sweep},
         "cpu_baseline": None,
     }
+    try:
+        out["host_buffers"] = host_buffer_rate(lib, n)
+    except Exception as e:  # reported, not fatal
+        out["host_buffers"] = {"error": str(e)}
     weights = None
     if a.learn_epochs > 0:
         print("bench: learned start (harvest, training, statistics)", file=sys.stderr, flush=True)
