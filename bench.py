@@ -312,6 +312,7 @@ def cosmo_bench(lib, n, steps, weights, cfl=0.4):
                        "static acceleration, 6 species, dual energy, periodic",
            "steps": steps, "generation_seconds": gen_s}
     g0 = g.copy()
+    batched_state = None
     forms = [("batched", None)] + ([("batched_learned", weights)] if weights is not None else [])
     for name, w in forms:
         say(f"{name} form")
@@ -338,6 +339,8 @@ def cosmo_bench(lib, n, steps, weights, cfl=0.4):
             lv.close()
         out[name] = {"value": cells * steps / (t2 - t1), "ms_per_step": (t2 - t1) * 1e3 / steps,
                      "fallback_cells": int(nfb), "a_end": model(t)[0], "tile_shape": shapes[3]}
+        if name == "batched":
+            batched_state = g
         del lv, g
     say("plain per-grid form")
     g = g0
@@ -358,6 +361,12 @@ def cosmo_bench(lib, n, steps, weights, cfl=0.4):
         dg.close()
     out["plain_per_grid"] = {"value": cells * steps / (t2 - t1),
                              "ms_per_step": (t2 - t1) * 1e3 / steps}
+    # the batched and per-grid forms took the same steps from the same state
+    import numpy as np
+    out["bitwise_equal_batched_vs_per_grid"] = all(
+        np.array_equal(a.view(np.int64), b.view(np.int64))
+        for a, b in zip(batched_state.fields, g.fields))
+    del batched_state
     key = "batched_learned" if weights is not None else "batched"
     out["speedup"] = out[key]["value"] / out["plain_per_grid"]["value"]
     # algorithmic bytes per cell-update: 8 x [3 (R + W) + P] with R = 7 + C + G,
