@@ -64,6 +64,15 @@ __global__ __launch_bounds__(kFbTS) void ppm_fallback_kernel(SweepParams P) {
 }
 
 // ------------------------------------------------------------------------
+// RiemannSolver HLL: every pencil group of the launch on the fallback list
+// (the one-pencil workgroups then take flux_hll's fluxes everywhere).
+__global__ void fb_all_kernel(SweepParams P, unsigned ngroups) {
+  for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < ngroups; k += gridDim.x * blockDim.x)
+    P.fb_list[k] = k;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *P.fb_count = ngroups;
+}
+
+// ------------------------------------------------------------------------
 // Per-sweep preparation (prep_item in ppm_device.hpp), one item per thread.
 __global__ void sweep_prep_kernel(FlatParams F) {
   prep_item(F, (long long)blockIdx.x * blockDim.x + threadIdx.x);
@@ -248,7 +257,8 @@ int check_para(const hipPPMParameter *P) {
   }
   if (P->ConservativeReconstruction != 0 || P->PositiveReconstruction != 0)
     return ENZO_PPM_ERR_NOT_SUPPORTED;  // off on this path (inteuler.F:232-239)
-  if (P->RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK) return ENZO_PPM_ERR_NOT_SUPPORTED;
+  if (P->RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK && P->RiemannSolver != ENZO_PPM_RIEMANN_HLL)
+    return ENZO_PPM_ERR_NOT_SUPPORTED;
   if (P->PPMFlatteningParameter != 0 && P->PPMFlatteningParameter != 1)
     return ENZO_PPM_ERR_NOT_SUPPORTED;
   if (P->PPMDiffusionParameter != 0 && P->PPMDiffusionParameter != 1)
@@ -361,6 +371,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
   S.fb_tag = 0;
   S.tp = kTP;
   S.fb_ok = pencil_fits<kFbTS>(S) ? 1 : 0;
+  S.hll_all = P->RiemannSolver == ENZO_PPM_RIEMANN_HLL ? 1 : 0;
   S.learn = D->learned ? D->learn : nullptr;
   if (D->harvest_cap > 0) {
     S.harvest = D->harvest;
@@ -378,6 +389,15 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
       case 2: hipLaunchKernelGGL((ppm_sweep_kernel<0, 1>), g, b, 0, D->stream, S); break;
       default: hipLaunchKernelGGL((ppm_sweep_kernel<1, 1>), g, b, 0, D->stream, S); break;
     }
+    return hipGetLastError();
+  };
+  // the two-shock tiles, or with the HLL solver every group to the
+  // one-pencil workgroups (which then do the whole sweep)
+  auto launch_main_or_all = [&](unsigned nblk) {
+    if (!S.hll_all) return launch_main(nblk);
+    const unsigned ngroups = nblk / (unsigned)S.nseg;
+    hipLaunchKernelGGL(fb_all_kernel, dim3((ngroups + 255) / 256), dim3(256), 0, D->stream, S,
+                       ngroups);
     return hipGetLastError();
   };
   auto launch_fb = [&]() {
@@ -416,9 +436,10 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
       D->ev_used += 2;
       HIPCHK(hipEventRecord(e0, D->stream));
     }
-    HIPCHK(launch_main((unsigned)nblk));
+    HIPCHK(launch_main_or_all((unsigned)nblk));
+    if (S.hll_all) HIPCHK(launch_fb());
     if (D->profile) HIPCHK(hipEventRecord(e1, D->stream));
-    HIPCHK(launch_fb());
+    if (!S.hll_all) HIPCHK(launch_fb());
   } else {
     // diffusion reads the live transverse velocities of the previous slice
     // (calcdiss.F:207-288 on BaryonField): slices in order, one launch each
@@ -431,7 +452,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
     for (int k = 0; k < D->GD[oax]; k++) {
       S.t2_0 = k;
       long long nblk = (long long)S.nseg * S.nblk_t1;
-      HIPCHK(launch_main((unsigned)nblk));
+      HIPCHK(launch_main_or_all((unsigned)nblk));
       HIPCHK(launch_fb());  // the next slice reads this one's results
     }
   }

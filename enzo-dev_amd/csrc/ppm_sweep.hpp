@@ -121,6 +121,8 @@ struct SweepParams {
   unsigned *fb_list, *fb_count, *fb_done;
   unsigned fb_tag;  // added to list entries (the grid's first group in a batched level)
   int fb_ok;        // a one-pencil workgroup holds the whole sweep axis
+  int hll_all;      // RiemannSolver HLL: every cell takes flux_hll's fluxes
+                    // (Grid_xEulerSweep.C:245-256), in one-pencil workgroups
   int tp;           // pencils per group (the sweep tile's TP)
   // learned two-shock start: network parameters (ppm_learn.hpp), read by
   // the LEARN instantiation only
@@ -1031,6 +1033,19 @@ PPM_HD int ph_fb_decide(const SweepParams &P, const Tile &T, double *sm, int tid
   int nfb = 0, err = 0, prev = 0;
   LDS<TP, TS, UNI>(sm, A_COL, 0, P.i1 - 1 - T.L0) = 0.0;
   LDS<TP, TS, UNI>(sm, A_COL, 0, P.i2 + 1 - T.L0) = 0.0;
+  if (P.hll_all) {
+    // the HLL solver on the whole pencil: every interface takes the HLL
+    // fluxes and flux_hll's own check (flux_hll.F:336-384) is the only one;
+    // no cell counts as a fallback
+    for (int cell = P.i1; cell <= P.i2; cell++) {
+      const int c = cell - T.L0;
+      LDS<TP, TS, UNI>(sm, A_COL, 0, c) = 1.0;
+      if ((LDS<TP, TS, UNI>(sm, A_D, 0, c) + LDSH<TP, TS, UNI>(sm, A_HA, c)) -
+              LDSH<TP, TS, UNI>(sm, A_HA, c + 1) <= 0.0)
+        err = 1;
+    }
+    return err ? kFbError : 0;
+  }
   for (int cell = P.i1; cell <= P.i2; cell++) {
     const int c = cell - T.L0;
     const double d = LDS<TP, TS, UNI>(sm, A_D, 0, c), e = LDS<TP, TS, UNI>(sm, A_E, 0, c);
@@ -1408,14 +1423,14 @@ PPM_HD TileCounts sweep_body(const SweepParams &P, const Tile &T, double *sm, X 
   bool fbon = false;
   int fberr = 0;
   if constexpr (FB != 0) {
-    if (tc.bad) {  // block-uniform
+    if (tc.bad || P.hll_all) {  // block-uniform
     const int dec = ex.first([&](int tid, Regs &r) {
       (void)r;
       return ph_fb_decide<TP, TS, UNI>(P, T, sm, tid);
     });
     tc.nfb = dec % kFbError;
     fberr = dec / kFbError;
-    fbon = tc.nfb > 0;
+    fbon = tc.nfb > 0 || P.hll_all;
     if (fbon) ex.run([&](int tid, Regs &r) { ph_fb_apply<TP, TS, UNI>(P, T, sm, tid, r); });
     }
   }

@@ -22,7 +22,7 @@ DEFAULT_GLOBALS = dict(
     DualEnergyFormalismEta2=0.1,
     PressureFree=0,
     tiny_number=1e-20,
-    RiemannSolver=0,          # TwoShock
+    RiemannSolver=5,          # TwoShock (typedefs.h: HLL 1, HLLC 4, TwoShock 5)
     RiemannSolverFallback=1,
     ReconstructionMethod=1,   # PPM
     ConservativeReconstruction=0,

@@ -52,8 +52,11 @@ typedef int64_t enzo_long_int; /* long_int (macros_and_parameters.h:202-217) */
 #define ENZO_PPM_ERR_NOT_SUPPORTED (-4) /* option the reference rejects on this path */
 #define ENZO_PPM_ERR_NO_DEVICE (-5)
 
-/* Riemann solver and reconstruction enums (typedefs.h) used on this path */
-#define ENZO_PPM_RIEMANN_TWOSHOCK 0 /* TwoShock: ReadParameterFile.C:1574-1578 */
+/* Riemann solver and reconstruction enums (typedefs.h:278-280) used on this
+   path; TwoShock is PPM_DirectEuler's default (ReadParameterFile.C:1574-1578) */
+#define ENZO_PPM_RIEMANN_HLL 1
+#define ENZO_PPM_RIEMANN_HLLC 4
+#define ENZO_PPM_RIEMANN_TWOSHOCK 5
 #define ENZO_PPM_RECONSTRUCTION_PPM 1
 
 /* struct fluxes (src/enzo/Fluxes.h:29-37), identical layout */
@@ -135,7 +138,15 @@ typedef struct hipPPMData hipPPMData;
 
 /* ---- lifecycle mirroring cuPPM.h:53-102 ---------------------------------- */
 
-/* cuPPMInitParameter (cuPPM.h:53-79); adds GridRank and SmallRho. */
+/* cuPPMInitParameter (cuPPM.h:53-79); adds GridRank and SmallRho.
+   RiemannSolver takes the reference's values (typedefs.h:279-280):
+   TwoShock (5, the default of PPM_DirectEuler) with the per-cell HLL
+   fallback when RiemannSolverFallback is set, or HLL (1): flux_hll over
+   whole pencils (Grid_xEulerSweep.C:245-256), its negative-density check
+   giving ENZO_PPM_ERR_NEGATIVE; each pencil is then computed by one
+   workgroup that holds it whole, so pencils of up to 1025 cells.  HLLC and
+   the others: ENZO_PPM_ERR_NOT_SUPPORTED.  The batched level
+   (hipPPMLevelCreate) takes TwoShock only. */
 int hipPPMInitParameter(hipPPMParameter *Para, enzo_int DensNum, enzo_int TENum,
                         enzo_int Vel1Num, enzo_int Vel2Num, enzo_int Vel3Num,
                         enzo_int GENum, enzo_int MaxNumberOfBaryonFields,

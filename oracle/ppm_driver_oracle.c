@@ -94,6 +94,13 @@ void KER(flux_twoshock)(const R *, const R *, const R *, const R *, const R *,
                         const R *, const R *, const R *, const R *, R *, R *, R *,
                         R *, R *, R *, R *, const I *, const R *, const R *,
                         const R *, R *);
+void KER(flux_hll)(const R *, const R *, const R *, const R *, const R *, const R *,
+                   const R *, const R *, const I *, const I *, const I *, const I *,
+                   const I *, const I *, const R *, const R *, const I *, const I *,
+                   const R *, const I *, const R *, const R *, const R *, const R *,
+                   const R *, const R *, const R *, const R *, const R *, const R *,
+                   const R *, const R *, R *, R *, R *, R *, R *, R *, R *, const I *,
+                   const R *, const R *, const R *, R *);
 void KER(euler)(R *, R *, const R *, R *, R *, R *, R *, const R *, const R *,
                 const I *, const I *, const I *, const I *, const I *, const I *,
                 const R *, const R *, const I *, const I *, const I *, const R *,
@@ -358,6 +365,15 @@ static I sweep_slice(const sweepctx *c, I dim, I s, slicebuf *b) {
                 &gl->PositiveReconstruction, &dt, &gl->Gamma, &gl->PressureFree, b->dls, b->drs,
                 b->pls, b->prs, b->gels, b->gers, b->uls, b->urs, b->vls, b->vrs, b->wls,
                 b->wrs, &NumberOfColours, b->col, b->colls, b->colrs);
+  if (gl->RiemannSolver == ENZO_PPM_RIEMANN_HLL) {
+    /* Grid_xEulerSweep.C:245-256: HLL fluxes over the whole slice */
+    KER(flux_hll)(b->d, b->e, b->ge, b->u, b->v, b->w, dxs, b->diffcoef, &idim, &jdim, &is, &ie,
+                  &js, &je, &dt, &gl->Gamma, &gl->PPMDiffusionParameter,
+                  &gl->DualEnergyFormalism, &gl->DualEnergyFormalismEta1,
+                  &gl->RiemannSolverFallback, b->dls, b->drs, b->pls, b->prs, b->uls, b->urs,
+                  b->vls, b->vrs, b->wls, b->wrs, b->gels, b->gers, b->df, b->uf, b->vf, b->wf,
+                  b->ef, b->gef, b->ges, &NumberOfColours, b->col, b->colls, b->colrs, b->colf);
+  } else {
   KER(twoshock)(b->dls, b->drs, b->pls, b->prs, b->uls, b->urs, &idim, &jdim, &is, &ie_p1, &js,
                 &je, &dt, &gl->Gamma, &MinimumPressure, &gl->PressureFree, b->pbar, b->ubar,
                 &GravityOn, b->gr, &gl->DualEnergyFormalism, &gl->DualEnergyFormalismEta1);
@@ -368,6 +384,7 @@ static I sweep_slice(const sweepctx *c, I dim, I s, slicebuf *b) {
                      b->gers, b->uls, b->urs, b->vls, b->vrs, b->wls, b->wrs, b->pbar, b->ubar,
                      b->df, b->ef, b->uf, b->vf, b->wf, b->gef, b->ges, &NumberOfColours, b->col,
                      b->colls, b->colrs, b->colf);
+  }
   KER(euler)(b->d, b->e, b->gr, b->ge, b->u, b->v, b->w, dxs, b->diffcoef, &idim, &jdim, &is,
              &ie, &js, &je, &dt, &gl->Gamma, &gl->PPMDiffusionParameter, &GravityOn,
              &gl->DualEnergyFormalism, &gl->DualEnergyFormalismEta1,
@@ -497,7 +514,7 @@ int PFX(solve_ppm_de)(const enzo_ppm_grid *g, const enzo_ppm_globals *gl, I Cycl
                       const I *GridGlobalStart, I GravityOn, I NumberOfColours,
                       const I *colnum, R MinimumSupportEnergyCoefficient, I nthreads,
                       I *fallback_count) {
-  if (gl->RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK ||
+  if ((gl->RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK && gl->RiemannSolver != ENZO_PPM_RIEMANN_HLL) ||
       gl->ReconstructionMethod != ENZO_PPM_RECONSTRUCTION_PPM)
     return ENZO_PPM_ERR_NOT_SUPPORTED;
   if (NumberOfColours > ENZO_PPM_MAX_COLOR) return ENZO_PPM_ERR_ARGUMENT;

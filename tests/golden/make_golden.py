@@ -32,7 +32,18 @@ CASES = {
     "fluxes_10": (lambda: problems.random_smooth([10, 9, 8], seed=25, ncolours=1), 0.4, 1,
                   [((5, 4, 4), (8, 7, 6)), ((3, 3, 3), (12, 11, 10))]),
     "blast_10": (lambda: problems.amr_level(1, seed=26, nmin=8, nmax=12)[0][0], 0.4, 2, None),
+    # RiemannSolver HLL (typedefs.h: 1): flux_hll over whole slices
+    "hll_turbulent_10": (lambda: with_glob(problems.turbulent(10, seed=27), RiemannSolver=1),
+                         0.4, 2, None),
+    "hll_diffusion_8": (lambda: with_glob(problems.random_smooth([8, 9, 7], seed=28, ncolours=1,
+                                                                flags=(1, 1, 1)), RiemannSolver=1),
+                        0.4, 2, None),
 }
+
+
+def with_glob(case, **kw):
+    g, glob, run = case
+    return g, dict(glob, **kw), run
 
 
 def grid_arrays(g):
@@ -118,9 +129,14 @@ def make_fallback(name):
 if __name__ == "__main__":
     if not oracle.available("ref"):
         oracle.build("ref")
+    only = sys.argv[1:]          # case names to (re)write; all by default
     for n in CASES:
+        if only and n not in only:
+            continue
         make(n)
         print("wrote", n)
+    if only:
+        sys.exit(0)
     make_twoshock()
     print("wrote twoshock")
     for n in ("1d_ok", "3d_y_ok", "3d_z_stretched_ok"):

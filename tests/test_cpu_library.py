@@ -386,3 +386,28 @@ def test_external_boundary_refusals_write_nothing(cpulib):
 def test_level_external_boundary_matches_oracle(cpulib, types):
     import boundary_checks as bc
     bc.run_level_external(cpulib, types)
+
+
+HLL_CASES = {
+    "turbulent": lambda: problems.turbulent(12, seed=5),
+    "diffusion": lambda: problems.random_smooth([10, 12, 8], seed=4, ncolours=1, flags=(1, 1, 1)),
+    "gravity": lambda: problems.random_smooth([10, 12, 8], seed=2, ncolours=1, gravity=True),
+    "no_dual_energy": lambda: problems.random_smooth([10, 12, 8], seed=3, dual_energy=False),
+    "sod_2d": lambda: problems.sod_x(20, rank=2, gamma=1.4, flags=(1, 1, 1)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(HLL_CASES))
+def test_hll_solver_steps(cpulib, name):
+    """RiemannSolver HLL (flux_hll over whole slices): the device path equals
+    the oracle bitwise over steps of every sweep order, with no fallback."""
+    g, glob, _ = HLL_CASES[name]()
+    glob = dict(glob, RiemannSolver=1)
+    for cycle in (0, 1, 2):
+        fill_bc(g, "periodic")
+        g.dt = oracle.compute_time_step("port", g, glob, 0.4)
+        a, b = g.copy(), g.copy()
+        assert oracle.solve_ppm_de("port", a, glob, cycle=cycle) == (1, 0)
+        assert run(cpulib, b, glob, cycle) == (1, 0)
+        assert all(r[1] for r in field_report(b, a))
+        g = a
