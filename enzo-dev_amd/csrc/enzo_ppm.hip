@@ -259,6 +259,14 @@ int check_para(const hipPPMParameter *P) {
     return ENZO_PPM_ERR_NOT_SUPPORTED;  // off on this path (inteuler.F:232-239)
   if (P->RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK && P->RiemannSolver != ENZO_PPM_RIEMANN_HLL)
     return ENZO_PPM_ERR_NOT_SUPPORTED;
+  // the HLL solver runs in one-pencil workgroups: every swept pencil must fit
+  if (P->RiemannSolver == ENZO_PPM_RIEMANN_HLL)
+    for (int d = 0; d < P->GridRank; d++)
+      if (P->GridDimension[d] > kFbTS + 1) {
+        snprintf(g_last_error, sizeof(g_last_error),
+                 "RiemannSolver HLL: pencils longer than %d cells are not supported", kFbTS + 1);
+        return ENZO_PPM_ERR_NOT_SUPPORTED;
+      }
   if (P->PPMFlatteningParameter != 0 && P->PPMFlatteningParameter != 1)
     return ENZO_PPM_ERR_NOT_SUPPORTED;
   if (P->PPMDiffusionParameter != 0 && P->PPMDiffusionParameter != 1)

@@ -164,7 +164,8 @@ def test_device_matches_golden(lib, path):
 
 
 HLL_CASES = ["turbulent_40", "diffusion", "gravity", "no_dual_energy", "sod_2d", "stretched_grid",
-             "longest_x"]
+             "longest_hll"]
+CASES["longest_hll"] = lambda: problems.random_smooth([2, 1019, 3], seed=12, ncolours=1)
 
 
 @pytest.mark.parametrize("name", HLL_CASES)
@@ -178,3 +179,15 @@ def test_hll_solver_matches_oracle(lib, name, cycle):
     assert fbk == 0
     rep = field_report(a, b)
     assert all(r[1] for r in rep), rep
+
+
+def test_hll_solver_refuses_longer_pencils(lib):
+    """The HLL solver's one-pencil workgroups hold 1025 cells: a longer
+    pencil (up to the reference's 1031) is refused before anything runs."""
+    g, glob, run = CASES["longest_x"]()
+    glob = dict(glob, RiemannSolver=1)
+    fill_bc(g, "periodic")
+    g.dt = oracle.compute_time_step("port", g, glob, run["cfl"])
+    a = g.copy()
+    assert device.solve_ppm_de(a, glob, cycle=0, lib=lib) == abi.ERR_NOT_SUPPORTED
+    assert b"HLL" in lib.hipPPMLastError()
