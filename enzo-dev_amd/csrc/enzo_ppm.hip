@@ -257,14 +257,16 @@ int check_para(const hipPPMParameter *P) {
   }
   if (P->ConservativeReconstruction != 0 || P->PositiveReconstruction != 0)
     return ENZO_PPM_ERR_NOT_SUPPORTED;  // off on this path (inteuler.F:232-239)
-  if (P->RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK && P->RiemannSolver != ENZO_PPM_RIEMANN_HLL)
+  if (P->RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK && P->RiemannSolver != ENZO_PPM_RIEMANN_HLL &&
+      P->RiemannSolver != ENZO_PPM_RIEMANN_HLLC)
     return ENZO_PPM_ERR_NOT_SUPPORTED;
-  // the HLL solver runs in one-pencil workgroups: every swept pencil must fit
-  if (P->RiemannSolver == ENZO_PPM_RIEMANN_HLL)
+  // the HLL and HLLC solvers run in one-pencil workgroups: every swept
+  // pencil must fit
+  if (P->RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK)
     for (int d = 0; d < P->GridRank; d++)
       if (P->GridDimension[d] > kFbTS + 1) {
         snprintf(g_last_error, sizeof(g_last_error),
-                 "RiemannSolver HLL: pencils longer than %d cells are not supported", kFbTS + 1);
+                 "RiemannSolver HLL/HLLC: pencils longer than %d cells are not supported", kFbTS + 1);
         return ENZO_PPM_ERR_NOT_SUPPORTED;
       }
   if (P->PPMFlatteningParameter != 0 && P->PPMFlatteningParameter != 1)
@@ -380,6 +382,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
   S.tp = kTP;
   S.fb_ok = pencil_fits<kFbTS>(S) ? 1 : 0;
   S.hll_all = P->RiemannSolver == ENZO_PPM_RIEMANN_HLL ? 1 : 0;
+  S.hllc = P->RiemannSolver == ENZO_PPM_RIEMANN_HLLC ? 1 : 0;
   S.learn = D->learned ? D->learn : nullptr;
   if (D->harvest_cap > 0) {
     S.harvest = D->harvest;
@@ -401,8 +404,9 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
   };
   // the two-shock tiles, or with the HLL solver every group to the
   // one-pencil workgroups (which then do the whole sweep)
+  const bool whole_pencils = S.hll_all || S.hllc;
   auto launch_main_or_all = [&](unsigned nblk) {
-    if (!S.hll_all) return launch_main(nblk);
+    if (!whole_pencils) return launch_main(nblk);
     const unsigned ngroups = nblk / (unsigned)S.nseg;
     hipLaunchKernelGGL(fb_all_kernel, dim3((ngroups + 255) / 256), dim3(256), 0, D->stream, S,
                        ngroups);
@@ -445,9 +449,9 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
       HIPCHK(hipEventRecord(e0, D->stream));
     }
     HIPCHK(launch_main_or_all((unsigned)nblk));
-    if (S.hll_all) HIPCHK(launch_fb());
+    if (whole_pencils) HIPCHK(launch_fb());
     if (D->profile) HIPCHK(hipEventRecord(e1, D->stream));
-    if (!S.hll_all) HIPCHK(launch_fb());
+    if (!whole_pencils) HIPCHK(launch_fb());
   } else {
     // diffusion reads the live transverse velocities of the previous slice
     // (calcdiss.F:207-288 on BaryonField): slices in order, one launch each

@@ -123,6 +123,8 @@ struct SweepParams {
   int fb_ok;        // a one-pencil workgroup holds the whole sweep axis
   int hll_all;      // RiemannSolver HLL: every cell takes flux_hll's fluxes
                     // (Grid_xEulerSweep.C:245-256), in one-pencil workgroups
+  int hllc;         // RiemannSolver HLLC: flux_hllc's fluxes with its per-cell
+                    // flux_hll fallback (Grid_xEulerSweep.C:258-269), likewise
   int tp;           // pencils per group (the sweep tile's TP)
   // learned two-shock start: network parameters (ppm_learn.hpp), read by
   // the LEARN instantiation only
@@ -193,6 +195,9 @@ struct Regs {
   // two-shock fluxes
   double h_df, h_uf, h_vf, h_wf, h_ef, h_gef, h_sl, h_sr, h_bm0, h_bp0;
   int hll;
+  // HLLC workgroups: the HLLC weights of the interface, the gas-energy
+  // source terms sl*gesl and sr*gesr, and the cell's gas-energy source
+  double c_sl, c_sr, c_ga, c_gb, c_ges;
 };
 
 template <int TP, int TS, int UNI = 0>
@@ -782,6 +787,77 @@ PPM_HD void hll_flux(const HllIn &h, Regs &r) {
   r.h_bp0 = bp0;
 }
 
+// HLLC fluxes of one interface (flux_hllc.F:104-340): the Roe averages and
+// Davis wave speeds of flux_hll, the contact speed cw and pressure cp, the
+// HLLC weights (chosen by the sign of cw), left/right fluxes, the contact
+// terms sm*cp and sm*cp*cw, dt/dx absorbed; they replace the interface's
+// primary fluxes.  Also the weights and gas-energy source terms the cell
+// phases need.
+PPM_HD void hllc_flux(const HllIn &h, Regs &r) {
+  const double gamma1 = h.gamma - 1.0;
+  const double gamma1i = 1.0 / gamma1;
+  const double sqrtdl = sqrt(h.dls);
+  const double sqrtdr = sqrt(h.drs);
+  const double isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+  const double vroe1 = (sqrtdl * h.uls + sqrtdr * h.urs) * isdlpdr;
+  const double vroe2 = (sqrtdl * h.vls + sqrtdr * h.vrs) * isdlpdr;
+  const double vroe3 = (sqrtdl * h.wls + sqrtdr * h.wrs) * isdlpdr;
+  const double v2 = vroe1 * vroe1 + vroe2 * vroe2 + vroe3 * vroe3;
+  const double el = gamma1i * h.pls + 0.5 * h.dls * (h.uls * h.uls + h.vls * h.vls + h.wls * h.wls);
+  const double er = gamma1i * h.prs + 0.5 * h.drs * (h.urs * h.urs + h.vrs * h.vrs + h.wrs * h.wrs);
+  const double hroe = ((el + h.pls) / sqrtdl + (er + h.prs) / sqrtdr) * isdlpdr;
+  const double cs = sqrt(gamma1 * fmx((hroe - 0.5 * v2), kTiny));
+  const double char1 = vroe1 - cs, char2 = vroe1 + cs;
+  const double csl0 = sqrt(h.gamma * h.pls / h.dls);
+  const double csr0 = sqrt(h.gamma * h.prs / h.drs);
+  const double csl = fmn(h.uls - csl0, char1);
+  const double csr = fmx(h.urs + csr0, char2);
+  const double bm = fmn(csl, 0.0), bp = fmx(csr, 0.0);
+  // contact wave (flux_hllc.F:158-180)
+  const double tl = h.pls - (csl - h.uls) * h.dls * h.uls;
+  const double tr = h.prs - (csr - h.urs) * h.drs * h.urs;
+  const double dl = h.dls * (csl - h.uls);
+  const double dr = -h.drs * (csr - h.urs);
+  const double q1 = 1.0 / (dl + dr);
+  const double cw = (tr - tl) * q1;
+  double cp = (dl * tr + dr * tl) * q1;
+  double sl, sr, sm;
+  if (cw >= 0.0) {
+    sl = cw / (cw - bm);
+    sr = 0.0;
+    sm = -bm / (cw - bm);
+  } else {
+    sl = 0.0;
+    sr = -cw / (bp - cw);
+    sm = bp / (bp - cw);
+  }
+  cp = fmx(cp, 0.0);
+  // left/right fluxes (:222-244), HLLC flux plus diffusion and contact
+  // terms (:272-302), dt/dx (:321-340)
+  const double dubl = h.dls * h.uls, dubr = h.drs * h.urs;
+  const double bm0 = h.uls - bm, bp0 = h.urs - bp;
+  const double dfl = dubl - bm * h.dls, dfr = dubr - bp * h.drs;
+  const double ufl = dubl * bm0 + h.pls, ufr = dubr * bp0 + h.prs;
+  const double vfl = h.dls * h.vls * bm0, vfr = h.drs * h.vrs * bp0;
+  const double wfl = h.dls * h.wls * bm0, wfr = h.drs * h.wrs * bp0;
+  const double efl = el * bm0 + h.pls * h.uls, efr = er * bp0 + h.prs * h.urs;
+  r.df = h.qc * (sl * dfl + sr * dfr + h.diffd);
+  r.uf = h.qc * ((sl * ufl + sr * ufr + h.diffuu) + sm * cp);
+  r.vf = h.qc * (sl * vfl + sr * vfr + h.diffuv);
+  r.wf = h.qc * (sl * wfl + sr * wfr + h.diffuw);
+  r.ef = h.qc * ((sl * efl + sr * efr + h.diffue) + sm * cp * cw);
+  r.gef = 0.0;
+  r.c_ga = r.c_gb = 0.0;
+  if (h.idual == 1) {
+    const double gefl = bm0 * h.gels * h.dls, gefr = bp0 * h.gers * h.drs;
+    r.gef = h.qc * (sl * gefl + sr * gefr + h.diffuge);
+    r.c_ga = sl * bm0;
+    r.c_gb = sr * bp0;
+  }
+  r.c_sl = sl;
+  r.c_sr = sr;
+}
+
 // Phase: Eulerian corrections (inteuler.F:321-551), two-shock solve and
 // Godunov fluxes (flux_twoshock.F:84-396) for the thread's interface; in
 // fallback workgroups (FB) also the HLL fluxes of the interface.
@@ -967,6 +1043,7 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
     h.qc = qc;
     h.idual = P.idual;
     hll_flux(h, r);
+    if (P.hllc) hllc_flux(h, r);
   }
 }
 
@@ -986,6 +1063,41 @@ PPM_HD void ph_store_flux(const SweepParams &P, const Tile &T, double *sm, int t
   LDS<TP, TS, UNI>(sm, F_WF, p, cb) = r.wf;
   LDS<TP, TS, UNI>(sm, F_GEF, p, cb) = r.gef;
   LDS<TP, TS, UNI>(sm, F_UB, p, cb) = r.ub;
+  if constexpr (FB != 0) {
+    if (P.hllc) {  // the HLLC gas-energy source terms (F_UB is unused here)
+      LDS<TP, TS, UNI>(sm, F_UB, p, cb) = r.c_ga;
+      LDSH<TP, TS, UNI>(sm, A_HB, cb) = r.c_gb;
+    }
+  }
+}
+
+PPM_HD void upd_range(const SweepParams &P, const Tile &T, int &c0, int &c1);
+
+// HLLC workgroups: each updated cell's gas-energy source
+// qc*pcent*(sl*gesl(c) + sr*gesr(c) - sl*gesl(c+1) - sr*gesr(c+1))
+// (flux_hllc.F:306-316) into a register, then (after a barrier) into F_UB,
+// where the cell phase reads it; A_HB is free again for the fallback.
+template <int TP, int TS, int UNI = 0>
+PPM_HD void ph_hllc_ges(const SweepParams &P, const Tile &T, double *sm, int tid, Regs &r) {
+  r.c_ges = 0.0;
+  int c0, c1;
+  upd_range(P, T, c0, c1);
+  if (!P.idual || tid > c1 - c0) return;
+  const int cell = c0 + tid, c = cell - T.L0;
+  const double pcent = fmx((P.gamma - 1.0) * LDS<TP, TS, UNI>(sm, A_GE, 0, c) * LDS<TP, TS, UNI>(sm, A_D, 0, c), kTiny);
+  const double qc = qc_of<UNI>(P, T.dt, cell);
+  r.c_ges = qc * pcent *
+            (LDS<TP, TS, UNI>(sm, F_UB, 0, c) + LDSH<TP, TS, UNI>(sm, A_HB, c) -
+             LDS<TP, TS, UNI>(sm, F_UB, 0, c + 1) - LDSH<TP, TS, UNI>(sm, A_HB, c + 1));
+}
+
+template <int TP, int TS, int UNI = 0>
+PPM_HD void ph_hllc_ges_store(const SweepParams &P, const Tile &T, double *sm, int tid,
+                              const Regs &r) {
+  int c0, c1;
+  upd_range(P, T, c0, c1);
+  if (!P.idual || tid > c1 - c0) return;
+  LDS<TP, TS, UNI>(sm, F_UB, 0, c0 + tid - T.L0) = r.c_ges;
 }
 
 // updated cells of the tile: [IA, IB-1] within [i1, i2]
@@ -1052,7 +1164,10 @@ PPM_HD int ph_fb_decide(const SweepParams &P, const Tile &T, double *sm, int tid
     const double dfl = prev ? LDSH<TP, TS, UNI>(sm, A_HA, c) : LDS<TP, TS, UNI>(sm, F_DF, 0, c);
     const double dfr = LDS<TP, TS, UNI>(sm, F_DF, 0, c + 1);
     int fb = 0;
-    if (d + (dfl - dfr) <= 0.0 || e < 0.0) {
+    // flux_twoshock.F:401-403 checks d + (df(i) - df(i+1)) and e; flux_hllc.F:
+    // 344 only (d + df(i)) - df(i+1)
+    const bool bad = P.hllc ? (d + dfl) - dfr <= 0.0 : (d + (dfl - dfr) <= 0.0 || e < 0.0);
+    if (bad) {
       if (P.ifallback == 0) {
         err = 1;
       } else {
@@ -1129,6 +1244,7 @@ PPM_HD int ph_euler(const SweepParams &P, const Tile &T, double *sm, int tid, bo
       double qc = qc_of<UNI>(P, T.dt, cell);
       double ges = qc * pcent * (LDS<TP, TS, UNI>(sm, F_UB, p, c) - LDS<TP, TS, UNI>(sm, F_UB, p, c + 1));
       if constexpr (FB != 0) {
+        if (P.hllc) ges = LDS<TP, TS, UNI>(sm, F_UB, p, c);  // ph_hllc_ges
         if (fbon &&
             (LDS<TP, TS, UNI>(sm, A_COL, p, c) != 0.0 || LDS<TP, TS, UNI>(sm, A_COL, p, c - 1) != 0.0))
           ges = qc * pcent *
@@ -1307,6 +1423,8 @@ PPM_HD void ph_col_flux(const SweepParams &P, const Tile &T, double *sm, int tid
   double colf = T.dt * r.ub * colb;
   if (FB != 0 && r.hll)  // flux_hll.F:231-236, 272-276, 327-331
     colf = T.dt * (r.h_sl * (r.h_bm0 * colls) + r.h_sr * (r.h_bp0 * colrs));
+  else if (FB != 0 && P.hllc)  // flux_hllc.F:252-258, 286-290, 336-340
+    colf = T.dt * (r.c_sl * (r.h_bm0 * colls) + r.c_sr * (r.h_bp0 * colrs));
   colf = colf / TS_<UNI>(P, T_DX, i);
   LDS<TP, TS, UNI>(sm, F_COLF, p, cb) = colf;
   if (P.face_head && i >= T.o && i < T.o + T.w) {
@@ -1419,11 +1537,17 @@ PPM_HD TileCounts sweep_body(const SweepParams &P, const Tile &T, double *sm, X 
     ph_solve<TP, TS, UNI, FB, LEARN>(P, T, sm, tid, r);
     ph_store_flux<TP, TS, UNI, FB>(P, T, sm, tid, r);
   });
+  if constexpr (FB != 0) {
+    if (P.hllc) {
+      ex.run([&](int tid, Regs &r) { ph_hllc_ges<TP, TS, UNI>(P, T, sm, tid, r); });
+      ex.run([&](int tid, Regs &r) { ph_hllc_ges_store<TP, TS, UNI>(P, T, sm, tid, r); });
+    }
+  }
   tc.bad = ex.count([&](int tid, Regs &r) { (void)r; return ph_check<TP, TS, UNI>(P, T, sm, tid); });
   bool fbon = false;
   int fberr = 0;
   if constexpr (FB != 0) {
-    if (tc.bad || P.hll_all) {  // block-uniform
+    if (tc.bad || P.hll_all || P.hllc) {  // block-uniform
     const int dec = ex.first([&](int tid, Regs &r) {
       (void)r;
       return ph_fb_decide<TP, TS, UNI>(P, T, sm, tid);

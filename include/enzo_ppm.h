@@ -141,11 +141,12 @@ typedef struct hipPPMData hipPPMData;
 /* cuPPMInitParameter (cuPPM.h:53-79); adds GridRank and SmallRho.
    RiemannSolver takes the reference's values (typedefs.h:279-280):
    TwoShock (5, the default of PPM_DirectEuler) with the per-cell HLL
-   fallback when RiemannSolverFallback is set, or HLL (1): flux_hll over
-   whole pencils (Grid_xEulerSweep.C:245-256), its negative-density check
-   giving ENZO_PPM_ERR_NEGATIVE; each pencil is then computed by one
-   workgroup that holds it whole, so pencils of up to 1025 cells.  HLLC and
-   the others: ENZO_PPM_ERR_NOT_SUPPORTED.  The batched level
+   fallback when RiemannSolverFallback is set, HLL (1): flux_hll over whole
+   pencils (Grid_xEulerSweep.C:245-256), its negative-density check giving
+   ENZO_PPM_ERR_NEGATIVE, or HLLC (4): flux_hllc (:258-269) with its own
+   per-cell flux_hll fallback; with HLL or HLLC each pencil is computed by
+   one workgroup that holds it whole, so pencils of up to 1025 cells.  The
+   others: ENZO_PPM_ERR_NOT_SUPPORTED.  The batched level
    (hipPPMLevelCreate) takes TwoShock only. */
 int hipPPMInitParameter(hipPPMParameter *Para, enzo_int DensNum, enzo_int TENum,
                         enzo_int Vel1Num, enzo_int Vel2Num, enzo_int Vel3Num,

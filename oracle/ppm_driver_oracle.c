@@ -59,9 +59,15 @@ void f_error_(const char *file, const I *line, long len) {
   ref_error_code = *line > 0 ? *line : 1;
 }
 void f_warning_(const char *file, const I *line, long len) {
-  (void)line;
-  /* flux_twoshock.F:443-445 warns once per cell before calling flux_hll */
-  if (len >= 13 && strstr(file, "flux_twoshock") != NULL) ref_fallback_count++;
+  /* flux_twoshock.F:446 and flux_hllc.F:380 warn once per cell before
+   * calling flux_hll (their other warnings are about colours) */
+  char name[512];  /* a Fortran string: len characters, no terminator */
+  const size_t n = len > 0 ? ((size_t)len < sizeof(name) ? (size_t)len : sizeof(name) - 1) : 0;
+  memcpy(name, file, n);
+  name[n] = 0;
+  if ((strstr(name, "flux_twoshock") != NULL && *line == 446) ||
+      (strstr(name, "flux_hllc") != NULL && *line == 380))
+    ref_fallback_count++;
 }
 #define ERRCODE ref_error_code
 #define FALLBACKS ref_fallback_count
@@ -101,6 +107,13 @@ void KER(flux_hll)(const R *, const R *, const R *, const R *, const R *, const 
                    const R *, const R *, const R *, const R *, const R *, const R *,
                    const R *, const R *, R *, R *, R *, R *, R *, R *, R *, const I *,
                    const R *, const R *, const R *, R *);
+void KER(flux_hllc)(const R *, const R *, const R *, const R *, const R *, const R *,
+                    const R *, const R *, const I *, const I *, const I *, const I *,
+                    const I *, const I *, const R *, const R *, const I *, const I *,
+                    const R *, const I *, const R *, const R *, const R *, const R *,
+                    const R *, const R *, const R *, const R *, const R *, const R *,
+                    const R *, const R *, R *, R *, R *, R *, R *, R *, R *, const I *,
+                    const R *, const R *, const R *, R *);
 void KER(euler)(R *, R *, const R *, R *, R *, R *, R *, const R *, const R *,
                 const I *, const I *, const I *, const I *, const I *, const I *,
                 const R *, const R *, const I *, const I *, const I *, const R *,
@@ -365,7 +378,15 @@ static I sweep_slice(const sweepctx *c, I dim, I s, slicebuf *b) {
                 &gl->PositiveReconstruction, &dt, &gl->Gamma, &gl->PressureFree, b->dls, b->drs,
                 b->pls, b->prs, b->gels, b->gers, b->uls, b->urs, b->vls, b->vrs, b->wls,
                 b->wrs, &NumberOfColours, b->col, b->colls, b->colrs);
-  if (gl->RiemannSolver == ENZO_PPM_RIEMANN_HLL) {
+  if (gl->RiemannSolver == ENZO_PPM_RIEMANN_HLLC) {
+    /* Grid_xEulerSweep.C:258-269: HLLC fluxes over the whole slice */
+    KER(flux_hllc)(b->d, b->e, b->ge, b->u, b->v, b->w, dxs, b->diffcoef, &idim, &jdim, &is, &ie,
+                   &js, &je, &dt, &gl->Gamma, &gl->PPMDiffusionParameter,
+                   &gl->DualEnergyFormalism, &gl->DualEnergyFormalismEta1,
+                   &gl->RiemannSolverFallback, b->dls, b->drs, b->pls, b->prs, b->uls, b->urs,
+                   b->vls, b->vrs, b->wls, b->wrs, b->gels, b->gers, b->df, b->uf, b->vf, b->wf,
+                   b->ef, b->gef, b->ges, &NumberOfColours, b->col, b->colls, b->colrs, b->colf);
+  } else if (gl->RiemannSolver == ENZO_PPM_RIEMANN_HLL) {
     /* Grid_xEulerSweep.C:245-256: HLL fluxes over the whole slice */
     KER(flux_hll)(b->d, b->e, b->ge, b->u, b->v, b->w, dxs, b->diffcoef, &idim, &jdim, &is, &ie,
                   &js, &je, &dt, &gl->Gamma, &gl->PPMDiffusionParameter,
@@ -514,7 +535,8 @@ int PFX(solve_ppm_de)(const enzo_ppm_grid *g, const enzo_ppm_globals *gl, I Cycl
                       const I *GridGlobalStart, I GravityOn, I NumberOfColours,
                       const I *colnum, R MinimumSupportEnergyCoefficient, I nthreads,
                       I *fallback_count) {
-  if ((gl->RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK && gl->RiemannSolver != ENZO_PPM_RIEMANN_HLL) ||
+  if ((gl->RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK && gl->RiemannSolver != ENZO_PPM_RIEMANN_HLL &&
+       gl->RiemannSolver != ENZO_PPM_RIEMANN_HLLC) ||
       gl->ReconstructionMethod != ENZO_PPM_RECONSTRUCTION_PPM)
     return ENZO_PPM_ERR_NOT_SUPPORTED;
   if (NumberOfColours > ENZO_PPM_MAX_COLOR) return ENZO_PPM_ERR_ARGUMENT;

@@ -842,6 +842,175 @@ void orc_flux_hll(const R *dslice, const R *eslice, const R *geslice,
 }
 
 /* ------------------------------------------------------------------ */
+/* flux_hllc.F:6-421 -- HLLC fluxes over the slice (RiemannSolver HLLC):
+ * Roe averages and Davis wave speeds as in flux_hll, the contact speed cw
+ * and pressure cp, HLLC weights sl / sr / sm chosen by the sign of cw, the
+ * left/right fluxes with dfl = dubl - bm*dls, the contact terms sm*cp (x
+ * momentum) and sm*cp*cw (energy), the gas-energy source, dt/dx, then the
+ * density check with the per-cell flux_hll fallback in cell order. */
+void orc_flux_hllc(const R *dslice, const R *eslice, const R *geslice,
+                   const R *uslice, const R *vslice, const R *wslice, const R *dx,
+                   const R *diffcoef, const I *pidim, const I *pjdim, const I *pi1,
+                   const I *pi2, const I *pj1, const I *pj2, const R *pdt,
+                   const R *pgamma, const I *pidiff, const I *pidual, const R *eta1,
+                   const I *pifallback, const R *dls, const R *drs, const R *pls,
+                   const R *prs, const R *uls, const R *urs, const R *vls,
+                   const R *vrs, const R *wls, const R *wrs, const R *gels,
+                   const R *gers, R *df, R *uf, R *vf, R *wf, R *ef, R *gef,
+                   R *ges, const I *pncolor, const R *colslice, const R *colls,
+                   const R *colrs, R *colf) {
+  (void)eta1;
+  const I idim = *pidim, jdim = *pjdim, i1 = *pi1, i2 = *pi2, j1 = *pj1, j2 = *pj2;
+  const I idiff = *pidiff, idual = *pidual, ncolor = *pncolor;
+  const R dt = *pdt, gamma = *pgamma;
+  const R gamma1 = gamma - 1.0;
+  const R gamma1i = 1.0 / gamma1;
+  I i, j, n;
+  (void)jdim;
+  R el[IJKN + 4], er[IJKN + 4], char1[IJKN + 4], char2[IJKN + 4];
+  R csl[IJKN + 4], csr[IJKN + 4], bm[IJKN + 4], bp[IJKN + 4];
+  R cw[IJKN + 4], cp[IJKN + 4], sl[IJKN + 4], sr[IJKN + 4], sm[IJKN + 4];
+  R gesl[IJKN + 4], gesr[IJKN + 4];
+  for (j = j1; j <= j2; j++) {
+    /* Roe averages, characteristics (:104-136) */
+    for (i = i1; i <= i2 + 1; i++) {
+      R sqrtdl = sqrt(S2(dls, i, j));
+      R sqrtdr = sqrt(S2(drs, i, j));
+      R isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+      R vroe1 = (sqrtdl * S2(uls, i, j) + sqrtdr * S2(urs, i, j)) * isdlpdr;
+      R vroe2 = (sqrtdl * S2(vls, i, j) + sqrtdr * S2(vrs, i, j)) * isdlpdr;
+      R vroe3 = (sqrtdl * S2(wls, i, j) + sqrtdr * S2(wrs, i, j)) * isdlpdr;
+      R v2 = vroe1 * vroe1 + vroe2 * vroe2 + vroe3 * vroe3;
+      el[i] = gamma1i * S2(pls, i, j) + 0.5 * S2(dls, i, j) *
+              (S2(uls, i, j) * S2(uls, i, j) + S2(vls, i, j) * S2(vls, i, j) +
+               S2(wls, i, j) * S2(wls, i, j));
+      er[i] = gamma1i * S2(prs, i, j) + 0.5 * S2(drs, i, j) *
+              (S2(urs, i, j) * S2(urs, i, j) + S2(vrs, i, j) * S2(vrs, i, j) +
+               S2(wrs, i, j) * S2(wrs, i, j));
+      R hroe = ((el[i] + S2(pls, i, j)) / sqrtdl + (er[i] + S2(prs, i, j)) / sqrtdr) * isdlpdr;
+      R cs = sqrt(gamma1 * fmx((hroe - 0.5 * v2), TINY));
+      char1[i] = vroe1 - cs;
+      char2[i] = vroe1 + cs;
+    }
+    /* min/max wave speeds (:140-153) */
+    for (i = i1; i <= i2 + 1; i++) {
+      R csl0 = sqrt(gamma * S2(pls, i, j) / S2(dls, i, j));
+      R csr0 = sqrt(gamma * S2(prs, i, j) / S2(drs, i, j));
+      csl[i] = fmn(S2(uls, i, j) - csl0, char1[i]);
+      csr[i] = fmx(S2(urs, i, j) + csr0, char2[i]);
+      bm[i] = fmn(csl[i], 0.0);
+      bp[i] = fmx(csr[i], 0.0);
+    }
+    /* contact speed and pressure, HLLC weights (:158-180) */
+    for (i = i1; i <= i2 + 1; i++) {
+      R tl = S2(pls, i, j) - (csl[i] - S2(uls, i, j)) * S2(dls, i, j) * S2(uls, i, j);
+      R tr = S2(prs, i, j) - (csr[i] - S2(urs, i, j)) * S2(drs, i, j) * S2(urs, i, j);
+      R dl = S2(dls, i, j) * (csl[i] - S2(uls, i, j));
+      R dr = -S2(drs, i, j) * (csr[i] - S2(urs, i, j));
+      R q1 = 1.0 / (dl + dr);
+      cw[i] = (tr - tl) * q1;
+      cp[i] = (dl * tr + dr * tl) * q1;
+      if (cw[i] >= 0.0) {
+        sl[i] = cw[i] / (cw[i] - bm[i]);
+        sr[i] = 0.0;
+        sm[i] = -bm[i] / (cw[i] - bm[i]);
+      } else {
+        sl[i] = 0.0;
+        sr[i] = -cw[i] / (bp[i] - cw[i]);
+        sm[i] = bp[i] / (bp[i] - cw[i]);
+      }
+      cp[i] = fmx(cp[i], 0.0);
+    }
+    for (i = i1; i <= i2 + 1; i++) {
+      R diffd = 0, diffuu = 0, diffuv = 0, diffuw = 0, diffue = 0, diffuge = 0;
+      if (idiff != 0) { /* :185-215 */
+        R dc = S2(diffcoef, i, j);
+        diffd = dc * (S2(dslice, i - 1, j) - S2(dslice, i, j));
+        diffuu = dc * (S2(dslice, i - 1, j) * S2(uslice, i - 1, j) - S2(dslice, i, j) * S2(uslice, i, j));
+        diffuv = dc * (S2(dslice, i - 1, j) * S2(vslice, i - 1, j) - S2(dslice, i, j) * S2(vslice, i, j));
+        diffuw = dc * (S2(dslice, i - 1, j) * S2(wslice, i - 1, j) - S2(dslice, i, j) * S2(wslice, i, j));
+        diffue = dc * (S2(dslice, i - 1, j) * S2(eslice, i - 1, j) - S2(dslice, i, j) * S2(eslice, i, j));
+        if (idual == 1)
+          diffuge = dc * (S2(dslice, i - 1, j) * S2(geslice, i - 1, j) - S2(dslice, i, j) * S2(geslice, i, j));
+      }
+      /* left/right fluxes (:222-244) */
+      R dubl = S2(dls, i, j) * S2(uls, i, j);
+      R dubr = S2(drs, i, j) * S2(urs, i, j);
+      R dfl = dubl - bm[i] * S2(dls, i, j);
+      R dfr = dubr - bp[i] * S2(drs, i, j);
+      R ufl = dubl * (S2(uls, i, j) - bm[i]) + S2(pls, i, j);
+      R ufr = dubr * (S2(urs, i, j) - bp[i]) + S2(prs, i, j);
+      R vfl = S2(dls, i, j) * S2(vls, i, j) * (S2(uls, i, j) - bm[i]);
+      R vfr = S2(drs, i, j) * S2(vrs, i, j) * (S2(urs, i, j) - bp[i]);
+      R wfl = S2(dls, i, j) * S2(wls, i, j) * (S2(uls, i, j) - bm[i]);
+      R wfr = S2(drs, i, j) * S2(wrs, i, j) * (S2(urs, i, j) - bp[i]);
+      R efl = el[i] * (S2(uls, i, j) - bm[i]) + S2(pls, i, j) * S2(uls, i, j);
+      R efr = er[i] * (S2(urs, i, j) - bp[i]) + S2(prs, i, j) * S2(urs, i, j);
+      /* HLLC flux plus diffusion (:272-290), contact terms (:299-302) */
+      S2(df, i, j) = sl[i] * dfl + sr[i] * dfr + diffd;
+      S2(uf, i, j) = sl[i] * ufl + sr[i] * ufr + diffuu;
+      S2(vf, i, j) = sl[i] * vfl + sr[i] * vfr + diffuv;
+      S2(wf, i, j) = sl[i] * wfl + sr[i] * wfr + diffuw;
+      S2(ef, i, j) = sl[i] * efl + sr[i] * efr + diffue;
+      if (idual == 1) {
+        R gefl = (S2(uls, i, j) - bm[i]) * S2(gels, i, j) * S2(dls, i, j);
+        R gefr = (S2(urs, i, j) - bp[i]) * S2(gers, i, j) * S2(drs, i, j);
+        gesl[i] = S2(uls, i, j) - bm[i];
+        gesr[i] = S2(urs, i, j) - bp[i];
+        S2(gef, i, j) = sl[i] * gefl + sr[i] * gefr + diffuge;
+      }
+      for (n = 1; n <= ncolor; n++) {
+        R colfl = (S2(uls, i, j) - bm[i]) * S3(colls, i, j, n);
+        R colfr = (S2(urs, i, j) - bp[i]) * S3(colrs, i, j, n);
+        S3(colf, i, j, n) = sl[i] * colfl + sr[i] * colfr;
+      }
+      S2(uf, i, j) = S2(uf, i, j) + sm[i] * cp[i];
+      S2(ef, i, j) = S2(ef, i, j) + sm[i] * cp[i] * cw[i];
+    }
+    /* gas-energy source (:306-316); the i = i2+1 value reads the locals
+       at i2+2, which the reference never sets and euler never uses */
+    if (idual == 1) {
+      sl[i2 + 2] = sr[i2 + 2] = gesl[i2 + 2] = gesr[i2 + 2] = 0.0;
+      for (i = i1; i <= i2 + 1; i++) {
+        R pcent = fmx((gamma - 1.0) * S2(geslice, i, j) * S2(dslice, i, j), TINY);
+        R qc = dt / F1(dx, i);
+        S2(ges, i, j) = qc * pcent *
+                        (sl[i] * gesl[i] + sr[i] * gesr[i] - sl[i + 1] * gesl[i + 1] -
+                         sr[i + 1] * gesr[i + 1]);
+      }
+    }
+    /* absorb dt/dx (:321-340) */
+    for (i = i1; i <= i2 + 1; i++) {
+      R qc = dt / F1(dx, i);
+      S2(df, i, j) = qc * S2(df, i, j);
+      S2(uf, i, j) = qc * S2(uf, i, j);
+      S2(vf, i, j) = qc * S2(vf, i, j);
+      S2(wf, i, j) = qc * S2(wf, i, j);
+      S2(ef, i, j) = qc * S2(ef, i, j);
+    }
+    if (idual == 1)
+      for (i = i1; i <= i2 + 1; i++) S2(gef, i, j) = dt / F1(dx, i) * S2(gef, i, j);
+    for (n = 1; n <= ncolor; n++)
+      for (i = i1; i <= i2 + 1; i++) S3(colf, i, j, n) = dt * S3(colf, i, j, n);
+    /* density check and per-cell flux_hll fallback, in cell order (:344-391) */
+    for (i = i1; i <= i2; i++) {
+      if (S2(dslice, i, j) + S2(df, i, j) - S2(df, i + 1, j) <= 0.0) {
+        if (*pifallback == 0) {
+          orc_error_code = 9;
+        } else {
+          orc_fallback_count++;
+          orc_flux_hll_impl(dslice, eslice, geslice, uslice, vslice, wslice, dx,
+                            diffcoef, idim, jdim, i, i, j, j, dt, gamma, idiff,
+                            idual, dls, drs, pls, prs, uls, urs, vls, vrs, wls,
+                            wrs, gels, gers, df, uf, vf, wf, ef, gef, ges, ncolor,
+                            colslice, colls, colrs, colf, 1);
+        }
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
 /* flux_twoshock.F:6-490 -- Godunov fluxes from the two-shock solution. */
 void orc_flux_twoshock(const R *dslice, const R *eslice, const R *geslice,
                        const R *uslice, const R *vslice, const R *wslice,

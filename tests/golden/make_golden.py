@@ -38,6 +38,12 @@ CASES = {
     "hll_diffusion_8": (lambda: with_glob(problems.random_smooth([8, 9, 7], seed=28, ncolours=1,
                                                                 flags=(1, 1, 1)), RiemannSolver=1),
                         0.4, 2, None),
+    # RiemannSolver HLLC (4): flux_hllc over whole slices
+    "hllc_turbulent_10": (lambda: with_glob(problems.turbulent(10, seed=29), RiemannSolver=4),
+                          0.4, 2, None),
+    "hllc_gravity_8": (lambda: with_glob(problems.random_smooth([8, 7, 9], seed=30, ncolours=1,
+                                                               gravity=True), RiemannSolver=4),
+                       0.4, 2, None),
 }
 
 
@@ -105,6 +111,28 @@ def make_twoshock():
                         uls=uls, urs=urs, pbar=pbar, ubar=ubar)
 
 
+def make_hllc_fallback():
+    """One SolvePPM_DE call with RiemannSolver HLLC in which four cells take
+    flux_hllc's per-cell flux_hll fallback (flux_hllc.F:344-391): a two-stream
+    3-D grid streaming along z, three Courant steps long."""
+    g, glob, _ = problems.two_stream(32, axis=2, rank=3, seed=3, ncolours=1)
+    glob = dict(glob, RiemannSolver=4)
+    g.dt = oracle.compute_time_step("ref", g, glob, 0.4) * 3.0
+    out = {"glob_keys": np.array(sorted(glob)), "glob_vals": np.array([float(glob[k]) for k in sorted(glob)]),
+           "active": np.array(g.active), "rank": g.rank, "ncolours": g.ncolours, "dx": np.array(g.dx),
+           "dt": g.dt, "cycle": 2}
+    for d in range(3):
+        out[f"cellwidth{d}"] = g.cellwidth[d].copy()
+    for k, f in enumerate(g.fields):
+        out[f"in_f{k}"] = f.copy()
+    st, fb = oracle.solve_ppm_de("ref", g, glob, cycle=2)
+    assert st == 1 and fb > 0, (st, fb)
+    out["status"], out["fallbacks"] = st, fb
+    for k, f in enumerate(g.fields):
+        out[f"out_f{k}"] = f.copy()
+    np.savez_compressed(os.path.join(HERE, "fallback_hllc_3d_z_ok.npz"), **out)
+
+
 def make_fallback(name):
     """One SolvePPM_DE call of a per-cell fallback case (tests/helpers.py)
     with the reference's status, fallback count and fields.  The gas energy
@@ -135,6 +163,9 @@ if __name__ == "__main__":
             continue
         make(n)
         print("wrote", n)
+    if not only or "hllc_fallback" in only:
+        make_hllc_fallback()
+        print("wrote hllc fallback")
     if only:
         sys.exit(0)
     make_twoshock()

@@ -397,12 +397,14 @@ HLL_CASES = {
 }
 
 
+@pytest.mark.parametrize("solver", [1, 4], ids=["hll", "hllc"])
 @pytest.mark.parametrize("name", sorted(HLL_CASES))
-def test_hll_solver_steps(cpulib, name):
-    """RiemannSolver HLL (flux_hll over whole slices): the device path equals
-    the oracle bitwise over steps of every sweep order, with no fallback."""
+def test_hll_solver_steps(cpulib, name, solver):
+    """RiemannSolver HLL (1: flux_hll over whole slices) and HLLC (4:
+    flux_hllc): the device path equals the oracle bitwise over steps of
+    every sweep order, with no fallback."""
     g, glob, _ = HLL_CASES[name]()
-    glob = dict(glob, RiemannSolver=1)
+    glob = dict(glob, RiemannSolver=solver)
     for cycle in (0, 1, 2):
         fill_bc(g, "periodic")
         g.dt = oracle.compute_time_step("port", g, glob, 0.4)
@@ -411,3 +413,19 @@ def test_hll_solver_steps(cpulib, name):
         assert run(cpulib, b, glob, cycle) == (1, 0)
         assert all(r[1] for r in field_report(b, a))
         g = a
+
+
+
+@pytest.mark.parametrize("case", [("3d_z_ok", 2, 3, 3.0, 1, 4), ("3d_x_fails", 0, 0, 3.210948911011219, -3, 6)])
+def test_hllc_fallback(cpulib, case):
+    """flux_hllc's per-cell flux_hll fallback (flux_hllc.F:344-391) on a
+    two-stream grid: status, fallback count and every field as the oracle."""
+    name, axis, seed, factor, status, nfb = case
+    g, glob, _ = problems.two_stream(32, axis=axis, rank=3, seed=seed, ncolours=1)
+    glob = dict(glob, RiemannSolver=4)
+    g.dt = oracle.compute_time_step("port", g, glob, 0.4) * factor
+    a, b = g.copy(), g.copy()
+    assert oracle.solve_ppm_de("port", a, glob, cycle=axis) == (status, nfb)
+    assert run(cpulib, b, glob, axis) == (status, nfb)
+    if status == 1:
+        assert all(r[1] for r in field_report(b, a))

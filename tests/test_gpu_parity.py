@@ -168,13 +168,15 @@ HLL_CASES = ["turbulent_40", "diffusion", "gravity", "no_dual_energy", "sod_2d",
 CASES["longest_hll"] = lambda: problems.random_smooth([2, 1019, 3], seed=12, ncolours=1)
 
 
+@pytest.mark.parametrize("solver", [1, 4], ids=["hll", "hllc"])
 @pytest.mark.parametrize("name", HLL_CASES)
 @pytest.mark.parametrize("cycle", [0, 1, 2])
-def test_hll_solver_matches_oracle(lib, name, cycle):
-    """RiemannSolver HLL (typedefs.h: 1; Grid_xEulerSweep.C:245-256): every
-    interface takes flux_hll's fluxes; bitwise against the oracle."""
+def test_hll_solver_matches_oracle(lib, name, cycle, solver):
+    """RiemannSolver HLL (typedefs.h: 1; Grid_xEulerSweep.C:245-256) and
+    HLLC (4; :258-269): every interface takes flux_hll's / flux_hllc's
+    fluxes; bitwise against the oracle."""
     g, glob, run = CASES[name]()
-    glob = dict(glob, RiemannSolver=1)
+    glob = dict(glob, RiemannSolver=solver)
     a, b, _, _, fbk = _one_step(lib, g, glob, run["cfl"], cycle)
     assert fbk == 0
     rep = field_report(a, b)
@@ -191,3 +193,57 @@ def test_hll_solver_refuses_longer_pencils(lib):
     a = g.copy()
     assert device.solve_ppm_de(a, glob, cycle=0, lib=lib) == abi.ERR_NOT_SUPPORTED
     assert b"HLL" in lib.hipPPMLastError()
+
+
+
+@pytest.mark.parametrize("case", [("3d_z_ok", 2, 3, 3.0, 1, 4), ("3d_x_fails", 0, 0, 3.210948911011219, -3, 6)])
+def test_hllc_fallback_matches_oracle(lib, case):
+    """flux_hllc's per-cell flux_hll fallback (flux_hllc.F:344-391): status,
+    fallback count and, when the step succeeds, every field as the oracle."""
+    name, axis, seed, factor, status, nfb = case
+    g, glob, _ = problems.two_stream(32, axis=axis, rank=3, seed=seed, ncolours=1)
+    glob = dict(glob, RiemannSolver=4)
+    g.dt = oracle.compute_time_step("port", g, glob, 0.4) * factor
+    a, b = g.copy(), g.copy()
+    assert oracle.solve_ppm_de("port", a, glob, cycle=axis) == (status, nfb)
+    h = C.c_void_p(None)
+    try:
+        st = device.solve_ppm_de(b, glob, cycle=axis, handle=h, lib=lib)
+        assert (st, lib.hipPPMFallbackCount(h)) == (status, nfb)
+    finally:
+        if h.value:
+            lib.hipPPMDestroy(h)
+    if status == 1:
+        assert all(r[1] for r in field_report(b, a))
+
+
+GOLD_FALLBACKS = sorted(__import__("glob").glob(__import__("os").path.join(
+    __import__("os").path.dirname(__import__("os").path.abspath(__file__)), "golden", "fallback_*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLD_FALLBACKS, ids=[p.rsplit("/", 1)[-1] for p in GOLD_FALLBACKS])
+def test_device_matches_golden_fallback(lib, path):
+    """Per-cell fallback steps written by the reference (TwoShock and HLLC):
+    the device's status, fallback count and fields, except the gas energy of
+    the cells after fallback cells (read from unset locals there)."""
+    from enzo_ppm.grid import Grid
+    from helpers import fallback_differences
+    z = np.load(path, allow_pickle=False)
+    glob = {str(k): (int(v) if str(k) != "Gamma" else float(v))
+            for k, v in zip(z["glob_keys"], z["glob_vals"])}
+    g = Grid(list(z["active"]), rank=int(z["rank"]), ncolours=int(z["ncolours"]), dual_energy=True,
+             dx=list(z["dx"]))
+    for d in range(3):
+        g.cellwidth[d][...] = z[f"cellwidth{d}"]
+    for k in range(g.nfields):
+        g.fields[k][...] = z[f"in_f{k}"]
+    g.dt = float(z["dt"])
+    h = C.c_void_p(None)
+    try:
+        st = device.solve_ppm_de(g, glob, cycle=int(z["cycle"]), handle=h, lib=lib)
+        fb = lib.hipPPMFallbackCount(h)
+    finally:
+        if h.value:
+            lib.hipPPMDestroy(h)
+    assert (st, fb) == (int(z["status"]), int(z["fallbacks"]))
+    fallback_differences(g, [z[f"out_f{k}"] for k in range(g.nfields)], fb)
