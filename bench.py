@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "enzo-dev_amd"))
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector/matrix peak
 
 
 def parse():
@@ -427,6 +428,18 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_sweep},
         "cpu_baseline": None,
     }
+    if n == 256:
+        # FP64 work of one 256^3 sweep of this workload, counted by rocprofv3
+        # (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x 64 lanes, an FMA two flops;
+        # profiles/ppm_sweep_kernel_turbulent256.md) over the live sweep time:
+        # the arithmetic side of the kernel's bound
+        flops = (0.187e9 + 0.250e9 + 2 * 0.231e9 + 0.038e9) * 64
+        out["fp64_valu"] = {"flop_per_launch": flops, "achieved_tflops": flops / sweep_avg_s / 1e12,
+                            "peak_tflops": PEAK_FP64_TFLOPS,
+                            "frac": flops / sweep_avg_s / 1e12 / PEAK_FP64_TFLOPS,
+                            "valu_instructions_per_launch": 1.407e9,
+                            "note": "instruction counts from the committed rocprofv3 pass; "
+                                    "time measured here"}
     try:
         out["host_buffers"] = host_buffer_rate(lib, n)
     except Exception as e:  # reported, not fatal
