@@ -16,6 +16,22 @@ TESTS = {
               ("density", "velocity_x", "pressure"), ((0.0, 0.25), (0.65, 0.7)), 5e-2, 1e-7),
     "toro2": (0.5, (1.0, -2.0, 0.4), (1.0, 2.0, 0.4), 0.15, "Toro-2-ShockTube_t0.15_exact.txt",
               ("density", "pressure"), ((0.2, 0.3), (0.7, 0.8)), 1e-1, 1e-7),
+    "toro3": (0.5, (1.0, 0.0, 1000.0), (1.0, 0.0, 0.01), 0.012, "Toro-3-ShockTube_t0.012_exact.txt",
+              ("density", "velocity_x", "pressure", "specific_thermal_energy"),
+              ((0.1, 0.25), (0.6, 0.7)), 1e-1, 1e-7),
+    "toro4": (0.4, (5.99924, 19.5975, 460.894), (5.99242, -6.19633, 46.0950), 0.035,
+              "Toro-4-ShockTube_t0.035_exact.txt",
+              ("density", "velocity_x", "pressure", "specific_thermal_energy"),
+              ((0.5, 0.6), (0.78, 0.8)), 1e-1, 1e-7),
+    "toro5": (0.8, (1.0, -19.59745, 1000.0), (1.0, -19.59745, 0.01), 0.012,
+              "Toro-5-ShockTube_t0.012_exact.txt",
+              ("density", "pressure", "specific_thermal_energy"), ((0.25, 0.4),), 1e-2, 1e-7),
+    "toro6": (0.5, (1.4, 0.0, 1.0), (1.0, 0.0, 1.0), 2.0, "Toro-6-ShockTube_t2.0_exact.txt",
+              ("density", "velocity_x", "pressure", "specific_thermal_energy"), ((0.0, 1.0),),
+              1e-6, 1e-7),
+    "toro7": (0.5, (1.4, 0.1, 1.0), (1.0, 0.1, 1.0), 2.0, "Toro-7-ShockTube_t2.0_exact.txt",
+              ("density", "velocity_x", "pressure", "specific_thermal_energy"),
+              ((0.0, 0.6), (0.8, 1.0)), 1e-3, 1e-7),
 }
 
 GLOBALS = dict(Gamma=1.4, DualEnergyFormalism=0)  # Enzo defaults otherwise

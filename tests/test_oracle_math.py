@@ -2,7 +2,7 @@
 
 Each check is chosen so that a plausible slip in the restatement (a dropped
 term, a sign, an index, a swapped operand) fails it:
-  - the reference's own answer tests: Sod and Toro 1-2 shock tubes against the
+  - the reference's own answer tests: Sod and Toro 1-7 shock tubes against the
     exact Riemann solutions shipped with them (run/Hydro/Hydro-1D), with their
     windows and tolerances;
   - twoshock: the converged p* is the root of the two-shock pressure function
