@@ -77,8 +77,10 @@ def test_steps(cpulib, name):
         g = a
 
 
-def test_face_fluxes(cpulib):
+@pytest.mark.parametrize("solver", [5, 1, 4], ids=["twoshock", "hll", "hllc"])
+def test_face_fluxes(cpulib, solver):
     g, glob, _ = problems.random_smooth([20, 18, 16], seed=5, ncolours=2)
+    glob = dict(glob, RiemannSolver=solver)
     boxes = [((7, 5, 4), (12, 9, 8)), ((3, 3, 3), (22, 20, 18)), ((10, 3, 3), (10, 20, 18))]
     g.fill_periodic()
     g.dt = oracle.compute_time_step("port", g, glob, 0.4)
@@ -87,6 +89,7 @@ def test_face_fluxes(cpulib):
     assert oracle.solve_ppm_de("port", a, glob, cycle=1, fluxes=fa)[0] == 1
     assert run(cpulib, b, glob, 1, fluxes=fb)[0] == 1
     assert bit_equal(fa.flat(), fb.flat())
+    assert all(r[1] for r in field_report(b, a))
 
 
 def test_device_resident_steps(cpulib):

@@ -89,8 +89,10 @@ def test_config1_sod_64(lib):
     assert abs(a.d[sl].sum() - m0) <= 1e-12 * m0
 
 
-def test_subgrid_face_fluxes(lib):
+@pytest.mark.parametrize("solver", [5, 1, 4], ids=["twoshock", "hll", "hllc"])
+def test_subgrid_face_fluxes(lib, solver):
     g, glob, run = problems.random_smooth([20, 18, 16], seed=5, ncolours=2)
+    glob = dict(glob, RiemannSolver=solver)
     boxes = [((7, 5, 4), (12, 9, 8)), ((3, 3, 3), (22, 20, 18)), ((10, 3, 3), (10, 20, 18))]
     for cycle in range(3):
         a, b, fa, fb, _ = _one_step(lib, g, glob, run["cfl"], cycle, boxes=boxes)
