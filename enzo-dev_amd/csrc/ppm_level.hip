@@ -170,6 +170,14 @@ __global__ void level_periodic_kernel(const BCParams *__restrict__ B) {
     periodic_item(b, gid);
 }
 
+// RiemannSolver HLL / HLLC: the launch's group list onto the fallback list
+__global__ void level_fb_fill_kernel(const unsigned *__restrict__ src, unsigned n, unsigned *list,
+                                     unsigned *count) {
+  for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    list[k] = src[k];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *count = n;
+}
+
 // one domain face of every grid in the list (blockIdx.y)
 __global__ void level_ext_bc_kernel(const ExtBCParams *__restrict__ B) {
   const ExtBCParams &b = B[blockIdx.y];
@@ -259,6 +267,11 @@ struct hipPPMLevel {
   FlatParams *dflat = nullptr;      // [parity 2][axis 3][grid]
   PresParams *dpres = nullptr;      // [parity 2][grid]
   Work *dwork = nullptr;            // all work lists
+  // RiemannSolver HLL / HLLC: every pencil group goes to the one-pencil
+  // workgroups; per axis and slice, the level-wide group list
+  int whole = 0;
+  unsigned *dfball = nullptr;
+  std::vector<long long> fball_first[3];
   Work *dpwork = nullptr;           // pressure chunks
   long long npwork = 0;
   // work lists: per axis, per launch: (shape, uni, first item, count, slice)
@@ -307,7 +320,7 @@ void free_level(hipPPMLevel *L) {
                   L->dtmin, L->counters, L->fb_flag, L->fb_list, L->fb_state, L->grp_grid,
                   L->dgrp_off, L->dparams,
                   L->dflat, L->dpres, L->dwork, L->dpwork, L->learn, L->cws_dt, L->old,
-                  L->dexp, L->dboxes, L->dgeom};
+                  L->dexp, L->dboxes, L->dgeom, L->dfball};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (L->own_stream && L->stream) (void)hipStreamDestroy(L->stream);
@@ -429,6 +442,7 @@ int build_plan(hipPPMLevel *L) {
   L->hparams.assign((size_t)2 * 3 * G, SweepParams());
   std::vector<FlatParams> hflat((size_t)2 * 3 * G);
   std::vector<Work> work;
+  std::vector<unsigned> fball;
   long long ngroups = 0;
   L->grp_off.assign(G, 0);
   for (int g = 0; g < G; g++) {  // flags per grid: the largest group count of any axis
@@ -519,6 +533,8 @@ int build_plan(hipPPMLevel *L) {
         S.fb_done = L->fb_state + 1;
         S.fb_tag = (unsigned)L->grp_off[g];
         S.fb_ok = pencil_fits<kFbTS>(S) ? 1 : 0;
+        S.hll_all = P0.RiemannSolver == ENZO_PPM_RIEMANN_HLL ? 1 : 0;
+        S.hllc = P0.RiemannSolver == ENZO_PPM_RIEMANN_HLLC ? 1 : 0;
         FlatParams &F = hflat[((size_t)par * 3 + d) * G + g];
         memset(&F, 0, sizeof(F));
         F.dim = d;
@@ -545,7 +561,21 @@ int build_plan(hipPPMLevel *L) {
     }
     // work lists: per slice (diffusion) or one, then per (shape, uni)
     const int nslices = L->diffusion ? max_slices : 1;
+    L->fball_first[d].clear();
     for (int k = 0; k < nslices; k++) {
+      L->fball_first[d].push_back((long long)fball.size());
+      if (L->whole)
+        for (int g = 0; g < G; g++) {
+          const SweepParams &S = L->hparams[((size_t)0 * 3 + d) * G + g];
+          if (L->diffusion) {
+            if (k >= S.n_t2) continue;
+            for (int r = 0; r < S.nblk_t1; r++)
+              fball.push_back((unsigned)(L->grp_off[g] + r + (long long)S.nblk_t1 * k));
+          } else {
+            for (long long r = 0; r < (long long)S.nblk_t1 * S.n_t2; r++)
+              fball.push_back((unsigned)(L->grp_off[g] + r));
+          }
+        }
       L->launch_slices[d].push_back((long long)L->launches[d].size());
       for (int s = 0; s < kShapes; s++)
         for (int u = 1; u >= 0; u--) {
@@ -568,6 +598,7 @@ int build_plan(hipPPMLevel *L) {
         }
     }
     L->launch_slices[d].push_back((long long)L->launches[d].size());
+    L->fball_first[d].push_back((long long)fball.size());
   }
   // host check of every item against its grid's tiles
   for (const Work &w : work) {
@@ -595,6 +626,18 @@ int build_plan(hipPPMLevel *L) {
   LCHK(hipMemcpy(L->dflat, hflat.data(), sizeof(FlatParams) * hflat.size(), hipMemcpyHostToDevice));
   if (!work.empty())
     LCHK(hipMemcpy(L->dwork, work.data(), sizeof(Work) * work.size(), hipMemcpyHostToDevice));
+  // whole-pencil group lists: every entry a group of the level, each list no
+  // longer than the fallback list
+  for (int d = 0; d < L->rank; d++)
+    for (size_t k = 0; k + 1 < L->fball_first[d].size(); k++)
+      if (L->fball_first[d][k + 1] - L->fball_first[d][k] > L->ngroups) return ENZO_PPM_ERR_ARGUMENT;
+  for (unsigned e : fball)
+    if ((long long)e >= L->ngroups) return ENZO_PPM_ERR_ARGUMENT;
+  if (L->dfball) (void)hipFree(L->dfball);
+  L->dfball = nullptr;
+  if (dev_alloc(&L->dfball, fball.size())) return ENZO_PPM_ERR_HIP;
+  if (!fball.empty())
+    LCHK(hipMemcpy(L->dfball, fball.data(), sizeof(unsigned) * fball.size(), hipMemcpyHostToDevice));
   drop_graphs(L);
   return ENZO_PPM_SUCCESS;
 }
@@ -608,6 +651,22 @@ int launch_level_sweep(hipPPMLevel *L, int d) {
   LCHK(hipGetLastError());
   const auto &sl = L->launch_slices[d];
   for (size_t k = 0; k + 1 < sl.size(); k++) {
+    if (L->whole) {
+      // HLL / HLLC: every group of the slice to the one-pencil workgroups
+      const long long f0 = L->fball_first[d][k], n = L->fball_first[d][k + 1] - f0;
+      if (n == 0) continue;
+      hipLaunchKernelGGL(level_fb_fill_kernel, dim3((unsigned)std::min<long long>((n + 255) / 256, 1024)),
+                         dim3(256), 0, L->stream, L->dfball + f0, (unsigned)n, L->fb_list, L->fb_state);
+      LCHK(hipGetLastError());
+      if (L->learned)
+        hipLaunchKernelGGL(level_fallback_kernel<1>, dim3(kFbBlocks), dim3(kFbTS), 0, L->stream, P,
+                           L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
+      else
+        hipLaunchKernelGGL(level_fallback_kernel<0>, dim3(kFbBlocks), dim3(kFbTS), 0, L->stream, P,
+                           L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
+      LCHK(hipGetLastError());
+      continue;
+    }
     for (long long li = sl[k]; li < sl[k + 1]; li++) {
       const auto &ln = L->launches[d][li];
       const Work *w = L->dwork + ln.first;
@@ -690,10 +749,15 @@ int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMP
         return ENZO_PPM_ERR_ARGUMENT;
     }
     if (P.ConservativeReconstruction != 0 || P.PositiveReconstruction != 0 ||
-        P.RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK ||
+        (P.RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK && P.RiemannSolver != ENZO_PPM_RIEMANN_HLL &&
+         P.RiemannSolver != ENZO_PPM_RIEMANN_HLLC) ||
         (P.PPMFlatteningParameter != 0 && P.PPMFlatteningParameter != 1) ||
         (P.PPMDiffusionParameter != 0 && P.PPMDiffusionParameter != 1))
       return ENZO_PPM_ERR_NOT_SUPPORTED;
+    // HLL / HLLC run in one-pencil workgroups: every swept pencil must fit
+    if (P.RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK)
+      for (int d = 0; d < P.GridRank; d++)
+        if (P.GridDimension[d] > kFbTS + 1) return ENZO_PPM_ERR_NOT_SUPPORTED;
   }
   hipPPMLevel *L = new (std::nothrow) hipPPMLevel();
   if (!L) return ENZO_PPM_ERR_ARGUMENT;
@@ -706,6 +770,7 @@ int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMP
   L->nslot = S_COL0 + L->ncol;
   L->gravity = Para[0].GravityOn != 0;
   L->diffusion = Para[0].PPMDiffusionParameter != 0;
+  L->whole = Para[0].RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK ? 1 : 0;
   L->GD.resize(3 * G);
   L->cells.resize(G);
   L->off.resize(G);

@@ -147,7 +147,8 @@ typedef struct hipPPMData hipPPMData;
    per-cell flux_hll fallback; with HLL or HLLC each pencil is computed by
    one workgroup that holds it whole, so pencils of up to 1025 cells.  The
    others: ENZO_PPM_ERR_NOT_SUPPORTED.  The batched level
-   (hipPPMLevelCreate) takes TwoShock only. */
+   (hipPPMLevelCreate) takes the same three; with HLL or HLLC every pencil
+   group of the level goes to the one-pencil workgroups. */
 int hipPPMInitParameter(hipPPMParameter *Para, enzo_int DensNum, enzo_int TENum,
                         enzo_int Vel1Num, enzo_int Vel2Num, enzo_int Vel3Num,
                         enzo_int GENum, enzo_int MaxNumberOfBaryonFields,

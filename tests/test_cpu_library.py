@@ -432,3 +432,13 @@ def test_hllc_fallback(cpulib, case):
     assert run(cpulib, b, glob, axis) == (status, nfb)
     if status == 1:
         assert all(r[1] for r in field_report(b, a))
+
+
+@pytest.mark.parametrize("kind,solver", [("blast", 4), ("diffusion", 1)],
+                         ids=["blast-hllc", "diffusion-hll"])
+def test_level_alternative_solvers(cpulib, kind, solver):
+    """RiemannSolver HLL / HLLC on a batched level (every pencil group in
+    the one-pencil workgroups, slice by slice with diffusion): fields and
+    Courant steps equal to the per-grid oracle loop."""
+    grids, glob = level_grids(kind, 3, seed=13)
+    level_check(cpulib, grids, dict(glob, RiemannSolver=solver), 2)

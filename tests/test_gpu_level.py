@@ -167,3 +167,15 @@ def test_level_decomposed_box_equals_whole(lib):
         sl = h.active_slice()
         for x, y in zip(h.fields, whole.fields):
             assert bit_equal(x[sl], y[w])
+
+
+@pytest.mark.parametrize("solver", [1, 4], ids=["hll", "hllc"])
+@pytest.mark.parametrize("kind", ["blast", "diffusion", "gravity"])
+@pytest.mark.parametrize("graph", [True, False])
+def test_level_alternative_solvers(lib, kind, solver, graph):
+    """RiemannSolver HLL / HLLC on a batched level: every pencil group in
+    the one-pencil workgroups (slice by slice with diffusion), graph replay
+    or direct launches; fields and Courant steps equal to the per-grid
+    oracle loop."""
+    grids, glob = level_grids(kind, 4, seed=13)
+    level_check(lib, grids, dict(glob, RiemannSolver=solver), 3, graph=graph)
