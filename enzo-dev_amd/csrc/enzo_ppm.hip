@@ -534,17 +534,6 @@ int launch_pressure(hipPPMData *D, const hipPPMParameter *P, double min_support,
   return ENZO_PPM_SUCCESS;
 }
 
-// Enzo field number of each device slot (-1: none)
-void slot_fields(const hipPPMParameter *P, std::vector<long long> &f) {
-  f.assign(nslot_of(P), -1);
-  f[S_D] = P->DensNum;
-  f[S_E] = P->TENum;
-  f[S_VX] = P->Vel1Num;
-  if (P->GridRank > 1) f[S_VY] = P->Vel2Num;
-  if (P->GridRank > 2) f[S_VZ] = P->Vel3Num;
-  if (P->DualEnergyFormalism) f[S_GE] = P->GENum;
-  for (int n = 0; n < P->NumberOfColours; n++) f[S_COL0 + n] = P->colnum[n];
-}
 
 }  // namespace
 
@@ -747,57 +736,13 @@ int hipPPMSetSubgridFluxes(hipPPMData *D, const hipPPMParameter *P, enzo_fluxes 
     D->faces[dim] = nullptr;
     D->face_out[dim] = nullptr;
     D->face_total[dim] = 0;
-    if (dim >= P->GridRank) continue;
-    const int N = D->GD[dim];
-    const int i1 = (int)P->GridStartIndex[dim] + 1, i2 = (int)P->GridEndIndex[dim] + 1;
-    const int ai = dim == 0 ? 1 : 0, aj = dim == 2 ? 1 : 2;
-    std::vector<std::pair<int, int>> order;  // (pos, record)
+    std::vector<int> head;
     long long total = 0;
-    for (int n = 0; n < NumberOfSubgrids; n++) {
-      const enzo_fluxes *F = SubgridFluxes[n];
-      if (!F) return ENZO_PPM_ERR_ARGUMENT;
-      for (int side = 0; side < 2; side++) {
-        FaceRec R;
-        memset(&R, 0, sizeof(R));
-        long long face = side == 0 ? F->LeftFluxStartGlobalIndex[dim][dim] - GridGlobalStart[dim]
-                                   : F->RightFluxStartGlobalIndex[dim][dim] - GridGlobalStart[dim] + 1;
-        R.pos = (int)face + 1;  // 1-based interface at the left edge of cell `face`
-        R.lo_i = (int)(F->RightFluxStartGlobalIndex[dim][ai] - GridGlobalStart[ai]);
-        R.hi_i = (int)(F->RightFluxEndGlobalIndex[dim][ai] - GridGlobalStart[ai]);
-        R.lo_j = (int)(F->RightFluxStartGlobalIndex[dim][aj] - GridGlobalStart[aj]);
-        R.hi_j = (int)(F->RightFluxEndGlobalIndex[dim][aj] - GridGlobalStart[aj]);
-        if (R.pos < i1 || R.pos > i2 + 1 || R.hi_i < R.lo_i || R.hi_j < R.lo_j)
-          return ENZO_PPM_ERR_ARGUMENT;
-        R.nface = (long long)(R.hi_i - R.lo_i + 1) * (R.hi_j - R.lo_j + 1);
-        order.push_back({R.pos, (int)D->hfaces[dim].size()});
-        D->hfaces[dim].push_back(R);
-        D->hface_sub[dim].push_back(n);
-        D->hface_side[dim].push_back(side);
-      }
-    }
-    // sort records by position, assign staging offsets
-    std::stable_sort(order.begin(), order.end());
-    std::vector<FaceRec> sorted;
-    std::vector<int> sub, side;
-    for (auto &pr : order) {
-      FaceRec R = D->hfaces[dim][pr.second];
-      R.base = total;
-      total += R.nface * nfld;
-      sorted.push_back(R);
-      sub.push_back(D->hface_sub[dim][pr.second]);
-      side.push_back(D->hface_side[dim][pr.second]);
-    }
-    D->hfaces[dim] = sorted;
-    D->hface_sub[dim] = sub;
-    D->hface_side[dim] = side;
-    std::vector<int> head(N + 2, 0);
-    {
-      size_t r = 0;
-      for (int pos = 0; pos <= N + 1; pos++) {
-        while (r < sorted.size() && sorted[r].pos < pos) r++;
-        head[pos] = (int)r;
-      }
-    }
+    const int st = build_face_records(D->GD, P, SubgridFluxes, (int)NumberOfSubgrids,
+                                      GridGlobalStart, nfld, dim, D->hfaces[dim],
+                                      D->hface_sub[dim], D->hface_side[dim], head, total);
+    if (st != ENZO_PPM_SUCCESS) return st;
+    const std::vector<FaceRec> &sorted = D->hfaces[dim];
     D->face_total[dim] = total;
     if (total > 0) {
       if (alloc_dev((void **)&D->face_head[dim], sizeof(int) * head.size()) ||
@@ -836,19 +781,8 @@ int hipPPMSaveSubgridFluxes(enzo_fluxes *SubgridFluxes[], enzo_int NumberOfSubgr
                              P->GridEndIndex[2] - P->GridStartIndex[2] + 1};
   std::vector<long long> f;
   slot_fields(P, f);
-  for (size_t r = 0; r < D->hfaces[dir].size(); r++) {
-    const FaceRec &R = D->hfaces[dir][r];
-    enzo_fluxes *F = SubgridFluxes[D->hface_sub[dir][r]];
-    const int side = D->hface_side[dir][r];
-    for (int s = 0; s < (int)f.size(); s++) {
-      if (f[s] < 0) continue;
-      // velocity guards of Grid_[xyz]EulerSweep.C:343-357
-      if (s >= S_VX && s <= S_VZ && (s - S_VX) != dir && nact[s - S_VX] <= 1) continue;
-      enzo_float *out = side == 0 ? F->LeftFluxes[f[s]][dir] : F->RightFluxes[f[s]][dir];
-      if (!out) continue;
-      memcpy(out, h.data() + R.base + (size_t)s * R.nface, sizeof(double) * R.nface);
-    }
-  }
+  scatter_face_fluxes(h, D->hfaces[dir], D->hface_sub[dir], D->hface_side[dir], f, nact, (int)dir,
+                      SubgridFluxes);
   return ENZO_PPM_SUCCESS;
 }
 

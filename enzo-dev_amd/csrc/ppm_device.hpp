@@ -7,6 +7,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <utility>
+#include <vector>
+
 #include "ppm_sweep.hpp"
 
 namespace ppm {
@@ -425,6 +430,102 @@ __device__ inline void prep_item(const FlatParams &F, long long gid) {
   const int outer = (int)(k / ncell);
   const int cell = F.i1 - 1 + (int)(k % ncell);
   F.flat[(long long)outer * F.N + (cell - 1)] = flat_value(F, outer, cell);
+}
+
+
+// ------------------------------------------------------------------------
+// Host: Enzo field number of each device slot (-1: none)
+inline void slot_fields(const hipPPMParameter *P, std::vector<long long> &f) {
+  f.assign(S_COL0 + (int)P->NumberOfColours, -1);
+  f[S_D] = P->DensNum;
+  f[S_E] = P->TENum;
+  f[S_VX] = P->Vel1Num;
+  if (P->GridRank > 1) f[S_VY] = P->Vel2Num;
+  if (P->GridRank > 2) f[S_VZ] = P->Vel3Num;
+  if (P->DualEnergyFormalism) f[S_GE] = P->GENum;
+  for (int n = 0; n < P->NumberOfColours; n++) f[S_COL0 + n] = P->colnum[n];
+}
+
+// ------------------------------------------------------------------------
+// Host: the face records of one sweep axis for a grid's SubgridFluxes
+// (Grid_[xyz]EulerSweep.C:313-376 capture positions): one record per
+// subgrid face crossing the axis, sorted by interface position, with
+// staging offsets of nfld fields each, and head[pos] = first record at or
+// after interface pos.  ENZO_PPM_ERR_ARGUMENT for a face outside the grid.
+inline int build_face_records(const int GD[3], const hipPPMParameter *P,
+                              enzo_fluxes *const *SubgridFluxes, int nsub,
+                              const enzo_long_int *GridGlobalStart, int nfld, int dim,
+                              std::vector<FaceRec> &sorted, std::vector<int> &sub,
+                              std::vector<int> &side, std::vector<int> &head, long long &total) {
+  sorted.clear();
+  sub.clear();
+  side.clear();
+  head.clear();
+  total = 0;
+  if (dim >= P->GridRank) return ENZO_PPM_SUCCESS;
+  const int N = GD[dim];
+  const int i1 = (int)P->GridStartIndex[dim] + 1, i2 = (int)P->GridEndIndex[dim] + 1;
+  const int ai = dim == 0 ? 1 : 0, aj = dim == 2 ? 1 : 2;
+  std::vector<FaceRec> recs;
+  std::vector<int> rsub, rside;
+  std::vector<std::pair<int, int>> order;  // (pos, record)
+  for (int n = 0; n < nsub; n++) {
+    const enzo_fluxes *F = SubgridFluxes[n];
+    if (!F) return ENZO_PPM_ERR_ARGUMENT;
+    for (int sd = 0; sd < 2; sd++) {
+      FaceRec R;
+      memset(&R, 0, sizeof(R));
+      long long face = sd == 0 ? F->LeftFluxStartGlobalIndex[dim][dim] - GridGlobalStart[dim]
+                               : F->RightFluxStartGlobalIndex[dim][dim] - GridGlobalStart[dim] + 1;
+      R.pos = (int)face + 1;  // 1-based interface at the left edge of cell `face`
+      R.lo_i = (int)(F->RightFluxStartGlobalIndex[dim][ai] - GridGlobalStart[ai]);
+      R.hi_i = (int)(F->RightFluxEndGlobalIndex[dim][ai] - GridGlobalStart[ai]);
+      R.lo_j = (int)(F->RightFluxStartGlobalIndex[dim][aj] - GridGlobalStart[aj]);
+      R.hi_j = (int)(F->RightFluxEndGlobalIndex[dim][aj] - GridGlobalStart[aj]);
+      if (R.pos < i1 || R.pos > i2 + 1 || R.hi_i < R.lo_i || R.hi_j < R.lo_j)
+        return ENZO_PPM_ERR_ARGUMENT;
+      R.nface = (long long)(R.hi_i - R.lo_i + 1) * (R.hi_j - R.lo_j + 1);
+      order.push_back({R.pos, (int)recs.size()});
+      recs.push_back(R);
+      rsub.push_back(n);
+      rside.push_back(sd);
+    }
+  }
+  std::stable_sort(order.begin(), order.end());
+  for (auto &pr : order) {
+    FaceRec R = recs[pr.second];
+    R.base = total;
+    total += R.nface * nfld;
+    sorted.push_back(R);
+    sub.push_back(rsub[pr.second]);
+    side.push_back(rside[pr.second]);
+  }
+  head.assign(N + 2, 0);
+  size_t r = 0;
+  for (int pos = 0; pos <= N + 1; pos++) {
+    while (r < sorted.size() && sorted[r].pos < pos) r++;
+    head[pos] = (int)r;
+  }
+  return ENZO_PPM_SUCCESS;
+}
+
+// Host: a grid's captured fluxes of one axis (staging `h`) into its
+// SubgridFluxes, with the velocity guards of Grid_[xyz]EulerSweep.C:343-357.
+inline void scatter_face_fluxes(const std::vector<double> &h, const std::vector<FaceRec> &recs,
+                                const std::vector<int> &sub, const std::vector<int> &side,
+                                const std::vector<long long> &slot_field, const long long nact[3],
+                                int dim, enzo_fluxes *const *SubgridFluxes) {
+  for (size_t r = 0; r < recs.size(); r++) {
+    const FaceRec &R = recs[r];
+    enzo_fluxes *F = SubgridFluxes[sub[r]];
+    for (int s = 0; s < (int)slot_field.size(); s++) {
+      if (slot_field[s] < 0) continue;
+      if (s >= S_VX && s <= S_VZ && (s - S_VX) != dim && nact[s - S_VX] <= 1) continue;
+      enzo_float *out = side[r] == 0 ? F->LeftFluxes[slot_field[s]][dim] : F->RightFluxes[slot_field[s]][dim];
+      if (!out) continue;
+      memcpy(out, h.data() + R.base + (size_t)s * R.nface, sizeof(double) * R.nface);
+    }
+  }
 }
 
 }  // namespace ppm

@@ -301,6 +301,18 @@ struct hipPPMLevel {
   double *old = nullptr;            // (S_GE + 1) * cells per grid at ooff
   std::vector<long long> ooff;
   ExpandParams *dexp = nullptr;     // [parity 2][grid]
+  // subgrid face fluxes (hipPPMLevelSetSubgridFluxes): per grid, per axis
+  // the face records, their owners and the device staging
+  struct Faces {
+    int nsub = 0;
+    std::vector<FaceRec> recs[3];
+    std::vector<int> sub[3], side[3];
+    long long total[3] = {0, 0, 0};
+    int *head[3] = {nullptr, nullptr, nullptr};
+    FaceRec *recs_dev[3] = {nullptr, nullptr, nullptr};
+    double *out[3] = {nullptr, nullptr, nullptr};
+  };
+  std::vector<Faces> faces;
   // sibling copies (hipPPMLevelSetSiblings)
   CopyBox *dboxes = nullptr;
   GridGeom *dgeom = nullptr;
@@ -314,8 +326,33 @@ void drop_graphs(hipPPMLevel *L) {
   L->graphs.clear();
 }
 
+void free_faces(hipPPMLevel::Faces &F) {
+  for (int d = 0; d < 3; d++) {
+    if (F.head[d]) (void)hipFree(F.head[d]);
+    if (F.recs_dev[d]) (void)hipFree(F.recs_dev[d]);
+    if (F.out[d]) (void)hipFree(F.out[d]);
+    F.head[d] = nullptr;
+    F.recs_dev[d] = nullptr;
+    F.out[d] = nullptr;
+    F.total[d] = 0;
+    F.recs[d].clear();
+    F.sub[d].clear();
+    F.side[d].clear();
+  }
+  F.nsub = 0;
+}
+
+// a grid's sweep parameters point at its face records (or at none)
+void face_params(const hipPPMLevel *L, int g, int d, SweepParams &S) {
+  const bool on = g < (int)L->faces.size() && L->faces[g].total[d] > 0;
+  S.face_head = on ? L->faces[g].head[d] : nullptr;
+  S.faces = on ? L->faces[g].recs_dev[d] : nullptr;
+  S.face_out = on ? L->faces[g].out[d] : nullptr;
+}
+
 void free_level(hipPPMLevel *L) {
   drop_graphs(L);
+  for (auto &F : L->faces) free_faces(F);
   void *ptrs[] = {L->A, L->B, L->pres, L->accel, L->tabs, L->cws, L->flat, L->qc, L->segs, L->dt,
                   L->dtmin, L->counters, L->fb_flag, L->fb_list, L->fb_state, L->grp_grid,
                   L->dgrp_off, L->dparams,
@@ -523,9 +560,7 @@ int build_plan(hipPPMLevel *L) {
         S.gravity = (int)P0.GravityOn;
         S.ifallback = (int)P0.RiemannSolverFallback;
         S.nact_out = (int)(Pg.GridEndIndex[oax] - Pg.GridStartIndex[oax] + 1);
-        S.face_head = nullptr;
-        S.faces = nullptr;
-        S.face_out = nullptr;
+        face_params(L, g, d, S);
         S.counters = L->counters;
         S.fb_flag = L->fb_flag + L->grp_off[g];
         S.fb_list = L->fb_list;
@@ -771,6 +806,7 @@ int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMP
   L->gravity = Para[0].GravityOn != 0;
   L->diffusion = Para[0].PPMDiffusionParameter != 0;
   L->whole = Para[0].RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK ? 1 : 0;
+  L->faces.resize(G);
   L->GD.resize(3 * G);
   L->cells.resize(G);
   L->off.resize(G);
@@ -992,6 +1028,69 @@ int hipPPMLevelStep(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dtFixed, en
   LCHK(hipGraphLaunch(it->second, L->stream));
   L->parity ^= (L->rank & 1);  // one swap per sweep
   L->old_valid = L->comoving;
+  return ENZO_PPM_SUCCESS;
+}
+
+int hipPPMLevelSetSubgridFluxes(hipPPMLevel *L, enzo_int grid, enzo_fluxes *SubgridFluxes[],
+                                enzo_int NumberOfSubgrids, const enzo_long_int GridGlobalStart[]) {
+  if (!L || grid < 0 || grid >= L->ngrids || NumberOfSubgrids < 0) return ENZO_PPM_ERR_ARGUMENT;
+  if (NumberOfSubgrids > 0 && (!SubgridFluxes || !GridGlobalStart)) return ENZO_PPM_ERR_ARGUMENT;
+  const int g = (int)grid, G = L->ngrids;
+  LCHK(hipStreamSynchronize(L->stream));  // no step may still use the old records
+  hipPPMLevel::Faces &F = L->faces[g];
+  free_faces(F);
+  int st = ENZO_PPM_SUCCESS;
+  for (int d = 0; d < L->rank && st == ENZO_PPM_SUCCESS; d++) {
+    std::vector<int> head;
+    long long total = 0;
+    st = build_face_records(&L->GD[3 * g], &L->para[g], SubgridFluxes, (int)NumberOfSubgrids,
+                            GridGlobalStart, L->nslot, d, F.recs[d], F.sub[d], F.side[d], head,
+                            total);
+    if (st != ENZO_PPM_SUCCESS || total == 0) continue;
+    if (dev_alloc(&F.head[d], head.size()) || dev_alloc(&F.recs_dev[d], F.recs[d].size()) ||
+        dev_alloc(&F.out[d], (size_t)total) ||
+        hipMemcpy(F.head[d], head.data(), sizeof(int) * head.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(F.recs_dev[d], F.recs[d].data(), sizeof(FaceRec) * F.recs[d].size(),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(F.out[d], 0, sizeof(double) * total) != hipSuccess) {
+      st = ENZO_PPM_ERR_HIP;
+      continue;
+    }
+    F.total[d] = total;
+  }
+  if (st != ENZO_PPM_SUCCESS) free_faces(F);
+  else F.nsub = (int)NumberOfSubgrids;
+  // the grid's sweep parameters of both buffer parities; the kernels read
+  // them from device memory at every launch, so captured steps see them too
+  for (int par = 0; par < 2; par++)
+    for (int d = 0; d < 3; d++) {
+      const size_t k = ((size_t)par * 3 + d) * G + g;
+      face_params(L, g, d, L->hparams[k]);
+      LCHK(hipMemcpy(L->dparams + k, &L->hparams[k], sizeof(SweepParams), hipMemcpyHostToDevice));
+    }
+  return st;
+}
+
+int hipPPMLevelSaveSubgridFluxes(hipPPMLevel *L, enzo_int grid, enzo_fluxes *SubgridFluxes[],
+                                 enzo_int NumberOfSubgrids) {
+  if (!L || grid < 0 || grid >= L->ngrids) return ENZO_PPM_ERR_ARGUMENT;
+  const hipPPMLevel::Faces &F = L->faces[grid];
+  if (NumberOfSubgrids != F.nsub || (NumberOfSubgrids > 0 && !SubgridFluxes))
+    return ENZO_PPM_ERR_ARGUMENT;
+  LCHK(hipStreamSynchronize(L->stream));
+  const hipPPMParameter &P = L->para[grid];
+  const long long nact[3] = {P.GridEndIndex[0] - P.GridStartIndex[0] + 1,
+                             P.GridEndIndex[1] - P.GridStartIndex[1] + 1,
+                             P.GridEndIndex[2] - P.GridStartIndex[2] + 1};
+  std::vector<long long> f;
+  slot_fields(&P, f);
+  for (int d = 0; d < L->rank; d++) {
+    if (F.total[d] == 0) continue;
+    std::vector<double> h((size_t)F.total[d]);
+    LCHK(hipMemcpy(h.data(), F.out[d], sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    scatter_face_fluxes(h, F.recs[d], F.sub[d], F.side[d], f, nact, d, SubgridFluxes);
+  }
   return ENZO_PPM_SUCCESS;
 }
 

@@ -173,6 +173,10 @@ PROTOTYPES = {
     "hipPPMLevelSetBaryon": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(PF), C.POINTER(PF)]),
     "hipPPMLevelGetBaryon": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(PF)]),
     "hipPPMLevelStep": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_double]),
+    "hipPPMLevelSetSubgridFluxes": (C.c_int, [C.c_void_p, C.c_int64,
+                                              C.POINTER(C.POINTER(EnzoFluxes)), C.c_int64, PI]),
+    "hipPPMLevelSaveSubgridFluxes": (C.c_int, [C.c_void_p, C.c_int64,
+                                               C.POINTER(C.POINTER(EnzoFluxes)), C.c_int64]),
     "hipPPMLevelStatus": (C.c_int, [C.c_void_p, PF, PI]),
     "hipPPMLevelUseGraph": (C.c_int, [C.c_void_p, C.c_int64]),
     "hipPPMLevelSetShape": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int64]),

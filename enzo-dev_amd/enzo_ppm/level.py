@@ -53,6 +53,22 @@ class Level:
         """One level step; dt <= 0 uses the level's Courant step."""
         self._check(self.lib.hipPPMLevelStep(self.handle, cycle, dt, cfl), "LevelStep")
 
+    def set_subgrid_fluxes(self, grid, fluxes):
+        """Record the face fluxes of `fluxes` (a grid.SubgridFluxes of grid
+        `grid`) from the next step on; None switches the grid's capture off."""
+        if fluxes is None:
+            self._check(self.lib.hipPPMLevelSetSubgridFluxes(self.handle, grid, None, 0, None),
+                        "LevelSetSubgridFluxes")
+            return
+        self._check(self.lib.hipPPMLevelSetSubgridFluxes(self.handle, grid, fluxes.ptrs, fluxes.n,
+                                                         fluxes.global_start),
+                    "LevelSetSubgridFluxes")
+
+    def save_subgrid_fluxes(self, grid, fluxes):
+        """The fluxes of the last step into the arrays of `fluxes`."""
+        self._check(self.lib.hipPPMLevelSaveSubgridFluxes(self.handle, grid, fluxes.ptrs, fluxes.n),
+                    "LevelSaveSubgridFluxes")
+
     def status(self):
         """(status, dt of the last step, fallback cells since the last call)."""
         dt = C.c_double(0.0)

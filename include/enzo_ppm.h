@@ -465,6 +465,22 @@ int hipPPMLevelGetBaryon(hipPPMLevel *Level, enzo_int grid, enzo_float *BaryonFi
 int hipPPMLevelStep(hipPPMLevel *Level, enzo_int CycleNumber, enzo_float dtFixed,
                     enzo_float cfl);
 
+/* Face list of one grid's subgrids (SubgridFluxes of
+   Grid_SolveHydroEquations.C:320-369 passed to the sweeps of Grid_SolvePPM_DE.C:88-143):
+   from the next hipPPMLevelStep on, every sweep of that grid records the
+   fluxes of those faces (Grid_xEulerSweep.C:313-376).  Waits for the level's
+   stream; NumberOfSubgrids = 0 switches the grid's capture off. */
+int hipPPMLevelSetSubgridFluxes(hipPPMLevel *Level, enzo_int grid,
+                                enzo_fluxes *SubgridFluxes[], enzo_int NumberOfSubgrids,
+                                const enzo_long_int GridGlobalStart[]);
+
+/* The fluxes of the last hipPPMLevelStep along every axis into the
+   SubgridFluxes arrays registered for the grid (the per-grid loop of
+   cuPPMSaveSubgridFluxes, cuPPM.h:101-103, over x, y and z).  Waits for
+   the level's stream. */
+int hipPPMLevelSaveSubgridFluxes(hipPPMLevel *Level, enzo_int grid,
+                                 enzo_fluxes *SubgridFluxes[], enzo_int NumberOfSubgrids);
+
 /* Waits for the level's stream; returns ENZO_PPM_SUCCESS or the error of
    any step since the last call, the dt of the last step and the fallback
    cells since the last call (the counts then restart). */

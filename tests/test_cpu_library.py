@@ -385,6 +385,14 @@ def test_external_boundary_refusals_write_nothing(cpulib):
         assert bit_equal(x, y)
 
 
+@pytest.mark.parametrize("graph", [True, False])
+def test_level_subgrid_fluxes(cpulib, graph):
+    """Face fluxes of subgrids recorded by the batched level equal the
+    oracle's per-grid SolvePPM_DE with the same SubgridFluxes."""
+    from helpers import level_flux_check
+    level_flux_check(cpulib, "blast", graph=graph)
+
+
 @pytest.mark.parametrize("types", [[0, 1, 3, 3, 2, 0], [1, 1, 0, 0, 4, 4]])
 def test_level_external_boundary_matches_oracle(cpulib, types):
     import boundary_checks as bc

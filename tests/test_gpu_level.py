@@ -179,3 +179,15 @@ def test_level_alternative_solvers(lib, kind, solver, graph):
     oracle loop."""
     grids, glob = level_grids(kind, 4, seed=13)
     level_check(lib, grids, dict(glob, RiemannSolver=solver), 3, graph=graph)
+
+
+@pytest.mark.parametrize("solver", [5, 1, 4], ids=["twoshock", "hll", "hllc"])
+@pytest.mark.parametrize("kind", ["blast", "diffusion", "gravity", "stretched"])
+@pytest.mark.parametrize("graph", [True, False])
+def test_level_subgrid_fluxes(lib, kind, solver, graph):
+    """Subgrid face fluxes recorded by the batched level (captured graph or
+    direct launches) equal the oracle's per-grid SolvePPM_DE with the same
+    SubgridFluxes, bitwise, and so do the fields."""
+    from helpers import level_flux_check
+    print(f"level fluxes {kind} solver={solver} graph={graph}", flush=True)
+    level_flux_check(lib, kind, graph=graph, solver=solver)
