@@ -1039,13 +1039,19 @@ int hipPPMProjectBoundaryFluxes(const enzo_fluxes *Fine, enzo_int rank, enzo_int
   return st;
 }
 
-int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
-                                  const enzo_fluxes *InitialFluxes,
-                                  const enzo_fluxes *RefinedFluxes,
-                                  enzo_fluxes *BoundaryFluxesThisTimeStep, enzo_int SUBlingGrid,
-                                  const enzo_int *GlobalDim, const enzo_int *PeriodicFace,
-                                  enzo_int CorrectParentBoundaryFlux, const enzo_int *ColourKind) {
-  if (!D || !P || !InitialFluxes || !RefinedFluxes) return ENZO_PPM_ERR_ARGUMENT;
+}  // extern "C"
+
+namespace ppm {
+
+// Flux correction of one grid's device state A (cells per slot, dims GD) on
+// `stream`; shared by the per-grid and the level entry points.
+int correct_refined_fluxes(double *A, long long cells, const int GD[3], hipStream_t stream,
+                           const hipPPMParameter *P, const enzo_fluxes *InitialFluxes,
+                           const enzo_fluxes *RefinedFluxes,
+                           enzo_fluxes *BoundaryFluxesThisTimeStep, enzo_int SUBlingGrid,
+                           const enzo_int *GlobalDim, const enzo_int *PeriodicFace,
+                           enzo_int CorrectParentBoundaryFlux, const enzo_int *ColourKind) {
+  if (!A || !P || !InitialFluxes || !RefinedFluxes) return ENZO_PPM_ERR_ARGUMENT;
   if (CorrectParentBoundaryFlux && !BoundaryFluxesThisTimeStep) return ENZO_PPM_ERR_ARGUMENT;
   if (SUBlingGrid && (!GlobalDim || !PeriodicFace)) return ENZO_PPM_ERR_ARGUMENT;
   if (P->GridRank != 3) return ENZO_PPM_ERR_NOT_SUPPORTED;  // the device holds Vel3 in 3-D only
@@ -1054,9 +1060,9 @@ int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
     if (!ColourKind || ColourKind[c] < 0 || ColourKind[c] > 2) return ENZO_PPM_ERR_ARGUMENT;
   FluxCorrParams F;
   memset(&F, 0, sizeof(F));
-  F.A = D->A;
-  F.cells = D->cells;
-  for (int d = 0; d < 3; d++) F.GD[d] = D->GD[d];
+  F.A = A;
+  F.cells = cells;
+  for (int d = 0; d < 3; d++) F.GD[d] = GD[d];
   F.idual = (int)P->DualEnergyFormalism;
   F.rank = (int)P->GridRank;
   F.tiny_number = P->MinimumPressure;
@@ -1092,7 +1098,7 @@ int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
   size_t cap = 0;
   int st = ENZO_PPM_SUCCESS;
   for (int dim = 0; dim < 3 && st == ENZO_PPM_SUCCESS; dim++) {
-    if (D->GD[dim] <= 1) continue;
+    if (GD[dim] <= 1) continue;
     // Refined's index ranges of this direction, edited for a SUBling as the
     // reference edits them in place (:147-222)
     long long RLS[3], RLE[3], RRS[3];
@@ -1132,7 +1138,7 @@ int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
     for (int i = 0; i < 3 && st == ENZO_PPM_SUCCESS; i++) {
       Start[i] = std::max((long long)In->LeftFluxStartGlobalIndex[dim][i], RLS[i]) - P->GridGlobalStart[i];
       End[i] = std::min((long long)In->LeftFluxEndGlobalIndex[dim][i], RLE[i]) - P->GridGlobalStart[i];
-      if (Start[i] < 0 || End[i] > D->GD[i]) st = ENZO_PPM_ERR_ARGUMENT;
+      if (Start[i] < 0 || End[i] > GD[i]) st = ENZO_PPM_ERR_ARGUMENT;
     }
     if (st != ENZO_PPM_SUCCESS) break;
     Start[dim] = std::max(Start[dim] - 1, 0ll);
@@ -1140,7 +1146,7 @@ int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
     long long Dim[3];
     for (int i = 0; i < 3; i++) Dim[i] = End[i] - Start[i] + 1;
     long long Offset = In->RightFluxStartGlobalIndex[dim][dim] - In->LeftFluxStartGlobalIndex[dim][dim] + 2;
-    Offset = std::min(Offset, (long long)D->GD[dim] - 1);
+    Offset = std::min(Offset, (long long)GD[dim] - 1);
     if (SUBlingGrid) {
       // this grid's own first / last active layer; Initial spans its whole
       // face (:287-303)
@@ -1165,8 +1171,8 @@ int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
     for (int i = 0; i < 3; i++) Ext[i] = End[i] - Start[i] + 1;
     if (Ext[0] <= 0 || Ext[1] <= 0 || Ext[2] <= 0) continue;  // faces do not overlap
     for (int i = 0; i < 3; i++)
-      if (Start[i] < 0 || End[i] >= D->GD[i]) st = ENZO_PPM_ERR_ARGUMENT;
-    if (Start[dim] + Offset >= D->GD[dim] || Start[dim] + Offset < 0) st = ENZO_PPM_ERR_ARGUMENT;
+      if (Start[i] < 0 || End[i] >= GD[i]) st = ENZO_PPM_ERR_ARGUMENT;
+    if (Start[dim] + Offset >= GD[dim] || Start[dim] + Offset < 0) st = ENZO_PPM_ERR_ARGUMENT;
     // every flux index within the arrays: Initial of extent Dim (the
     // grid's own face for a SUBling), Refined of extent RefinedDim
     for (int i = 0; i < 3; i++) {
@@ -1207,7 +1213,7 @@ int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
               if (bndR) BoundaryFluxesThisTimeStep->RightFluxes[lf[l]][dim][gfi] = Rf->RightFluxes[lf[l]][dim][fi];
             }
     }
-    for (int i = 0; i < dim; i++) Offset *= D->GD[i];
+    for (int i = 0; i < dim; i++) Offset *= GD[i];
     // stage [list field][side][initial, refined][loop position]: the left
     // side pairs initial left with refined left (proper subgrid) or refined
     // right (SUBling), the right side the other way round
@@ -1237,7 +1243,7 @@ int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
       if (alloc_dev((void **)&dflux, sizeof(double) * cap)) { st = ENZO_PPM_ERR_HIP; break; }
     }
     if (hipMemcpyAsync(dflux, stage.data(), sizeof(double) * stage.size(), hipMemcpyHostToDevice,
-                       D->stream) != hipSuccess) { st = ENZO_PPM_ERR_HIP; break; }
+                       stream) != hipSuccess) { st = ENZO_PPM_ERR_HIP; break; }
     for (int i = 0; i < 3; i++) {
       F.Start[i] = (int)Start[i];
       F.Dim[i] = (int)Ext[i];
@@ -1249,12 +1255,28 @@ int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
     F.flux = dflux;
     F.nface = nface;
     hipLaunchKernelGGL(fluxcorr_kernel, dim3((unsigned)((nface + 255) / 256)), dim3(256), 0,
-                       D->stream, F);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(D->stream) != hipSuccess)
+                       stream, F);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(stream) != hipSuccess)
       st = ENZO_PPM_ERR_HIP;
   }
   if (dflux) (void)hipFree(dflux);
   return st;
+}
+
+}  // namespace ppm
+
+extern "C" {
+
+int hipPPMCorrectForRefinedFluxes(hipPPMData *D, const hipPPMParameter *P,
+                                  const enzo_fluxes *InitialFluxes,
+                                  const enzo_fluxes *RefinedFluxes,
+                                  enzo_fluxes *BoundaryFluxesThisTimeStep, enzo_int SUBlingGrid,
+                                  const enzo_int *GlobalDim, const enzo_int *PeriodicFace,
+                                  enzo_int CorrectParentBoundaryFlux, const enzo_int *ColourKind) {
+  if (!D) return ENZO_PPM_ERR_ARGUMENT;
+  return correct_refined_fluxes(D->A, D->cells, D->GD, D->stream, P, InitialFluxes, RefinedFluxes,
+                                BoundaryFluxesThisTimeStep, SUBlingGrid, GlobalDim, PeriodicFace,
+                                CorrectParentBoundaryFlux, ColourKind);
 }
 
 int hipPPMPeriodicBoundary(hipPPMData *D, const hipPPMParameter *P) {

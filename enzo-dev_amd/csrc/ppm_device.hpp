@@ -434,6 +434,17 @@ __device__ inline void prep_item(const FlatParams &F, long long gid) {
 
 
 // ------------------------------------------------------------------------
+// Host: flux correction of one grid's device state (A: the grid's slots,
+// `cells` apart; GD: its dimensions) on `stream`, as
+// hipPPMCorrectForRefinedFluxes describes; waits for the stream.
+int correct_refined_fluxes(double *A, long long cells, const int GD[3], hipStream_t stream,
+                           const hipPPMParameter *P, const enzo_fluxes *InitialFluxes,
+                           const enzo_fluxes *RefinedFluxes,
+                           enzo_fluxes *BoundaryFluxesThisTimeStep, enzo_int SUBlingGrid,
+                           const enzo_int *GlobalDim, const enzo_int *PeriodicFace,
+                           enzo_int CorrectParentBoundaryFlux, const enzo_int *ColourKind);
+
+// ------------------------------------------------------------------------
 // Host: Enzo field number of each device slot (-1: none)
 inline void slot_fields(const hipPPMParameter *P, std::vector<long long> &f) {
   f.assign(S_COL0 + (int)P->NumberOfColours, -1);

@@ -1094,6 +1094,23 @@ int hipPPMLevelSaveSubgridFluxes(hipPPMLevel *L, enzo_int grid, enzo_fluxes *Sub
   return ENZO_PPM_SUCCESS;
 }
 
+int hipPPMLevelCorrectForRefinedFluxes(hipPPMLevel *L, enzo_int grid,
+                                       const enzo_fluxes *InitialFluxes,
+                                       const enzo_fluxes *RefinedFluxes,
+                                       enzo_fluxes *BoundaryFluxesThisTimeStep,
+                                       enzo_int SUBlingGrid, const enzo_int *GlobalDim,
+                                       const enzo_int *PeriodicFace,
+                                       enzo_int CorrectParentBoundaryFlux,
+                                       const enzo_int *ColourKind) {
+  if (!L || grid < 0 || grid >= L->ngrids) return ENZO_PPM_ERR_ARGUMENT;
+  LCHK(hipStreamSynchronize(L->stream));  // the grid's state after every queued step
+  double *state = (L->parity == 0 ? L->A : L->B) + L->off[grid];
+  return correct_refined_fluxes(state, L->cells[grid], &L->GD[3 * grid], L->stream,
+                                &L->para[grid], InitialFluxes, RefinedFluxes,
+                                BoundaryFluxesThisTimeStep, SUBlingGrid, GlobalDim, PeriodicFace,
+                                CorrectParentBoundaryFlux, ColourKind);
+}
+
 int hipPPMLevelStatus(hipPPMLevel *L, enzo_float *dt_used, enzo_long_int *fallback_cells) {
   if (!L) return ENZO_PPM_ERR_ARGUMENT;
   LCHK(hipStreamSynchronize(L->stream));

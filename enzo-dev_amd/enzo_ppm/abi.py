@@ -202,6 +202,10 @@ PROTOTYPES = {
                                                 C.POINTER(EnzoFluxes), C.POINTER(EnzoFluxes),
                                                 C.POINTER(EnzoFluxes), C.c_int64, PI, PI,
                                                 C.c_int64, PI]),
+    "hipPPMLevelCorrectForRefinedFluxes": (C.c_int, [C.c_void_p, C.c_int64,
+                                                C.POINTER(EnzoFluxes), C.POINTER(EnzoFluxes),
+                                                C.POINTER(EnzoFluxes), C.c_int64, PI, PI,
+                                                C.c_int64, PI]),
     "hipPPMSetExpansionFactors": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_double,
                                             C.c_double]),
     "hipPPMComovingExpansionTerms": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter),

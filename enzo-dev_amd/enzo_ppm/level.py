@@ -14,13 +14,16 @@ from .grid import make_globals
 
 
 class Level:
-    def __init__(self, grids, glob, lib=None, stream=None):
+    def __init__(self, grids, glob, lib=None, stream=None, global_starts=None):
+        """global_starts: per grid, the global index of its first cell (the
+        GridGlobalStart the flux correction reads); zeros by default."""
         self.lib = L = lib or abi.load_library()
         self.grids = list(grids)
         _, gd = make_globals(**glob)
         self.paras = (abi.HipPPMParameter * len(self.grids))()
         for i, g in enumerate(self.grids):
-            self.paras[i] = g.hip_parameter(L, gd)
+            gs = (0, 0, 0) if global_starts is None else tuple(global_starts[i])
+            self.paras[i] = g.hip_parameter(L, gd, global_start=gs)
         self.handle = C.c_void_p(None)
         self._check(L.hipPPMLevelCreate(C.byref(self.handle), len(self.grids), self.paras, stream),
                     "LevelCreate")
@@ -68,6 +71,20 @@ class Level:
         """The fluxes of the last step into the arrays of `fluxes`."""
         self._check(self.lib.hipPPMLevelSaveSubgridFluxes(self.handle, grid, fluxes.ptrs, fluxes.n),
                     "LevelSaveSubgridFluxes")
+
+    def correct_for_refined_fluxes(self, grid, initial, refined, boundary=None, colour_kind=(),
+                                   correct_parent_boundary_flux=0, subling=0, global_dim=None,
+                                   periodic_face=None):
+        """Flux correction of grid `grid` on the level's device state
+        (hipPPMLevelCorrectForRefinedFluxes); arguments as
+        DeviceGrid.correct_for_refined_fluxes."""
+        kinds = abi.iarr(list(colour_kind), max(1, len(colour_kind)))
+        self._check(self.lib.hipPPMLevelCorrectForRefinedFluxes(
+            self.handle, grid, C.byref(initial), C.byref(refined),
+            C.byref(boundary) if boundary is not None else None, int(subling),
+            abi.iarr(global_dim, 3) if global_dim is not None else None,
+            abi.iarr(periodic_face, 6) if periodic_face is not None else None,
+            int(correct_parent_boundary_flux), kinds), "LevelCorrectForRefinedFluxes")
 
     def status(self):
         """(status, dt of the last step, fallback cells since the last call)."""

@@ -481,6 +481,19 @@ int hipPPMLevelSetSubgridFluxes(hipPPMLevel *Level, enzo_int grid,
 int hipPPMLevelSaveSubgridFluxes(hipPPMLevel *Level, enzo_int grid,
                                  enzo_fluxes *SubgridFluxes[], enzo_int NumberOfSubgrids);
 
+/* hipPPMCorrectForRefinedFluxes on grid `grid` of the level, in place on
+   the level's device state (grid::CorrectForRefinedFluxes,
+   Grid_CorrectForRefinedFluxes.C, as UpdateFromFinerGrids.C:150-198 calls
+   it): same arguments and rules.  Waits for the level's stream. */
+int hipPPMLevelCorrectForRefinedFluxes(hipPPMLevel *Level, enzo_int grid,
+                                       const enzo_fluxes *InitialFluxes,
+                                       const enzo_fluxes *RefinedFluxes,
+                                       enzo_fluxes *BoundaryFluxesThisTimeStep,
+                                       enzo_int SUBlingGrid, const enzo_int *GlobalDim,
+                                       const enzo_int *PeriodicFace,
+                                       enzo_int CorrectParentBoundaryFlux,
+                                       const enzo_int *ColourKind);
+
 /* Waits for the level's stream; returns ENZO_PPM_SUCCESS or the error of
    any step since the last call, the dt of the last step and the fallback
    cells since the last call (the counts then restart). */

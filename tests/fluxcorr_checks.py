@@ -64,6 +64,44 @@ def run_case(lib, name, kinds=(0, 1, 2)):
     return changed
 
 
+def run_level_case(lib, name, steps=0, kinds=(0, 1, 2)):
+    """The same correction on grid 1 of a two-grid batched level
+    (hipPPMLevelCorrectForRefinedFluxes), after `steps` level steps of a
+    fixed dt (both buffer parities): grid 1 equals the oracle's step(s) and
+    correction bitwise, grid 0 the oracle's step(s) alone."""
+    from enzo_ppm.level import Level
+    from helpers import level_reference
+    g, glob, ini, ref, own, cpbf = case(name)
+    other, _, _ = problems.turbulent(12, seed=3, ncolours=3)
+    grids = [other, g]
+    for x in grids:
+        x.fill_periodic()
+    b = [x.copy() for x in grids]
+    if steps:
+        level_reference(b, glob, steps, dt=1e-3)
+    own_b = SubgridFluxes(g, [((0, 0, 0), (15, 15, 15))])
+    ref_o = SubgridFluxes(g, [CASES[name][0]])
+    for key, (L, R) in ref.arrays[0].items():
+        ref_o.arrays[0][key][0][:] = L
+        ref_o.arrays[0][key][1][:] = R
+    st = oracle.correct_for_refined_fluxes(b[1], GS, ini.structs[0], ref_o.structs[0],
+                                           own_b.structs[0], glob, list(kinds), cpbf)
+    assert st == 1
+    lv = Level([x.copy() for x in grids], glob, lib=lib, global_starts=[(0, 0, 0), GS])
+    try:
+        for s in range(steps):
+            lv.step(s, 1e-3, 0.4)
+        lv.correct_for_refined_fluxes(1, ini.structs[0], ref.structs[0], own.structs[0], kinds, cpbf)
+        assert lv.status()[0] == 1
+        lv.download()
+        for x, y in zip(lv.grids, b):
+            rep = field_report(x, y)
+            assert all(r[1] for r in rep), rep
+    finally:
+        lv.close()
+    assert np.array_equal(own.flat(), own_b.flat())
+
+
 def run_projection(lib, r):
     """hipPPMProjectBoundaryFluxes against the oracle, bitwise."""
     parent, _, _ = problems.turbulent(16, seed=4, ncolours=1)

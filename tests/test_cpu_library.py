@@ -339,6 +339,12 @@ def test_flux_correction_matches_oracle(cpulib, name):
     assert fc.run_case(cpulib, name) > 0
 
 
+def test_level_flux_correction_matches_oracle(cpulib):
+    """hipPPMLevelCorrectForRefinedFluxes on grid 1 of a two-grid level."""
+    import fluxcorr_checks as fc
+    fc.run_level_case(cpulib, "undo")
+
+
 @pytest.mark.parametrize("r", [2, 4])
 def test_flux_projection_matches_oracle(cpulib, r):
     import fluxcorr_checks as fc

@@ -33,3 +33,10 @@ def test_flux_projection_matches_oracle(lib, r):
 @pytest.mark.parametrize("name", sorted(fc.SUBLING_CASES))
 def test_subling_flux_correction_matches_oracle(lib, name):
     assert fc.run_subling_case(lib, name) > 0
+
+
+@pytest.mark.parametrize("steps", [0, 1])
+@pytest.mark.parametrize("name", sorted(fc.CASES))
+def test_level_flux_correction_matches_oracle(lib, name, steps):
+    print(f"level flux correction {name} steps={steps}", flush=True)
+    fc.run_level_case(lib, name, steps)
