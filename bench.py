@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--cosmo-n", type=int, default=512,
                     help="active cells per axis of the comoving workload (0: skip it)")
     ap.add_argument("--cosmo-steps", type=int, default=10)
+    ap.add_argument("--solver-steps", type=int, default=5,
+                    help="timed steps of the same workload with the HLL and HLLC solvers (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (1 GiB read + 1 GiB write, "
                          "3 times) to scale HBM byte counters")
@@ -177,12 +179,15 @@ def level_bench(lib, ngrids, steps, cfl=0.4, weights=None):
                if batched_learned else {})}
 
 
-def time_grid(lib, n, steps, warmup, weights=None):
+def time_grid(lib, n, steps, warmup, weights=None, solver=None):
     """Device-resident steps of the n^3 turbulent workload: `warmup` untimed,
     then `steps` timed, with HIP events around every sweep launch.  With
-    `weights` the sweeps use the learned two-shock start."""
+    `weights` the sweeps use the learned two-shock start; `solver` sets
+    RiemannSolver (1 HLL, 4 HLLC; default two-shock)."""
     from enzo_ppm import device, problems
     g, glob, run = problems.turbulent(n, seed=2024)
+    if solver is not None:
+        glob = dict(glob, RiemannSolver=solver)
     dg = device.DeviceGrid(g, glob, lib=lib)
     try:
         if weights is not None:
@@ -440,6 +445,18 @@ def main():
                             "valu_instructions_per_launch": 1.407e9,
                             "note": "instruction counts from the committed rocprofv3 pass; "
                                     "time measured here"}
+    if a.solver_steps > 0:
+        # RiemannSolver HLL / HLLC on the same workload: whole pencils in the
+        # one-pencil workgroups (not the metric; the two-shock solver is)
+        print("bench: HLL and HLLC solvers", file=sys.stderr, flush=True)
+        out["other_solvers"] = {}
+        for name, rs in (("hll", 1), ("hllc", 4)):
+            try:
+                r = time_grid(lib, n, a.solver_steps, 1, solver=rs)
+                r.pop("grid")
+                out["other_solvers"][name] = r
+            except Exception as e:  # reported, not fatal
+                out["other_solvers"][name] = {"error": str(e)}
     try:
         out["host_buffers"] = host_buffer_rate(lib, n)
     except Exception as e:  # reported, not fatal

@@ -413,7 +413,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
     return hipGetLastError();
   };
   auto launch_fb = [&]() {
-    const dim3 g(kFbBlocks), b(kFbTS);
+    const dim3 g(whole_pencils ? kFbBlocksWhole : kFbBlocks), b(kFbTS);
     switch (variant) {
       case 0: hipLaunchKernelGGL((ppm_fallback_kernel<0, 0>), g, b, 0, D->stream, S); break;
       case 1: hipLaunchKernelGGL((ppm_fallback_kernel<1, 0>), g, b, 0, D->stream, S); break;

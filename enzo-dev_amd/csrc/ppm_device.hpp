@@ -46,6 +46,9 @@ struct DevExec {
 // One-pencil fallback workgroups: kFbTS threads, the whole pencil in LDS.
 constexpr int kFbTS = 1024;
 constexpr int kFbBlocks = 64;
+// RiemannSolver HLL / HLLC send every group there: one workgroup per CU
+// (a workgroup's LDS fills most of a CU's 160 KB)
+constexpr int kFbBlocksWhole = 256;
 using FbCfg = Cfg<1, kFbTS>;
 
 // Work-item 0 of a sweep workgroup whose tile saw a failing cell or an

@@ -694,10 +694,10 @@ int launch_level_sweep(hipPPMLevel *L, int d) {
                          dim3(256), 0, L->stream, L->dfball + f0, (unsigned)n, L->fb_list, L->fb_state);
       LCHK(hipGetLastError());
       if (L->learned)
-        hipLaunchKernelGGL(level_fallback_kernel<1>, dim3(kFbBlocks), dim3(kFbTS), 0, L->stream, P,
+        hipLaunchKernelGGL(level_fallback_kernel<1>, dim3(kFbBlocksWhole), dim3(kFbTS), 0, L->stream, P,
                            L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
       else
-        hipLaunchKernelGGL(level_fallback_kernel<0>, dim3(kFbBlocks), dim3(kFbTS), 0, L->stream, P,
+        hipLaunchKernelGGL(level_fallback_kernel<0>, dim3(kFbBlocksWhole), dim3(kFbTS), 0, L->stream, P,
                            L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
       LCHK(hipGetLastError());
       continue;
