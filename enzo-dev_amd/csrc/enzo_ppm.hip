@@ -54,12 +54,13 @@ void ppm_sweep_kernel(SweepParams P) {
 // fallback applied in the reference's cell order; its results replace the
 // sweep kernel's for those pencils.  Launched after every sweep launch; with
 // no flagged group each workgroup only reads the count.
-template <int UNI, int LEARN>
-__global__ __launch_bounds__(kFbTS) void ppm_fallback_kernel(SweepParams P) {
-  __shared__ double sm[NARR_FB * FbCfg::LW];
+template <int UNI, int LEARN, int TS = kFbTS>
+__global__ __launch_bounds__(TS) void ppm_fallback_kernel(SweepParams P) {
+  __shared__ double sm[NARR_FB * Cfg<1, TS>::LW];
   DevExec ex;
   const unsigned n = *P.fb_count;
-  for (unsigned k = blockIdx.x; k < n; k += gridDim.x) fallback_group<UNI, LEARN>(P, (int)P.fb_list[k], sm, ex);
+  for (unsigned k = blockIdx.x; k < n; k += gridDim.x)
+    fallback_group<UNI, LEARN, TS>(P, (int)P.fb_list[k], sm, ex);
   fallback_launch_done(P.fb_count, P.fb_done);
 }
 
@@ -413,6 +414,17 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
     return hipGetLastError();
   };
   auto launch_fb = [&]() {
+    // whole pencils of at most kFbTSHalf + 1 cells: half-size workgroups
+    if (whole_pencils && pencil_fits<kFbTSHalf>(S)) {
+      const dim3 g(kFbBlocksWhole * (kFbTS / kFbTSHalf)), b(kFbTSHalf);
+      switch (variant) {
+        case 0: hipLaunchKernelGGL((ppm_fallback_kernel<0, 0, kFbTSHalf>), g, b, 0, D->stream, S); break;
+        case 1: hipLaunchKernelGGL((ppm_fallback_kernel<1, 0, kFbTSHalf>), g, b, 0, D->stream, S); break;
+        case 2: hipLaunchKernelGGL((ppm_fallback_kernel<0, 1, kFbTSHalf>), g, b, 0, D->stream, S); break;
+        default: hipLaunchKernelGGL((ppm_fallback_kernel<1, 1, kFbTSHalf>), g, b, 0, D->stream, S); break;
+      }
+      return hipGetLastError();
+    }
     const dim3 g(whole_pencils ? kFbBlocksWhole : kFbBlocks), b(kFbTS);
     switch (variant) {
       case 0: hipLaunchKernelGGL((ppm_fallback_kernel<0, 0>), g, b, 0, D->stream, S); break;

@@ -49,6 +49,9 @@ constexpr int kFbBlocks = 64;
 // RiemannSolver HLL / HLLC send every group there: one workgroup per CU
 // (a workgroup's LDS fills most of a CU's 160 KB)
 constexpr int kFbBlocksWhole = 256;
+// and pencils of at most kFbTSHalf + 1 cells take half-size workgroups, two
+// per CU
+constexpr int kFbTSHalf = 512;
 using FbCfg = Cfg<1, kFbTS>;
 
 // Work-item 0 of a sweep workgroup whose tile saw a failing cell or an
@@ -62,7 +65,7 @@ __device__ inline void flag_group(const SweepParams &P, int grp) {
 
 // Every pencil of group `grp` (P.tp pencils along t1) redone whole with the
 // per-cell fallback, then the group flag cleared.
-template <int UNI, int LEARN = 0>
+template <int UNI, int LEARN = 0, int TS = kFbTS>
 __device__ inline void fallback_group(const SweepParams &P, int grp, double *sm, DevExec &ex) {
   const int tb = grp % P.nblk_t1, t2 = P.t2_0 + grp / P.nblk_t1;
   for (int p = 0; p < P.tp && tb * P.tp + p < P.n_t1; p++) {
@@ -70,7 +73,7 @@ __device__ inline void fallback_group(const SweepParams &P, int grp, double *sm,
       if (threadIdx.x == 0) atomicAdd(&P.counters[3], 1ull);
       break;
     }
-    TileCounts tc = sweep_pencil_fb<kFbTS, UNI, DevExec, LEARN>(P, tb * P.tp + p, t2, sm, ex);
+    TileCounts tc = sweep_pencil_fb<TS, UNI, DevExec, LEARN>(P, tb * P.tp + p, t2, sm, ex);
     if (threadIdx.x == 0) {
       if (tc.nfb) atomicAdd(&P.counters[0], (unsigned long long)tc.nfb);
       if (tc.err) atomicAdd(&P.counters[2], (unsigned long long)tc.err);

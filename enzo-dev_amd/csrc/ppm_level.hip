@@ -74,21 +74,21 @@ void level_sweep_kernel(const SweepParams *__restrict__ params, const Work *__re
 
 // fallback over the level's list: entries are level-wide group indices
 // (the grid's first group + the group), grp_grid maps them to their grid
-template <int LEARN>
-__global__ __launch_bounds__(kFbTS) void level_fallback_kernel(const SweepParams *__restrict__ params,
+template <int LEARN, int TS = kFbTS>
+__global__ __launch_bounds__(TS) void level_fallback_kernel(const SweepParams *__restrict__ params,
                                                               const unsigned *list,
                                                               const int *grp_grid,
                                                               const unsigned *grp_off,
                                                               unsigned *count, unsigned *done) {
-  __shared__ double sm[NARR_FB * FbCfg::LW];
+  __shared__ double sm[NARR_FB * Cfg<1, TS>::LW];
   DevExec ex;
   const unsigned n = *count;
   for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
     const unsigned e = list[k];
     const int g = grp_grid[e];
     const SweepParams &P = params[g];
-    if (P.uni[0]) fallback_group<1, LEARN>(P, (int)(e - grp_off[g]), sm, ex);
-    else fallback_group<0, LEARN>(P, (int)(e - grp_off[g]), sm, ex);
+    if (P.uni[0]) fallback_group<1, LEARN, TS>(P, (int)(e - grp_off[g]), sm, ex);
+    else fallback_group<0, LEARN, TS>(P, (int)(e - grp_off[g]), sm, ex);
   }
   fallback_launch_done(count, done);
 }
@@ -272,6 +272,7 @@ struct hipPPMLevel {
   int whole = 0;
   unsigned *dfball = nullptr;
   std::vector<long long> fball_first[3];
+  int fb_half[3] = {0, 0, 0};       // every grid's pencils along the axis fit kFbTSHalf
   Work *dpwork = nullptr;           // pressure chunks
   long long npwork = 0;
   // work lists: per axis, per launch: (shape, uni, first item, count, slice)
@@ -635,6 +636,11 @@ int build_plan(hipPPMLevel *L) {
     L->launch_slices[d].push_back((long long)L->launches[d].size());
     L->fball_first[d].push_back((long long)fball.size());
   }
+  for (int d = 0; d < 3; d++) {
+    L->fb_half[d] = 1;
+    for (int g = 0; g < G; g++)
+      if (!pencil_fits<kFbTSHalf>(L->hparams[((size_t)0 * 3 + d) * G + g])) L->fb_half[d] = 0;
+  }
   // host check of every item against its grid's tiles
   for (const Work &w : work) {
     if (w.grid < 0 || w.grid >= G) return ENZO_PPM_ERR_ARGUMENT;
@@ -693,7 +699,14 @@ int launch_level_sweep(hipPPMLevel *L, int d) {
       hipLaunchKernelGGL(level_fb_fill_kernel, dim3((unsigned)std::min<long long>((n + 255) / 256, 1024)),
                          dim3(256), 0, L->stream, L->dfball + f0, (unsigned)n, L->fb_list, L->fb_state);
       LCHK(hipGetLastError());
-      if (L->learned)
+      const dim3 gh(kFbBlocksWhole * (kFbTS / kFbTSHalf)), bh(kFbTSHalf);
+      if (L->fb_half[d] && L->learned)
+        hipLaunchKernelGGL((level_fallback_kernel<1, kFbTSHalf>), gh, bh, 0, L->stream, P,
+                           L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
+      else if (L->fb_half[d])
+        hipLaunchKernelGGL((level_fallback_kernel<0, kFbTSHalf>), gh, bh, 0, L->stream, P,
+                           L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
+      else if (L->learned)
         hipLaunchKernelGGL(level_fallback_kernel<1>, dim3(kFbBlocksWhole), dim3(kFbTS), 0, L->stream, P,
                            L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
       else
