@@ -414,16 +414,25 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
     return hipGetLastError();
   };
   auto launch_fb = [&]() {
-    // whole pencils of at most kFbTSHalf + 1 cells: half-size workgroups
-    if (whole_pencils && pencil_fits<kFbTSHalf>(S)) {
-      const dim3 g(kFbBlocksWhole * (kFbTS / kFbTSHalf)), b(kFbTSHalf);
-      switch (variant) {
-        case 0: hipLaunchKernelGGL((ppm_fallback_kernel<0, 0, kFbTSHalf>), g, b, 0, D->stream, S); break;
-        case 1: hipLaunchKernelGGL((ppm_fallback_kernel<1, 0, kFbTSHalf>), g, b, 0, D->stream, S); break;
-        case 2: hipLaunchKernelGGL((ppm_fallback_kernel<0, 1, kFbTSHalf>), g, b, 0, D->stream, S); break;
-        default: hipLaunchKernelGGL((ppm_fallback_kernel<1, 1, kFbTSHalf>), g, b, 0, D->stream, S); break;
-      }
-      return hipGetLastError();
+    if (whole_pencils) {
+      const int ts = whole_pencil_width([&](int w) {
+        return w == 128 ? pencil_fits<128>(S) : w == 256 ? pencil_fits<256>(S) : pencil_fits<512>(S);
+      });
+#define FB_WHOLE(TS)                                                                       \
+  if (ts == TS) {                                                                          \
+    const dim3 g(kFbBlocksWhole * (kFbTS / TS)), b(TS);                                     \
+    switch (variant) {                                                                     \
+      case 0: hipLaunchKernelGGL((ppm_fallback_kernel<0, 0, TS>), g, b, 0, D->stream, S); break; \
+      case 1: hipLaunchKernelGGL((ppm_fallback_kernel<1, 0, TS>), g, b, 0, D->stream, S); break; \
+      case 2: hipLaunchKernelGGL((ppm_fallback_kernel<0, 1, TS>), g, b, 0, D->stream, S); break; \
+      default: hipLaunchKernelGGL((ppm_fallback_kernel<1, 1, TS>), g, b, 0, D->stream, S); break; \
+    }                                                                                      \
+    return hipGetLastError();                                                              \
+  }
+      FB_WHOLE(128)
+      FB_WHOLE(256)
+      FB_WHOLE(512)
+#undef FB_WHOLE
     }
     const dim3 g(whole_pencils ? kFbBlocksWhole : kFbBlocks), b(kFbTS);
     switch (variant) {

@@ -49,9 +49,14 @@ constexpr int kFbBlocks = 64;
 // RiemannSolver HLL / HLLC send every group there: one workgroup per CU
 // (a workgroup's LDS fills most of a CU's 160 KB)
 constexpr int kFbBlocksWhole = 256;
-// and pencils of at most kFbTSHalf + 1 cells take half-size workgroups, two
-// per CU
-constexpr int kFbTSHalf = 512;
+// and take the narrowest workgroup of 128, 256, 512 or 1024 work-items that
+// holds the pencil (kFbBlocksWhole * kFbTS / TS workgroups)
+template <class FITS>
+inline int whole_pencil_width(FITS fits) {
+  for (int ts = 128; ts < kFbTS; ts *= 2)
+    if (fits(ts)) return ts;
+  return kFbTS;
+}
 using FbCfg = Cfg<1, kFbTS>;
 
 // Work-item 0 of a sweep workgroup whose tile saw a failing cell or an

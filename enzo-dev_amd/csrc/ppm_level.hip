@@ -272,7 +272,7 @@ struct hipPPMLevel {
   int whole = 0;
   unsigned *dfball = nullptr;
   std::vector<long long> fball_first[3];
-  int fb_half[3] = {0, 0, 0};       // every grid's pencils along the axis fit kFbTSHalf
+  int fb_ts[3] = {kFbTS, kFbTS, kFbTS};  // workgroup width holding every grid's pencils
   Work *dpwork = nullptr;           // pressure chunks
   long long npwork = 0;
   // work lists: per axis, per launch: (shape, uni, first item, count, slice)
@@ -636,11 +636,15 @@ int build_plan(hipPPMLevel *L) {
     L->launch_slices[d].push_back((long long)L->launches[d].size());
     L->fball_first[d].push_back((long long)fball.size());
   }
-  for (int d = 0; d < 3; d++) {
-    L->fb_half[d] = 1;
-    for (int g = 0; g < G; g++)
-      if (!pencil_fits<kFbTSHalf>(L->hparams[((size_t)0 * 3 + d) * G + g])) L->fb_half[d] = 0;
-  }
+  for (int d = 0; d < 3; d++)
+    L->fb_ts[d] = whole_pencil_width([&](int w) {
+      for (int g = 0; g < G; g++) {
+        const SweepParams &S = L->hparams[((size_t)0 * 3 + d) * G + g];
+        if (!(w == 128 ? pencil_fits<128>(S) : w == 256 ? pencil_fits<256>(S) : pencil_fits<512>(S)))
+          return false;
+      }
+      return true;
+    });
   // host check of every item against its grid's tiles
   for (const Work &w : work) {
     if (w.grid < 0 || w.grid >= G) return ENZO_PPM_ERR_ARGUMENT;
@@ -699,13 +703,21 @@ int launch_level_sweep(hipPPMLevel *L, int d) {
       hipLaunchKernelGGL(level_fb_fill_kernel, dim3((unsigned)std::min<long long>((n + 255) / 256, 1024)),
                          dim3(256), 0, L->stream, L->dfball + f0, (unsigned)n, L->fb_list, L->fb_state);
       LCHK(hipGetLastError());
-      const dim3 gh(kFbBlocksWhole * (kFbTS / kFbTSHalf)), bh(kFbTSHalf);
-      if (L->fb_half[d] && L->learned)
-        hipLaunchKernelGGL((level_fallback_kernel<1, kFbTSHalf>), gh, bh, 0, L->stream, P,
-                           L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
-      else if (L->fb_half[d])
-        hipLaunchKernelGGL((level_fallback_kernel<0, kFbTSHalf>), gh, bh, 0, L->stream, P,
-                           L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
+      const int ts = L->fb_ts[d];
+      const dim3 gh(kFbBlocksWhole * (kFbTS / ts)), bh(ts);
+#define LEVEL_FB_WHOLE(TS)                                                                  \
+  if (ts == TS) {                                                                           \
+    if (L->learned)                                                                         \
+      hipLaunchKernelGGL((level_fallback_kernel<1, TS>), gh, bh, 0, L->stream, P, L->fb_list, \
+                         L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);           \
+    else                                                                                    \
+      hipLaunchKernelGGL((level_fallback_kernel<0, TS>), gh, bh, 0, L->stream, P, L->fb_list, \
+                         L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);           \
+  }
+      LEVEL_FB_WHOLE(128)
+      else LEVEL_FB_WHOLE(256)
+      else LEVEL_FB_WHOLE(512)
+#undef LEVEL_FB_WHOLE
       else if (L->learned)
         hipLaunchKernelGGL(level_fallback_kernel<1>, dim3(kFbBlocksWhole), dim3(kFbTS), 0, L->stream, P,
                            L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
