@@ -92,6 +92,36 @@ def cpu_baseline(n, threads):
                        f"{threads} threads, {t1 - t0:.2f} s")
 
 
+def level_solver_rates(lib, ngrids, steps, cfl=0.4):
+    """The same AMR level batched with RiemannSolver HLL and HLLC (whole
+    pencils in one-pencil workgroups sized to the level's pencils): steps
+    of the level's Courant dt after two untimed ones."""
+    import numpy as np
+    from enzo_ppm import problems
+    from enzo_ppm.level import Level
+    grids, glob, _ = problems.amr_level(ngrids, seed=7, nmin=8, nmax=64, ncolours=0)
+    cells = int(sum(int(np.prod(g.active)) for g in grids))
+    out = {}
+    for name, rs in (("hll", 1), ("hllc", 4)):
+        lv = Level([g.copy() for g in grids], dict(glob, RiemannSolver=rs), lib=lib)
+        try:
+            for c in range(2):
+                lv.step(c, 0.0, cfl)
+            if lv.status()[0] != 1:
+                raise RuntimeError("level step failed")
+            t0 = time.perf_counter()
+            for c in range(2, 2 + steps):
+                lv.step(c, 0.0, cfl)
+            st, _, _ = lv.status()
+            t1 = time.perf_counter()
+            if st != 1:
+                raise RuntimeError(f"level step status {st}")
+            out[name] = {"value": cells * steps / (t1 - t0), "ms_per_step": (t1 - t0) * 1e3 / steps}
+        finally:
+            lv.close()
+    return out
+
+
 def level_bench(lib, ngrids, steps, cfl=0.4, weights=None):
     """The AMR-level workload: `ngrids` grids, each axis log-uniform in
     [8, 64] active cells (seeded), smooth sinusoidal density and velocity with
@@ -475,6 +505,12 @@ def main():
             out["level"] = level_bench(lib, a.level_grids, a.level_steps, weights=weights)
         except Exception as e:  # reported, not fatal
             out["level"] = {"error": str(e)}
+        if a.solver_steps > 0 and "error" not in out["level"]:
+            try:
+                out["level"]["other_solvers"] = level_solver_rates(lib, a.level_grids,
+                                                                   a.level_steps)
+            except Exception as e:  # reported, not fatal
+                out["level"]["other_solvers"] = {"error": str(e)}
     if a.cosmo_n > 0:
         try:
             out["cosmological"] = cosmo_bench(lib, a.cosmo_n, a.cosmo_steps, weights)
