@@ -36,6 +36,7 @@ template <int UNI, int LEARN>
 __global__ __launch_bounds__(kTP *kTS) __attribute__((amdgpu_waves_per_eu(4)))
 void ppm_sweep_kernel(SweepParams P) {
   __shared__ double sm[SweepCfg::SMEM];
+  PPM_LDS_ENTER(sm);
   DevExec ex;
   const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   TileCounts tc = sweep_tile<kTP, kTS, UNI, DevExec, LEARN>(P, tile, sm, ex);
@@ -57,6 +58,7 @@ void ppm_sweep_kernel(SweepParams P) {
 template <int UNI, int LEARN, int TS = kFbTS>
 __global__ __launch_bounds__(TS) void ppm_fallback_kernel(SweepParams P) {
   __shared__ double sm[NARR_FB * Cfg<1, TS>::LW];
+  PPM_LDS_ENTER(sm);
   DevExec ex;
   const unsigned n = *P.fb_count;
   for (unsigned k = blockIdx.x; k < n; k += gridDim.x)
@@ -413,35 +415,23 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
                        ngroups);
     return hipGetLastError();
   };
-  auto launch_fb = [&]() {
-    if (whole_pencils) {
-      const int ts = whole_pencil_width([&](int w) {
-        return w == 128 ? pencil_fits<128>(S) : w == 256 ? pencil_fits<256>(S) : pencil_fits<512>(S);
-      });
-#define FB_WHOLE(TS)                                                                       \
-  if (ts == TS) {                                                                          \
-    const dim3 g(kFbBlocksWhole * (kFbTS / TS)), b(TS);                                     \
-    switch (variant) {                                                                     \
-      case 0: hipLaunchKernelGGL((ppm_fallback_kernel<0, 0, TS>), g, b, 0, D->stream, S); break; \
-      case 1: hipLaunchKernelGGL((ppm_fallback_kernel<1, 0, TS>), g, b, 0, D->stream, S); break; \
-      case 2: hipLaunchKernelGGL((ppm_fallback_kernel<0, 1, TS>), g, b, 0, D->stream, S); break; \
-      default: hipLaunchKernelGGL((ppm_fallback_kernel<1, 1, TS>), g, b, 0, D->stream, S); break; \
-    }                                                                                      \
-    return hipGetLastError();                                                              \
-  }
-      FB_WHOLE(128)
-      FB_WHOLE(256)
-      FB_WHOLE(512)
-#undef FB_WHOLE
-    }
-    const dim3 g(whole_pencils ? kFbBlocksWhole : kFbBlocks), b(kFbTS);
-    switch (variant) {
-      case 0: hipLaunchKernelGGL((ppm_fallback_kernel<0, 0>), g, b, 0, D->stream, S); break;
-      case 1: hipLaunchKernelGGL((ppm_fallback_kernel<1, 0>), g, b, 0, D->stream, S); break;
-      case 2: hipLaunchKernelGGL((ppm_fallback_kernel<0, 1>), g, b, 0, D->stream, S); break;
-      default: hipLaunchKernelGGL((ppm_fallback_kernel<1, 1>), g, b, 0, D->stream, S); break;
-    }
-    return hipGetLastError();
+  // one-pencil workgroups: the narrowest width that holds the pencils
+  // (fb_ok above: the widest does)
+  const int fbts = S.fb_ok ? fb_width_of(S) : kFbTS;
+  // ngroups: pencil groups of the launch, the most that can be on the list
+  // (any number of workgroups is correct: each takes every gridDim-th entry)
+  auto launch_fb = [&](unsigned ngroups) {
+    const unsigned nblk = std::max(1u, std::min(ngroups, fb_blocks(fbts, (whole_pencils ? kFbBlocksWhole : kFbBlocks) * kFbTS)));
+    return with_fb_width(fbts, [&](auto W) {
+      constexpr int TS = decltype(W)::value;
+      const dim3 g(nblk), b(TS);
+      switch (variant) {
+        case 0: hipLaunchKernelGGL((ppm_fallback_kernel<0, 0, TS>), g, b, 0, D->stream, S); break;
+        case 1: hipLaunchKernelGGL((ppm_fallback_kernel<1, 0, TS>), g, b, 0, D->stream, S); break;
+        case 2: hipLaunchKernelGGL((ppm_fallback_kernel<0, 1, TS>), g, b, 0, D->stream, S); break;
+        default: hipLaunchKernelGGL((ppm_fallback_kernel<1, 1, TS>), g, b, 0, D->stream, S); break;
+      }
+    });
   };
   if (S.idiff == 0) {
     // all pencils at once: t1 is the contiguous transverse axis for y/z
@@ -470,9 +460,9 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
       HIPCHK(hipEventRecord(e0, D->stream));
     }
     HIPCHK(launch_main_or_all((unsigned)nblk));
-    if (whole_pencils) HIPCHK(launch_fb());
+    if (whole_pencils) HIPCHK(launch_fb((unsigned)(nblk / S.nseg)));
     if (D->profile) HIPCHK(hipEventRecord(e1, D->stream));
-    if (!whole_pencils) HIPCHK(launch_fb());
+    if (!whole_pencils) HIPCHK(launch_fb((unsigned)(nblk / S.nseg)));
   } else {
     // diffusion reads the live transverse velocities of the previous slice
     // (calcdiss.F:207-288 on BaryonField): slices in order, one launch each
@@ -486,7 +476,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
       S.t2_0 = k;
       long long nblk = (long long)S.nseg * S.nblk_t1;
       HIPCHK(launch_main_or_all((unsigned)nblk));
-      HIPCHK(launch_fb());  // the next slice reads this one's results
+      HIPCHK(launch_fb((unsigned)S.nblk_t1));  // the next slice reads this one's results
     }
   }
   std::swap(D->A, D->B);

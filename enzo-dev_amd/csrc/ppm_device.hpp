@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -43,20 +44,47 @@ struct DevExec {
   }
 };
 
-// One-pencil fallback workgroups: kFbTS threads, the whole pencil in LDS.
+// One-pencil fallback workgroups: the whole pencil in LDS, one work-item per
+// interface, in workgroups of 128, 256, 512 or kFbTS = 1024 work-items (the
+// narrowest that holds every pencil of the launch: fewer registers spilled
+// and more workgroups per CU than the widest).
 constexpr int kFbTS = 1024;
+constexpr int kFbWidths[4] = {128, 256, 512, 1024};
+static_assert(kFbWidths[3] == kFbTS, "the widest one-pencil workgroup is kFbTS");
+// The per-cell fallback launches kFbBlocks * kFbTS work-items in all (with
+// no flagged group each workgroup only reads the count); RiemannSolver HLL /
+// HLLC send every group there, kFbBlocksWhole * kFbTS work-items: a CU's
+// worth of LDS per 1024 work-items.
 constexpr int kFbBlocks = 64;
-// RiemannSolver HLL / HLLC send every group there: one workgroup per CU
-// (a workgroup's LDS fills most of a CU's 160 KB)
 constexpr int kFbBlocksWhole = 256;
-// and take the narrowest workgroup of 128, 256, 512 or 1024 work-items that
-// holds the pencil (kFbBlocksWhole * kFbTS / TS workgroups)
-template <class FITS>
-inline int whole_pencil_width(FITS fits) {
-  for (int ts = 128; ts < kFbTS; ts *= 2)
-    if (fits(ts)) return ts;
-  return kFbTS;
+
+// narrowest one-pencil width whose tile holds the sweep axis of S (0: none)
+PPM_HD bool pencil_fits_width(const SweepParams &S, int ts) {
+  return ts == 128 ? pencil_fits<128>(S) : ts == 256 ? pencil_fits<256>(S)
+       : ts == 512 ? pencil_fits<512>(S) : ts == 1024 ? pencil_fits<1024>(S) : false;
 }
+inline int fb_width_of(const SweepParams &S) {
+  for (int ts : kFbWidths)
+    if (pencil_fits_width(S, ts)) return ts;
+  return 0;
+}
+
+// f(std::integral_constant<int, TS>()) for the one-pencil width ts, which
+// must be one of kFbWidths (each is instantiated here, so a width that is
+// not in the list cannot reach a narrower kernel)
+template <class F>
+inline hipError_t with_fb_width(int ts, F f) {
+  switch (ts) {
+    case 128: f(std::integral_constant<int, 128>()); break;
+    case 256: f(std::integral_constant<int, 256>()); break;
+    case 512: f(std::integral_constant<int, 512>()); break;
+    case 1024: f(std::integral_constant<int, 1024>()); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+// workgroups of width ts for a launch of `total` work-items
+inline unsigned fb_blocks(int ts, int total) { return (unsigned)(total / ts); }
 using FbCfg = Cfg<1, kFbTS>;
 
 // Work-item 0 of a sweep workgroup whose tile saw a failing cell or an

@@ -62,6 +62,7 @@ template <int TP, int TS, int UNI, int LEARN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void level_sweep_kernel(const SweepParams *__restrict__ params, const Work *__restrict__ work) {
   __shared__ double sm[Cfg<TP, TS>::SMEM];
+  PPM_LDS_ENTER(sm);
   DevExec ex;
   const Work w = work[xcd_tile((int)blockIdx.x, (int)gridDim.x)];
   const SweepParams &P = params[w.grid];
@@ -81,6 +82,7 @@ __global__ __launch_bounds__(TS) void level_fallback_kernel(const SweepParams *_
                                                               const unsigned *grp_off,
                                                               unsigned *count, unsigned *done) {
   __shared__ double sm[NARR_FB * Cfg<1, TS>::LW];
+  PPM_LDS_ENTER(sm);
   DevExec ex;
   const unsigned n = *count;
   for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
@@ -637,14 +639,13 @@ int build_plan(hipPPMLevel *L) {
     L->fball_first[d].push_back((long long)fball.size());
   }
   for (int d = 0; d < 3; d++)
-    L->fb_ts[d] = whole_pencil_width([&](int w) {
-      for (int g = 0; g < G; g++) {
-        const SweepParams &S = L->hparams[((size_t)0 * 3 + d) * G + g];
-        if (!(w == 128 ? pencil_fits<128>(S) : w == 256 ? pencil_fits<256>(S) : pencil_fits<512>(S)))
-          return false;
-      }
-      return true;
-    });
+    for (int ts : kFbWidths) {  // the narrowest one-pencil width holding every grid's pencils
+      L->fb_ts[d] = ts;
+      bool all = true;
+      for (int g = 0; g < G && all; g++)
+        all = pencil_fits_width(L->hparams[((size_t)0 * 3 + d) * G + g], ts);
+      if (all) break;
+    }
   // host check of every item against its grid's tiles
   for (const Work &w : work) {
     if (w.grid < 0 || w.grid >= G) return ENZO_PPM_ERR_ARGUMENT;
@@ -687,6 +688,24 @@ int build_plan(hipPPMLevel *L) {
   return ENZO_PPM_SUCCESS;
 }
 
+// the level's fallback launch in one-pencil workgroups of width ts,
+// blocks1024 * kFbTS work-items in all
+// and at most `ngroups` workgroups (the most entries the list can hold)
+hipError_t launch_level_fallback(hipPPMLevel *L, const SweepParams *P, int ts, int blocks1024,
+                                 long long ngroups) {
+  const unsigned nblk =
+      (unsigned)std::max(1LL, std::min(ngroups, (long long)fb_blocks(ts, blocks1024 * kFbTS)));
+  return with_fb_width(ts, [&](auto W) {
+    constexpr int TS = decltype(W)::value;
+    if (L->learned)
+      hipLaunchKernelGGL((level_fallback_kernel<1, TS>), dim3(nblk), dim3(TS), 0, L->stream, P,
+                         L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
+    else
+      hipLaunchKernelGGL((level_fallback_kernel<0, TS>), dim3(nblk), dim3(TS), 0, L->stream, P,
+                         L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
+  });
+}
+
 int launch_level_sweep(hipPPMLevel *L, int d) {
   const int G = L->ngrids;
   const SweepParams *P = L->dparams + ((size_t)L->parity * 3 + d) * G;
@@ -703,28 +722,7 @@ int launch_level_sweep(hipPPMLevel *L, int d) {
       hipLaunchKernelGGL(level_fb_fill_kernel, dim3((unsigned)std::min<long long>((n + 255) / 256, 1024)),
                          dim3(256), 0, L->stream, L->dfball + f0, (unsigned)n, L->fb_list, L->fb_state);
       LCHK(hipGetLastError());
-      const int ts = L->fb_ts[d];
-      const dim3 gh(kFbBlocksWhole * (kFbTS / ts)), bh(ts);
-#define LEVEL_FB_WHOLE(TS)                                                                  \
-  if (ts == TS) {                                                                           \
-    if (L->learned)                                                                         \
-      hipLaunchKernelGGL((level_fallback_kernel<1, TS>), gh, bh, 0, L->stream, P, L->fb_list, \
-                         L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);           \
-    else                                                                                    \
-      hipLaunchKernelGGL((level_fallback_kernel<0, TS>), gh, bh, 0, L->stream, P, L->fb_list, \
-                         L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);           \
-  }
-      LEVEL_FB_WHOLE(128)
-      else LEVEL_FB_WHOLE(256)
-      else LEVEL_FB_WHOLE(512)
-#undef LEVEL_FB_WHOLE
-      else if (L->learned)
-        hipLaunchKernelGGL(level_fallback_kernel<1>, dim3(kFbBlocksWhole), dim3(kFbTS), 0, L->stream, P,
-                           L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
-      else
-        hipLaunchKernelGGL(level_fallback_kernel<0>, dim3(kFbBlocksWhole), dim3(kFbTS), 0, L->stream, P,
-                           L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
-      LCHK(hipGetLastError());
+      LCHK(launch_level_fallback(L, P, L->fb_ts[d], kFbBlocksWhole, n));
       continue;
     }
     for (long long li = sl[k]; li < sl[k + 1]; li++) {
@@ -751,13 +749,7 @@ int launch_level_sweep(hipPPMLevel *L, int d) {
 #undef LEVEL_LAUNCH
       LCHK(hipGetLastError());
     }
-    if (L->learned)
-      hipLaunchKernelGGL(level_fallback_kernel<1>, dim3(kFbBlocks), dim3(kFbTS), 0, L->stream, P,
-                         L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
-    else
-      hipLaunchKernelGGL(level_fallback_kernel<0>, dim3(kFbBlocks), dim3(kFbTS), 0, L->stream, P,
-                         L->fb_list, L->grp_grid, L->dgrp_off, L->fb_state, L->fb_state + 1);
-    LCHK(hipGetLastError());
+    LCHK(launch_level_fallback(L, P, L->fb_ts[d], kFbBlocks, L->ngroups));
   }
   L->parity ^= 1;
   return ENZO_PPM_SUCCESS;

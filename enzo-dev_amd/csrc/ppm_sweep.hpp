@@ -200,18 +200,46 @@ struct Regs {
   double c_sl, c_sr, c_ga, c_gb, c_ges;
 };
 
+// Tile LDS accessors.  A checking build (PPM_LDS_TRACK, defined by the CPU
+// execution of the HIP source in tests/emu) hands out a reference that
+// reports every read and write to ppm_lds_note_read / ppm_lds_note_write,
+// so that a read of a cell the workgroup has not written is caught; a kernel
+// announces its tile array with PPM_LDS_ENTER (nothing in the HIP build).
+#ifdef PPM_LDS_TRACK
+struct LdsRef {
+  double *q;
+  operator double() const {
+    ppm_lds_note_read(q);
+    return *q;
+  }
+  LdsRef &operator=(double v) {
+    ppm_lds_note_write(q);
+    *q = v;
+    return *this;
+  }
+  LdsRef &operator=(const LdsRef &o) { return *this = (double)o; }
+};
+#define PPM_LDS_REF LdsRef
+#define PPM_LDS_AT(...) LdsRef{&(__VA_ARGS__)}
+#define PPM_LDS_ENTER(arr) ppm_lds_enter((arr), sizeof(arr))
+#else
+#define PPM_LDS_ENTER(arr) ((void)0)
+#define PPM_LDS_REF double &
+#define PPM_LDS_AT(...) (__VA_ARGS__)
+#endif
+
 template <int TP, int TS, int UNI = 0>
-PPM_HD double &LDS(double *sm, int arr, int p, int c) {
+PPM_HD PPM_LDS_REF LDS(double *sm, int arr, int p, int c) {
   PPM_ASSERT(arr >= 0 && arr < NARR && p >= 0 && p < TP && c >= 0 && c < Cfg<TP, TS, UNI>::LW);
-  return sm[(arr * TP + p) * Cfg<TP, TS, UNI>::LW + c];
+  return PPM_LDS_AT(sm[(arr * TP + p) * Cfg<TP, TS, UNI>::LW + c]);
 }
 
 // the fallback arrays A_HA / A_HB (one-pencil tiles only)
 template <int TP, int TS, int UNI = 0>
-PPM_HD double &LDSH(double *sm, int arr, int c) {
+PPM_HD PPM_LDS_REF LDSH(double *sm, int arr, int c) {
   static_assert(TP == 1, "fallback arrays exist in one-pencil tiles only");
   PPM_ASSERT((arr == A_HA || arr == A_HB) && c >= 0 && c < Cfg<TP, TS, UNI>::LW);
-  return sm[arr * Cfg<TP, TS, UNI>::LW + c];
+  return PPM_LDS_AT(sm[arr * Cfg<TP, TS, UNI>::LW + c]);
 }
 
 PPM_HD double dtval(const SweepParams &P) { return P.dtp ? *P.dtp : P.dt; }
@@ -368,7 +396,7 @@ PPM_HD double flat_of(const SweepParams &P, const Tile &T, int p, int cell);
 // the thread's parabola cell (same mapping as ph_parab: one item per thread)
 template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_flat_reg(const SweepParams &P, const Tile &T, int tid, Regs &r) {
-  const int n = T.NI + 2;
+  const int n = T.NI + 1;
   r.flat_c = 0.0;
   const int p = tid / TS, k = tid % TS;
   if (P.iflatten == 0 || p >= T.np || k >= n) return;
@@ -402,7 +430,7 @@ PPM_HD void ph_load(const SweepParams &P, const Tile &T, double *sm, int tid) {
 // from (u, cs) where they are used (char_* below), bit for bit.
 template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_chars(const SweepParams &P, const Tile &T, double *sm, int tid) {
-  const int nch = T.NI + 2;
+  const int nch = T.NI + 1;
   for (int p = tid / TS, k = tid % TS; p < T.np && k < nch; k += TS) {
     int cell = T.IA - 1 + k;
     int c = cell - T.L0;
@@ -412,7 +440,7 @@ PPM_HD void ph_chars(const SweepParams &P, const Tile &T, double *sm, int tid) {
     LDS<TP, TS, UNI>(sm, A_CS, p, c) = cs;
   }
   if (P.isteep) {
-    const int n2 = T.NI + 4;
+    const int n2 = T.NI + 3;
     for (int p = tid / TS, k = tid % TS; p < T.np && k < n2; k += TS) {
       int cell = T.IA - 2 + k;
       int c = cell - T.L0;
@@ -429,7 +457,7 @@ PPM_HD void ph_chars(const SweepParams &P, const Tile &T, double *sm, int tid) {
 template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_steepen(const SweepParams &P, const Tile &T, double *sm, int tid) {
   if (!P.isteep) return;
-  const int n = T.NI + 2;
+  const int n = T.NI + 1;
   for (int p = tid / TS, k = tid % TS; p < T.np && k < n; k += TS) {
     int cell = T.IA - 1 + k;
     int c = cell - T.L0;
@@ -452,7 +480,7 @@ PPM_HD void ph_steepen(const SweepParams &P, const Tile &T, double *sm, int tid)
 // cells [IA-2, IB+1], into A_DQ.
 template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_slope(const SweepParams &P, const Tile &T, double *sm, int tid, int arr) {
-  const int n = T.NI + 4;
+  const int n = T.NI + 3;
   for (int p = tid / TS, k = tid % TS; p < T.np && k < n; k += TS) {
     int cell = T.IA - 2 + k;
     int c = cell - T.L0;
@@ -485,7 +513,7 @@ PPM_HD double flat_of(const SweepParams &P, const Tile &T, int p, int cell) {
 template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_parab(const SweepParams &P, const Tile &T, double *sm, int tid, int arr,
                      int steep, const Regs &rg) {
-  const int n = T.NI + 2;
+  const int n = T.NI + 1;
   PPM_ASSERT(n <= TS);  // one cell per thread, as ph_flat_reg (build_segments)
   for (int p = tid / TS, k = tid % TS; p < T.np && k < n; k += TS) {
     int cell = T.IA - 1 + k;

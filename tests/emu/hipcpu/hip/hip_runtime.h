@@ -10,16 +10,25 @@
 //  * Every hipMalloc buffer is placed against an inaccessible guard page, by
 //    default right after its last byte (HIPCPU_GUARD=low: right before its
 //    first), so an access out of bounds stops the process at once.
+//  * Every __shared__ array is filled with a NaN pattern (0xff bytes) when a
+//    workgroup starts, as LDS holds whatever the previous occupant left on
+//    the GPU; with HIPCPU_LDS_CHECK=1 a read of an LDS cell through the tile
+//    accessors that the current workgroup has not written stops the process.
+//  * HIPCPU_ORDER=rev runs the work-items of every barrier interval in
+//    reverse order, HIPCPU_ORDER=shuf:<seed> in a fresh random order per
+//    interval; results must not depend on it (no ordering inside a phase).
 // Streams are ignored (everything runs in call order) and atomics are plain
 // read-modify-writes (fibers switch only at barriers).  Not part of the
 // product: the product is the HIP build of the same source.
 #pragma once
 
+#include <execinfo.h>
 #include <setjmp.h>
 #include <sys/mman.h>
 #include <ucontext.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -34,7 +43,10 @@
 #define __global__
 #define __device__
 #define __host__
-#define __shared__ static
+// every __shared__ array in one section, so that a workgroup start can poison
+// all of them (the linker defines the section's bounds)
+#define __shared__ static __attribute__((section("hipcpu_lds")))
+extern "C" char __start_hipcpu_lds[], __stop_hipcpu_lds[];
 #define __launch_bounds__(...)
 
 struct dim3 {
@@ -69,6 +81,66 @@ inline std::vector<long long> g_slots;          // values deposited for shuffles
 inline std::function<void()> g_body;
 constexpr size_t kStack = 256 * 1024;
 
+// LDS poisoning and read-before-write tracking.  Static __shared__ locals of
+// non-template functions sit in the section hipcpu_lds (poisoned at every
+// workgroup start); g++ ignores the section attribute of template statics,
+// so the tile kernels announce their arrays with PPM_LDS_ENTER, which
+// poisons an array the first time a work-item of a workgroup reaches it and
+// keeps a per-double record of the workgroup that last wrote it.
+inline int g_lds_check = -1;      // HIPCPU_LDS_CHECK
+inline int g_order = -1;          // HIPCPU_ORDER: 0 forward, 1 reverse, 2 shuffled
+inline unsigned long long g_rng = 0;
+inline uint32_t g_epoch = 0;      // workgroups started
+inline unsigned long long g_lds_reads = 0, g_lds_bad = 0;
+struct LdsRegion {
+  size_t bytes = 0;
+  uint32_t poisoned = 0;          // epoch of the last poisoning
+  std::vector<uint32_t> wrote;    // epoch of the last write, per double
+};
+inline std::map<const char *, LdsRegion> g_regions;
+inline const char *g_last_base = nullptr;
+inline LdsRegion *g_last_region = nullptr;
+
+// read at every launch, so that one process can switch modes between calls
+inline void lds_env() {
+  const char *c = getenv("HIPCPU_LDS_CHECK");
+  g_lds_check = (c && c[0] == '1') ? 1 : 0;
+  const char *o = getenv("HIPCPU_ORDER");
+  g_order = 0;
+  if (o && !strncmp(o, "rev", 3)) g_order = 1;
+  if (o && !strncmp(o, "shuf", 4)) {
+    g_order = 2;
+    const char *col = strchr(o, ':');
+    const unsigned long long seed = col ? strtoull(col + 1, nullptr, 10) * 2654435761ull + 1 : 1;
+    static unsigned long long last_seed = 0;
+    if (seed != last_seed) g_rng = last_seed = seed;
+  }
+}
+
+inline unsigned long long next_rand() {  // splitmix64
+  unsigned long long z = (g_rng += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// the announced region holding p, and p's double index in it
+inline LdsRegion &lds_region(const void *p, size_t &k) {
+  const char *c = (const char *)p;
+  if (!(g_last_region && c >= g_last_base && c < g_last_base + g_last_region->bytes)) {
+    auto it = g_regions.upper_bound(c);
+    if (it == g_regions.begin() || c >= std::prev(it)->first + std::prev(it)->second.bytes) {
+      fprintf(stderr, "hipcpu: tile LDS accessor outside every announced LDS array\n");
+      abort();
+    }
+    --it;
+    g_last_base = it->first;
+    g_last_region = &it->second;
+  }
+  k = (size_t)(c - g_last_base) / 8;
+  return *g_last_region;
+}
+
 inline void fiber_entry() {
   g_body();
   g_fib[g_cur].state = 2;
@@ -98,8 +170,15 @@ inline void run_block(int nthreads) {
     f.started = false;
   }
   g_slots.assign(nthreads, 0);
+  memset(__start_hipcpu_lds, 0xff, (size_t)(__stop_hipcpu_lds - __start_hipcpu_lds));
+  g_epoch++;  // a new workgroup: announced arrays are poisoned again on entry
+  std::vector<int> order(nthreads);
+  for (int t = 0; t < nthreads; t++) order[t] = g_order == 1 ? nthreads - 1 - t : t;
   for (;;) {
-    for (int t = 0; t < nthreads; t++) {
+    if (g_order == 2)
+      for (int t = nthreads - 1; t > 0; t--) std::swap(order[t], order[next_rand() % (t + 1)]);
+    for (int ti = 0; ti < nthreads; ti++) {
+      const int t = order[ti];
       if (g_fib[t].state == 2) continue;
       g_fib[t].state = 0;
       g_cur = t;
@@ -132,6 +211,7 @@ inline std::vector<std::function<void()>> *g_capture = nullptr;
 
 template <class K, class... A>
 void launch_now(K kernel, dim3 grid, dim3 block, A... args) {
+  lds_env();
   g_gdim = grid;
   g_bdim = block;
   const int nt = (int)(block.x * block.y * block.z);
@@ -169,6 +249,48 @@ inline void *guarded_alloc(size_t n) {
 }
 
 }  // namespace hipcpu
+
+// reads of LDS before any write by the workgroup, counted since the library
+// was loaded (HIPCPU_LDS_CHECK=1 with HIPCPU_LDS_CONTINUE set: count, not abort)
+extern "C" __attribute__((weak)) unsigned long long hipcpu_lds_violations() { return hipcpu::g_lds_bad; }
+
+// hooks of the tile accessors (ppm_sweep.hpp, PPM_LDS_TRACK)
+#define PPM_LDS_TRACK 1
+inline void ppm_lds_enter(void *base, size_t bytes) {
+  hipcpu::LdsRegion &r = hipcpu::g_regions[(const char *)base];
+  if (r.bytes == 0) {
+    r.bytes = bytes;
+    r.wrote.assign(bytes / 8 + 1, 0);
+    hipcpu::g_last_region = nullptr;
+  }
+  if (r.poisoned != hipcpu::g_epoch) {  // first work-item of this workgroup here
+    memset(base, 0xff, bytes);
+    r.poisoned = hipcpu::g_epoch;
+  }
+}
+inline void ppm_lds_note_write(const void *p) {
+  if (hipcpu::g_lds_check <= 0) return;
+  size_t k;
+  hipcpu::LdsRegion &r = hipcpu::lds_region(p, k);
+  r.wrote[k] = hipcpu::g_epoch;
+}
+inline void ppm_lds_note_read(const void *p) {
+  if (hipcpu::g_lds_check <= 0) return;
+  hipcpu::g_lds_reads++;
+  size_t k;
+  hipcpu::LdsRegion &r = hipcpu::lds_region(p, k);
+  if (r.wrote[k] != hipcpu::g_epoch) {
+    hipcpu::g_lds_bad++;
+    if (hipcpu::g_lds_bad <= 20)
+      fprintf(stderr, "hipcpu: LDS read before write: double %zu of an array of %zu bytes, block %u, work-item %u\n",
+            k, r.bytes, hipcpu::g_block.x, hipcpu::g_thread.x);
+    if (getenv("HIPCPU_LDS_TRACE")) {  // call sites, for addr2line
+      void *bt[12];
+      backtrace_symbols_fd(bt, backtrace(bt, 12), 2);
+    }
+    if (!getenv("HIPCPU_LDS_CONTINUE")) abort();
+  }
+}
 
 #define threadIdx hipcpu::g_thread
 #define blockIdx hipcpu::g_block

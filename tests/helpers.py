@@ -148,6 +148,13 @@ def level_grids(kind, n=4, seed=7):
             g.fill_periodic()
             out.append(g)
         return out, glob
+    if kind == "long":  # ~200- and ~400-cell pencils: 256- and 512-wide one-pencil workgroups
+        out = []
+        for i, act in enumerate(([200, 4, 3], [5, 400, 2], [6, 3, 5])[:n]):
+            g, glob, _ = problems.random_smooth(act, seed=seed + i, ncolours=1)
+            g.fill_periodic()
+            out.append(g)
+        return out, glob
     if kind == "diffusion":
         out = []
         for i in range(n):

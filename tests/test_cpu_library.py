@@ -411,6 +411,10 @@ HLL_CASES = {
     "gravity": lambda: problems.random_smooth([10, 12, 8], seed=2, ncolours=1, gravity=True),
     "no_dual_energy": lambda: problems.random_smooth([10, 12, 8], seed=3, dual_energy=False),
     "sod_2d": lambda: problems.sod_x(20, rank=2, gamma=1.4, flags=(1, 1, 1)),
+    # pencils of 206 and 406 cells: the 256- and 512-wide one-pencil workgroups
+    "pencil_200_diffusion": lambda: problems.random_smooth([2, 200, 3], seed=21, ncolours=1,
+                                                           flags=(1, 1, 1)),
+    "pencil_400": lambda: problems.random_smooth([400, 3, 2], seed=22, ncolours=1),
 }
 
 
@@ -448,11 +452,70 @@ def test_hllc_fallback(cpulib, case):
         assert all(r[1] for r in field_report(b, a))
 
 
-@pytest.mark.parametrize("kind,solver", [("blast", 4), ("diffusion", 1)],
-                         ids=["blast-hllc", "diffusion-hll"])
+@pytest.mark.parametrize("kind,solver", [("blast", 4), ("diffusion", 1), ("long", 1), ("long", 4)],
+                         ids=["blast-hllc", "diffusion-hll", "long-hll", "long-hllc"])
 def test_level_alternative_solvers(cpulib, kind, solver):
     """RiemannSolver HLL / HLLC on a batched level (every pencil group in
     the one-pencil workgroups, slice by slice with diffusion): fields and
     Courant steps equal to the per-grid oracle loop."""
     grids, glob = level_grids(kind, 3, seed=13)
     level_check(cpulib, grids, dict(glob, RiemannSolver=solver), 2)
+
+
+# Work-item order and LDS initialisation.  The CPU build runs the work-items
+# of a barrier interval one after another; on the GPU they run at once and
+# LDS starts with whatever the previous workgroup left.  Here every __shared__
+# array starts as NaN, a read of an LDS cell the workgroup has not written is
+# counted (HIPCPU_LDS_CHECK), and each case runs with the work-items of every
+# interval in forward, reverse and two shuffled orders: the results must be
+# bitwise the same and no read may precede its write.
+ORDER_CASES = ["1d_ok", "3d_x_hll_fails", "3d_y_ok", "3d_y_diffusion_hll_fails", "3d_z_stretched_ok"]
+
+
+def _run_fields(lib, g, glob, cycle, fluxes=None):
+    b = g.copy()
+    fl = SubgridFluxes(b, fluxes) if fluxes else None
+    st, n = run(lib, b, glob, cycle, fluxes=fl)
+    return st, n, [f.copy() for f in b.fields], (fl.flat().copy() if fl else None)
+
+
+@pytest.mark.parametrize("name", ORDER_CASES)
+def test_work_item_order_and_lds_reads(cpulib, name, monkeypatch):
+    cpulib.hipcpu_lds_violations.restype = C.c_ulonglong
+    monkeypatch.setenv("HIPCPU_LDS_CHECK", "1")
+    monkeypatch.setenv("HIPCPU_LDS_CONTINUE", "1")
+    g, glob, cycle = fallback_case(name, ncolours=2)
+    boxes = [((3, 15, 3), (8, 18, 7)), ((4, 20, 4), (6, 25, 6))] if name.startswith("3d_y") else None
+    before = cpulib.hipcpu_lds_violations()
+    out = {}
+    for order in ("fwd", "rev", "shuf:1", "shuf:5"):
+        monkeypatch.setenv("HIPCPU_ORDER", order)
+        out[order] = _run_fields(cpulib, g, glob, cycle, boxes)
+    assert cpulib.hipcpu_lds_violations() == before, "LDS read before write"
+    st, n, fields, flux = out["fwd"]
+    assert n > 0 and st == FALLBACK_CASES[name][-1]
+    for order, (st2, n2, fields2, flux2) in out.items():
+        assert (st2, n2) == (st, n), order
+        assert all(bit_equal(x, y) for x, y in zip(fields, fields2)), order
+        if flux is not None:
+            assert bit_equal(flux, flux2), order
+
+
+@pytest.mark.parametrize("name", ["diffusion", "turbulent", "ragged"])
+def test_steps_lds_reads(cpulib, name, monkeypatch):
+    """Plain steps (the main tile kernel, slice-serial diffusion launches)
+    with every LDS read checked against the workgroup's own writes and the
+    work-items run in reverse order."""
+    cpulib.hipcpu_lds_violations.restype = C.c_ulonglong
+    monkeypatch.setenv("HIPCPU_LDS_CHECK", "1")
+    monkeypatch.setenv("HIPCPU_LDS_CONTINUE", "1")
+    monkeypatch.setenv("HIPCPU_ORDER", "rev")
+    before = cpulib.hipcpu_lds_violations()
+    g, glob, _ = CASES[name]()
+    fill_bc(g, "periodic")
+    g.dt = oracle.compute_time_step("port", g, glob, 0.4)
+    a, b = g.copy(), g.copy()
+    assert oracle.solve_ppm_de("port", a, glob, cycle=1)[0] == 1
+    assert run(cpulib, b, glob, 1) == (1, 0)
+    assert all(r[1] for r in field_report(b, a))
+    assert cpulib.hipcpu_lds_violations() == before, "LDS read before write"

@@ -170,7 +170,7 @@ def test_level_decomposed_box_equals_whole(lib):
 
 
 @pytest.mark.parametrize("solver", [1, 4], ids=["hll", "hllc"])
-@pytest.mark.parametrize("kind", ["blast", "diffusion", "gravity"])
+@pytest.mark.parametrize("kind", ["blast", "diffusion", "gravity", "long"])
 @pytest.mark.parametrize("graph", [True, False])
 def test_level_alternative_solvers(lib, kind, solver, graph):
     """RiemannSolver HLL / HLLC on a batched level: every pencil group in

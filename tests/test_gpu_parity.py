@@ -166,8 +166,12 @@ def test_device_matches_golden(lib, path):
 
 
 HLL_CASES = ["turbulent_40", "diffusion", "gravity", "no_dual_energy", "sod_2d", "stretched_grid",
-             "longest_hll"]
+             "longest_hll", "pencil_200_diffusion", "pencil_400"]
 CASES["longest_hll"] = lambda: problems.random_smooth([2, 1019, 3], seed=12, ncolours=1)
+# pencils of 206 and 406 cells: the 256- and 512-wide one-pencil workgroups
+CASES["pencil_200_diffusion"] = lambda: problems.random_smooth([2, 200, 3], seed=21, ncolours=1,
+                                                               flags=(1, 1, 1))
+CASES["pencil_400"] = lambda: problems.random_smooth([400, 3, 2], seed=22, ncolours=1)
 
 
 @pytest.mark.parametrize("solver", [1, 4], ids=["hll", "hllc"])
