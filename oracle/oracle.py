@@ -44,6 +44,11 @@ def lib(kind="port"):
     f.argtypes = [C.POINTER(abi.EnzoPPMGrid), C.POINTER(abi.EnzoPPMGlobals), C.c_int64, C.c_int64,
                   C.POINTER(C.POINTER(abi.EnzoFluxes)), C.POINTER(abi.PF), abi.PI, C.c_int64,
                   C.c_int64, abi.PI, C.c_double, C.c_int64, C.POINTER(C.c_int64)]
+    f = getattr(L, p + "sweep_direction")
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(abi.EnzoPPMGrid), C.POINTER(abi.EnzoPPMGlobals), C.c_int64, C.c_int64,
+                  C.POINTER(C.POINTER(abi.EnzoFluxes)), C.POINTER(abi.PF), abi.PI, C.c_int64,
+                  C.c_int64, abi.PI, abi.PF, C.c_int64, C.POINTER(C.c_int64)]
     f = getattr(L, p + "compute_pressure")
     f.restype = C.c_int
     f.argtypes = [C.POINTER(abi.EnzoPPMGrid), C.POINTER(abi.EnzoPPMGlobals), C.c_double, abi.PF]
@@ -77,6 +82,28 @@ def solve_ppm_de(kind, grid, glob, cycle=0, fluxes=None, nthreads=8, min_support
     st = getattr(L, PREFIX[kind] + "solve_ppm_de")(
         C.byref(cg), C.byref(gs), cycle, nsub, fptr, cw, gstart, int(grid.gravity),
         grid.ncolours, colnum, min_support, nthreads, C.byref(fb))
+    return st, fb.value
+
+
+def sweep_direction(kind, grid, glob, dim, pressure, fluxes=None, nthreads=8):
+    """One directional sweep of every slice along `dim` on `grid` (in place),
+    with `pressure` (an array of the grid's shape) as the step's stale
+    pressure.  Returns (status, fallbacks)."""
+    import numpy as np
+    L = lib(kind)
+    gs, _ = make_globals(**glob)
+    cg, keep = grid.ctypes_grid()
+    cw = grid.ctypes_cellwidth()
+    colnum = abi.iarr(grid.colnum, max(1, len(grid.colnum)))
+    nsub = fluxes.n if fluxes is not None else 0
+    fptr = fluxes.ptrs if fluxes is not None else (C.POINTER(abi.EnzoFluxes) * 1)()
+    gstart = fluxes.global_start if fluxes is not None else abi.iarr([0, 0, 0])
+    p = np.ascontiguousarray(pressure, dtype=np.float64)
+    assert p.shape == grid.shape
+    fb = C.c_int64(0)
+    st = getattr(L, PREFIX[kind] + "sweep_direction")(
+        C.byref(cg), C.byref(gs), dim, nsub, fptr, cw, gstart, int(grid.gravity), grid.ncolours,
+        colnum, abi.dptr(p), nthreads, C.byref(fb))
     return st, fb.value
 
 

@@ -565,3 +565,23 @@ int PFX(solve_ppm_de)(const enzo_ppm_grid *g, const enzo_ppm_globals *gl, I Cycl
                                                    : ENZO_PPM_ERR_NEGATIVE;
   return ENZO_PPM_SUCCESS;
 }
+
+/* One directional sweep, grid::[xyz]EulerSweep over every slice of `dim`
+   (Grid_SolvePPM_DE.C:88-144 for one direction), with the caller's pressure
+   array standing for the stale pressure of the step.  Lets tests compose a
+   step from single sweeps in an order they state themselves. */
+int PFX(sweep_direction)(const enzo_ppm_grid *g, const enzo_ppm_globals *gl, I dim,
+                         I NumberOfSubgrids, enzo_fluxes **SubgridFluxes, R **CellWidthTemp,
+                         const I *GridGlobalStart, I GravityOn, I NumberOfColours,
+                         const I *colnum, R *pressure, I nthreads, I *fallback_count) {
+  if (dim < 0 || dim >= g->GridRank || !pressure) return ENZO_PPM_ERR_ARGUMENT;
+  if (NumberOfColours > ENZO_PPM_MAX_COLOR) return ENZO_PPM_ERR_ARGUMENT;
+  sweepctx c = {g, gl, NumberOfSubgrids, SubgridFluxes, CellWidthTemp, GridGlobalStart,
+                GravityOn, NumberOfColours, colnum, pressure};
+  I fb = 0;
+  I err = run_sweep(&c, dim, nthreads, &fb);
+  if (fallback_count) *fallback_count = fb;
+  if (err) return err == 2 || err == 3 || err == 8 ? ENZO_PPM_ERR_NOT_SUPPORTED
+                                                   : ENZO_PPM_ERR_NEGATIVE;
+  return ENZO_PPM_SUCCESS;
+}
