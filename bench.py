@@ -17,9 +17,10 @@ roofline: HBM-bound.  Algorithmic bytes of one sweep launch = active cells x
 achieved = those bytes / the sweep kernel's mean launch time, timed with HIP
 events around each launch on the library's stream.
 cpu_baseline: the reference Fortran kernels (flang build in oracle/_ref,
-driven by the oracle's slice driver on a thread pool) on one step of the same
-256^3 workload; falls back to the oracle's C restatement ("port") when the
-reference build is absent.
+driven by the oracle's slice driver on a thread pool over the host CPU share,
+at most 16 threads) on one step of the same 256^3 workload, median of 5 runs;
+falls back to the oracle's C restatement ("port") when the reference build is
+absent.
 """
 import argparse
 import json
@@ -41,7 +42,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256, help="active cells per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0: the process's CPU share, at most 16)")
+    ap.add_argument("--cpu-repeats", type=int, default=5)
     ap.add_argument("--level-grids", type=int, default=512,
                     help="grids of the batched AMR-level comparison (0: skip it)")
     ap.add_argument("--level-steps", type=int, default=5)
@@ -72,24 +75,51 @@ def copy_calibration():
                       "launches": 3}), flush=True)
 
 
-def cpu_baseline(n, threads):
+def host_cpu_share():
+    """(threads, nproc, affinity): the CPUs this process may run on
+    (sched_getaffinity), at most 16, the host CPUs a one-GPU box allots to
+    its GPU; nproc is os.cpu_count() for the record."""
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = nproc
+    return max(1, min(aff, 16)), nproc, aff
+
+
+def cpu_baseline(n, threads, repeats=5):
+    """The reference Fortran kernels (oracle/_ref, else the oracle's C
+    restatement) on one full step of the n^3 workload (Courant step,
+    pressure and the three sweeps), slices of each sweep spread over
+    `threads` threads (0: the host CPU share); the median of `repeats` runs,
+    each from the same initial state."""
+    import statistics
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from enzo_ppm import problems
+    share, nproc, aff = host_cpu_share()
+    threads = threads or share
     kind = "ref" if oracle.available("ref") else "port"
-    g, glob, run = problems.turbulent(n, seed=2024)
-    g.fill_periodic()
-    t0 = time.perf_counter()
-    g.dt = oracle.compute_time_step(kind, g, glob, run["cfl"])
-    st, _ = oracle.solve_ppm_de(kind, g, glob, cycle=0, nthreads=threads)
-    t1 = time.perf_counter()
-    if st != 1:
-        raise RuntimeError(f"cpu baseline status {st}")
+    g0, glob, run = problems.turbulent(n, seed=2024)
+    g0.fill_periodic()
+    times = []
+    for _ in range(repeats):
+        g = g0.copy()
+        t0 = time.perf_counter()
+        g.dt = oracle.compute_time_step(kind, g, glob, run["cfl"])
+        st, _ = oracle.solve_ppm_de(kind, g, glob, cycle=0, nthreads=threads)
+        t1 = time.perf_counter()
+        if st != 1:
+            raise RuntimeError(f"cpu baseline status {st}")
+        times.append(t1 - t0)
+        del g
+    med = statistics.median(times)
     cells = n ** 3
-    return dict(value=cells / (t1 - t0), unit="cell-updates/s", cores=threads,
-                kind="reference" if kind == "ref" else "port",
-                sample=f"one full step (pressure + Courant + 3 sweeps) of the {n}^3 workload, "
-                       f"{threads} threads, {t1 - t0:.2f} s")
+    return dict(value=cells / med, unit="cell-updates/s", cores=threads, nproc=nproc,
+                affinity=aff, kind="reference" if kind == "ref" else "port",
+                repeats=repeats, seconds=[round(x, 3) for x in times],
+                sample=f"one full step (Courant step, pressure, 3 sweeps) of the {n}^3 workload "
+                       f"on {threads} threads, median of {repeats} runs: {med:.2f} s")
 
 
 def level_solver_rates(lib, ngrids, steps, cfl=0.4):
@@ -519,7 +549,7 @@ def main():
     if not a.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         try:
-            out["cpu_baseline"] = cpu_baseline(n, a.cpu_threads)
+            out["cpu_baseline"] = cpu_baseline(n, a.cpu_threads, a.cpu_repeats)
         except Exception as e:  # reported, not fatal
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)

@@ -1456,7 +1456,11 @@ int hipPPMGetHarvest(hipPPMData *D, enzo_float *records, enzo_long_int max_recor
   return ENZO_PPM_SUCCESS;
 }
 
-const char *hipPPMVersion(void) { return "enzo_ppm 0.1 gfx950 fp64"; }
+#define PPM_STR2(x) #x
+#define PPM_STR(x) PPM_STR2(x)
+const char *hipPPMVersion(void) {
+  return "enzo_ppm 0.2 gfx950 fp64 integers-" PPM_STR(ENZO_PPM_INTEGER_BITS);
+}
 
 const char *hipPPMLastError(void) { return g_last_error; }
 

@@ -3,7 +3,10 @@
 The struct layouts follow the reference types they stand in for:
 ``enzo_fluxes`` is ``struct fluxes`` (src/enzo/Fluxes.h:29-37) with
 ``long_int`` = int64 and ``float`` = double (precision-64 build,
-src/enzo/macros_and_parameters.h:216-217,257).
+src/enzo/macros_and_parameters.h:151,257).  ``int`` is int64 (integers-64,
+:216-217) unless the environment sets ENZO_PPM_INTEGERS=32 before this
+module is imported: then it is int32 (integers-32, :201-213) and the
+library is libenzo_ppm_hip_i32.so, the header's ENZO_PPM_INTEGERS_32 build.
 """
 import ctypes as C
 import os
@@ -21,12 +24,19 @@ ERR_NEGATIVE = -3
 ERR_NOT_SUPPORTED = -4
 ERR_NO_DEVICE = -5
 
-enzo_int = C.c_int64
+BITS = int(os.environ.get("ENZO_PPM_INTEGERS", "64"))
+if BITS not in (32, 64):
+    raise ValueError(f"ENZO_PPM_INTEGERS must be 32 or 64, not {BITS}")
+enzo_int = C.c_int32 if BITS == 32 else C.c_int64   # `int`
+enzo_long_int = C.c_int64                           # long_int
 enzo_float = C.c_double
+I = enzo_int
+LI = enzo_long_int
 PF = C.POINTER(C.c_double)
-PI = C.POINTER(C.c_int64)
+PI = C.POINTER(enzo_int)
+PL = C.POINTER(enzo_long_int)
 
-Idx33 = (C.c_int64 * MAX_DIMENSION) * MAX_DIMENSION
+Idx33 = (LI * MAX_DIMENSION) * MAX_DIMENSION
 Ptr = (PF * MAX_DIMENSION) * MAX_NUMBER_OF_BARYON_FIELDS
 
 
@@ -46,73 +56,73 @@ class EnzoPPMGlobals(C.Structure):
     """Hydro globals read on the path (global_data.h)."""
     _fields_ = [
         ("Gamma", C.c_double),
-        ("PPMDiffusionParameter", C.c_int64),
-        ("PPMFlatteningParameter", C.c_int64),
-        ("PPMSteepeningParameter", C.c_int64),
-        ("DualEnergyFormalism", C.c_int64),
+        ("PPMDiffusionParameter", I),
+        ("PPMFlatteningParameter", I),
+        ("PPMSteepeningParameter", I),
+        ("DualEnergyFormalism", I),
         ("DualEnergyFormalismEta1", C.c_double),
         ("DualEnergyFormalismEta2", C.c_double),
-        ("PressureFree", C.c_int64),
+        ("PressureFree", I),
         ("tiny_number", C.c_double),
-        ("RiemannSolver", C.c_int64),
-        ("RiemannSolverFallback", C.c_int64),
-        ("ReconstructionMethod", C.c_int64),
-        ("ConservativeReconstruction", C.c_int64),
-        ("PositiveReconstruction", C.c_int64),
+        ("RiemannSolver", I),
+        ("RiemannSolverFallback", I),
+        ("ReconstructionMethod", I),
+        ("ConservativeReconstruction", I),
+        ("PositiveReconstruction", I),
         ("SmallRho", C.c_double),
-        ("EOSType", C.c_int64),
+        ("EOSType", I),
     ]
 
 
 class EnzoPPMGrid(C.Structure):
     """Grid state that grid::SolvePPM_DE reads."""
     _fields_ = [
-        ("GridRank", C.c_int64),
-        ("GridDimension", C.c_int64 * 3),
-        ("GridStartIndex", C.c_int64 * 3),
-        ("GridEndIndex", C.c_int64 * 3),
+        ("GridRank", I),
+        ("GridDimension", I * 3),
+        ("GridStartIndex", I * 3),
+        ("GridEndIndex", I * 3),
         ("BaryonField", C.POINTER(PF)),
         ("AccelerationField", C.POINTER(PF)),
-        ("DensNum", C.c_int64),
-        ("GENum", C.c_int64),
-        ("Vel1Num", C.c_int64),
-        ("Vel2Num", C.c_int64),
-        ("Vel3Num", C.c_int64),
-        ("TENum", C.c_int64),
+        ("DensNum", I),
+        ("GENum", I),
+        ("Vel1Num", I),
+        ("Vel2Num", I),
+        ("Vel3Num", I),
+        ("TENum", I),
         ("dtFixed", C.c_double),
     ]
 
 
 class HipPPMParameter(C.Structure):
-    """hipPPMParameter (cuPPMParameter, cuPPM.h:24-50, FP64/int64)."""
+    """hipPPMParameter (cuPPMParameter, cuPPM.h:24-50, FP64, `int` = enzo_int)."""
     _fields_ = [
-        ("DensNum", C.c_int64), ("TENum", C.c_int64), ("Vel1Num", C.c_int64),
-        ("Vel2Num", C.c_int64), ("Vel3Num", C.c_int64), ("GENum", C.c_int64),
-        ("MaxNumberOfBaryonFields", C.c_int64),
-        ("GridGlobalStart", C.c_int64 * 3),
-        ("GridStartIndex", C.c_int64 * 3),
-        ("GridEndIndex", C.c_int64 * 3),
-        ("GridDimension", C.c_int64 * 3),
-        ("GridRank", C.c_int64),
+        ("DensNum", I), ("TENum", I), ("Vel1Num", I),
+        ("Vel2Num", I), ("Vel3Num", I), ("GENum", I),
+        ("MaxNumberOfBaryonFields", I),
+        ("GridGlobalStart", LI * 3),
+        ("GridStartIndex", I * 3),
+        ("GridEndIndex", I * 3),
+        ("GridDimension", I * 3),
+        ("GridRank", I),
         ("dx", C.c_double * 3),
         ("CellWidth", PF * 3),
         ("Gamma", C.c_double),
-        ("PPMDiffusionParameter", C.c_int64),
-        ("PPMFlatteningParameter", C.c_int64),
-        ("PPMSteepeningParameter", C.c_int64),
-        ("ConservativeReconstruction", C.c_int64),
-        ("PositiveReconstruction", C.c_int64),
-        ("GravityOn", C.c_int64),
-        ("DualEnergyFormalism", C.c_int64),
+        ("PPMDiffusionParameter", I),
+        ("PPMFlatteningParameter", I),
+        ("PPMSteepeningParameter", I),
+        ("ConservativeReconstruction", I),
+        ("PositiveReconstruction", I),
+        ("GravityOn", I),
+        ("DualEnergyFormalism", I),
         ("DualEnergyFormalismEta1", C.c_double),
         ("DualEnergyFormalismEta2", C.c_double),
-        ("PressureFree", C.c_int64),
+        ("PressureFree", I),
         ("MinimumPressure", C.c_double),
-        ("RiemannSolver", C.c_int64),
-        ("RiemannSolverFallback", C.c_int64),
+        ("RiemannSolver", I),
+        ("RiemannSolverFallback", I),
         ("SmallRho", C.c_double),
-        ("NumberOfColours", C.c_int64),
-        ("colnum", C.c_int64 * MAX_COLOR),
+        ("NumberOfColours", I),
+        ("colnum", I * MAX_COLOR),
     ]
 
 
@@ -123,102 +133,97 @@ def dptr(a):
 
 
 def iarr(vals, n=None):
+    """enzo_int array (`int` arguments and members)."""
     n = len(vals) if n is None else n
-    arr = (C.c_int64 * n)()
+    arr = (I * n)()
+    for i, v in enumerate(vals):
+        arr[i] = int(v)
+    return arr
+
+
+def larr(vals, n=None):
+    """long_int array (GridGlobalStart and the like)."""
+    n = len(vals) if n is None else n
+    arr = (LI * n)()
     for i, v in enumerate(vals):
         arr[i] = int(v)
     return arr
 
 
 PACKAGE_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_NAME = "libenzo_ppm_hip.so"
+LIB_NAME = "libenzo_ppm_hip_i32.so" if BITS == 32 else "libenzo_ppm_hip.so"
 LIB_PATH = os.path.join(PACKAGE_DIR, LIB_NAME)
 
-# exported entry points and their prototypes (include/enzo_ppm.h)
+# exported entry points and their prototypes (include/enzo_ppm.h): I is
+# enzo_int, LI long_int (tests/test_capi.py checks this table against the
+# header's declarations)
+_P = C.POINTER(HipPPMParameter)
+_FL = C.POINTER(C.POINTER(EnzoFluxes))
+_F1 = C.POINTER(EnzoFluxes)
+_H = C.c_void_p
+_D = C.c_double
 PROTOTYPES = {
-    "hipPPMInitParameter": (C.c_int, [C.POINTER(HipPPMParameter)] + [C.c_int64] * 7
-                            + [PI] * 4 + [C.c_double] * 4 + [C.c_int64] * 7
-                            + [C.c_double] * 2 + [C.c_int64, C.c_double] + [C.c_int64] * 3
-                            + [PI, C.c_int64, C.c_double]),
-    "hipPPMSetCellWidth": (C.c_int, [C.POINTER(HipPPMParameter), C.c_int64, PF]),
-    "hipPPMInitData": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(HipPPMParameter), C.c_void_p]),
-    "hipPPMSetBaryon": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.POINTER(PF), PF,
-                                  C.POINTER(PF)]),
-    "hipPPMSetSubgridFluxes": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter),
-                                         C.POINTER(C.POINTER(EnzoFluxes)), C.c_int64, PI]),
-    "hipPPMSweep": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_double, C.c_int64]),
-    "hipPPMSaveSubgridFluxes": (C.c_int, [C.POINTER(C.POINTER(EnzoFluxes)), C.c_int64, PI,
-                                          C.c_void_p, C.POINTER(HipPPMParameter), C.c_int64]),
-    "hipPPMGetBaryon": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.POINTER(PF)]),
-    "hipPPMDestroy": (C.c_int, [C.c_void_p]),
-    "hipPPMFallbackCount": (C.c_int64, [C.c_void_p]),
-    "hipPPMSolvePPM_DE": (C.c_int, [C.POINTER(EnzoPPMGrid), C.POINTER(EnzoPPMGlobals), C.c_int64,
-                                    C.c_int64, C.POINTER(C.POINTER(EnzoFluxes)), C.POINTER(PF),
-                                    PI, C.c_int64, C.c_int64, PI, C.c_double,
-                                    C.POINTER(C.c_void_p), C.c_void_p]),
-    "hipPPMDeviceField": (PF, [C.c_void_p, C.c_int64]),
-    "hipPPMPressureAndTimeStep": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_double,
-                                            C.c_double, PF]),
-    "hipPPMStep": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_int64, C.c_double]),
-    "hipPPMPeriodicBoundary": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter)]),
-    "hipPPMProfile": (C.c_int, [C.c_void_p, C.c_int64]),
-    "hipPPMProfileRead": (C.c_int, [C.c_void_p, PF, PI]),
-    "hipPPMSynchronize": (C.c_int, [C.c_void_p]),
+    "hipPPMInitParameter": (C.c_int, [_P] + [I] * 7 + [PL] + [PI] * 3 + [_D] * 4 + [I] * 7
+                            + [_D] * 2 + [I, _D] + [I] * 3 + [PI, I, _D]),
+    "hipPPMSetCellWidth": (C.c_int, [_P, I, PF]),
+    "hipPPMInitData": (C.c_int, [C.POINTER(_H), _P, _H]),
+    "hipPPMSetBaryon": (C.c_int, [_H, _P, C.POINTER(PF), PF, C.POINTER(PF)]),
+    "hipPPMSetSubgridFluxes": (C.c_int, [_H, _P, _FL, I, PL]),
+    "hipPPMSweep": (C.c_int, [_H, _P, _D, I]),
+    "hipPPMSaveSubgridFluxes": (C.c_int, [_FL, I, PL, _H, _P, I]),
+    "hipPPMGetBaryon": (C.c_int, [_H, _P, C.POINTER(PF)]),
+    "hipPPMDestroy": (C.c_int, [_H]),
+    "hipPPMFallbackCount": (LI, [_H]),
+    "hipPPMSolvePPM_DE": (C.c_int, [C.POINTER(EnzoPPMGrid), C.POINTER(EnzoPPMGlobals), I, I, _FL,
+                                    C.POINTER(PF), PL, I, I, PI, _D, C.POINTER(_H), _H]),
+    "hipPPMDeviceField": (PF, [_H, I]),
+    "hipPPMPressureAndTimeStep": (C.c_int, [_H, _P, _D, _D, PF]),
+    "hipPPMStep": (C.c_int, [_H, _P, I, _D]),
+    "hipPPMPeriodicBoundary": (C.c_int, [_H, _P]),
+    "hipPPMExternalBoundary": (C.c_int, [_H, _P, PI, PI, PI, I, PF]),
+    "hipPPMProjectBoundaryFluxes": (C.c_int, [_F1, I, I, PI, _F1]),
+    "hipPPMCorrectForRefinedFluxes": (C.c_int, [_H, _P, _F1, _F1, _F1, I, PI, PI, I, PI]),
+    "hipPPMSetComoving": (C.c_int, [_H, _P, I]),
+    "hipPPMSetExpansionFactors": (C.c_int, [_H, _P, _D, _D]),
+    "hipPPMComovingExpansionTerms": (C.c_int, [_H, _P, _D, _D]),
+    "hipPPMProfile": (C.c_int, [_H, I]),
+    "hipPPMProfileRead": (C.c_int, [_H, PF, PL]),
+    "hipPPMSynchronize": (C.c_int, [_H]),
+    "hipPPMSetLearnedGuess": (C.c_int, [_H, C.c_void_p]),
+    "hipPPMSetHarvest": (C.c_int, [_H, LI, I]),
+    "hipPPMGetHarvest": (C.c_int, [_H, PF, LI, PL, PL]),
+    "hipPPMLearnTrain": (C.c_int, [PF, LI, _D, I, I, _D, I, C.c_void_p, PF]),
+    "hipPPMLearnGradient": (C.c_int, [PF, LI, _D, C.c_void_p, PF, PF]),
+    "hipPPMLearnPredict": (C.c_int, [PF, LI, _D, C.c_void_p, C.c_void_p, PF]),
+    "hipPPMLearnNewtonStats": (C.c_int, [PF, LI, _D, _D, C.c_void_p, PF]),
+    "hipPPMLearnParameterCount": (C.c_int, []),
+    "hipPPMLearnLastError": (C.c_char_p, []),
     "hipPPMVersion": (C.c_char_p, []),
     "hipPPMLastError": (C.c_char_p, []),
     "hipPPMDeviceCount": (C.c_int, []),
     # batched AMR level
-    "hipPPMLevelCreate": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64, C.POINTER(HipPPMParameter),
-                                    C.c_void_p]),
-    "hipPPMLevelSetBaryon": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(PF), C.POINTER(PF)]),
-    "hipPPMLevelGetBaryon": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(PF)]),
-    "hipPPMLevelStep": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_double]),
-    "hipPPMLevelSetSubgridFluxes": (C.c_int, [C.c_void_p, C.c_int64,
-                                              C.POINTER(C.POINTER(EnzoFluxes)), C.c_int64, PI]),
-    "hipPPMLevelSaveSubgridFluxes": (C.c_int, [C.c_void_p, C.c_int64,
-                                               C.POINTER(C.POINTER(EnzoFluxes)), C.c_int64]),
-    "hipPPMLevelStatus": (C.c_int, [C.c_void_p, PF, PI]),
-    "hipPPMLevelUseGraph": (C.c_int, [C.c_void_p, C.c_int64]),
-    "hipPPMLevelSetShape": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int64]),
-    "hipPPMLevelCommitShapes": (C.c_int, [C.c_void_p]),
-    "hipPPMLevelTune": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_int64, PI]),
-    "hipPPMLevelSetLearnedGuess": (C.c_int, [C.c_void_p, C.c_void_p]),
-    "hipPPMLevelSetComoving": (C.c_int, [C.c_void_p, C.c_int64]),
-    "hipPPMLevelSetExpansionFactors": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
-    "hipPPMLevelTimeStep": (C.c_int, [C.c_void_p, C.c_double, PF]),
-    "hipPPMLevelComovingExpansionTerms": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
-    "hipPPMLevelPeriodicBoundary": (C.c_int, [C.c_void_p]),
-    "hipPPMLevelSetSiblings": (C.c_int, [C.c_void_p, PI, PI, C.c_int64]),
-    "hipPPMLevelCopyZones": (C.c_int, [C.c_void_p]),
-    "hipPPMLevelDestroy": (C.c_int, [C.c_void_p]),
-    "hipPPMSetLearnedGuess": (C.c_int, [C.c_void_p, C.c_void_p]),
-    "hipPPMSetComoving": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_int64]),
-    "hipPPMProjectBoundaryFluxes": (C.c_int, [C.POINTER(EnzoFluxes), C.c_int64, C.c_int64, PI,
-                                              C.POINTER(EnzoFluxes)]),
-    "hipPPMExternalBoundary": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), PI, PI, PI,
-                                         C.c_int64, PF]),
-    "hipPPMLevelExternalBoundary": (C.c_int, [C.c_void_p, PI, PI, PI, C.c_int64, PF]),
-    "hipPPMCorrectForRefinedFluxes": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter),
-                                                C.POINTER(EnzoFluxes), C.POINTER(EnzoFluxes),
-                                                C.POINTER(EnzoFluxes), C.c_int64, PI, PI,
-                                                C.c_int64, PI]),
-    "hipPPMLevelCorrectForRefinedFluxes": (C.c_int, [C.c_void_p, C.c_int64,
-                                                C.POINTER(EnzoFluxes), C.POINTER(EnzoFluxes),
-                                                C.POINTER(EnzoFluxes), C.c_int64, PI, PI,
-                                                C.c_int64, PI]),
-    "hipPPMSetExpansionFactors": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter), C.c_double,
-                                            C.c_double]),
-    "hipPPMComovingExpansionTerms": (C.c_int, [C.c_void_p, C.POINTER(HipPPMParameter),
-                                               C.c_double, C.c_double]),
-    "hipPPMSetHarvest": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64]),
-    "hipPPMGetHarvest": (C.c_int, [C.c_void_p, PF, C.c_int64, PI, PI]),
-    "hipPPMLearnTrain": (C.c_int, [PF, C.c_int64, C.c_double, C.c_int64, C.c_int64, C.c_double,
-                                   C.c_int64, C.c_void_p, PF]),
-    "hipPPMLearnGradient": (C.c_int, [PF, C.c_int64, C.c_double, C.c_void_p, PF, PF]),
-    "hipPPMLearnPredict": (C.c_int, [PF, C.c_int64, C.c_double, C.c_void_p, C.c_void_p, PF]),
-    "hipPPMLearnNewtonStats": (C.c_int, [PF, C.c_int64, C.c_double, C.c_double, C.c_void_p, PF]),
-    "hipPPMLearnParameterCount": (C.c_int, []),
-    "hipPPMLearnLastError": (C.c_char_p, []),
+    "hipPPMLevelCreate": (C.c_int, [C.POINTER(_H), I, _P, _H]),
+    "hipPPMLevelSetBaryon": (C.c_int, [_H, I, C.POINTER(PF), C.POINTER(PF)]),
+    "hipPPMLevelGetBaryon": (C.c_int, [_H, I, C.POINTER(PF)]),
+    "hipPPMLevelStep": (C.c_int, [_H, I, _D, _D]),
+    "hipPPMLevelSetSubgridFluxes": (C.c_int, [_H, I, _FL, I, PL]),
+    "hipPPMLevelSaveSubgridFluxes": (C.c_int, [_H, I, _FL, I]),
+    "hipPPMLevelCorrectForRefinedFluxes": (C.c_int, [_H, I, _F1, _F1, _F1, I, PI, PI, I, PI]),
+    "hipPPMLevelStatus": (C.c_int, [_H, PF, PL]),
+    "hipPPMLevelUseGraph": (C.c_int, [_H, I]),
+    "hipPPMLevelSetShape": (C.c_int, [_H, I, I, I]),
+    "hipPPMLevelCommitShapes": (C.c_int, [_H]),
+    "hipPPMLevelTune": (C.c_int, [_H, I, _D, I, PI]),
+    "hipPPMLevelSetComoving": (C.c_int, [_H, I]),
+    "hipPPMLevelSetExpansionFactors": (C.c_int, [_H, _D, _D]),
+    "hipPPMLevelTimeStep": (C.c_int, [_H, _D, PF]),
+    "hipPPMLevelComovingExpansionTerms": (C.c_int, [_H, _D, _D]),
+    "hipPPMLevelPeriodicBoundary": (C.c_int, [_H]),
+    "hipPPMLevelExternalBoundary": (C.c_int, [_H, PI, PI, PI, I, PF]),
+    "hipPPMLevelSetSiblings": (C.c_int, [_H, PL, PL, I]),
+    "hipPPMLevelCopyZones": (C.c_int, [_H]),
+    "hipPPMLevelSetLearnedGuess": (C.c_int, [_H, C.c_void_p]),
+    "hipPPMLevelDestroy": (C.c_int, [_H]),
     "hipPPMLevelLastError": (C.c_char_p, []),
 }
 

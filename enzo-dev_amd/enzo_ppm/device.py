@@ -28,7 +28,7 @@ def solve_ppm_de(grid, glob, cycle=0, fluxes=None, handle=None, min_support=0.0,
     colnum = abi.iarr(grid.colnum, max(1, len(grid.colnum)))
     nsub = fluxes.n if fluxes is not None else 0
     fptr = fluxes.ptrs if fluxes is not None else (C.POINTER(abi.EnzoFluxes) * 1)()
-    gstart = fluxes.global_start if fluxes is not None else abi.iarr([0, 0, 0])
+    gstart = fluxes.global_start if fluxes is not None else abi.larr([0, 0, 0])
     st = L.hipPPMSolvePPM_DE(C.byref(cg), C.byref(gs), cycle, nsub, fptr, cw, gstart,
                              int(grid.gravity), grid.ncolours, colnum, min_support,
                              C.byref(handle) if handle is not None else None, None)

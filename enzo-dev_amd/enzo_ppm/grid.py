@@ -186,7 +186,7 @@ class Grid:
         G = globals_dict
         para = abi.HipPPMParameter()
         colnum = abi.iarr(self.colnum, max(1, len(self.colnum)))
-        gstart = abi.iarr(global_start, 3)
+        gstart = abi.larr(global_start, 3)
         st = lib.hipPPMInitParameter(
             C.byref(para), self.DensNum, self.TENum, self.Vel1Num, self.Vel2Num, self.Vel3Num,
             self.GENum, abi.MAX_NUMBER_OF_BARYON_FIELDS, gstart, abi.iarr(self.start, 3),
@@ -283,7 +283,7 @@ class SubgridFluxes:
         for i, F in enumerate(self.structs):
             self.ptrs[i] = C.pointer(F)
         self.n = n
-        self.global_start = abi.iarr(global_start, 3)
+        self.global_start = abi.larr(global_start, 3)
 
     def zero(self):
         for arrs in self.arrays:

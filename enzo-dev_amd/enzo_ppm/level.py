@@ -103,7 +103,7 @@ class Level:
         self._check(self.lib.hipPPMLevelCommitShapes(self.handle), "LevelCommitShapes")
 
     def tune(self, cycle, dt, repeats=2):
-        out = (C.c_int64 * 4)()
+        out = (abi.I * 4)()
         self._check(self.lib.hipPPMLevelTune(self.handle, cycle, dt, repeats, out), "LevelTune")
         return list(out)
 
@@ -147,8 +147,8 @@ class Level:
     def set_siblings(self, global_starts, domain_active, periodic=True):
         """Sibling copy boxes from each grid's global start (index of its
         first cell, ghosts included) and the domain's active cells."""
-        starts = abi.iarr([v for s in global_starts for v in s])
-        self._check(self.lib.hipPPMLevelSetSiblings(self.handle, starts, abi.iarr(domain_active, 3),
+        starts = abi.larr([v for s in global_starts for v in s])
+        self._check(self.lib.hipPPMLevelSetSiblings(self.handle, starts, abi.larr(domain_active, 3),
                                                     int(periodic)), "LevelSetSiblings")
 
     def copy_zones(self):

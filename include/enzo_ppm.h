@@ -4,9 +4,10 @@
  * Drop-in for the GPU switch inside grid::SolvePPM_DE
  * (src/enzo/Grid_SolvePPM_DE.C:63-82,99-102,118-121,137-141,148-152), which in
  * the reference calls the cuPPM* lifecycle declared in src/enzo/cuPPM.h:53-102.
- * This header exports the same lifecycle for the default precision-64 /
- * integers-64 build (`float` = double, `int` = int64_t:
- * src/enzo/macros_and_parameters.h:216-217,257), with these additions:
+ * This header exports the same lifecycle for the precision-64 build
+ * (`float` = double: src/enzo/macros_and_parameters.h:257) with either
+ * integer width (`int` = int64_t by default, int32_t with
+ * ENZO_PPM_INTEGERS_32, see enzo_int below), with these additions:
  *   - every call returns a status (ENZO_PPM_SUCCESS, or a negative error
  *     code; the reference calls exit(1) or aborts),
  *   - an explicit hipStream_t,
@@ -31,9 +32,22 @@
 extern "C" {
 #endif
 
-typedef int64_t enzo_int;      /* `int` with integers-64 (macros_and_parameters.h:216-217) */
+/* `int` follows the reference's CONFIG_INTEGERS: 64 by default (LARGE_INTS,
+   macros_and_parameters.h:216-217), 32 when the caller defines
+   ENZO_PPM_INTEGERS_32 (SMALL_INTS, :201-213; the test-suite build,
+   Make.settings.test-suite:2).  Each width is its own library:
+   libenzo_ppm_hip.so (64) and libenzo_ppm_hip_i32.so (32), with the same
+   entry points; hipPPMVersion() names the width.  long_int is 64 bits in
+   both (macros_and_parameters.h:151). */
+#ifdef ENZO_PPM_INTEGERS_32
+typedef int32_t enzo_int;
+#define ENZO_PPM_INTEGER_BITS 32
+#else
+typedef int64_t enzo_int;
+#define ENZO_PPM_INTEGER_BITS 64
+#endif
 typedef double enzo_float;     /* `float` with precision-64 (macros_and_parameters.h:257) */
-typedef int64_t enzo_long_int; /* long_int (macros_and_parameters.h:202-217) */
+typedef int64_t enzo_long_int; /* long_int (macros_and_parameters.h:151) */
 
 #define ENZO_PPM_MAX_DIMENSION 3                 /* MAX_DIMENSION (macros_and_parameters.h:113) */
 #ifndef ENZO_PPM_MAX_NUMBER_OF_BARYON_FIELDS

@@ -35,6 +35,11 @@ def available(kind):
 def lib(kind="port"):
     if kind in _libs:
         return _libs[kind]
+    if abi.BITS != 64:
+        # both builds take 64-bit integers (LARGE_INTS, like the Fortran
+        # kernels they wrap): a process bound to the 32-bit ABI cannot call them
+        raise RuntimeError("the oracle is built with 64-bit integers; this process imported "
+                           "enzo_ppm.abi with ENZO_PPM_INTEGERS=32")
     if not os.path.exists(PATHS[kind]):
         raise RuntimeError(f"oracle library {PATHS[kind]} not built")
     L = C.CDLL(PATHS[kind])
@@ -42,12 +47,12 @@ def lib(kind="port"):
     f = getattr(L, p + "solve_ppm_de")
     f.restype = C.c_int
     f.argtypes = [C.POINTER(abi.EnzoPPMGrid), C.POINTER(abi.EnzoPPMGlobals), C.c_int64, C.c_int64,
-                  C.POINTER(C.POINTER(abi.EnzoFluxes)), C.POINTER(abi.PF), abi.PI, C.c_int64,
+                  C.POINTER(C.POINTER(abi.EnzoFluxes)), C.POINTER(abi.PF), abi.PL, C.c_int64,
                   C.c_int64, abi.PI, C.c_double, C.c_int64, C.POINTER(C.c_int64)]
     f = getattr(L, p + "sweep_direction")
     f.restype = C.c_int
     f.argtypes = [C.POINTER(abi.EnzoPPMGrid), C.POINTER(abi.EnzoPPMGlobals), C.c_int64, C.c_int64,
-                  C.POINTER(C.POINTER(abi.EnzoFluxes)), C.POINTER(abi.PF), abi.PI, C.c_int64,
+                  C.POINTER(C.POINTER(abi.EnzoFluxes)), C.POINTER(abi.PF), abi.PL, C.c_int64,
                   C.c_int64, abi.PI, abi.PF, C.c_int64, C.POINTER(C.c_int64)]
     f = getattr(L, p + "compute_pressure")
     f.restype = C.c_int
@@ -77,7 +82,7 @@ def solve_ppm_de(kind, grid, glob, cycle=0, fluxes=None, nthreads=8, min_support
     colnum = abi.iarr(grid.colnum, max(1, len(grid.colnum)))
     nsub = fluxes.n if fluxes is not None else 0
     fptr = fluxes.ptrs if fluxes is not None else (C.POINTER(abi.EnzoFluxes) * 1)()
-    gstart = fluxes.global_start if fluxes is not None else abi.iarr([0, 0, 0])
+    gstart = fluxes.global_start if fluxes is not None else abi.larr([0, 0, 0])
     fb = C.c_int64(0)
     st = getattr(L, PREFIX[kind] + "solve_ppm_de")(
         C.byref(cg), C.byref(gs), cycle, nsub, fptr, cw, gstart, int(grid.gravity),
@@ -97,7 +102,7 @@ def sweep_direction(kind, grid, glob, dim, pressure, fluxes=None, nthreads=8):
     colnum = abi.iarr(grid.colnum, max(1, len(grid.colnum)))
     nsub = fluxes.n if fluxes is not None else 0
     fptr = fluxes.ptrs if fluxes is not None else (C.POINTER(abi.EnzoFluxes) * 1)()
-    gstart = fluxes.global_start if fluxes is not None else abi.iarr([0, 0, 0])
+    gstart = fluxes.global_start if fluxes is not None else abi.larr([0, 0, 0])
     p = np.ascontiguousarray(pressure, dtype=np.float64)
     assert p.shape == grid.shape
     fb = C.c_int64(0)

@@ -55,7 +55,7 @@ def test_bad_arguments_rejected():
     _, gd = make_globals(**glob)
     para = abi.HipPPMParameter()
     # 2 ghost zones only: rejected (the stencil reaches 3 cells out)
-    st = lib.hipPPMInitParameter(C.byref(para), 0, 1, 2, 3, 4, 5, 30, abi.iarr([0, 0, 0]),
+    st = lib.hipPPMInitParameter(C.byref(para), 0, 1, 2, 3, 4, 5, 30, abi.larr([0, 0, 0]),
                                  abi.iarr([2, 2, 2]), abi.iarr([9, 9, 9]), abi.iarr([12, 12, 12]),
                                  0.1, 0.1, 0.1, 5 / 3, 0, 1, 1, 0, 0, 0, 1, 1e-3, 0.1, 0, 1e-20, 0, 1,
                                  0, abi.iarr([0]), 3, 1e-30)

@@ -36,7 +36,7 @@ def emu_solve(L, g, glob, cycle=0, fluxes=None):
     colnum = abi.iarr(g.colnum, max(1, len(g.colnum)))
     nsub = fluxes.n if fluxes is not None else 0
     fptr = fluxes.ptrs if fluxes is not None else (C.POINTER(abi.EnzoFluxes) * 1)()
-    gstart = fluxes.global_start if fluxes is not None else abi.iarr([0, 0, 0])
+    gstart = fluxes.global_start if fluxes is not None else abi.larr([0, 0, 0])
     cnt = (C.c_int64 * 4)()
     st = L.emu_solve_ppm_de(C.byref(cg), C.byref(gs), C.c_int64(cycle), C.c_int64(nsub), fptr, cw,
                             gstart, C.c_int64(int(g.gravity)), C.c_int64(g.ncolours), colnum,
