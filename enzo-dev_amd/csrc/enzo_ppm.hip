@@ -358,6 +358,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
     S.kt_row[t] = D->kt[rax][t];
     S.kt_out[t] = D->kt[oax][t];
   }
+  set_dx_inverse(S);
   S.gamma = P->Gamma;
   S.eta1 = P->DualEnergyFormalismEta1;
   S.eta2 = P->DualEnergyFormalismEta2;

@@ -27,10 +27,19 @@ struct DevExec {
     f((int)threadIdx.x, r);
     __syncthreads();
   }
+  // a wavefront vote per wavefront, its population count in LDS, the sum
+  // over the workgroup's wavefronts (at most 16: 1024 work-items)
   template <class F>
   __device__ int count(F f) {
+    __shared__ int part[16];
     int v = f((int)threadIdx.x, r);
-    return __syncthreads_count(v);
+    const unsigned long long b = __ballot(v != 0);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    int n = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) n += part[w];
+    __syncthreads();  // part is free for the next call
+    return n;
   }
   template <class F>
   __device__ int first(F f) {

@@ -548,6 +548,7 @@ int build_plan(hipPPMLevel *L) {
           S.kt_row[t] = L->kt[((size_t)3 * g + rax) * NTAB + t];
           S.kt_out[t] = L->kt[((size_t)3 * g + oax) * NTAB + t];
         }
+        set_dx_inverse(S);
         S.gamma = P0.Gamma;
         S.eta1 = P0.DualEnergyFormalismEta1;
         S.eta2 = P0.DualEnergyFormalismEta2;
@@ -1284,6 +1285,7 @@ int hipPPMLevelSetExpansionFactors(hipPPMLevel *L, enzo_float a_courant, enzo_fl
           S.kt_row[t] = L->kt[((size_t)3 * g + rax) * NTAB + t];
           S.kt_out[t] = L->kt[((size_t)3 * g + oax) * NTAB + t];
         }
+        set_dx_inverse(S);
       }
   LCHK(hipMemcpy(L->dparams, L->hparams.data(), sizeof(SweepParams) * L->hparams.size(),
                  hipMemcpyHostToDevice));

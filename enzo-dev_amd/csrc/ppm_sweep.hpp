@@ -23,6 +23,8 @@
 #pragma once
 
 #include <math.h>
+
+#include <cmath>
 #include <stdint.h>
 
 #ifdef __HIPCC__
@@ -104,6 +106,12 @@ struct SweepParams {
   // that no table is read.  uni[]: sweep, row, outer.
   int uni[3];
   double kt[NTAB], kt_row[NTAB], kt_out[NTAB];
+  // a uniform sweep axis whose width is a power of two: 1/dx and 1/(2 dx)
+  // are exact, so x / dx == x * inv_dx and x / (2 dx) == x * inv_2dx bit for
+  // bit (both are the correctly rounded value of the same real number);
+  // set by set_dx_inverse
+  int dx_pow2;
+  double inv_dx, inv_2dx;
   double gamma, eta1, eta2, pmin, dfloor, dt;
   const double *dtp;      // if non-null, dt = *dtp
   int isteep, iflatten, idiff, idual, ipresfree, gravity, ifallback;
@@ -177,7 +185,22 @@ struct Tile {
   int t1_0, t2;      // first pencil (t1) of the block, pencil row t2
   int np;            // pencils in this block
   double dt;         // time step, read once per block (it may live on the device)
+  double qcu;        // dt/dx of a uniform sweep axis (qc_of)
 };
+
+// characteristic distances of one cell (inteuler.F:201-208)
+struct Chars {
+  double ch1, ch2, cm, c0, cp;
+};
+PPM_HD Chars chars_of(double u, double cs, double dt, double dx2i) {
+  Chars r;
+  r.ch1 = fmx(0.0, dt * (u + cs)) * dx2i;
+  r.ch2 = fmx(0.0, -(dt * (u - cs))) * dx2i;
+  r.cm = dt * (u - cs) * dx2i;
+  r.c0 = dt * (u)*dx2i;
+  r.cp = dt * (u + cs) * dx2i;
+  return r;
+}
 
 struct Regs {
   // interface averages (inteuler.F before the corrections)
@@ -189,6 +212,7 @@ struct Regs {
   double dls_pre, drs_pre, dls, drs, db, ub;
   double df, ef, uf, vf, wf, gef;
   int ovl, ovr, upl, upr;  // DE overrides; upwind choice (u(i-1)<=0, u(i)>=0)
+  Chars ca, cb;            // characteristic distances of cells i-1 and i
   double flat_c;           // flattening at this thread's parabola cell
   // fallback workgroups only: HLL fluxes of the interface (dt/dx absorbed),
   // HLL weights and wave-speed offsets, and whether they replace the
@@ -244,20 +268,6 @@ PPM_HD PPM_LDS_REF LDSH(double *sm, int arr, int c) {
 
 PPM_HD double dtval(const SweepParams &P) { return P.dtp ? *P.dtp : P.dt; }
 
-// characteristic distances of one cell (inteuler.F:201-208)
-struct Chars {
-  double ch1, ch2, cm, c0, cp;
-};
-PPM_HD Chars chars_of(double u, double cs, double dt, double dx2i) {
-  Chars r;
-  r.ch1 = fmx(0.0, dt * (u + cs)) * dx2i;
-  r.ch2 = fmx(0.0, -(dt * (u - cs))) * dx2i;
-  r.cm = dt * (u - cs) * dx2i;
-  r.c0 = dt * (u)*dx2i;
-  r.cp = dt * (u + cs) * dx2i;
-  return r;
-}
-
 PPM_HD double tabv(const double *tab, int N, int t, int cell) {
   PPM_ASSERT(cell >= 1 && cell <= N && t >= 0 && t < NTAB);
   return tab[t * N + (cell - 1)];
@@ -276,10 +286,30 @@ PPM_HD double TR_(const SweepParams &P, int t, int cell, int n) {
 PPM_HD double TO_(const SweepParams &P, int t, int cell, int n) {
   return P.uni[2] ? P.kt_out[t] : tabv(P.tab_out, n, t, cell);
 }
+// whether x is a power of two (then 1/x is exact)
+inline bool is_pow2(double x) {
+  int e;
+  return x > 0.0 && std::frexp(x, &e) == 0.5;
+}
+
+// SweepParams::dx_pow2 / inv_dx / inv_2dx from uni[0] and kt[T_DX]
+inline void set_dx_inverse(SweepParams &S) {
+  S.dx_pow2 = S.uni[0] && is_pow2(S.kt[T_DX]) && S.kt[T_SDXP] == 2.0 * S.kt[T_DX] &&
+              S.kt[T_SDXM] == 2.0 * S.kt[T_DX];
+  S.inv_dx = S.dx_pow2 ? 1.0 / S.kt[T_DX] : 0.0;
+  S.inv_2dx = S.dx_pow2 ? 1.0 / S.kt[T_SDXP] : 0.0;
+}
+
+// dt/dx of a uniform sweep axis, once per tile
+PPM_HD double qc_uniform(const SweepParams &P, double dt) {
+  if (!P.uni[0]) return 0.0;
+  return P.dx_pow2 ? dt * P.inv_dx : dt / P.kt[T_DX];
+}
+
 // dt/dx(i) (flux_twoshock.F:366)
 template <int UNI>
-PPM_HD double qc_of(const SweepParams &P, double dt, int cell) {
-  if (UNI) return dt / P.kt[T_DX];
+PPM_HD double qc_of(const SweepParams &P, const Tile &T, int cell) {
+  if (UNI) return T.qcu;
   return P.qc[cell - 1];
 }
 
@@ -350,6 +380,7 @@ PPM_HD Tile make_tile(const SweepParams &P, int block) {
   T.w = P.seg_w[seg];
   tile_ranges(P, T);
   T.dt = dtval(P);
+  T.qcu = qc_uniform(P, T.dt);
   return T;
 }
 
@@ -365,6 +396,7 @@ PPM_HD Tile make_pencil_tile(const SweepParams &P, int t1, int t2) {
   T.w = P.N;
   tile_ranges(P, T);
   T.dt = dtval(P);
+  T.qcu = qc_uniform(P, T.dt);
   return T;
 }
 
@@ -446,8 +478,15 @@ PPM_HD void ph_chars(const SweepParams &P, const Tile &T, double *sm, int tid) {
       int c = cell - T.L0;
       double dm = LDS<TP, TS, UNI>(sm, A_D, p, c - 1), d0 = LDS<TP, TS, UNI>(sm, A_D, p, c),
              dp = LDS<TP, TS, UNI>(sm, A_D, p, c + 1);
-      double d2d = (dp - d0) / TS_<UNI>(P, T_SDXP, cell);
-      d2d = (d2d - (d0 - dm) / TS_<UNI>(P, T_SDXM, cell)) / TS_<UNI>(P, T_QA3, cell);
+      double d2d, d2m;
+      if (UNI && P.dx_pow2) {  // x / (2 dx) == x * inv_2dx exactly
+        d2d = (dp - d0) * P.inv_2dx;
+        d2m = (d0 - dm) * P.inv_2dx;
+      } else {
+        d2d = (dp - d0) / TS_<UNI>(P, T_SDXP, cell);
+        d2m = (d0 - dm) / TS_<UNI>(P, T_SDXM, cell);
+      }
+      d2d = (d2d - d2m) / TS_<UNI>(P, T_QA3, cell);
       LDS<TP, TS, UNI>(sm, A_DQ, p, c) = d2d;
     }
   }
@@ -562,14 +601,9 @@ struct IfaceAvg {
 };
 
 template <int TP, int TS, int UNI = 0>
-PPM_HD IfaceAvg iface_avg(const SweepParams &P, const Tile &T, double *sm, int p, int ca,
-                          int cb, int with_mp) {
+PPM_HD IfaceAvg iface_avg(const Chars &A, const Chars &B, double *sm, int p, int ca, int cb,
+                          int with_mp) {
   IfaceAvg r;
-  const double dt = T.dt;
-  const Chars A = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, ca), LDS<TP, TS, UNI>(sm, A_CS, p, ca), dt,
-                           TS_<UNI>(P, T_DX2I, ca + T.L0));
-  const Chars B = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, cb), LDS<TP, TS, UNI>(sm, A_CS, p, cb), dt,
-                           TS_<UNI>(P, T_DX2I, cb + T.L0));
   double qr_a = LDS<TP, TS, UNI>(sm, A_QR, p, ca), ql_a = LDS<TP, TS, UNI>(sm, A_QL, p, ca);
   double q6_a = LDS<TP, TS, UNI>(sm, A_Q6, p, ca);
   double ql_b = LDS<TP, TS, UNI>(sm, A_QL, p, cb), qr_b = LDS<TP, TS, UNI>(sm, A_QR, p, cb);
@@ -594,30 +628,40 @@ PPM_HD IfaceAvg iface_avg(const SweepParams &P, const Tile &T, double *sm, int p
   return r;
 }
 
-// Phase: interface averages of variable `var` (0 d, 1 p, 2 u, 3 v, 4 w, 5 ge)
-// for the thread's interface; results into Regs.
-template <int TP, int TS, int UNI = 0>
-PPM_HD void ph_iface(const SweepParams &P, const Tile &T, double *sm, int tid, int var,
-                     Regs &r) {
+// Phase: interface averages of variable VAR (0 d, 1 p, 2 u, 3 v, 4 w, 5 ge)
+// for the thread's interface; results into Regs.  VAR is a compile-time
+// constant, so each variable's phase is its own straight-line code.
+template <int VAR, int TP, int TS, int UNI = 0>
+PPM_HD void ph_iface(const SweepParams &P, const Tile &T, double *sm, int tid, Regs &r) {
+  (void)P;
   int p = tid / TS, q = tid % TS;
   if (p >= T.np || q >= T.NI) return;
   int i = T.IA + q;
   int ca = i - 1 - T.L0, cb = i - T.L0;
-  IfaceAvg a = iface_avg<TP, TS, UNI>(P, T, sm, p, ca, cb, var == 1 || var == 2);
-  if (var == 0) { r.dla = a.qla; r.dra = a.qra; r.dl0 = a.ql0; r.dr0 = a.qr0; }
-  else if (var == 1) {
+  if constexpr (VAR == 0) {
+    // the characteristic distances of cells i-1 and i, kept in registers
+    // for the other variables, the Riemann solve and the colours
+    r.ca = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, ca), LDS<TP, TS, UNI>(sm, A_CS, p, ca), T.dt,
+                    TS_<UNI>(P, T_DX2I, i - 1));
+    r.cb = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, cb), LDS<TP, TS, UNI>(sm, A_CS, p, cb), T.dt,
+                    TS_<UNI>(P, T_DX2I, i));
+  }
+  IfaceAvg a = iface_avg<TP, TS, UNI>(r.ca, r.cb, sm, p, ca, cb, VAR == 1 || VAR == 2);
+  if constexpr (VAR == 0) {
+    r.dla = a.qla; r.dra = a.qra; r.dl0 = a.ql0; r.dr0 = a.qr0;
+  } else if constexpr (VAR == 1) {
     r.pla = a.qla; r.pra = a.qra; r.pl0 = a.ql0; r.pr0 = a.qr0;
     r.plm = a.qlm; r.prm = a.qrm; r.plp = a.qlp; r.prp = a.qrp;
-  } else if (var == 2) {
+  } else if constexpr (VAR == 2) {
     r.ula = a.qla; r.ura = a.qra;
     r.ulm = a.qlm; r.urm = a.qrm; r.ulp = a.qlp; r.urp = a.qrp;
     // upwind choice of the advected states (inteuler.F:414-434)
     r.upl = LDS<TP, TS, UNI>(sm, A_U, p, ca) <= 0.0;
     r.upr = LDS<TP, TS, UNI>(sm, A_U, p, cb) >= 0.0;
-  } else if (var == 3) {
+  } else if constexpr (VAR == 3) {
     r.vls = r.upl ? a.qla : a.ql0;
     r.vrs = r.upr ? a.qra : a.qr0;
-  } else if (var == 4) {
+  } else if constexpr (VAR == 4) {
     r.wls = r.upl ? a.qla : a.ql0;
     r.wrs = r.upr ? a.qra : a.qr0;
   } else {
@@ -897,10 +941,7 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
   const int i = T.IA + q;
   const int ca = i - 1 - T.L0, cb = i - T.L0;
   const double gamma = P.gamma, dt = T.dt;
-  const Chars CA = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, ca), LDS<TP, TS, UNI>(sm, A_CS, p, ca), dt,
-                            TS_<UNI>(P, T_DX2I, i - 1));
-  const Chars CB = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, cb), LDS<TP, TS, UNI>(sm, A_CS, p, cb), dt,
-                            TS_<UNI>(P, T_DX2I, i));
+  const Chars &CA = r.ca, &CB = r.cb;
   // correction terms (3.7), inteuler.F:321-383
   double cla = sqrt(fmx(gamma * r.pla * r.dla, 0.0));
   double cra = sqrt(fmx(gamma * r.pra * r.dra, 0.0));
@@ -1043,7 +1084,7 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
     dueb = dub * eb;
     if (P.idual == 1) dugeb = dub * geb;
   }
-  double qc = qc_of<UNI>(P, T.dt, i);
+  double qc = qc_of<UNI>(P, T, i);
   r.df = qc * dub;
   r.ef = qc * (dueb + upb);
   r.uf = qc * (duub + pb);
@@ -1113,7 +1154,7 @@ PPM_HD void ph_hllc_ges(const SweepParams &P, const Tile &T, double *sm, int tid
   if (!P.idual || tid > c1 - c0) return;
   const int cell = c0 + tid, c = cell - T.L0;
   const double pcent = fmx((P.gamma - 1.0) * LDS<TP, TS, UNI>(sm, A_GE, 0, c) * LDS<TP, TS, UNI>(sm, A_D, 0, c), kTiny);
-  const double qc = qc_of<UNI>(P, T.dt, cell);
+  const double qc = qc_of<UNI>(P, T, cell);
   r.c_ges = qc * pcent *
             (LDS<TP, TS, UNI>(sm, F_UB, 0, c) + LDSH<TP, TS, UNI>(sm, A_HB, c) -
              LDS<TP, TS, UNI>(sm, F_UB, 0, c + 1) - LDSH<TP, TS, UNI>(sm, A_HB, c + 1));
@@ -1269,7 +1310,7 @@ PPM_HD int ph_euler(const SweepParams &P, const Tile &T, double *sm, int tid, bo
     if (P.idual == 1) {
       double ge = LDS<TP, TS, UNI>(sm, A_GE, p, c);
       double pcent = fmx((gamma - 1.0) * ge * d, kTiny);
-      double qc = qc_of<UNI>(P, T.dt, cell);
+      double qc = qc_of<UNI>(P, T, cell);
       double ges = qc * pcent * (LDS<TP, TS, UNI>(sm, F_UB, p, c) - LDS<TP, TS, UNI>(sm, F_UB, p, c + 1));
       if constexpr (FB != 0) {
         if (P.hllc) ges = LDS<TP, TS, UNI>(sm, F_UB, p, c);  // ph_hllc_ges
@@ -1440,7 +1481,7 @@ PPM_HD void ph_col_flux(const SweepParams &P, const Tile &T, double *sm, int tid
   if (p >= T.np || q >= T.NI) return;
   int i = T.IA + q;
   int ca = i - 1 - T.L0, cb = i - T.L0;
-  IfaceAvg a = iface_avg<TP, TS, UNI>(P, T, sm, p, ca, cb, 0);
+  IfaceAvg a = iface_avg<TP, TS, UNI>(r.ca, r.cb, sm, p, ca, cb, 0);
   double colls = r.upl ? a.qla * r.dls_pre / r.dla : a.ql0 * r.dls_pre / r.dl0;
   double colrs = r.upr ? a.qra * r.drs_pre / r.dra : a.qr0 * r.drs_pre / r.dr0;
   if (r.ovl) colls = colls * r.dla / r.dls_pre;
@@ -1453,7 +1494,7 @@ PPM_HD void ph_col_flux(const SweepParams &P, const Tile &T, double *sm, int tid
     colf = T.dt * (r.h_sl * (r.h_bm0 * colls) + r.h_sr * (r.h_bp0 * colrs));
   else if (FB != 0 && P.hllc)  // flux_hllc.F:252-258, 286-290, 336-340
     colf = T.dt * (r.c_sl * (r.h_bm0 * colls) + r.c_sr * (r.h_bp0 * colrs));
-  colf = colf / TS_<UNI>(P, T_DX, i);
+  colf = (UNI && P.dx_pow2) ? colf * P.inv_dx : colf / TS_<UNI>(P, T_DX, i);
   LDS<TP, TS, UNI>(sm, F_COLF, p, cb) = colf;
   if (P.face_head && i >= T.o && i < T.o + T.w) {
     int h0 = P.face_head[i], h1 = P.face_head[i + 1];
@@ -1543,6 +1584,17 @@ struct TileCounts {
   int bad, chains, err, nfb;
 };
 
+// Reconstruction of variable V (0 d, 1 p, 2 u, 3 v, 4 w, 5 ge): slopes,
+// parabolae (steepened for the density), interface averages.
+template <int V, int TP, int TS, int UNI, class X>
+PPM_HD void sweep_var(const SweepParams &P, const Tile &T, double *sm, X &ex) {
+  constexpr int arr = V == 0 ? A_D : V == 1 ? A_P : V == 2 ? A_U : V == 3 ? A_V : V == 4 ? A_W : A_GE;
+  const int steep = (V == 0 && P.isteep) ? 1 : 0;
+  ex.run([&](int tid, Regs &r) { (void)r; ph_slope<TP, TS, UNI>(P, T, sm, tid, arr); });
+  ex.run([&](int tid, Regs &r) { ph_parab<TP, TS, UNI>(P, T, sm, tid, arr, steep, r); });
+  ex.run([&](int tid, Regs &r) { ph_iface<V, TP, TS, UNI>(P, T, sm, tid, r); });
+}
+
 template <int TP, int TS, int UNI, int FB, class X, int LEARN = 0>
 PPM_HD TileCounts sweep_body(const SweepParams &P, const Tile &T, double *sm, X &ex) {
   TileCounts tc;
@@ -1553,14 +1605,12 @@ PPM_HD TileCounts sweep_body(const SweepParams &P, const Tile &T, double *sm, X 
   });
   ex.run([&](int tid, Regs &r) { (void)r; ph_chars<TP, TS, UNI>(P, T, sm, tid); });
   ex.run([&](int tid, Regs &r) { (void)r; ph_steepen<TP, TS, UNI>(P, T, sm, tid); });
-  const int nv = P.idual ? 6 : 5;
-  for (int v = 0; v < nv; v++) {
-    const int arr = v == 0 ? A_D : v == 1 ? A_P : v == 2 ? A_U : v == 3 ? A_V : v == 4 ? A_W : A_GE;
-    const int steep = (v == 0 && P.isteep) ? 1 : 0;
-    ex.run([&](int tid, Regs &r) { (void)r; ph_slope<TP, TS, UNI>(P, T, sm, tid, arr); });
-    ex.run([&](int tid, Regs &r) { ph_parab<TP, TS, UNI>(P, T, sm, tid, arr, steep, r); });
-    ex.run([&](int tid, Regs &r) { ph_iface<TP, TS, UNI>(P, T, sm, tid, v, r); });
-  }
+  sweep_var<0, TP, TS, UNI>(P, T, sm, ex);
+  sweep_var<1, TP, TS, UNI>(P, T, sm, ex);
+  sweep_var<2, TP, TS, UNI>(P, T, sm, ex);
+  sweep_var<3, TP, TS, UNI>(P, T, sm, ex);
+  sweep_var<4, TP, TS, UNI>(P, T, sm, ex);
+  if (P.idual) sweep_var<5, TP, TS, UNI>(P, T, sm, ex);
   ex.run([&](int tid, Regs &r) {
     ph_solve<TP, TS, UNI, FB, LEARN>(P, T, sm, tid, r);
     ph_store_flux<TP, TS, UNI, FB>(P, T, sm, tid, r);

@@ -317,6 +317,22 @@ inline double __shfl_xor(double v, int lane_mask, int width = 64) {
   return r;
 }
 
+// wavefront vote: bit l of the result is the predicate of work-item
+// (t & ~63) + l, for the 64 work-items of t's wavefront (called by every
+// work-item of the workgroup, as the kernels do)
+inline unsigned long long __ballot(int pred) {
+  const int t = hipcpu::g_cur;
+  hipcpu::g_slots[t] = pred != 0;
+  hipcpu::barrier(0);
+  unsigned long long m = 0;
+  const int w0 = t & ~63;
+  for (int l = 0; l < 64 && w0 + l < (int)hipcpu::g_slots.size(); l++)
+    if (hipcpu::g_slots[w0 + l]) m |= 1ull << l;
+  hipcpu::barrier(0);
+  return m;
+}
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+
 template <class T>
 inline T atomicAdd(T *p, T v) { T o = *p; *p = o + v; return o; }
 template <class T>

@@ -193,6 +193,7 @@ extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *
     S.qc = qc.data();
     S.uni[0] = uni[dim]; S.uni[1] = uni[rax]; S.uni[2] = uni[oax];
     for (int t = 0; t < NTAB; t++) { S.kt[t] = kt[dim][t]; S.kt_row[t] = kt[rax][t]; S.kt_out[t] = kt[oax][t]; }
+    set_dx_inverse(S);
     S.gamma = gl->Gamma; S.eta1 = gl->DualEnergyFormalismEta1; S.eta2 = gl->DualEnergyFormalismEta2;
     S.pmin = gl->tiny_number; S.dfloor = gl->SmallRho; S.dt = g->dtFixed; S.dtp = nullptr;
     S.isteep = (int)gl->PPMSteepeningParameter; S.iflatten = (int)gl->PPMFlatteningParameter;
