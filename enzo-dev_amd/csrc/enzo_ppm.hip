@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -149,6 +150,34 @@ __global__ void zero_u64_kernel(unsigned long long *p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
+// zero fill of n 32-bit words (used where a caller may capture the stream,
+// instead of hipMemsetAsync)
+__global__ void zero_u32_kernel(unsigned *p, long long n) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x)
+    p[k] = 0u;
+}
+
+// the caller's separate field arrays <-> the slot-contiguous state
+struct SlotPtrs {
+  double *f[S_COL0 + ENZO_PPM_MAX_COLOR];  // per slot, or null (gather: zeros)
+};
+__global__ void gather_slots_kernel(double *A, long long cells, int nslot, SlotPtrs F) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nslot * cells;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int s = (int)(k / cells);
+    const double *src = F.f[s];
+    A[k] = src ? src[k - s * cells] : 0.0;
+  }
+}
+__global__ void scatter_slots_kernel(const double *A, long long cells, int nslot, SlotPtrs F) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nslot * cells;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int s = (int)(k / cells);
+    if (F.f[s]) F.f[s][k - s * cells] = A[k];
+  }
+}
+
 // ------------------------------------------------------------------------
 // Periodic ghost fill along one axis for every slot (the periodic case of
 // the external boundary).  Thread per (slot, ghost cell).
@@ -176,6 +205,7 @@ struct hipPPMData {
   double *A = nullptr, *B = nullptr;
   double *pres = nullptr;
   double *accel = nullptr;  // 3 * cells when gravity
+  const double *accel_ext[3] = {nullptr, nullptr, nullptr};  // the caller's (no-alloc entry)
   double *tab[3] = {nullptr, nullptr, nullptr};
   double *cwdev[3] = {nullptr, nullptr, nullptr};
   double *flat = nullptr;
@@ -344,7 +374,8 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
   S.B = D->B;
   S.ncol = (int)P->NumberOfColours;
   S.pres = D->pres;
-  S.gr = P->GravityOn ? D->accel + (long long)dim * D->cells : nullptr;
+  S.gr = !P->GravityOn ? nullptr
+         : D->accel_ext[dim] ? D->accel_ext[dim] : D->accel + (long long)dim * D->cells;
   S.flat = D->flat;
   S.tab = D->tab[dim];
   S.tab_row = D->tab[rax];
@@ -489,25 +520,58 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
 // takes it) and their coefficient tables; the Courant pass's widths
 // (CellWidth itself) when comoving
 int upload_geometry(hipPPMData *D, const hipPPMParameter *Para) {
+  // host staging of one axis (no heap allocation: the entry point that
+  // allocates nothing uses this too)
+  constexpr int kN = ENZO_PPM_MAX_ANY_SINGLE_DIRECTION;
+  static thread_local double cw[kN], dx[kN], tab[(size_t)NTAB * kN];
   for (int d = 0; d < 3; d++) {
     const int N = D->GD[d];
-    std::vector<double> cw(N), dx(N), tab;
     for (int i = 0; i < N; i++)
       cw[i] = Para->CellWidth[d] ? Para->CellWidth[d][i] : (d < Para->GridRank ? Para->dx[d] : 1.0);
     for (int i = 0; i < N; i++)
       dx[i] = D->comoving && d < Para->GridRank ? D->a_sweep * cw[i] : cw[i];
-    build_tables(dx.data(), N, tab);
-    D->uni[d] = uniform_axis(dx.data(), N, tab, D->kt[d]) ? 1 : 0;
-    HIPCHK(hipMemcpyAsync(D->tab[d], tab.data(), sizeof(double) * tab.size(),
+    build_tables(dx, N, tab);
+    D->uni[d] = uniform_axis(dx, N, tab, D->kt[d]) ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(D->tab[d], tab, sizeof(double) * NTAB * (size_t)N,
                           hipMemcpyHostToDevice, D->stream));
-    HIPCHK(hipMemcpyAsync(D->cwdev[d], dx.data(), sizeof(double) * N, hipMemcpyHostToDevice,
+    HIPCHK(hipMemcpyAsync(D->cwdev[d], dx, sizeof(double) * N, hipMemcpyHostToDevice,
                           D->stream));
     if (D->comoving)
-      HIPCHK(hipMemcpyAsync(D->cwdt[d], cw.data(), sizeof(double) * N, hipMemcpyHostToDevice,
+      HIPCHK(hipMemcpyAsync(D->cwdt[d], cw, sizeof(double) * N, hipMemcpyHostToDevice,
                             D->stream));
-    HIPCHK(hipStreamSynchronize(D->stream));  // host vectors go out of scope
+    HIPCHK(hipStreamSynchronize(D->stream));  // the staging is reused for the next axis
   }
   return ENZO_PPM_SUCCESS;
+}
+
+// Sweep segments of every axis into segs[] (o[nseg] then w[nseg] per axis,
+// D->segoff / D->nseg set); returns the number of ints or -1 when an axis
+// cannot be split.  segs holds 2 * 3 * ENZO_PPM_MAX_ANY_SINGLE_DIRECTION.
+int plan_segments(hipPPMData *D, const hipPPMParameter *Para, int *segs) {
+  int n = 0;
+  for (int d = 0; d < 3; d++) {
+    const int N = D->GD[d];
+    const int i1 = (int)Para->GridStartIndex[d] + 1, i2 = (int)Para->GridEndIndex[d] + 1;
+    int ns = 0;
+    if (d < Para->GridRank && N > 1) {
+      ns = build_segments<kTP, kTS>(N, i1, i2, segs + n, segs + n + N, N);
+      if (ns == 0) return -1;
+      for (int k = 0; k < ns; k++) segs[n + ns + k] = segs[n + N + k];  // w after o
+    }
+    D->segoff[d] = n;
+    D->nseg[d] = ns;
+    n += 2 * ns;
+  }
+  return n;
+}
+
+// pencil groups of any sweep launch: ceil(n_t1 / kTP) * n_t2
+long long fallback_groups(const int GD[3]) {
+  long long ngrp = 1;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      if (a != b) ngrp = std::max(ngrp, (long long)((GD[a] + kTP - 1) / kTP) * GD[b]);
+  return ngrp;
 }
 
 int launch_pressure(hipPPMData *D, const hipPPMParameter *P, double min_support, int want_dt,
@@ -654,11 +718,7 @@ int hipPPMInitData(hipPPMData **Data, const hipPPMParameter *Para, void *stream)
         alloc_dev((void **)&D->dtmin, sizeof(unsigned long long)) ||
         alloc_dev((void **)&D->counters, 4 * sizeof(unsigned long long)))
       return ENZO_PPM_ERR_HIP;
-    // pencil groups of any sweep launch: ceil(n_t1 / kTP) * n_t2
-    long long ngrp = 1;
-    for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++)
-        if (a != b) ngrp = std::max(ngrp, (long long)((D->GD[a] + kTP - 1) / kTP) * D->GD[b]);
+    const long long ngrp = fallback_groups(D->GD);
     D->fb_groups = ngrp;
     if (alloc_dev((void **)&D->fb_flag, sizeof(int) * ngrp) ||
         alloc_dev((void **)&D->fb_list, sizeof(unsigned) * ngrp) ||
@@ -669,25 +729,15 @@ int hipPPMInitData(hipPPMData **Data, const hipPPMParameter *Para, void *stream)
   // geometry tables and segments (cheap; rebuilt on every init)
   st = upload_geometry(D, Para);
   if (st != ENZO_PPM_SUCCESS) return st;
-  std::vector<int> segs;
-  for (int d = 0; d < 3; d++) {
-    const int N = D->GD[d];
-    const int i1 = (int)Para->GridStartIndex[d] + 1, i2 = (int)Para->GridEndIndex[d] + 1;
-    std::vector<int> o, w;
-    if (d < Para->GridRank && N > 1) {
-      if (!build_segments<kTP, kTS>(N, i1, i2, o, w)) return ENZO_PPM_ERR_ARGUMENT;
-    }
-    D->segoff[d] = (int)segs.size();
-    D->nseg[d] = (int)o.size();
-    segs.insert(segs.end(), o.begin(), o.end());
-    segs.insert(segs.end(), w.begin(), w.end());
-  }
+  std::vector<int> segs(2 * 3 * (size_t)ENZO_PPM_MAX_ANY_SINGLE_DIRECTION);
+  const int nsegs = plan_segments(D, Para, segs.data());
+  if (nsegs < 0) return ENZO_PPM_ERR_ARGUMENT;
   if (D->segbuf) (void)hipFree(D->segbuf);
   D->segbuf = nullptr;
-  if (alloc_dev((void **)&D->segbuf, sizeof(int) * std::max<size_t>(1, segs.size())))
+  if (alloc_dev((void **)&D->segbuf, sizeof(int) * std::max(1, nsegs)))
     return ENZO_PPM_ERR_HIP;
-  if (!segs.empty())
-    HIPCHK(hipMemcpy(D->segbuf, segs.data(), sizeof(int) * segs.size(), hipMemcpyHostToDevice));
+  if (nsegs > 0)
+    HIPCHK(hipMemcpy(D->segbuf, segs.data(), sizeof(int) * nsegs, hipMemcpyHostToDevice));
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, D->stream, D->counters, 4);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemsetAsync(D->fb_flag, 0, sizeof(int) * D->fb_groups, D->stream));
@@ -889,6 +939,187 @@ int hipPPMSolvePPM_DE(const enzo_ppm_grid *g, const enzo_ppm_globals *gl, enzo_i
   if (st == ENZO_PPM_SUCCESS) st = hipPPMGetBaryon(D, &Para, g->BaryonField);
   if (!Data) hipPPMDestroy(local);
   return st;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// The entry point that allocates nothing (hipPPMSolvePPM_DE_NoAlloc): its
+// device buffers are carved from `scratch`, one static array per GPU that
+// the GPU holds until the process ends.  Host code never passes `scratch`
+// itself (in host code the name is a stand-in, not a device address): the
+// device address comes from hipGetSymbolAddress, once per device, with that
+// device current.
+constexpr size_t kScratchBytes = size_t(ENZO_PPM_NOALLOC_SCRATCH_MIB) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's scratch address (looked up on its first use)
+int scratch_of_current_device(void **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return ENZO_PPM_ERR_ARGUMENT;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ENZO_PPM_SUCCESS;
+}
+
+// consecutive 256-byte aligned pieces of the scratch array
+struct ScratchCarve {
+  unsigned char *base;
+  size_t used = 0;
+  template <class T>
+  T *take(size_t n) {
+    used = (used + 255) & ~(size_t)255;
+    T *p = reinterpret_cast<T *>(base + used);
+    used += n * sizeof(T);
+    return p;
+  }
+};
+
+unsigned grid_blocks(long long items) {
+  long long nb = (items + 255) / 256;
+  return (unsigned)std::max(1LL, std::min(nb, 8192LL));
+}
+
+}  // namespace
+
+extern "C" {
+
+int hipPPMSolvePPM_DE_NoAlloc(const enzo_ppm_grid *g, const enzo_ppm_globals *gl,
+                              enzo_int CycleNumber, enzo_float *CellWidthTemp[],
+                              enzo_int GravityOn, enzo_int NumberOfColours, const enzo_int colnum[],
+                              enzo_float MinimumSupportEnergyCoefficient, void *stream,
+                              enzo_long_int *FallbackCells) {
+  if (!g || !gl || !g->BaryonField) return ENZO_PPM_ERR_ARGUMENT;
+  if (gl->EOSType != 0 || gl->ReconstructionMethod != ENZO_PPM_RECONSTRUCTION_PPM)
+    return ENZO_PPM_ERR_NOT_SUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return ENZO_PPM_ERR_NO_DEVICE;
+  hipPPMParameter Para;
+  const enzo_long_int ggs[3] = {0, 0, 0};
+  int st = hipPPMInitParameter(
+      &Para, g->DensNum, g->TENum, g->Vel1Num, g->Vel2Num, g->Vel3Num, g->GENum,
+      ENZO_PPM_MAX_NUMBER_OF_BARYON_FIELDS, ggs, g->GridStartIndex, g->GridEndIndex,
+      g->GridDimension, CellWidthTemp ? CellWidthTemp[0][0] : 1.0,
+      CellWidthTemp ? CellWidthTemp[1][0] : 1.0, CellWidthTemp ? CellWidthTemp[2][0] : 1.0,
+      gl->Gamma, gl->PPMDiffusionParameter, gl->PPMFlatteningParameter,
+      gl->PPMSteepeningParameter, gl->ConservativeReconstruction, gl->PositiveReconstruction,
+      GravityOn, gl->DualEnergyFormalism, gl->DualEnergyFormalismEta1,
+      gl->DualEnergyFormalismEta2, gl->PressureFree, gl->tiny_number, gl->RiemannSolver,
+      gl->RiemannSolverFallback, NumberOfColours, colnum, g->GridRank, gl->SmallRho);
+  if (st != ENZO_PPM_SUCCESS) return st;
+  if (CellWidthTemp)
+    for (int d = 0; d < 3; d++) hipPPMSetCellWidth(&Para, d, CellWidthTemp[d]);
+  st = check_para(&Para);
+  if (st != ENZO_PPM_SUCCESS) return st;
+  if (GravityOn && !g->AccelerationField) return ENZO_PPM_ERR_ARGUMENT;
+
+  hipPPMData D;  // no device buffers of its own: every pointer is into scratch
+  for (int d = 0; d < 3; d++) D.GD[d] = (int)Para.GridDimension[d];
+  D.cells = (long long)D.GD[0] * D.GD[1] * D.GD[2];
+  D.nslot = nslot_of(&Para);
+  D.ncol = (int)Para.NumberOfColours;
+  D.gravity = GravityOn != 0;
+  D.stream = (hipStream_t)stream;
+  HIPCHK(hipGetDevice(&D.device));
+  void *base = nullptr;
+  st = scratch_of_current_device(&base);
+  if (st != ENZO_PPM_SUCCESS) return st;
+  ScratchCarve sc{static_cast<unsigned char *>(base)};
+  const size_t cells = (size_t)D.cells;
+  D.A = sc.take<double>(cells * D.nslot);
+  D.B = sc.take<double>(cells * D.nslot);
+  D.pres = sc.take<double>(cells);
+  long long fcap = 0;
+  for (int d = 0; d < 3; d++) {
+    D.tab[d] = sc.take<double>((size_t)NTAB * D.GD[d]);
+    D.cwdev[d] = sc.take<double>((size_t)D.GD[d]);
+    fcap = std::max(fcap, (long long)D.GD[d] * D.GD[(d + 2) % 3]);
+  }
+  D.flat_cap = fcap;
+  D.flat = sc.take<double>((size_t)fcap);
+  D.qc = sc.take<double>((size_t)std::max(D.GD[0], std::max(D.GD[1], D.GD[2])));
+  D.dt = sc.take<double>(1);
+  D.dtmin = sc.take<unsigned long long>(1);
+  D.counters = sc.take<unsigned long long>(4);
+  D.fb_groups = fallback_groups(D.GD);
+  D.fb_flag = sc.take<int>((size_t)D.fb_groups);
+  D.fb_list = sc.take<unsigned>((size_t)D.fb_groups);
+  D.fb_state = sc.take<unsigned>(2);
+  static thread_local int segs[2 * 3 * ENZO_PPM_MAX_ANY_SINGLE_DIRECTION];
+  const int nsegs = plan_segments(&D, &Para, segs);
+  if (nsegs < 0) return ENZO_PPM_ERR_ARGUMENT;
+  D.segbuf = sc.take<int>((size_t)std::max(1, nsegs));
+  if (sc.used > kScratchBytes) {
+    snprintf(g_last_error, sizeof(g_last_error),
+             "hipPPMSolvePPM_DE_NoAlloc: the grid needs %zu bytes of scratch, more than its "
+             "%zu (%d MiB)", sc.used, kScratchBytes, ENZO_PPM_NOALLOC_SCRATCH_MIB);
+    return ENZO_PPM_ERR_ARGUMENT;
+  }
+  if (GravityOn)
+    for (int d = 0; d < 3; d++) {
+      if (d < Para.GridRank && !g->AccelerationField[d]) return ENZO_PPM_ERR_ARGUMENT;
+      D.accel_ext[d] = d < Para.GridRank ? g->AccelerationField[d] : nullptr;
+    }
+  SlotPtrs F;
+  for (int s = 0; s < D.nslot; s++) {
+    const long long f = slot_field(&Para, s);
+    F.f[s] = f >= 0 ? g->BaryonField[f] : nullptr;
+    if (f >= 0 && !F.f[s]) return ENZO_PPM_ERR_ARGUMENT;
+  }
+  // scratch keeps the last call's contents: clear what must start at zero
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, D.stream, D.counters, 4);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_blocks(D.fb_groups)), dim3(256), 0, D.stream,
+                     reinterpret_cast<unsigned *>(D.fb_flag), D.fb_groups);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, D.stream, D.fb_state, 2LL);
+  HIPCHK(hipGetLastError());
+  st = upload_geometry(&D, &Para);
+  if (st != ENZO_PPM_SUCCESS) return st;
+  if (nsegs > 0) {
+    HIPCHK(hipMemcpyAsync(D.segbuf, segs, sizeof(int) * nsegs, hipMemcpyHostToDevice, D.stream));
+    HIPCHK(hipStreamSynchronize(D.stream));  // segs is reused by the next call of this thread
+  }
+  hipLaunchKernelGGL(gather_slots_kernel, dim3(grid_blocks((long long)cells * D.nslot)), dim3(256),
+                     0, D.stream, D.A, D.cells, D.nslot, F);
+  HIPCHK(hipGetLastError());
+  // stale pressure (Grid_SolvePPM_DE.C:58-61), as hipPPMSolvePPM_DE takes it
+  st = launch_pressure(&D, &Para, 0.0, 0, 1.0);
+  if (st == ENZO_PPM_SUCCESS && MinimumSupportEnergyCoefficient != 0.0)
+    st = launch_pressure(&D, &Para, MinimumSupportEnergyCoefficient, 0, 1.0);
+  const int rank = (int)g->GridRank;
+  const int ixyz = (int)(CycleNumber % rank);
+  for (int n = ixyz; n < ixyz + rank && st == ENZO_PPM_SUCCESS; n++) {
+    const int dir = n % rank;
+    if (g->GridEndIndex[dir] - g->GridStartIndex[dir] + 1 <= 1) continue;
+    st = launch_sweep(&D, &Para, g->dtFixed, nullptr, dir);
+  }
+  if (st != ENZO_PPM_SUCCESS) return st;
+  hipLaunchKernelGGL(scatter_slots_kernel, dim3(grid_blocks((long long)cells * D.nslot)),
+                     dim3(256), 0, D.stream, D.A, D.cells, D.nslot, F);
+  HIPCHK(hipGetLastError());
+  unsigned long long c[4];
+  HIPCHK(hipMemcpyAsync(c, D.counters, sizeof(c), hipMemcpyDeviceToHost, D.stream));
+  HIPCHK(hipStreamSynchronize(D.stream));
+  if (FallbackCells) *FallbackCells = (enzo_long_int)c[0];
+  if (c[3] != 0) {
+    snprintf(g_last_error, sizeof(g_last_error),
+             "the per-cell fallback was needed on a pencil longer than %d cells", kFbTS + 1);
+    return ENZO_PPM_ERR_NOT_SUPPORTED;
+  }
+  if (c[2] != 0) return ENZO_PPM_ERR_NEGATIVE;
+  if (c[0] != 0 && Para.RiemannSolverFallback == 0) return ENZO_PPM_ERR_NEGATIVE;
+  return ENZO_PPM_SUCCESS;
 }
 
 enzo_float *hipPPMDeviceField(hipPPMData *D, enzo_int field) {

@@ -493,6 +493,18 @@ int correct_refined_fluxes(double *A, long long cells, const int GD[3], hipStrea
                            enzo_int CorrectParentBoundaryFlux, const enzo_int *ColourKind);
 
 // ------------------------------------------------------------------------
+// Host: BaryonField number of state slot s, or -1 (a slot the grid does
+// not hold)
+inline long long slot_field(const hipPPMParameter *P, int s) {
+  if (s == S_D) return P->DensNum;
+  if (s == S_E) return P->TENum;
+  if (s == S_VX) return P->Vel1Num;
+  if (s == S_VY) return P->GridRank > 1 ? P->Vel2Num : -1;
+  if (s == S_VZ) return P->GridRank > 2 ? P->Vel3Num : -1;
+  if (s == S_GE) return P->DualEnergyFormalism ? P->GENum : -1;
+  return s - S_COL0 < P->NumberOfColours ? P->colnum[s - S_COL0] : -1;
+}
+
 // Host: Enzo field number of each device slot (-1: none)
 inline void slot_fields(const hipPPMParameter *P, std::vector<long long> &f) {
   f.assign(S_COL0 + (int)P->NumberOfColours, -1);

@@ -12,8 +12,9 @@ namespace ppm {
 // Coefficient tables of one axis from its cell widths, with the reference's
 // expressions (inteuler.F:140-158 grid coefficients, :166-175 steepening
 // sums, calcdiss.F:382-392 diffusion factors, dx2i = 0.5/dx).
-inline void build_tables(const double *dx, int N, std::vector<double> &t) {
-  t.assign((size_t)NTAB * N, 0.0);
+// t holds NTAB * N doubles.
+inline void build_tables(const double *dx, int N, double *t) {
+  for (size_t k = 0; k < (size_t)NTAB * N; k++) t[k] = 0.0;
   auto X = [&](int i) { return dx[i - 1]; };
   auto S = [&](int tt, int i) -> double & { return t[(size_t)tt * N + (i - 1)]; };
   for (int i = 1; i <= N; i++) {
@@ -45,11 +46,15 @@ inline void build_tables(const double *dx, int N, std::vector<double> &t) {
     if (i - 1 >= 1) S(T_DIFFX, i) = 0.25 * (X(i) + X(i - 1));
   }
 }
+inline void build_tables(const double *dx, int N, std::vector<double> &t) {
+  t.assign((size_t)NTAB * N, 0.0);
+  build_tables(dx, N, t.data());
+}
 
 // If every width of the axis is the same double, every defined table entry is
 // the same double too: returns true and the entries (taken at an interior
 // cell) in k[NTAB].
-inline bool uniform_axis(const double *dx, int N, const std::vector<double> &t, double *k) {
+inline bool uniform_axis(const double *dx, int N, const double *t, double *k) {
   for (int i = 1; i < N; i++)
     if (dx[i] != dx[0]) return false;
   int c = N >= 5 ? N / 2 + 1 : 0;  // 1-based interior cell with every entry defined
@@ -57,14 +62,17 @@ inline bool uniform_axis(const double *dx, int N, const std::vector<double> &t, 
   if (!c) k[T_DX] = dx[0];
   return true;
 }
+inline bool uniform_axis(const double *dx, int N, const std::vector<double> &t, double *k) {
+  return uniform_axis(dx, N, t.data(), k);
+}
 
 // Split cells 1..N into segments owned by one tile each; every tile must
 // fit its interfaces in TS threads and its loaded cells in LW.
+// Raw form: o[] and w[] hold up to cap entries; returns the number of
+// segments, 0 when none fits (or more than cap would be needed).
 template <int TP, int TS>
-inline bool build_segments(int N, int i1, int i2, std::vector<int> &o, std::vector<int> &w) {
-  for (int nseg = (N + TS - 8) / (TS - 7); nseg <= N; nseg++) {
-    o.clear();
-    w.clear();
+inline int build_segments(int N, int i1, int i2, int *o, int *w, int cap) {
+  for (int nseg = (N + TS - 8) / (TS - 7); nseg <= N && nseg <= cap; nseg++) {
     int base = N / nseg, rem = N % nseg, start = 1;
     bool ok = true;
     for (int s = 0; s < nseg; s++) {
@@ -76,13 +84,22 @@ inline bool build_segments(int N, int i1, int i2, std::vector<int> &o, std::vect
       int L1 = std::max(IB + 2, std::min(start + ww, N));
       // one slope item per thread (cells [IA-2, IB+1]) and the loaded cells in LW
       if (NI + 4 > TS || L1 - L0 + 1 > Cfg<TP, TS>::LW || L0 < 1 || L1 > N) { ok = false; break; }
-      o.push_back(start);
-      w.push_back(ww);
+      o[s] = start;
+      w[s] = ww;
       start += ww;
     }
-    if (ok) return true;
+    if (ok) return nseg;
   }
-  return false;
+  return 0;
+}
+template <int TP, int TS>
+inline bool build_segments(int N, int i1, int i2, std::vector<int> &o, std::vector<int> &w) {
+  o.assign(N, 0);
+  w.assign(N, 0);
+  const int n = build_segments<TP, TS>(N, i1, i2, o.data(), w.data(), N);
+  o.resize(n);
+  w.resize(n);
+  return n > 0;
 }
 
 }  // namespace ppm

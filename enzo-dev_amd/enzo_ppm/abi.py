@@ -176,6 +176,8 @@ PROTOTYPES = {
     "hipPPMFallbackCount": (LI, [_H]),
     "hipPPMSolvePPM_DE": (C.c_int, [C.POINTER(EnzoPPMGrid), C.POINTER(EnzoPPMGlobals), I, I, _FL,
                                     C.POINTER(PF), PL, I, I, PI, _D, C.POINTER(_H), _H]),
+    "hipPPMSolvePPM_DE_NoAlloc": (C.c_int, [C.POINTER(EnzoPPMGrid), C.POINTER(EnzoPPMGlobals), I,
+                                            C.POINTER(PF), I, I, PI, _D, _H, PL]),
     "hipPPMDeviceField": (PF, [_H, I]),
     "hipPPMPressureAndTimeStep": (C.c_int, [_H, _P, _D, _D, PF]),
     "hipPPMStep": (C.c_int, [_H, _P, I, _D]),

@@ -241,6 +241,33 @@ int hipPPMSolvePPM_DE(const enzo_ppm_grid *grid, const enzo_ppm_globals *globals
                       enzo_float MinimumSupportEnergyCoefficient,
                       hipPPMData **Data, void *stream);
 
+/* ---- the entry point that allocates nothing ------------------------------
+   No counterpart in the reference (its GPU path allocates device buffers on
+   every call, cuPPM.cu:111-186).  For callers that cannot allocate device
+   memory -- tight serving loops, harnesses that forbid hipMalloc: the
+   hipPPMSolvePPM_DE step (pressure once, the cycle-rotated sweeps, the
+   per-cell fallback) on fields that are already on the current device,
+   BaryonField[] and AccelerationField[] holding DEVICE pointers (updated in
+   place; CellWidthTemp, colnum and the structs stay on the host).  No face
+   fluxes are captured (use hipPPMSolvePPM_DE for subgrid fluxes).  Its
+   device buffers come from one static array of
+   ENZO_PPM_NOALLOC_SCRATCH_MIB MiB per GPU, held until the process ends:
+   a grid needs about 8 * cells * (2 * (6 + NumberOfColours) + 1) bytes of it
+   (e.g. up to 121^3 cells, ghosts included, with 3 colours) and a larger one
+   is refused with ENZO_PPM_ERR_ARGUMENT.  Every call on a GPU uses that
+   GPU's array, so calls that overlap on one GPU (other threads or streams)
+   race; calls in turn do not.  Calls no hipMalloc/hipFree; waits for its
+   stream before returning.  FallbackCells (may be NULL) receives the cells
+   that took the per-cell HLL fallback.  Results are bitwise those of
+   hipPPMSolvePPM_DE with NumberOfSubgrids = 0. */
+#define ENZO_PPM_NOALLOC_SCRATCH_MIB 256
+int hipPPMSolvePPM_DE_NoAlloc(const enzo_ppm_grid *grid, const enzo_ppm_globals *globals,
+                              enzo_int CycleNumber, enzo_float *CellWidthTemp[],
+                              enzo_int GravityOn, enzo_int NumberOfColours,
+                              const enzo_int colnum[],
+                              enzo_float MinimumSupportEnergyCoefficient, void *stream,
+                              enzo_long_int *FallbackCells);
+
 /* ---- device-resident multi-step use --------------------------------------- */
 
 /* Device pointer of BaryonField[field] (current buffer) or, with field = -1,

@@ -48,6 +48,7 @@
 #define __shared__ static __attribute__((section("hipcpu_lds")))
 extern "C" char __start_hipcpu_lds[], __stop_hipcpu_lds[];
 #define __launch_bounds__(...)
+#define __align__(n) __attribute__((aligned(n)))
 
 struct dim3 {
   unsigned x, y, z;
@@ -370,6 +371,13 @@ inline hipError_t hipGetLastError() { return hipSuccess; }
 inline const char *hipGetErrorString(hipError_t e) { return e == hipSuccess ? "no error" : "hipcpu error"; }
 inline hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
 inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+// a __device__ variable is an ordinary host object here: its address is its
+// device address
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) {
+  *p = const_cast<void *>(sym);
+  return hipSuccess;
+}
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
 
 struct hipcpu_event { std::chrono::steady_clock::time_point t; };

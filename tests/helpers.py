@@ -286,3 +286,43 @@ def level_flux_check(lib, kind, steps=2, graph=True, capture=(0, 2), solver=None
             raise AssertionError("save with a stale face list was accepted")
     finally:
         lv.close()
+
+
+def noalloc_solve(lib, g, glob, cycle, field_addrs, accel_addrs=None, min_support=0.0):
+    """hipPPMSolvePPM_DE_NoAlloc on grid `g`'s description with the fields at
+    the given addresses (device memory on the GPU; on the CPU build, host
+    arrays); returns (status, fallback cells)."""
+    import ctypes as C
+    from enzo_ppm.grid import make_globals
+    gs, _ = make_globals(**glob)
+    cg, keep = g.ctypes_grid()
+    bf = (abi.PF * abi.MAX_NUMBER_OF_BARYON_FIELDS)()
+    for i, a in enumerate(field_addrs):
+        bf[i] = C.cast(C.c_void_p(a), abi.PF)
+    cg.BaryonField = C.cast(bf, C.POINTER(abi.PF))
+    acc = None
+    if accel_addrs is not None:
+        acc = (abi.PF * 3)()
+        for i, a in enumerate(accel_addrs):
+            acc[i] = C.cast(C.c_void_p(a), abi.PF)
+        cg.AccelerationField = C.cast(acc, C.POINTER(abi.PF))
+    cw = g.ctypes_cellwidth()
+    colnum = abi.iarr(g.colnum, max(1, len(g.colnum)))
+    nfb = abi.LI(0)
+    st = lib.hipPPMSolvePPM_DE_NoAlloc(C.byref(cg), C.byref(gs), cycle, cw, int(g.gravity),
+                                       g.ncolours, colnum, min_support, None, C.byref(nfb))
+    del keep, bf, acc
+    return st, nfb.value
+
+
+# cases of the entry point that allocates nothing: two calls in a row, the
+# second on a smaller grid (per-cell fallback in the first; colours,
+# gravity, stretched widths and diffusion in the second)
+def noalloc_cases():
+    from enzo_ppm import problems
+    g1, glob1, cycle1 = fallback_case("3d_x_ok", ncolours=2)
+    g2, glob2, _ = problems.random_smooth([11, 9, 12], seed=43, ncolours=1, gravity=True,
+                                          stretch=0.1, flags=(1, 1, 1))
+    g2.fill_periodic()
+    g2.dt = oracle.compute_time_step("port", g2, glob2, 0.4)
+    return [(g1, glob1, cycle1), (g2, glob2, 2)]
