@@ -498,11 +498,11 @@ def main():
         # (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x 64 lanes, an FMA two flops;
         # profiles/ppm_sweep_kernel_turbulent256.md) over the live sweep time:
         # the arithmetic side of the kernel's bound
-        flops = (0.187e9 + 0.250e9 + 2 * 0.231e9 + 0.038e9) * 64
+        flops = (0.173e9 + 0.218e9 + 2 * 0.217e9 + 0.035e9) * 64
         out["fp64_valu"] = {"flop_per_launch": flops, "achieved_tflops": flops / sweep_avg_s / 1e12,
                             "peak_tflops": PEAK_FP64_TFLOPS,
                             "frac": flops / sweep_avg_s / 1e12 / PEAK_FP64_TFLOPS,
-                            "valu_instructions_per_launch": 1.407e9,
+                            "valu_instructions_per_launch": 1.242e9,
                             "note": "instruction counts from the committed rocprofv3 pass; "
                                     "time measured here"}
     if a.solver_steps > 0:
