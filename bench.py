@@ -313,24 +313,47 @@ def host_buffer_rate(lib, n, calls=3):
             "what": "hipPPMSolvePPM_DE with host arrays: upload, pressure, 3 sweeps, download"}
 
 
+def _level_records(lib, ngrids, seed, keep, cap):
+    """Plain-solver interface states of an AMR level's grids (Sedov blasts
+    in sinusoidal flow, axes 8-64), one step of each grid with its own ghost
+    zones (no periodic fill)."""
+    import numpy as np
+    from enzo_ppm import learn, problems
+    grids, glob, _ = problems.amr_level(ngrids, seed=seed, nmin=8, nmax=64, ncolours=0)
+    out = []
+    for g in grids:
+        r, _ = learn.harvest(g, glob, steps=1, keep=keep, cap=cap, boundary=False, lib=lib)
+        out.append(r)
+    return np.concatenate(out)
+
+
 def learned_bench(lib, n, steps, warmup, epochs, cap):
     """The learned two-shock start end to end: harvest of plain-solver
-    interface states from the 64^3 Sod tube (one step, every interface) and
-    the n^3 turbulent box (one step, one interface in `keep` so that about
-    `cap` records are kept); training on the GPU; Newton statistics on a
-    held-out n^3 field (another seed, one interface in 16); and the n^3 step
-    timed with the learned start."""
+    interface states -- the 64^3 Sod tube (one step, every interface), the
+    n^3 turbulent box (one step, one interface in `keep` so that about `cap`
+    records are kept), the AMR level's Sedov grids (64 of them) and a 128^3
+    cosmological box (Mach 100, sigma_lnrho 2; one in 4) --, training on the
+    GPU, the gate fitted on a sample of the training states; Newton
+    statistics on held-out states of each kind (other seeds); and the n^3
+    step timed with the learned start."""
+    import numpy as np
     from enzo_ppm import learn, problems
     t0 = time.perf_counter()
     g1, glob1, _ = problems.sod_x(64)
     r1, _ = learn.harvest(g1, glob1, steps=1, lib=lib)
+    r3 = _level_records(lib, 64, 11, 1, cap)
+    g4, glob4, _ = problems.cosmological(128, seed=12)
+    r4, _ = learn.harvest(g4, glob4, steps=1, keep=4, cap=cap, lib=lib)
+    del g4
     g2, glob2, _ = problems.turbulent(n, seed=2024)
     swept = 3 * (n + 1) * (n + 6) ** 2
-    keep = max(1, -(-swept // max(1, cap - len(r1))))
+    keep = max(1, -(-swept // max(1, cap - len(r1) - len(r3) - len(r4))))
     r2, _ = learn.harvest(g2, glob2, steps=1, keep=keep, cap=cap, lib=lib)
-    import numpy as np
-    rec = np.concatenate([r1, r2])
-    del r1, r2
+    del g2
+    rec = np.concatenate([r1, r2, r3, r4])
+    sources = {"sod64": len(r1), f"turbulent{n}": len(r2), "amr_level": len(r3),
+               "cosmological128": len(r4)}
+    del r1, r2, r3, r4
     t1 = time.perf_counter()
     gamma = glob2["Gamma"]
     w, mse = learn.train(rec, gamma, epochs=epochs, batch=8192, lr=3e-3, seed=0, lib=lib)
@@ -338,15 +361,33 @@ def learned_bench(lib, n, steps, warmup, epochs, cap):
     w, gate_table = learn.fit_gate(rec[::16], gamma, 1e-20, w, lib=lib)
     n_train = len(rec)
     del rec
+    held = {}
     gh, globh, _ = problems.turbulent(n, seed=4242)
     rh, _ = learn.harvest(gh, globh, steps=1, keep=16, cap=cap, lib=lib)
-    _, stats = learn.newton_stats(rh, gamma, 1e-20, w, lib=lib)
-    del rh
+    del gh
+    held[f"turbulent{n}"] = rh
+    held["amr_level"] = _level_records(lib, 64, 13, 1, cap)
+    gc, globc, _ = problems.cosmological(128, seed=14)
+    held["cosmological128"], _ = learn.harvest(gc, globc, steps=1, keep=4, cap=cap, lib=lib)
+    del gc
+    stats = {}
+    w0 = w.copy()
+    w0[learn.OFF_GATE] = 0.0
+    for k, r in held.items():
+        _, s = learn.newton_stats(r, gamma, 1e-20, w, lib=lib)
+        _, s0 = learn.newton_stats(r, gamma, 1e-20, w0, lib=lib)
+        s["network_everywhere"] = {"newton_updates_learned_start": s0["newton_updates_learned_start"],
+                                   "restarted_fraction": s0["restarted_fraction"],
+                                   "network_evaluated_fraction": s0["network_evaluated_fraction"]}
+        stats[k] = s
+    del held
     print("bench: learned-start timing", file=sys.stderr, flush=True)
     t = time_grid(lib, n, steps, warmup, weights=w)
     t.pop("grid")
-    return {"harvest": {"records": n_train, "sources": f"64^3 Sod (1 step, all interfaces) + "
-                                                       f"{n}^3 turbulent (1 step, 1 in {keep})",
+    return {"harvest": {"records": n_train, "by_source": sources,
+                        "sources": f"64^3 Sod (1 step, all interfaces) + {n}^3 turbulent (1 step, "
+                                   f"1 in {keep}) + 64 AMR-level Sedov grids (1 step each) + 128^3 "
+                                   "cosmological Mach 100 (1 step, 1 in 4)",
                         "seconds": t1 - t0},
             "training": {"epochs": epochs, "batch": 8192, "lr": 3e-3, "seconds": t2 - t1,
                          "mse_per_epoch": [float(x) for x in mse],

@@ -302,8 +302,8 @@ int check_para(const hipPPMParameter *P) {
                  "RiemannSolver HLL/HLLC: pencils longer than %d cells are not supported", kFbTS + 1);
         return ENZO_PPM_ERR_NOT_SUPPORTED;
       }
-  if (P->PPMFlatteningParameter != 0 && P->PPMFlatteningParameter != 1)
-    return ENZO_PPM_ERR_NOT_SUPPORTED;
+  if (P->PPMFlatteningParameter < 0 || P->PPMFlatteningParameter > 3)
+    return ENZO_PPM_ERR_NOT_SUPPORTED;  // calcdiss.F has types 1-3
   if (P->PPMDiffusionParameter != 0 && P->PPMDiffusionParameter != 1)
     return ENZO_PPM_ERR_NOT_SUPPORTED;  // idiff = 2 reads undefined values in calcdiss.F
   return ENZO_PPM_SUCCESS;
@@ -350,6 +350,7 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
     F.cells = D->cells;
     F.n_outer = D->GD[oax];
     F.iflatten = iflat;
+    F.gamma = P->Gamma;
     F.A = D->A;
     F.pres = D->pres;
     F.flat = D->flat;
@@ -1147,8 +1148,11 @@ int hipPPMStep(hipPPMData *D, const hipPPMParameter *P, enzo_int CycleNumber,
   if (!D || !P) return ENZO_PPM_ERR_ARGUMENT;
   const int rank = (int)P->GridRank;
   if (D->comoving) {  // OldBaryonField of the time-centred expansion pressure
-    HIPCHK(hipMemcpyAsync(D->old, D->A, sizeof(double) * (size_t)(S_GE + 1) * D->cells,
-                          hipMemcpyDeviceToDevice, D->stream));
+    int slots[5];
+    const int ns = expansion_old_slots((int)P->DualEnergyFormalism, rank, slots);
+    for (int k = 0; k < ns; k++)
+      HIPCHK(hipMemcpyAsync(D->old + (size_t)slots[k] * D->cells, D->A + (size_t)slots[k] * D->cells,
+                            sizeof(double) * (size_t)D->cells, hipMemcpyDeviceToDevice, D->stream));
     D->old_valid = true;
   }
   const int ixyz = (int)(CycleNumber % rank);

@@ -17,14 +17,28 @@
 
 namespace ppm {
 
+// A workgroup-uniform int as such (the compiler then keeps what derives
+// from it in scalar registers), and a per-work-item int the compiler must
+// treat as new at this point (no code that depends on it is hoisted above
+// it).  The CPU execution of this source (tests/emu/hipcpu) runs work-items
+// one by one: there both are the value itself.
+#ifdef HIPCPU_RUNTIME
+#define PPM_UNIFORM(x) (x)
+#define PPM_OPAQUE_VGPR(x) ((void)(x))
+#else
+#define PPM_UNIFORM(x) __builtin_amdgcn_readfirstlane(x)
+#define PPM_OPAQUE_VGPR(x) asm volatile("" : "+v"(x))
+#endif
+
 // Workgroup executor of the tile phases (ppm_sweep.hpp): run = phase then
 // barrier; count = number of work-items with a nonzero result; first = the
 // result of work-item 0, broadcast.
 struct DevExec {
   Regs r;
+  int tid = (int)threadIdx.x;  // the work-item's index passed to the phases
   template <class F>
   __device__ void run(F f) {
-    f((int)threadIdx.x, r);
+    f(tid, r);
     __syncthreads();
   }
   // a wavefront vote per wavefront, its population count in LDS, the sum
@@ -32,7 +46,7 @@ struct DevExec {
   template <class F>
   __device__ int count(F f) {
     __shared__ int part[16];
-    int v = f((int)threadIdx.x, r);
+    int v = f(tid, r);
     const unsigned long long b = __ballot(v != 0);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = __popcll(b);
     __syncthreads();
@@ -44,7 +58,7 @@ struct DevExec {
   template <class F>
   __device__ int first(F f) {
     __shared__ int v0;
-    int v = f((int)threadIdx.x, r);
+    int v = f(tid, r);
     if (threadIdx.x == 0) v0 = v;
     __syncthreads();
     v = v0;
@@ -325,6 +339,22 @@ struct ExpandParams {
   double *A;    // current state (slots as in the sweeps)
   double *old;  // state at the step's start, slots S_D..S_GE (interp only)
 };
+
+// The slots of the step's starting state that expand_cell reads (the
+// interpolated pressure's inputs): density and gas energy with dual energy,
+// density, total energy and the rank's velocities without it.  Returns how
+// many, in s[] (at most 5).
+PPM_HD int expansion_old_slots(int idual, int rank, int s[5]) {
+  int n = 0;
+  s[n++] = S_D;
+  if (idual) {
+    s[n++] = S_GE;
+  } else {
+    s[n++] = S_E;
+    for (int d = 0; d < rank; d++) s[n++] = S_VX + d;
+  }
+  return n;
+}
 
 __device__ inline void expand_cell(const ExpandParams &X, long long g) {
   double *A = X.A;

@@ -73,6 +73,58 @@ void level_sweep_kernel(const SweepParams *__restrict__ params, const Work *__re
   }
 }
 
+// The same sweep as one persistent launch: a few workgroups per CU, each
+// pulling work items until the list is done.  The items are split into one
+// contiguous range per XCD (workgroup b runs on XCD b % kXCD, each XCD with
+// its own L2; with fewer than kXCD workgroups, one range per workgroup), and
+// the workgroups of a range take its items in order from a counter of their
+// own, queue[x]; queue[kXCD + x] counts the range's workgroups that have
+// finished, and the last of them puts both back to 0 for the next launch
+// (graph replays included).  Every workgroup leaves the loop once its pull
+// passes its range's end, so every item is taken exactly once.
+template <int TP, int TS, int UNI, int LEARN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void level_sweep_persistent_kernel(const SweepParams *__restrict__ params,
+                                   const Work *__restrict__ work, int nitems, unsigned *queue) {
+  __shared__ double sm[Cfg<TP, TS>::SMEM];
+  __shared__ int next;
+  PPM_LDS_ENTER(sm);
+  const int nparts = (int)gridDim.x < kXCD ? (int)gridDim.x : kXCD;
+  const int x = (int)(blockIdx.x % nparts);
+  const int q = nitems / nparts, rem = nitems % nparts;
+  const int lo = x * q + (x < rem ? x : rem), hi = lo + q + (x < rem ? 1 : 0);
+  for (;;) {
+    if (threadIdx.x == 0) next = lo + (int)atomicAdd(&queue[x], 1u);
+    __syncthreads();
+    // uniform over the workgroup: the item, and with it the grid's
+    // parameters, go to scalar registers as in the one-item kernel
+    const int k = PPM_UNIFORM(next);
+    __syncthreads();  // `next` is free for the next pull
+    if (k >= hi) break;
+    const Work w = work[k];
+    const SweepParams &P = params[w.grid];
+    DevExec ex;
+    // the work-item index as a value of this iteration: otherwise the
+    // compiler hoists every tid-derived address of the tile out of the loop
+    // and keeps them all live (spilling registers)
+    PPM_OPAQUE_VGPR(ex.tid);
+    TileCounts tc = sweep_tile<TP, TS, UNI, DevExec, LEARN>(P, w.tile, sm, ex);
+    if (threadIdx.x == 0) {
+      if (tc.bad || tc.err) flag_group(P, w.tile / P.nseg);
+      if (tc.chains) atomicAdd(&P.counters[1], (unsigned long long)tc.chains);
+    }
+  }
+  if (threadIdx.x == 0) {
+    const unsigned nx = gridDim.x / nparts + ((int)(gridDim.x % nparts) > x ? 1u : 0u);
+    __threadfence();
+    if (atomicAdd(&queue[kXCD + x], 1u) == nx - 1) {
+      queue[x] = 0u;
+      queue[kXCD + x] = 0u;
+      __threadfence();
+    }
+  }
+}
+
 // fallback over the level's list: entries are level-wide group indices
 // (the grid's first group + the group), grp_grid maps them to their grid
 template <int LEARN, int TS = kFbTS>
@@ -118,16 +170,21 @@ __global__ __launch_bounds__(256) void level_pressure_kernel(const PresParams *_
   if (q.want_dt) block_min_dt(local, q.dtmin);
 }
 
-// comoving: the state at the step's start (slots S_D..S_GE) kept per chunk
+// comoving: the slots of the state at the step's start that the expansion
+// terms read (expansion_old_slots), kept per chunk
 __global__ __launch_bounds__(256) void level_save_old_kernel(const ExpandParams *__restrict__ X,
                                                              const Work *__restrict__ work) {
   const Work w = work[blockIdx.x];
   const ExpandParams &x = X[w.grid];
   const long long lim = (long long)w.tile + kPresChunk;
   const long long end = x.cells < lim ? x.cells : lim;
-  for (int s = 0; s <= S_GE; s++)
+  int slots[5];
+  const int ns = expansion_old_slots(x.idual, x.rank, slots);
+  for (int k = 0; k < ns; k++) {
+    const long long s = slots[k];
     for (long long g = (long long)w.tile + threadIdx.x; g < end; g += blockDim.x)
       x.old[s * x.cells + g] = x.A[s * x.cells + g];
+  }
 }
 
 // comoving expansion terms per chunk
@@ -281,6 +338,7 @@ struct hipPPMLevel {
   struct Launch {
     int shape, uni;
     long long first, count;
+    long long queue;  // its 2 * kXCD counters of the persistent launch, at queues + 2 kXCD queue
   };
   std::vector<Launch> launches[3];
   std::vector<long long> launch_slices[3];  // index of the first launch of each slice
@@ -289,7 +347,16 @@ struct hipPPMLevel {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int parity = 0;                   // 0: state in A
+  // the pressure array holds the current state's pressure
+  // (hipPPMLevelTimeStep computed it and nothing has changed the state
+  // since), so the next hipPPMLevelStep need not compute it again
+  bool pres_current = false;
   bool use_graph = true;
+  // persistent sweep launches (level_sweep_persistent_kernel) instead of one
+  // workgroup per work item; persistent_blocks workgroups per launch
+  bool persistent = false;
+  int persistent_blocks = 1024;
+  unsigned *queues = nullptr;
   std::map<int, hipGraphExec_t> graphs;
   double graph_cfl = -1.0;          // cfl the Courant-step graphs were captured with
   long long fallback_total = 0;
@@ -360,7 +427,7 @@ void free_level(hipPPMLevel *L) {
                   L->dtmin, L->counters, L->fb_flag, L->fb_list, L->fb_state, L->grp_grid,
                   L->dgrp_off, L->dparams,
                   L->dflat, L->dpres, L->dwork, L->dpwork, L->learn, L->cws_dt, L->old,
-                  L->dexp, L->dboxes, L->dgeom, L->dfball};
+                  L->dexp, L->dboxes, L->dgeom, L->dfball, L->queues};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (L->own_stream && L->stream) (void)hipStreamDestroy(L->stream);
@@ -483,7 +550,7 @@ int build_plan(hipPPMLevel *L) {
   std::vector<FlatParams> hflat((size_t)2 * 3 * G);
   std::vector<Work> work;
   std::vector<unsigned> fball;
-  long long ngroups = 0;
+  long long ngroups = 0, nlaunch = 0;
   L->grp_off.assign(G, 0);
   for (int g = 0; g < G; g++) {  // flags per grid: the largest group count of any axis
     long long m = 1;
@@ -584,6 +651,7 @@ int build_plan(hipPPMLevel *L) {
         F.cells = L->cells[g];
         F.n_outer = gd[oax];
         F.iflatten = S.iflatten;
+        F.gamma = S.gamma;
         F.A = S.A;
         F.pres = S.pres;
         F.flat = L->flat + L->flat_off[g];
@@ -633,7 +701,7 @@ int build_plan(hipPPMLevel *L) {
             }
           }
           const long long cnt = (long long)work.size() - first;
-          if (cnt > 0) L->launches[d].push_back(hipPPMLevel::Launch{s, u, first, cnt});
+          if (cnt > 0) L->launches[d].push_back(hipPPMLevel::Launch{s, u, first, cnt, nlaunch++});
         }
     }
     L->launch_slices[d].push_back((long long)L->launches[d].size());
@@ -683,6 +751,12 @@ int build_plan(hipPPMLevel *L) {
   if (L->dfball) (void)hipFree(L->dfball);
   L->dfball = nullptr;
   if (dev_alloc(&L->dfball, fball.size())) return ENZO_PPM_ERR_HIP;
+  // the persistent launches' counters start at 0 (and their last workgroups
+  // leave them at 0)
+  if (L->queues) (void)hipFree(L->queues);
+  L->queues = nullptr;
+  if (dev_alloc(&L->queues, (size_t)std::max(1LL, nlaunch) * 2 * kXCD)) return ENZO_PPM_ERR_HIP;
+  LCHK(hipMemset(L->queues, 0, sizeof(unsigned) * (size_t)std::max(1LL, nlaunch) * 2 * kXCD));
   if (!fball.empty())
     LCHK(hipMemcpy(L->dfball, fball.data(), sizeof(unsigned) * fball.size(), hipMemcpyHostToDevice));
   drop_graphs(L);
@@ -730,24 +804,28 @@ int launch_level_sweep(hipPPMLevel *L, int d) {
       const auto &ln = L->launches[d][li];
       const Work *w = L->dwork + ln.first;
       const dim3 grid((unsigned)ln.count), blk(256);
+      const dim3 pgrid((unsigned)std::min<long long>(ln.count, L->persistent_blocks));
+      unsigned *queue = L->queues + (size_t)2 * kXCD * ln.queue;
+      const int nit = (int)ln.count;
+#define LEVEL_LAUNCH1(TP, TS, U, LRN)                                                         \
+  if (L->persistent)                                                                        \
+    hipLaunchKernelGGL((level_sweep_persistent_kernel<TP, TS, U, LRN>), pgrid, blk, 0,       \
+                       L->stream, P, w, nit, queue);                                         \
+  else                                                                                      \
+    hipLaunchKernelGGL((level_sweep_kernel<TP, TS, U, LRN>), grid, blk, 0, L->stream, P, w);
 #define LEVEL_LAUNCH(S, TP, TS)                                                             \
   if (ln.shape == S) {                                                                    \
     if (L->learned) {                                                                     \
-      if (ln.uni)                                                                         \
-        hipLaunchKernelGGL((level_sweep_kernel<TP, TS, 1, 1>), grid, blk, 0, L->stream, P, w); \
-      else                                                                                \
-        hipLaunchKernelGGL((level_sweep_kernel<TP, TS, 0, 1>), grid, blk, 0, L->stream, P, w); \
+      if (ln.uni) { LEVEL_LAUNCH1(TP, TS, 1, 1) } else { LEVEL_LAUNCH1(TP, TS, 0, 1) }   \
     } else {                                                                              \
-      if (ln.uni)                                                                         \
-        hipLaunchKernelGGL((level_sweep_kernel<TP, TS, 1, 0>), grid, blk, 0, L->stream, P, w); \
-      else                                                                                \
-        hipLaunchKernelGGL((level_sweep_kernel<TP, TS, 0, 0>), grid, blk, 0, L->stream, P, w); \
+      if (ln.uni) { LEVEL_LAUNCH1(TP, TS, 1, 0) } else { LEVEL_LAUNCH1(TP, TS, 0, 0) }   \
     }                                                                                     \
   }
       LEVEL_LAUNCH(0, 16, 16)
       LEVEL_LAUNCH(1, 8, 32)
       LEVEL_LAUNCH(2, 4, 64)
 #undef LEVEL_LAUNCH
+#undef LEVEL_LAUNCH1
       LCHK(hipGetLastError());
     }
     LCHK(launch_level_fallback(L, P, L->fb_ts[d], kFbBlocks, L->ngroups));
@@ -756,9 +834,11 @@ int launch_level_sweep(hipPPMLevel *L, int d) {
   return ENZO_PPM_SUCCESS;
 }
 
-// one level step: pressure (+ Courant dt) of every grid, then the sweeps in
-// the rotated order of CycleNumber
-int enqueue_step(hipPPMLevel *L, int cycle, int want_dt, double cfl) {
+// one level step: pressure (+ Courant dt) of every grid -- unless
+// `pressure_current` (the pressure array already holds this state's
+// pressure and dt is given) -- then the sweeps in the rotated order of
+// CycleNumber
+int enqueue_step(hipPPMLevel *L, int cycle, int want_dt, double cfl, bool pressure_current) {
   if (L->comoving) {
     hipLaunchKernelGGL(level_save_old_kernel, dim3((unsigned)L->npwork), dim3(256), 0, L->stream,
                        L->dexp + (size_t)L->parity * L->ngrids, L->dpwork);
@@ -766,8 +846,9 @@ int enqueue_step(hipPPMLevel *L, int cycle, int want_dt, double cfl) {
   }
   if (want_dt)
     hipLaunchKernelGGL(level_dt_init_kernel, dim3(1), dim3(64), 0, L->stream, L->dtmin);
-  hipLaunchKernelGGL(level_pressure_kernel, dim3((unsigned)L->npwork), dim3(256), 0, L->stream,
-                     L->dpres + (size_t)L->parity * L->ngrids, L->dpwork);
+  if (want_dt || !pressure_current)
+    hipLaunchKernelGGL(level_pressure_kernel, dim3((unsigned)L->npwork), dim3(256), 0, L->stream,
+                       L->dpres + (size_t)L->parity * L->ngrids, L->dpwork);
   LCHK(hipGetLastError());
   if (want_dt)
     hipLaunchKernelGGL(level_dt_final_kernel, dim3(1), dim3(64), 0, L->stream, L->dtmin, cfl, L->dt);
@@ -804,7 +885,7 @@ int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMP
     if (P.ConservativeReconstruction != 0 || P.PositiveReconstruction != 0 ||
         (P.RiemannSolver != ENZO_PPM_RIEMANN_TWOSHOCK && P.RiemannSolver != ENZO_PPM_RIEMANN_HLL &&
          P.RiemannSolver != ENZO_PPM_RIEMANN_HLLC) ||
-        (P.PPMFlatteningParameter != 0 && P.PPMFlatteningParameter != 1) ||
+        P.PPMFlatteningParameter < 0 || P.PPMFlatteningParameter > 3 ||
         (P.PPMDiffusionParameter != 0 && P.PPMDiffusionParameter != 1))
       return ENZO_PPM_ERR_NOT_SUPPORTED;
     // HLL / HLLC run in one-pencil workgroups: every swept pencil must fit
@@ -932,6 +1013,13 @@ int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMP
     if (hipStreamCreate(&L->stream) != hipSuccess) return fail(ENZO_PPM_ERR_HIP);
     L->own_stream = true;
   }
+  {  // persistent launches: 4 workgroups per CU (the sweep tiles' occupancy)
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        ncu > 0)
+      L->persistent_blocks = 4 * ncu;
+  }
   if (upload_geometry(L) != ENZO_PPM_SUCCESS) return fail(ENZO_PPM_ERR_HIP);
   if ((!L->hsegs.empty() && hipMemcpy(L->segs, L->hsegs.data(), sizeof(int) * L->hsegs.size(),
                                       hipMemcpyHostToDevice) != hipSuccess) ||
@@ -955,6 +1043,7 @@ int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMP
 int hipPPMLevelSetBaryon(hipPPMLevel *L, enzo_int grid, enzo_float *BaryonField[],
                          enzo_float *AccelerationField[]) {
   if (!L || !BaryonField || grid < 0 || grid >= L->ngrids) return ENZO_PPM_ERR_ARGUMENT;
+  L->pres_current = false;  // the state changes
   const hipPPMParameter &P = L->para[grid];
   long long fnum[S_COL0 + ENZO_PPM_MAX_COLOR];
   for (int s = 0; s < L->nslot; s++) fnum[s] = -1;
@@ -1008,18 +1097,20 @@ int hipPPMLevelGetBaryon(hipPPMLevel *L, enzo_int grid, enzo_float *BaryonField[
 int hipPPMLevelStep(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dtFixed, enzo_float cfl) {
   if (!L || CycleNumber < 0) return ENZO_PPM_ERR_ARGUMENT;
   const int want_dt = dtFixed > 0.0 ? 0 : 1;
+  const bool pres_done = L->pres_current && !want_dt;
+  L->pres_current = false;
   if (!want_dt) LCHK(hipMemcpyAsync(L->dt, &dtFixed, sizeof(double), hipMemcpyHostToDevice, L->stream));
   if (!L->use_graph) {
     L->old_valid = L->comoving;
-    return enqueue_step(L, (int)(CycleNumber % L->rank), want_dt, cfl);
+    return enqueue_step(L, (int)(CycleNumber % L->rank), want_dt, cfl, pres_done);
   }
-  // the launches of a step depend on the rotation, the buffer parity and the
-  // dt mode
+  // the launches of a step depend on the rotation, the buffer parity, the
+  // dt mode and whether the pressure is already there
   const int rot = (int)(CycleNumber % L->rank);
-  const int key = (rot * 2 + L->parity) * 2 + want_dt;
+  const int key = ((rot * 2 + L->parity) * 2 + want_dt) * 2 + (pres_done ? 1 : 0);
   if (want_dt && cfl != L->graph_cfl) {  // cfl is a kernel argument of those graphs
     for (auto g = L->graphs.begin(); g != L->graphs.end();) {
-      if (g->first % 2) {
+      if ((g->first / 2) % 2) {
         (void)hipGraphExecDestroy(g->second);
         g = L->graphs.erase(g);
       } else {
@@ -1033,7 +1124,7 @@ int hipPPMLevelStep(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dtFixed, en
     const int parity0 = L->parity;
     hipGraph_t graph;
     LCHK(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
-    int st = enqueue_step(L, rot, want_dt, cfl);
+    int st = enqueue_step(L, rot, want_dt, cfl, pres_done);
     hipError_t ec = hipStreamEndCapture(L->stream, &graph);
     if (st != ENZO_PPM_SUCCESS) return st;
     LCHK(ec);
@@ -1121,6 +1212,7 @@ int hipPPMLevelCorrectForRefinedFluxes(hipPPMLevel *L, enzo_int grid,
                                        enzo_int CorrectParentBoundaryFlux,
                                        const enzo_int *ColourKind) {
   if (!L || grid < 0 || grid >= L->ngrids) return ENZO_PPM_ERR_ARGUMENT;
+  L->pres_current = false;  // the state changes
   LCHK(hipStreamSynchronize(L->stream));  // the grid's state after every queued step
   double *state = (L->parity == 0 ? L->A : L->B) + L->off[grid];
   return correct_refined_fluxes(state, L->cells[grid], &L->GD[3 * grid], L->stream,
@@ -1150,6 +1242,15 @@ int hipPPMLevelUseGraph(hipPPMLevel *L, enzo_int enable) {
   return ENZO_PPM_SUCCESS;
 }
 
+int hipPPMLevelPersistent(hipPPMLevel *L, enzo_int enable) {
+  if (!L) return ENZO_PPM_ERR_ARGUMENT;
+  if ((enable == 0 || enable == 1) && L->persistent != (enable == 1)) {
+    L->persistent = enable == 1;
+    drop_graphs(L);  // the captured steps launch the other kernels
+  }
+  return L->persistent ? 1 : 0;
+}
+
 int hipPPMLevelSetShape(hipPPMLevel *L, enzo_int grid, enzo_int dim, enzo_int shape) {
   if (!L || grid < 0 || grid >= L->ngrids || dim < 0 || dim >= L->rank || shape < 0 ||
       shape >= kShapes)
@@ -1175,6 +1276,7 @@ int hipPPMLevelTune(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dt, enzo_in
   // saved before and restored after.  shape_by_length (4 entries) receives
   // the choice per class.
   if (!L || dt <= 0.0 || repeats < 1) return ENZO_PPM_ERR_ARGUMENT;
+  L->pres_current = false;  // the state changes (steps, then the saved state back)
   const int bounds[4] = {20, 40, 80, 1 << 30};
   auto cls = [&](int N) { int c = 0; while (N > bounds[c]) c++; return c; };
   long long tot = 0;
@@ -1222,6 +1324,21 @@ int hipPPMLevelTune(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dt, enzo_in
     choice[c] = best_s;
   }
   if (st == ENZO_PPM_SUCCESS) st = apply();
+  // then one workgroup per work item against the persistent launches
+  if (st == ENZO_PPM_SUCCESS) {
+    double tp[2] = {1e300, 1e300};
+    for (int pers = 0; pers < 2 && st == ENZO_PPM_SUCCESS; pers++) {
+      hipPPMLevelPersistent(L, pers);
+      if ((st = hipPPMLevelStep(L, CycleNumber, dt, 0.0)) != ENZO_PPM_SUCCESS) break;
+      if (hipStreamSynchronize(L->stream) != hipSuccess) { st = ENZO_PPM_ERR_HIP; break; }
+      auto t0 = std::chrono::steady_clock::now();
+      for (int r = 0; r < repeats && st == ENZO_PPM_SUCCESS; r++)
+        st = hipPPMLevelStep(L, CycleNumber, dt, 0.0);
+      if (hipStreamSynchronize(L->stream) != hipSuccess) st = ENZO_PPM_ERR_HIP;
+      tp[pers] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    hipPPMLevelPersistent(L, tp[1] < tp[0] ? 1 : 0);
+  }
   // restore the state and the counters
   L->parity = parity0;
   if (hipMemcpy(parity0 == 0 ? L->A : L->B, save, sizeof(double) * tot, hipMemcpyDeviceToDevice) != hipSuccess ||
@@ -1302,11 +1419,13 @@ int hipPPMLevelTimeStep(hipPPMLevel *L, enzo_float cfl, enzo_float *dt_host) {
   LCHK(hipGetLastError());
   LCHK(hipMemcpyAsync(dt_host, L->dt, sizeof(double), hipMemcpyDeviceToHost, L->stream));
   LCHK(hipStreamSynchronize(L->stream));
+  L->pres_current = true;
   return ENZO_PPM_SUCCESS;
 }
 
 int hipPPMLevelComovingExpansionTerms(hipPPMLevel *L, enzo_float coef, enzo_float Coefficient) {
   if (!L || !L->comoving) return ENZO_PPM_ERR_ARGUMENT;
+  L->pres_current = false;  // the state changes
   LCHK(hipStreamSynchronize(L->stream));
   int st = upload_params(L, coef, Coefficient);
   if (st != ENZO_PPM_SUCCESS) return st;
@@ -1380,6 +1499,7 @@ int hipPPMLevelSetSiblings(hipPPMLevel *L, const enzo_long_int *GridGlobalStart,
 
 int hipPPMLevelCopyZones(hipPPMLevel *L) {
   if (!L || !L->dgeom) return ENZO_PPM_ERR_ARGUMENT;
+  L->pres_current = false;  // the state changes
   if (L->nboxes == 0) return ENZO_PPM_SUCCESS;
   const unsigned bx =
       (unsigned)std::min<long long>((L->max_box * L->nslot + 255) / 256, 1024);
@@ -1391,6 +1511,7 @@ int hipPPMLevelCopyZones(hipPPMLevel *L) {
 
 int hipPPMLevelPeriodicBoundary(hipPPMLevel *L) {
   if (!L) return ENZO_PPM_ERR_ARGUMENT;
+  L->pres_current = false;  // the state changes
   const int G = L->ngrids;
   std::vector<BCParams> hb((size_t)3 * G);
   long long nmax = 1;
@@ -1433,6 +1554,7 @@ int hipPPMLevelExternalBoundary(hipPPMLevel *L, const enzo_int *BoundaryType,
                                 const enzo_int *GridOffset, const enzo_int *BoundaryDimension,
                                 enzo_int NumberOfBaryonFields, const enzo_float *InflowValue) {
   if (!L || !BoundaryType || !GridOffset || !BoundaryDimension) return ENZO_PPM_ERR_ARGUMENT;
+  L->pres_current = false;  // the state changes
   const int G = L->ngrids;
   // per face in the reference's order (x left, x right, y left, ...): the
   // grids that have it on the domain's edge; everything checked first

@@ -146,6 +146,12 @@ struct SweepParams {
   unsigned harvest_keep;
 };
 
+// Learned start in the sweep tile: the network's inputs of the interfaces
+// that pass the gate are listed in LDS, kNetStride floats each (their
+// corrections then overwrite the first), and the workgroup evaluates the
+// network on the list together (learn_eval).
+constexpr int kNetStride = 6;
+
 // harvest record: dls, drs, pls, prs, uls, urs (the two-shock inputs),
 // p* (converged), starting pressure, Newton updates, restarted (0/1)
 constexpr int kHarvestDoubles = 10;
@@ -213,6 +219,12 @@ struct Regs {
   double df, ef, uf, vf, wf, gef;
   int ovl, ovr, upl, upr;  // DE overrides; upwind choice (u(i-1)<=0, u(i)>=0)
   Chars ca, cb;            // characteristic distances of cells i-1 and i
+  // two-shock inputs kept between the split Riemann phases of the learned
+  // start (dls / drs are above), and the interface's slot in the workgroup's
+  // list of network evaluations (-1: none)
+  double s_pl, s_pr, s_ul, s_ur;
+  int ls;
+  float lc;  // the network's correction at that slot
   double flat_c;           // flattening at this thread's parabola cell
   // fallback workgroups only: HLL fluxes of the interface (dt/dx absorbed),
   // HLL weights and wave-speed offsets, and whether they replace the
@@ -628,6 +640,30 @@ PPM_HD IfaceAvg iface_avg(const Chars &A, const Chars &B, double *sm, int p, int
   return r;
 }
 
+// The upwind interface states of an advected quantity (inteuler.F:414-434):
+// left qla (upl) or ql0, right qra (upr) or qr0, each computed alone with
+// its characteristic distance selected first.  qr0's form is qra's with the
+// distance -c0 (a - c*y == a + (-c)*y and 1 + k*c == 1 - k*(-c) exactly), so
+// the results are bitwise iface_avg's selected ones.
+struct IfaceUp {
+  double ql, qr;
+};
+
+template <int TP, int TS, int UNI = 0>
+PPM_HD IfaceUp iface_up(const Chars &A, const Chars &B, double *sm, int p, int ca, int cb,
+                        int upl, int upr) {
+  const double qr_a = LDS<TP, TS, UNI>(sm, A_QR, p, ca), ql_a = LDS<TP, TS, UNI>(sm, A_QL, p, ca);
+  const double q6_a = LDS<TP, TS, UNI>(sm, A_Q6, p, ca);
+  const double ql_b = LDS<TP, TS, UNI>(sm, A_QL, p, cb), qr_b = LDS<TP, TS, UNI>(sm, A_QR, p, cb);
+  const double q6_b = LDS<TP, TS, UNI>(sm, A_Q6, p, cb);
+  const double dq_a = qr_a - ql_a, dq_b = qr_b - ql_b;
+  const double cl = upl ? A.ch1 : A.c0, cr = upr ? B.ch2 : -B.c0;
+  IfaceUp r;
+  r.ql = qr_a - cl * (dq_a - (1.0 - kFt * cl) * q6_a);
+  r.qr = ql_b + cr * (dq_b + (1.0 - kFt * cr) * q6_b);
+  return r;
+}
+
 // Phase: interface averages of variable VAR (0 d, 1 p, 2 u, 3 v, 4 w, 5 ge)
 // for the thread's interface; results into Regs.  VAR is a compile-time
 // constant, so each variable's phase is its own straight-line code.
@@ -646,6 +682,20 @@ PPM_HD void ph_iface(const SweepParams &P, const Tile &T, double *sm, int tid, R
     r.cb = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, cb), LDS<TP, TS, UNI>(sm, A_CS, p, cb), T.dt,
                     TS_<UNI>(P, T_DX2I, i));
   }
+  if constexpr (VAR >= 3) {
+    const IfaceUp a = iface_up<TP, TS, UNI>(r.ca, r.cb, sm, p, ca, cb, r.upl, r.upr);
+    if constexpr (VAR == 3) {
+      r.vls = a.ql;
+      r.vrs = a.qr;
+    } else if constexpr (VAR == 4) {
+      r.wls = a.ql;
+      r.wrs = a.qr;
+    } else {
+      r.gels = a.ql;
+      r.gers = a.qr;
+    }
+    return;
+  }
   IfaceAvg a = iface_avg<TP, TS, UNI>(r.ca, r.cb, sm, p, ca, cb, VAR == 1 || VAR == 2);
   if constexpr (VAR == 0) {
     r.dla = a.qla; r.dra = a.qra; r.dl0 = a.ql0; r.dr0 = a.qr0;
@@ -658,15 +708,6 @@ PPM_HD void ph_iface(const SweepParams &P, const Tile &T, double *sm, int tid, R
     // upwind choice of the advected states (inteuler.F:414-434)
     r.upl = LDS<TP, TS, UNI>(sm, A_U, p, ca) <= 0.0;
     r.upr = LDS<TP, TS, UNI>(sm, A_U, p, cb) >= 0.0;
-  } else if constexpr (VAR == 3) {
-    r.vls = r.upl ? a.qla : a.ql0;
-    r.vrs = r.upr ? a.qra : a.qr0;
-  } else if constexpr (VAR == 4) {
-    r.wls = r.upl ? a.qla : a.ql0;
-    r.wrs = r.upr ? a.qra : a.qr0;
-  } else {
-    r.gels = r.upl ? a.qla : a.ql0;
-    r.gers = r.upr ? a.qra : a.qr0;
   }
 }
 
@@ -722,18 +763,31 @@ PPM_HD int twoshock1(double dls, double drs, double &pls, double &prs, double ul
   return twoshock_newton(dls, drs, pls, prs, uls, urs, cl, cr, gamma, pmin, ps, pbar, ubar);
 }
 
-// The same solver started from the learned pressure (ppm_learn.hpp): the
-// acoustic start p_ac times e^c, the network's correction, at interfaces
-// whose acoustic wave strength passes the gate g (|p_ac - p_l| > g p_l or
-// |p_ac - p_r| > g p_r); elsewhere p_ac, whose error is second order in
-// that strength, is kept and the network is not evaluated.  Where p_ac is
-// at the floor pmin (it predicts cavitation) it is kept too.  When Newton
-// does not meet the tolerance from the learned start, p_ac starts it again
+// Whether the learned start is used at an interface: the acoustic start
+// p_ac (floored at pmin) above the floor, with an acoustic wave strength
+// that passes the gate g (|p_ac - p_l| > g p_l or |p_ac - p_r| > g p_r);
+// elsewhere p_ac, whose error is second order in that strength, is kept and
+// the network is not evaluated.  Where p_ac is at the floor (it predicts
+// cavitation) it is kept too.
+PPM_HD bool learn_gate(const float *w, double dls, double drs, double pls, double prs, double uls,
+                       double urs, double gamma, double pmin) {
+  const double cl = sqrt(gamma * pls * dls);
+  const double cr = sqrt(gamma * prs * drs);
+  double ps_ref = acoustic_start(dls, drs, pls, prs, uls, urs, cl, cr);
+  if (ps_ref < pmin) ps_ref = pmin;
+  const double g = w[kOffGate];
+  return ps_ref > pmin && (fabs(ps_ref - pls) > g * pls || fabs(ps_ref - prs) > g * prs);
+}
+
+// The same solver started from the learned pressure (ppm_learn.hpp): at an
+// interface that passed the gate (have_c), the acoustic start p_ac times
+// e^c, c being the network's correction; elsewhere p_ac.  When Newton does
+// not meet the tolerance from the learned start, p_ac starts it again
 // (restarted = 1), so a non-converging interface gets the reference
 // method's result.  Returns all updates taken.
-PPM_HD int twoshock_learned(const float *w, double dls, double drs, double &pls, double &prs,
-                            double uls, double urs, double gamma, double pmin, int ipresfree,
-                            double &pbar, double &ubar, double &ps0, int &restarted) {
+PPM_HD int twoshock_learned(bool have_c, float c, double dls, double drs, double &pls,
+                            double &prs, double uls, double urs, double gamma, double pmin,
+                            int ipresfree, double &pbar, double &ubar, double &ps0, int &restarted) {
   restarted = 0;
   if (ipresfree == 1) return twoshock1(dls, drs, pls, prs, uls, urs, gamma, pmin, ipresfree, pbar, ubar, ps0);
   const double cl = sqrt(gamma * pls * dls);
@@ -741,11 +795,8 @@ PPM_HD int twoshock_learned(const float *w, double dls, double drs, double &pls,
   double ps_ref = acoustic_start(dls, drs, pls, prs, uls, urs, cl, cr);
   if (ps_ref < pmin) ps_ref = pmin;
   double ps = ps_ref;
-  const double g = w[kOffGate];
-  if (ps_ref > pmin && (fabs(ps_ref - pls) > g * pls || fabs(ps_ref - prs) > g * prs)) {
-    float x[kMlpIn];
-    mlp_features(dls, drs, pls, prs, uls, urs, gamma, x);
-    ps = ps_ref * (double)mlp_exp(mlp_eval(w, x));
+  if (have_c) {  // the interface passed the gate (learn_gate)
+    ps = ps_ref * (double)mlp_exp(c);
     if (!(ps >= pmin)) ps = pmin;  // NaN too
   }
   ps0 = ps;
@@ -933,14 +984,20 @@ PPM_HD void hllc_flux(const HllIn &h, Regs &r) {
 // Phase: Eulerian corrections (inteuler.F:321-551), two-shock solve and
 // Godunov fluxes (flux_twoshock.F:84-396) for the thread's interface; in
 // fallback workgroups (FB) also the HLL fluxes of the interface.
-template <int TP, int TS, int UNI = 0, int FB = 0, int LEARN = 0>
+// STAGE 2: the whole phase.  The learned start splits it around the
+// network's evaluation: STAGE 0 computes the two-shock inputs into Regs,
+// STAGE 1 continues from them (with Regs::ls / lc, the interface's network
+// correction).
+template <int TP, int TS, int UNI = 0, int FB = 0, int LEARN = 0, int STAGE = 2>
 PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, Regs &r) {
   int p = tid / TS, q = tid % TS;
-  if constexpr (FB != 0) r.hll = 0;
+  if constexpr (FB != 0 && STAGE != 1) r.hll = 0;
   if (p >= T.np || q >= T.NI) return;
   const int i = T.IA + q;
   const int ca = i - 1 - T.L0, cb = i - T.L0;
   const double gamma = P.gamma, dt = T.dt;
+  double pls, prs, uls, urs, dls, drs;
+  if constexpr (STAGE != 1) {
   const Chars &CA = r.ca, &CB = r.cb;
   // correction terms (3.7), inteuler.F:321-383
   double cla = sqrt(fmx(gamma * r.pla * r.dla, 0.0));
@@ -978,12 +1035,12 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
   if (CB.cm >= 0) betarm = 0.0;
   if (CB.c0 >= 0) betar0 = 0.0;
   // corrected states (3.6), inteuler.F:387-396
-  double pls = r.pla + (betalp + betalm) * (cla * cla);
-  double prs = r.pra + (betarp + betarm) * (cra * cra);
-  double uls = r.ula + (betalp - betalm) * cla;
-  double urs = r.ura + (betarp - betarm) * cra;
-  double dls = 1.0 / (1.0 / r.dla - (betal0 + betalp + betalm));
-  double drs = 1.0 / (1.0 / r.dra - (betar0 + betarp + betarm));
+  pls = r.pla + (betalp + betalm) * (cla * cla);
+  prs = r.pra + (betarp + betarm) * (cra * cra);
+  uls = r.ula + (betalp - betalm) * cla;
+  urs = r.ura + (betarp - betarm) * cra;
+  dls = 1.0 / (1.0 / r.dla - (betal0 + betalp + betalm));
+  drs = 1.0 / (1.0 / r.dra - (betar0 + betarp + betarm));
   r.dls_pre = dls;
   r.drs_pre = drs;
   r.ovl = r.ovr = 0;
@@ -1017,13 +1074,29 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
   }
   r.dls = dls;
   r.drs = drs;
+  }  // STAGE != 1
+  if constexpr (STAGE == 0) {
+    r.s_pl = pls;
+    r.s_pr = prs;
+    r.s_ul = uls;
+    r.s_ur = urs;
+    return;
+  }
+  if constexpr (STAGE == 1) {
+    dls = r.dls;
+    drs = r.drs;
+    pls = r.s_pl;
+    prs = r.s_pr;
+    uls = r.s_ul;
+    urs = r.s_ur;
+  }
   // two-shock solution
   double pbar, ubar, ps0;
   const double pls_in = pls, prs_in = prs;
   int iters, restarted = 0;
-  if constexpr (LEARN != 0)
-    iters = twoshock_learned(P.learn, dls, drs, pls, prs, uls, urs, gamma, P.pmin, P.ipresfree,
-                             pbar, ubar, ps0, restarted);
+  if constexpr (LEARN != 0 && STAGE == 1)
+    iters = twoshock_learned(r.ls >= 0, r.lc, dls, drs, pls, prs, uls, urs, gamma, P.pmin,
+                             P.ipresfree, pbar, ubar, ps0, restarted);
   else
     iters = twoshock1(dls, drs, pls, prs, uls, urs, gamma, P.pmin, P.ipresfree, pbar, ubar, ps0);
   if (FB == 0 && P.harvest && i >= T.o && i < T.o + T.w) {
@@ -1481,14 +1554,17 @@ PPM_HD void ph_col_flux(const SweepParams &P, const Tile &T, double *sm, int tid
   if (p >= T.np || q >= T.NI) return;
   int i = T.IA + q;
   int ca = i - 1 - T.L0, cb = i - T.L0;
-  IfaceAvg a = iface_avg<TP, TS, UNI>(r.ca, r.cb, sm, p, ca, cb, 0);
-  double colls = r.upl ? a.qla * r.dls_pre / r.dla : a.ql0 * r.dls_pre / r.dl0;
-  double colrs = r.upr ? a.qra * r.drs_pre / r.dra : a.qr0 * r.drs_pre / r.dr0;
+  // each upwind choice selects the operands of one division (the same
+  // operations as dividing both ways and selecting the result)
+  const IfaceUp a = iface_up<TP, TS, UNI>(r.ca, r.cb, sm, p, ca, cb, r.upl, r.upr);
+  double colls = a.ql * r.dls_pre / (r.upl ? r.dla : r.dl0);
+  double colrs = a.qr * r.drs_pre / (r.upr ? r.dra : r.dr0);
   if (r.ovl) colls = colls * r.dla / r.dls_pre;
   if (r.ovr) colrs = colrs * r.dra / r.drs_pre;
   colls = fmx(colls, kColorFloor);
   colrs = fmx(colrs, kColorFloor);
-  double colb = r.ub > 0.0 ? colls * r.db / r.dls : colrs * r.db / r.drs;
+  const bool up = r.ub > 0.0;
+  double colb = (up ? colls : colrs) * r.db / (up ? r.dls : r.drs);
   double colf = T.dt * r.ub * colb;
   if (FB != 0 && r.hll)  // flux_hll.F:231-236, 272-276, 327-331
     colf = T.dt * (r.h_sl * (r.h_bm0 * colls) + r.h_sr * (r.h_bp0 * colrs));
@@ -1525,14 +1601,17 @@ PPM_HD void ph_col_update(const SweepParams &P, const Tile &T, double *sm, int t
   });
 }
 
-// flattening coefficient of line 1 of slice `outer` at `cell` (calcdiss.F:193-202,
-// 409-442, iflatten = 1); p and u are read along that line.
+// flattening coefficient of line 1 of slice `outer` at `cell` (calcdiss.F:
+// 193-202 shock flag; iflatten = 1: 409-442, CW84 A1-A2; 2: 447-485, the
+// Lagrangean A4-A6; 3: 489-566, the multidimensional A7-A9; the maximum of
+// 568-585), reading p, u, and for 2 and 3 also d and e along that line.
 struct FlatParams {
   int dim, N, i1, i2;
   int GD[3];
   long long cells;
   int n_outer;
   int iflatten;
+  double gamma;
   const double *A, *pres;
   double *flat;
   const double *dx;   // cell widths along the sweep (N)
@@ -1543,10 +1622,14 @@ struct FlatParams {
 
 PPM_HD double flat_value(const FlatParams &F, int outer, int cell) {
   const int rax = (F.dim + 1) % 3, oax = (F.dim + 2) % 3;
-  double pv[9], uv[9];
+  const bool de = F.iflatten >= 2;  // types 2 and 3 read density and energy too
+  double pv[9], uv[9], dv[9], ev[9];
   for (int k = -4; k <= 4; k++) {
     int cc = cell + k;
-    if (cc < 1 || cc > F.N) { pv[k + 4] = 1.0; uv[k + 4] = 0.0; continue; }
+    if (cc < 1 || cc > F.N) {
+      pv[k + 4] = 1.0; uv[k + 4] = 0.0; dv[k + 4] = 1.0; ev[k + 4] = 0.0;
+      continue;
+    }
     int c[3];
     c[F.dim] = cc - 1;
     c[rax] = 0;
@@ -1555,21 +1638,57 @@ PPM_HD double flat_value(const FlatParams &F, int outer, int cell) {
     PPM_ASSERT(g >= 0 && g < F.cells);
     pv[k + 4] = F.pres[g];
     uv[k + 4] = F.A[(S_VX + F.dim) * F.cells + g];
+    dv[k + 4] = de ? F.A[S_D * F.cells + g] : 1.0;
+    ev[k + 4] = de ? F.A[S_E * F.cells + g] : 0.0;
   }
   auto P = [&](int x) { PPM_ASSERT(x - cell >= -4 && x - cell <= 4); return pv[x - cell + 4]; };
-  auto U = [&](int x) { return uv[x - cell + 4]; };
+  auto U = [&](int x) { PPM_ASSERT(x - cell >= -4 && x - cell <= 4); return uv[x - cell + 4]; };
+  auto D = [&](int x) { PPM_ASSERT(x - cell >= -4 && x - cell <= 4); return dv[x - cell + 4]; };
+  auto E = [&](int x) { PPM_ASSERT(x - cell >= -4 && x - cell <= 4); return ev[x - cell + 4]; };
+  const double gamma = F.gamma;
   auto ftemp = [&](int x) {
-    if (x == F.i1 - 2) x = F.i1 - 1;  // calcdiss.F:424-425
+    if (x == F.i1 - 2) x = F.i1 - 1;  // calcdiss.F:424-425, 474-475, 569-570
     if (x == F.i2 + 2) x = F.i2 + 1;
     double qb = fabs(P(x + 1) - P(x - 1)) / fmn(P(x + 1), P(x - 1));
     double wflag = (qb > 0.33 && U(x - 1) > U(x + 1)) ? 1.0 : 0.0;
-    double qa;
-    if (fabs(P(x + 2) - P(x - 2)) / fmn(P(x + 2), P(x - 2)) < 0.33)
-      qa = 1.0;
-    else
-      qa = (P(x + 1) - P(x - 1)) / (P(x + 2) - P(x - 2));
-    double f = fmn(1.0, (qa - 0.75) * 10.0 * wflag);
-    return fmx(0.0, f);
+    if (F.iflatten == 1) {
+      double qa;
+      if (fabs(P(x + 2) - P(x - 2)) / fmn(P(x + 2), P(x - 2)) < 0.33)
+        qa = 1.0;
+      else
+        qa = (P(x + 1) - P(x - 1)) / (P(x + 2) - P(x - 2));
+      double f = fmn(1.0, (qa - 0.75) * 10.0 * wflag);
+      return fmx(0.0, f);
+    }
+    // di = 1/d at x +- 2, and Z (calcdiss.F:458-462, 530-534)
+    const double dim2 = 1.0 / D(x - 2), dip2 = 1.0 / D(x + 2);
+    const double Z = sqrt((fmx(P(x + 2), P(x - 2)) + 0.5 * (P(x + 2) + P(x - 2)) * (gamma - 1.0)) /
+                          fmx(dip2, dim2));
+    if (F.iflatten == 2) {
+      const int is = x + (int)copysign(2.0, P(x + 1) - P(x - 1));
+      const double omega =
+          fmx(0.0, 0.75 * (10.0 - (P(x + 1) - P(x - 1)) / (P(x + 2) - P(x - 2))));
+      const double kt = (Z + sqrt(gamma * P(is) * D(is))) / Z;
+      const double kappa = fmx(0.0, (kt - 2.0) / (kt + 0.01));
+      return fmn(wflag * omega, kappa);
+    }
+    const double dp1 = P(x + 1) - P(x - 1), dp2 = P(x + 2) - P(x - 2);
+    const double de1 = E(x + 1) - E(x - 1), de2 = E(x + 2) - E(x - 2);
+    double dpp = 0.0, dee = 0.0;
+    if (dp2 != 0.0) dpp = dp1 / dp2;
+    if (de2 != 0.0) dee = de1 / de2;
+    const double omega_tilde = fmx(dpp, dee);
+    const int ism = x + (int)copysign(2.0, dp1), isp = x - (int)copysign(2.0, dp1);
+    double s = -copysign(1.0, dp1);
+    if (dp1 == 0.0) s = 0.0;
+    const double sigma_tilde = wflag * fabs(dp2) / fmn(P(x + 2), P(x - 2));
+    const double sigma = fmx(0.0, (sigma_tilde - 0.5) / (sigma_tilde + 1.0));
+    const double omega = fmx(0.0, 10.0 * (omega_tilde - 0.75));
+    const double ZE = s * Z / D(ism) + U(ism) + kTiny;
+    const double cj2s = sqrt(gamma * P(isp) / D(isp));
+    const double kt = fabs((ZE - U(isp) + s * cj2s) / ZE);
+    const double kappa = fmx(0.0, (kt - 2.0) / (kt + 0.01));
+    return fmn(fmn(kappa, wflag * omega), wflag * sigma);
   };
   if (P(cell + 1) - P(cell - 1) < 0.0) return fmx(ftemp(cell), ftemp(cell + 1));
   return fmx(ftemp(cell), ftemp(cell - 1));
@@ -1583,6 +1702,91 @@ PPM_HD double flat_value(const FlatParams &F, int outer, int cell) {
 struct TileCounts {
   int bad, chains, err, nfb;
 };
+
+// ---------------------------------------------------------------------------
+// Learned start inside the tile.  Between the two stages of the Riemann
+// phase, the LDS arrays A_CS .. A_Q6 are free (the reconstruction is done,
+// the flux store has not begun); as floats they hold the list of network
+// inputs of the interfaces that pass the gate ([NT][kNetStride]), the
+// hidden layers of a chunk of CH of them ([CH][32] twice) and, in the last
+// float, the list's length.  The workgroup evaluates the network on the list
+// together, one (interface, hidden unit) pair per work-item, so that the
+// network costs registers nowhere and work only where the gate passes.
+template <int TP, int TS, int UNI>
+struct NetLds {
+  static constexpr int NT = TP * TS;
+  static constexpr int FLOATS = 2 * (A_Q6 - A_CS + 1) * TP * Cfg<TP, TS, UNI>::LW;
+  static constexpr int fits(int ch) { return kNetStride * NT + 2 * kMlpHidden * ch + 1 <= FLOATS; }
+  static constexpr int CH = fits(32) ? 32 : fits(16) ? 16 : fits(8) ? 8 : 4;
+  static_assert(fits(CH), "the learned start's lists fit the free tile arrays");
+  PPM_HD static float *base(double *sm) {
+    return reinterpret_cast<float *>(sm + (size_t)A_CS * TP * Cfg<TP, TS, UNI>::LW);
+  }
+  PPM_HD static int *count(double *sm) { return reinterpret_cast<int *>(base(sm) + FLOATS - 1); }
+};
+
+// Phase: the gate at the thread's interface (learn_gate); a passing
+// interface appends its network inputs to the list and keeps its slot.
+template <int TP, int TS, int UNI = 0>
+PPM_HD void ph_learn_gate(const SweepParams &P, const Tile &T, double *sm, int tid, Regs &r) {
+  r.ls = -1;
+  r.lc = 0.0f;
+  const int p = tid / TS, q = tid % TS;
+  if (p >= T.np || q >= T.NI || P.ipresfree == 1) return;
+  if (!learn_gate(P.learn, r.dls, r.drs, r.s_pl, r.s_pr, r.s_ul, r.s_ur, P.gamma, P.pmin)) return;
+  const int k = atomicAdd(NetLds<TP, TS, UNI>::count(sm), 1);
+  r.ls = k;
+  float x[kMlpIn];
+  mlp_features(r.dls, r.drs, r.s_pl, r.s_pr, r.s_ul, r.s_ur, P.gamma, x);
+  float *in = NetLds<TP, TS, UNI>::base(sm) + kNetStride * k;
+  for (int m = 0; m < kMlpIn; m++) in[m] = x[m];
+}
+
+// The network (mlp_eval's arithmetic, operation for operation) on the m
+// listed interfaces, chunk by chunk; each correction replaces its inputs'
+// first float.
+template <int TP, int TS, int UNI, class X>
+PPM_HD void learn_eval(const SweepParams &P, double *sm, int m, X &ex) {
+  using L = NetLds<TP, TS, UNI>;
+  constexpr int H = kMlpHidden;
+  float *list = L::base(sm), *h1 = list + kNetStride * L::NT, *h2 = h1 + L::CH * H;
+  const float *w = P.learn;
+  for (int c0 = 0; c0 < m; c0 += L::CH) {
+    const int mc = m - c0 < L::CH ? m - c0 : L::CH;
+    ex.run([&](int tid, Regs &r) {  // hidden layer 1
+      (void)r;
+      for (int idx = tid; idx < mc * H; idx += L::NT) {
+        const int e = idx / H, j = idx % H;
+        const float *x = list + kNetStride * (c0 + e);
+        float a = w[kOffB1 + j];
+        for (int k = 0; k < kMlpIn; k++)
+          a += w[kOffW1 + j * kMlpIn + k] * ((x[k] - w[kOffMean + k]) * w[kOffInvStd + k]);
+        h1[idx] = mlp_tanh(a);
+      }
+    });
+    ex.run([&](int tid, Regs &r) {  // hidden layer 2 times the output weights
+      (void)r;
+      for (int idx = tid; idx < mc * H; idx += L::NT) {
+        const int e = idx / H, j = idx % H;
+        float a = w[kOffB2 + j];
+        for (int k = 0; k < H; k++) a += w[kOffW2 + j * H + k] * h1[e * H + k];
+        h2[idx] = w[kOffW3 + j] * mlp_tanh(a);
+      }
+    });
+    ex.run([&](int tid, Regs &r) {  // the output, summed in mlp_eval's order
+      (void)r;
+      for (int e = tid; e < mc; e += L::NT) {
+        float y = w[kOffB3];
+        for (int j = 0; j < H; j++) y += h2[e * H + j];
+        list[kNetStride * (c0 + e)] = y;
+      }
+    });
+  }
+  ex.run([&](int tid, Regs &r) {  // every listed interface takes its correction
+    (void)tid;
+    if (r.ls >= 0) r.lc = list[kNetStride * r.ls];
+  });
+}
 
 // Reconstruction of variable V (0 d, 1 p, 2 u, 3 v, 4 w, 5 ge): slopes,
 // parabolae (steepened for the density), interface averages.
@@ -1611,10 +1815,25 @@ PPM_HD TileCounts sweep_body(const SweepParams &P, const Tile &T, double *sm, X 
   sweep_var<3, TP, TS, UNI>(P, T, sm, ex);
   sweep_var<4, TP, TS, UNI>(P, T, sm, ex);
   if (P.idual) sweep_var<5, TP, TS, UNI>(P, T, sm, ex);
-  ex.run([&](int tid, Regs &r) {
-    ph_solve<TP, TS, UNI, FB, LEARN>(P, T, sm, tid, r);
-    ph_store_flux<TP, TS, UNI, FB>(P, T, sm, tid, r);
-  });
+  if constexpr (LEARN != 0) {
+    // the Riemann phase in two stages around the network (NetLds)
+    ex.run([&](int tid, Regs &r) {
+      ph_solve<TP, TS, UNI, FB, LEARN, 0>(P, T, sm, tid, r);
+      if (tid == 0) *NetLds<TP, TS, UNI>::count(sm) = 0;
+    });
+    ex.run([&](int tid, Regs &r) { ph_learn_gate<TP, TS, UNI>(P, T, sm, tid, r); });
+    const int m = *NetLds<TP, TS, UNI>::count(sm);  // the same for every work-item
+    if (m > 0) learn_eval<TP, TS, UNI>(P, sm, m, ex);
+    ex.run([&](int tid, Regs &r) {
+      ph_solve<TP, TS, UNI, FB, LEARN, 1>(P, T, sm, tid, r);
+      ph_store_flux<TP, TS, UNI, FB>(P, T, sm, tid, r);
+    });
+  } else {
+    ex.run([&](int tid, Regs &r) {
+      ph_solve<TP, TS, UNI, FB, LEARN>(P, T, sm, tid, r);
+      ph_store_flux<TP, TS, UNI, FB>(P, T, sm, tid, r);
+    });
+  }
   if constexpr (FB != 0) {
     if (P.hllc) {
       ex.run([&](int tid, Regs &r) { ph_hllc_ges<TP, TS, UNI>(P, T, sm, tid, r); });

@@ -213,6 +213,7 @@ PROTOTYPES = {
     "hipPPMLevelCorrectForRefinedFluxes": (C.c_int, [_H, I, _F1, _F1, _F1, I, PI, PI, I, PI]),
     "hipPPMLevelStatus": (C.c_int, [_H, PF, PL]),
     "hipPPMLevelUseGraph": (C.c_int, [_H, I]),
+    "hipPPMLevelPersistent": (C.c_int, [_H, I]),
     "hipPPMLevelSetShape": (C.c_int, [_H, I, I, I]),
     "hipPPMLevelCommitShapes": (C.c_int, [_H]),
     "hipPPMLevelTune": (C.c_int, [_H, I, _D, I, PI]),

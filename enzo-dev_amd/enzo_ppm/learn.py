@@ -74,9 +74,11 @@ def init_weights(seed=0):
     return w
 
 
-def harvest(grid, glob, steps=1, keep=1, cap=1 << 22, cfl=0.4, lib=None, weights=None):
-    """`steps` device-resident steps of `grid` (periodic ghost fill, Courant
-    dt, rotated sweeps) with the harvest on; `grid` is updated in place.
+def harvest(grid, glob, steps=1, keep=1, cap=1 << 22, cfl=0.4, lib=None, weights=None,
+            boundary=True):
+    """`steps` device-resident steps of `grid` (periodic ghost fill unless
+    `boundary` is false -- then the ghosts stay as given --, Courant dt,
+    rotated sweeps) with the harvest on; `grid` is updated in place.
     Returns (records, interfaces selected).  With `weights` the sweeps use
     the learned start and the records show its Newton updates."""
     dg = device.DeviceGrid(grid, glob, lib=lib)
@@ -87,7 +89,8 @@ def harvest(grid, glob, steps=1, keep=1, cap=1 << 22, cfl=0.4, lib=None, weights
             dg.set_learned(weights)
         dg.set_harvest(cap, keep)
         for cycle in range(steps):
-            dg.periodic_boundary()
+            if boundary:
+                dg.periodic_boundary()
             dg.pressure_and_dt(cfl, want_host=False)
             dg.step(cycle)
             r, s = dg.get_harvest(cap)

@@ -96,6 +96,15 @@ class Level:
     def use_graph(self, on=True):
         self._check(self.lib.hipPPMLevelUseGraph(self.handle, int(on)), "LevelUseGraph")
 
+    def persistent(self, enable=None):
+        """Persistent sweep launches on (True) or off (False); None only
+        queries.  Returns the setting."""
+        v = -1 if enable is None else int(bool(enable))
+        st = self.lib.hipPPMLevelPersistent(self.handle, v)
+        if st < 0:
+            raise RuntimeError(f"LevelPersistent failed with status {st}")
+        return bool(st)
+
     def set_shape(self, grid, dim, shape):
         self._check(self.lib.hipPPMLevelSetShape(self.handle, grid, dim, shape), "LevelSetShape")
 

@@ -543,6 +543,13 @@ int hipPPMLevelStatus(hipPPMLevel *Level, enzo_float *dt_used, enzo_long_int *fa
 /* Replay captured hipGraphs (default) or launch each step directly. */
 int hipPPMLevelUseGraph(hipPPMLevel *Level, enzo_int enable);
 
+/* Sweep launches: enable 1 = persistent (a few workgroups per CU pull the
+   level's work items from per-XCD counters until the list is done), 0 = one
+   workgroup per work item (default); any other value leaves the setting.
+   Returns the setting (0 or 1).  hipPPMLevelTune times both and keeps the
+   faster.  Results are bitwise the same either way. */
+int hipPPMLevelPersistent(hipPPMLevel *Level, enzo_int enable);
+
 /* Tile shape of grid `grid` along sweep axis `dim`: 0 = 16 pencils x 16
    interfaces, 1 = 8 x 32 (default), 2 = 4 x 64; hipPPMLevelCommitShapes
    rebuilds the work lists. */

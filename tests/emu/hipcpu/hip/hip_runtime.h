@@ -379,6 +379,9 @@ inline hipError_t hipGetSymbolAddress(void **p, const void *sym) {
   return hipSuccess;
 }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 0 };
+// one "compute unit": the CPU runs one workgroup at a time
+inline hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int) { *v = 1; return hipSuccess; }
 
 struct hipcpu_event { std::chrono::steady_clock::time_point t; };
 inline hipError_t hipEventCreate(hipEvent_t *e) { *e = new hipcpu_event(); return hipSuccess; }

@@ -129,6 +129,7 @@ extern "C" int emu_solve_ppm_de(const enzo_ppm_grid *g, const enzo_ppm_globals *
       F.dim = dim; F.N = N; F.i1 = i1; F.i2 = i2;
       for (int d = 0; d < 3; d++) F.GD[d] = GD[d];
       F.cells = cells; F.n_outer = GD[oax]; F.A = A.data(); F.pres = pres.data(); F.flat = flat.data();
+      F.iflatten = (int)gl->PPMFlatteningParameter; F.gamma = gl->Gamma;
       for (int outer = 0; outer < GD[oax]; outer++)
         for (int cell = i1 - 1; cell <= i2 + 1; cell++)
           flat[(size_t)outer * N + cell - 1] = flat_value(F, outer, cell);

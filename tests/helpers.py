@@ -326,3 +326,39 @@ def noalloc_cases():
     g2.fill_periodic()
     g2.dt = oracle.compute_time_step("port", g2, glob2, 0.4)
     return [(g1, glob1, cycle1), (g2, glob2, 2)]
+
+
+class DeviceArrays:
+    """Device copies of float64 numpy arrays, made through the HIP runtime
+    the library itself links (libamdhip64.so.7), for tests that hand the
+    library device pointers."""
+
+    def __init__(self, arrays):
+        import ctypes as C
+        self.C = C
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.hip.hipFree.argtypes = [C.c_void_p]
+        self.hip.hipDeviceSynchronize.argtypes = []
+        self.ptrs, self.sizes = [], []
+        for a in arrays:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            p = C.c_void_p()
+            assert self.hip.hipMalloc(C.byref(p), a.nbytes) == 0
+            self.ptrs.append(p)
+            self.sizes.append(a.nbytes)
+            assert self.hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+
+    def addrs(self):
+        return [p.value for p in self.ptrs]
+
+    def read(self, k, out):
+        assert out.flags["C_CONTIGUOUS"] and out.nbytes == self.sizes[k]
+        assert self.hip.hipDeviceSynchronize() == 0
+        assert self.hip.hipMemcpy(out.ctypes.data, self.ptrs[k], out.nbytes, 2) == 0  # to host
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipFree(p)
+        self.ptrs = []

@@ -13,8 +13,7 @@ face fluxes of
   - hipPPMSolvePPM_DE with colours, diffusion and two subgrids' face fluxes,
   - device-resident steps (periodic fill, pressure + Courant, Step) with
     gravity and per-cell widths,
-  - a batched level of three grids (hipPPMLevel*, the autotuner's shape
-    query included),
+  - a batched level of three grids (hipPPMLevel*, the autotuner included),
   - the external-boundary fill (reflecting / outflow / inflow faces).
 """
 import os
@@ -77,13 +76,14 @@ def run_cases(lib):
     lv = Level([x.copy() for x in grids], glob, lib=lib)
     try:
         lv.step(0, 0.0, 0.4)
+        # the tuner's choice depends on timings, so only its range is checked
         shapes = lv.tune(1, 1.0e-5, repeats=1)
+        assert all(s in (0, 1, 2) for s in shapes), shapes
         lv.step(1, 0.0, 0.4)
         st, dt, nfb = lv.status()
         lv.download()
         out["level_status"] = np.array([st, nfb])
         out["level_dt"] = np.array([dt])
-        out["level_shapes"] = np.array(shapes)
         for i, x in enumerate(lv.grids):
             for k, f in enumerate(x.fields):
                 out[f"level_grid{i}_field{k}"] = f

@@ -104,6 +104,78 @@ def learned_vs_oracle(lib, g, glob, w, steps, cfl=0.4):
     return ref, h
 
 
+def check_sweep_uses_network(lib, n=12, seed=13):
+    """The sweep's learned start takes the network's output: a network whose
+    output is the constant c = 0.7 (every weight zero but the output bias)
+    moves Newton's start at every interface that passes the gate, so with
+    gate 0 one step differs from the reference start's in the last bits
+    (and agrees to the one-step tolerance), and with a gate no interface
+    passes it is bitwise the reference start's step."""
+    g0, glob, _ = problems.turbulent(n, seed=seed)
+
+    def run(w):
+        h = g0.copy()
+        dg = device.DeviceGrid(h, glob, lib=lib)
+        try:
+            if w is not None:
+                dg.set_learned(w)
+            dg.periodic_boundary()
+            dg.pressure_and_dt(0.4)
+            dg.step(0)
+            assert dg.download() == 1
+        finally:
+            dg.close()
+        return h
+
+    w = np.zeros(learn.N_FLOATS, np.float32)
+    w[learn.OFF_B3] = 0.7
+    w[learn.OFF_INV_STD:learn.OFF_GATE] = 1.0
+    base = run(None)
+    w[learn.OFF_GATE] = 0.0
+    moved = run(w)
+    w[learn.OFF_GATE] = 1.0e30
+    closed = run(w)
+    for a, b in zip(closed.fields, base.fields):
+        assert np.array_equal(a, b)
+    assert any(not np.array_equal(a, b) for a, b in zip(moved.fields, base.fields))
+    check_learned_run(base, moved, one_step=True)
+
+
+def check_level_uses_network(lib, grids, glob):
+    """check_sweep_uses_network for the batched level (hipPPMLevel*): the
+    constant network moves the step with gate 0 and leaves it bitwise the
+    reference start's with a closed gate."""
+    from enzo_ppm.level import Level
+
+    def run(w):
+        lv = Level([g.copy() for g in grids], glob, lib=lib)
+        try:
+            if w is not None:
+                lv.set_learned(w)
+            lv.step(0, 0.0, 0.4)
+            assert lv.status()[0] == 1
+            lv.download()
+            return lv.grids
+        finally:
+            lv.close()
+
+    w = np.zeros(learn.N_FLOATS, np.float32)
+    w[learn.OFF_B3] = 0.7
+    w[learn.OFF_INV_STD:learn.OFF_GATE] = 1.0
+    base = run(None)
+    w[learn.OFF_GATE] = 0.0
+    moved = run(w)
+    w[learn.OFF_GATE] = 1.0e30
+    closed = run(w)
+    differs = False
+    for gb, gm, gc in zip(base, moved, closed):
+        for a, b, c in zip(gb.fields, gm.fields, gc.fields):
+            assert np.array_equal(c, a)
+            differs |= not np.array_equal(b, a)
+            assert l1_rel(b, a) <= 1e-11
+    assert differs
+
+
 def l1_rel(a, b):
     """L1 relative difference; absolute where the reference field is zero."""
     den = float(np.sum(np.abs(b)))

@@ -61,6 +61,10 @@ CASES = {
     "longest_y": lambda: problems.random_smooth([3, 1025, 2], seed=11, ncolours=1),
     "one_cell": lambda: problems.random_smooth([1, 1, 1], seed=11, ncolours=1),
     "ragged": lambda: problems.random_smooth([5, 33, 2], seed=11, ncolours=1),
+    # flattening types 2 and 3 (calcdiss.F:447-566)
+    "flatten2_diffusion": lambda: problems.random_smooth([12, 10, 9], seed=9, ncolours=1,
+                                                         flags=(1, 2, 1)),
+    "flatten3": lambda: problems.random_smooth([12, 10, 9], seed=9, ncolours=1, flags=(1, 3, 0)),
 }
 
 
@@ -143,12 +147,13 @@ from enzo_ppm.level import Level  # noqa: E402
 from helpers import level_grids, level_reference  # noqa: E402
 
 
-def level_check(lib, grids, glob, steps, shapes=None, graph=True, dt=None):
+def level_check(lib, grids, glob, steps, shapes=None, graph=True, dt=None, persistent=False):
     ref = [g.copy() for g in grids]
     dts = level_reference(ref, glob, steps, dt=dt)
     lv = Level([g.copy() for g in grids], glob, lib=lib)
     try:
         lv.use_graph(graph)
+        assert lv.persistent(persistent) == persistent
         if shapes is not None:
             for i, g in enumerate(lv.grids):
                 for d in range(g.rank):
@@ -181,6 +186,16 @@ def test_level_direct_launches(cpulib):
 def test_level_kinds(cpulib, kind):
     grids, glob = level_grids(kind, 3)
     level_check(cpulib, grids, glob, 2)
+
+
+@pytest.mark.parametrize("kind", ["blast", "diffusion"])
+@pytest.mark.parametrize("graph", [True, False])
+def test_level_persistent(cpulib, kind, graph):
+    """Persistent sweep launches (workgroups pulling work items from per-XCD
+    counters, hipPPMLevelPersistent): every grid bitwise as the oracle's
+    per-grid loop, over steps that reuse the counters."""
+    grids, glob = level_grids(kind, 4)
+    level_check(cpulib, grids, glob, 3, persistent=True, graph=graph)
 
 
 def test_level_fallback(cpulib):

@@ -79,6 +79,15 @@ def test_newton_gain_on_held_out_states(lib, trained, held_out):
     assert s["restarted_fraction"] < 0.01
 
 
+def test_sweep_uses_network(lib):
+    lc.check_sweep_uses_network(lib, n=32)
+
+
+def test_level_uses_network(lib):
+    grids, glob = level_grids("blast", 24)
+    lc.check_level_uses_network(lib, grids, glob)
+
+
 def test_learned_one_step_against_oracle(lib, trained):
     g, glob, _ = problems.turbulent(64, seed=31)
     ref, got = lc.learned_vs_oracle(lib, g, glob, trained[0], steps=1)
@@ -132,3 +141,34 @@ def test_level_learned_blast(lib, trained):
     for a, b in zip(lv.grids, ref):
         for x, y in zip(a.fields, b.fields):
             assert lc.l1_rel(x, y) <= 1e-9
+
+
+@pytest.mark.parametrize("steps", [1, 20])
+def test_cosmological_level_learned_against_oracle(lib, trained, steps):
+    """The cosmological workload's kind of state (comoving at a = 0.1,
+    lognormal density sigma 2, Mach 100, static acceleration, 6 species) on
+    a 32^3 grid, batched with the learned start evaluated wherever the
+    acoustic start is above the pressure floor (gate 0), against the
+    oracle's comoving per-grid loop: one step max relative difference
+    <= 1e-11 per field, 20 steps L1 <= 1e-9 per field."""
+    from enzo_ppm import comoving
+    from helpers import oracle_comoving_level
+    w = trained[0].copy()
+    w[learn.OFF_GATE] = 0.0
+    g, glob, _ = problems.cosmological(32, seed=33)
+    model = comoving.EdS(0.1, 1.0)
+    ref = g.copy()
+    oracle_comoving_level([ref], glob, model, 1.0, steps, 0.4)
+    lv = Level([g.copy()], glob, lib=lib)
+    try:
+        lv.set_comoving(True)
+        lv.set_learned(w)
+        t = 1.0
+        for c in range(steps):
+            t, _ = comoving.level_step(lv, model, t, c, 0.4)
+            assert lv.status()[0] == 1
+        lv.download()
+    finally:
+        lv.close()
+    l1 = lc.check_learned_run(ref, lv.grids[0], one_step=steps == 1)
+    print(f"cosmological 32^3, {steps} steps: L1 per field", l1)

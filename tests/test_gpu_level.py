@@ -20,12 +20,13 @@ def lib():
     return L
 
 
-def level_check(lib, grids, glob, steps, shapes=None, graph=True):
+def level_check(lib, grids, glob, steps, shapes=None, graph=True, persistent=False):
     ref = [g.copy() for g in grids]
     dts = level_reference(ref, glob, steps)
     lv = Level([g.copy() for g in grids], glob, lib=lib)
     try:
         lv.use_graph(graph)
+        assert lv.persistent(persistent) == persistent
         if shapes is not None:
             for i, g in enumerate(lv.grids):
                 for d in range(g.rank):
@@ -60,10 +61,22 @@ def test_level_kinds(lib, kind):
     level_check(lib, grids, glob, 2)
 
 
-def test_level_many_grids(lib):
-    """48 grids, axes log-uniform in [8, 64]: the AMR-level workload shape."""
+@pytest.mark.parametrize("kind", ["blast", "diffusion"])
+@pytest.mark.parametrize("graph", [True, False])
+def test_level_persistent(lib, kind, graph):
+    """Persistent sweep launches (workgroups pulling work items from per-XCD
+    counters, hipPPMLevelPersistent): every grid bitwise as the oracle's
+    per-grid loop, over steps that reuse the counters."""
+    grids, glob = level_grids(kind, 4)
+    level_check(lib, grids, glob, 3, persistent=True, graph=graph)
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_level_many_grids(lib, persistent):
+    """48 grids, axes log-uniform in [8, 64]: the AMR-level workload shape
+    (with persistent launches: more work items than workgroups)."""
     grids, glob, _ = problems.amr_level(48, seed=21, nmin=8, nmax=64, ncolours=0)
-    level_check(lib, grids, glob, 2)
+    level_check(lib, grids, glob, 2, persistent=persistent)
 
 
 def test_level_fallback(lib):

@@ -62,7 +62,19 @@ CASES = {
     "one_cell": lambda: problems.random_smooth([1, 1, 1], seed=11, ncolours=1),
     "ragged": lambda: problems.random_smooth([5, 33, 2], seed=11, ncolours=1),
     "one_by_long": lambda: problems.random_smooth([1, 200, 1], seed=11, ncolours=1),
+    # PPMFlatteningParameter 2 (Lagrangean, calcdiss.F:447-485) and 3
+    # (multidimensional, :489-566), the first with diffusion
+    "flatten2_diffusion": lambda: problems.random_smooth([12, 10, 9], seed=9, ncolours=1,
+                                                         flags=(1, 2, 1)),
+    "flatten3": lambda: problems.random_smooth([12, 10, 9], seed=9, ncolours=1, flags=(1, 3, 0)),
+    "flatten3_blast": lambda: _first_grid(problems.amr_level(1, seed=44, nmin=16, nmax=20,
+                                                             ncolours=1, flags=(1, 3, 0))),
 }
+
+
+def _first_grid(level):
+    grids, glob, run = level
+    return grids[0], glob, run
 
 
 @pytest.mark.parametrize("name", sorted(CASES))

@@ -81,6 +81,15 @@ def test_learned_steps_against_oracle(cpulib, trained):
     lc.check_learned_run(ref, got, one_step=False)
 
 
+def test_sweep_uses_network(cpulib):
+    lc.check_sweep_uses_network(cpulib)
+
+
+def test_level_uses_network(cpulib):
+    grids, glob = level_grids("blast", 3)
+    lc.check_level_uses_network(cpulib, grids, glob)
+
+
 def test_untrained_network_still_converges(cpulib):
     """Any weights: Newton from a poor start restarts from the reference
     start where it does not converge, and the step stays within tolerance."""
