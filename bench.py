@@ -33,6 +33,14 @@ sys.path.insert(0, os.path.join(ROOT, "enzo-dev_amd"))
 
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector/matrix peak
+# VALU issue: 256 CUs x 4 SIMDs, one wave64 VALU instruction per SIMD every
+# 4 cycles (16 lanes per cycle), at the 2.4 GHz peak engine clock
+SIMDS, VALU_CYCLES_PER_INST, CLOCK_HZ = 1024, 4, 2.4e9
+# rocprofv3 counts of one 256^3 sweep of the turbulent workload with the
+# current build (profiles/ppm_sweep_kernel_turbulent256.md): wave-level VALU
+# instructions, waves, and the FP64 add / mul / fma / transcendental parts
+VALU_PER_SWEEP_256, WAVES_PER_SWEEP_256 = 1.242e9, 345840
+F64_ADD, F64_MUL, F64_FMA, F64_TRANS = 0.173e9, 0.218e9, 0.217e9, 0.035e9
 
 
 def parse():
@@ -539,13 +547,27 @@ def main():
         # (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x 64 lanes, an FMA two flops;
         # profiles/ppm_sweep_kernel_turbulent256.md) over the live sweep time:
         # the arithmetic side of the kernel's bound
-        flops = (0.173e9 + 0.218e9 + 2 * 0.217e9 + 0.035e9) * 64
+        flops = (F64_ADD + F64_MUL + 2 * F64_FMA + F64_TRANS) * 64
         out["fp64_valu"] = {"flop_per_launch": flops, "achieved_tflops": flops / sweep_avg_s / 1e12,
                             "peak_tflops": PEAK_FP64_TFLOPS,
                             "frac": flops / sweep_avg_s / 1e12 / PEAK_FP64_TFLOPS,
-                            "valu_instructions_per_launch": 1.242e9,
+                            "valu_instructions_per_launch": VALU_PER_SWEEP_256,
                             "note": "instruction counts from the committed rocprofv3 pass; "
                                     "time measured here"}
+        # the kernel's actual bound: VALU issue.  Interfaces per sweep: 257
+        # per pencil (i1 .. i2+1) on every pencil of the 262^2 cross-section
+        ifaces = (n + 1) * (n + 6) ** 2
+        issue_s = VALU_PER_SWEEP_256 * VALU_CYCLES_PER_INST / SIMDS / CLOCK_HZ
+        out["valu_issue"] = {"bound": "valu", "valu_wave_instructions": VALU_PER_SWEEP_256,
+                             "waves": WAVES_PER_SWEEP_256,
+                             "valu_per_wave": VALU_PER_SWEEP_256 / WAVES_PER_SWEEP_256,
+                             "interfaces": ifaces,
+                             "valu_wave_instructions_per_interface": VALU_PER_SWEEP_256 / ifaces,
+                             "issue_ms": issue_s * 1e3, "kernel_ms": sweep_avg_s * 1e3,
+                             "frac": issue_s / sweep_avg_s,
+                             "note": "issue time = instructions x 4 cycles / 1024 SIMDs / "
+                                     "2.4 GHz; frac = the share of the kernel's time the "
+                                     "SIMDs spend issuing VALU instructions"}
     if a.solver_steps > 0:
         # RiemannSolver HLL / HLLC on the same workload: whole pencils in the
         # one-pencil workgroups (not the metric; the two-shock solver is)

@@ -232,7 +232,8 @@ struct hipPPMData {
   int nsub = 0;
   long long fallback_total = 0;
   long long chain_total = 0;
-  // learned two-shock start (network parameters, kMlpFloats floats) when set
+  // learned two-shock start (network parameters in the device layout,
+  // kMlpDevFloats floats) when set
   float *learn = nullptr;
   bool learned = false;
   // comoving coordinates: sweeps use widths a_sweep * CellWidth, the Courant
@@ -1644,11 +1645,13 @@ int hipPPMSetLearnedGuess(hipPPMData *D, const float *weights) {
   }
   for (int k = 0; k < kMlpFloats; k++)
     if (!std::isfinite(weights[k])) return ENZO_PPM_ERR_ARGUMENT;
-  if (!D->learn && alloc_dev((void **)&D->learn, sizeof(float) * kMlpFloats))
+  if (!D->learn && alloc_dev((void **)&D->learn, sizeof(float) * kMlpDevFloats))
     return ENZO_PPM_ERR_HIP;
+  float dw[kMlpDevFloats];
+  learn_device_weights(weights, dw);
   // ordered after the stream's earlier work, visible to its later launches
   HIPCHK(hipStreamSynchronize(D->stream));
-  HIPCHK(hipMemcpy(D->learn, weights, sizeof(float) * kMlpFloats, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(D->learn, dw, sizeof(float) * kMlpDevFloats, hipMemcpyHostToDevice));
   D->learned = true;
   return ENZO_PPM_SUCCESS;
 }

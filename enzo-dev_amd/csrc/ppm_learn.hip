@@ -193,15 +193,16 @@ __global__ void learn_newton_kernel(const double *rec, long long n, double gamma
   pl = r[2];
   pr = r[3];
   int restarted = 0;
-  const bool gated = learn_gate(w, r[0], r[1], pl, pr, r[4], r[5], gamma, pmin);
+  const Acoustic A = acoustic_of(r[0], r[1], pl, pr, r[4], r[5], gamma, pmin);
+  const bool gated = learn_gate(w, A, pl, pr, pmin);
   float c = 0.0f;
   if (gated) {
     float x[kMlpIn];
     mlp_features(r[0], r[1], pl, pr, r[4], r[5], gamma, x);
     c = mlp_eval(w, x);
   }
-  const int it1 = twoshock_learned(gated, c, r[0], r[1], pl, pr, r[4], r[5], gamma, pmin, 0, pbar,
-                                   ubar, ps0, restarted);
+  const int it1 = twoshock_learned(gated, c, A, r[0], r[1], pl, pr, r[4], r[5], gamma, pmin, 0,
+                                   pbar, ubar, ps0, restarted);
   o[1] = it1;
   o[2] = restarted;
   o[4] = pbar;

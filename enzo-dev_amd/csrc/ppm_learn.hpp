@@ -37,6 +37,22 @@ constexpr int kOffMean = kMlpParams;
 constexpr int kOffInvStd = kOffMean + kMlpIn;
 constexpr int kOffGate = kOffInvStd + kMlpIn;  // wave-strength threshold for the network
 constexpr int kMlpFloats = kOffGate + 1;
+// the device copy appends W1 and W2 transposed ([IN][H] and [H][H], input
+// index outer), so that the work-items of consecutive hidden units read
+// consecutive weights (learn_device_weights)
+constexpr int kOffW1T = kMlpFloats;
+constexpr int kOffW2T = kOffW1T + kMlpIn * kMlpHidden;
+constexpr int kMlpDevFloats = kOffW2T + kMlpHidden * kMlpHidden;
+
+// the device layout of a parameter block w (kMlpFloats) into d (kMlpDevFloats)
+inline void learn_device_weights(const float *w, float *d) {
+  for (int k = 0; k < kMlpFloats; k++) d[k] = w[k];
+  for (int j = 0; j < kMlpHidden; j++)
+    for (int k = 0; k < kMlpIn; k++) d[kOffW1T + k * kMlpHidden + j] = w[kOffW1 + j * kMlpIn + k];
+  for (int j = 0; j < kMlpHidden; j++)
+    for (int k = 0; k < kMlpHidden; k++)
+      d[kOffW2T + k * kMlpHidden + j] = w[kOffW2 + j * kMlpHidden + k];
+}
 // training targets are clipped to [-kYClip, kYClip]: p* at the pressure
 // floor gives corrections near -46 and would dominate the squared error
 constexpr float kYClip = 8.0f;

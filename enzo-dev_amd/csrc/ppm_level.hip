@@ -1648,8 +1648,10 @@ int hipPPMLevelSetLearnedGuess(hipPPMLevel *L, const float *weights) {
   if (weights) {
     for (int k = 0; k < kMlpFloats; k++)
       if (!std::isfinite(weights[k])) return ENZO_PPM_ERR_ARGUMENT;
-    if (!L->learn && dev_alloc(&L->learn, kMlpFloats)) return ENZO_PPM_ERR_HIP;
-    LCHK(hipMemcpy(L->learn, weights, sizeof(float) * kMlpFloats, hipMemcpyHostToDevice));
+    if (!L->learn && dev_alloc(&L->learn, kMlpDevFloats)) return ENZO_PPM_ERR_HIP;
+    float dw[kMlpDevFloats];
+    learn_device_weights(weights, dw);
+    LCHK(hipMemcpy(L->learn, dw, sizeof(float) * kMlpDevFloats, hipMemcpyHostToDevice));
   }
   L->learned = weights != nullptr;
   return build_plan(L);  // parameters carry the network pointer; graphs are recaptured

@@ -166,6 +166,13 @@ enum Arr {
 constexpr int F_DF = A_DQ, F_EF = A_QL, F_UF = A_QR, F_VF = A_Q6, F_WF = A_ST,
               F_GEF = A_P, F_UB = A_X, F_COLF = A_ST, A_EOUT = A_X, A_GE1 = A_DQ,
               A_HOT = A_QR;
+// the learned start's acoustic quantities (Acoustic) of each interface,
+// between the two stages of the Riemann phase, in arrays dead then and
+// outside the network's lists (A_CS .. A_Q6): A_P after the pressure's
+// reconstruction, A_COL before the fallback flags and the colours, A_X
+// before the flux store (each work-item reads its own interface's entries
+// before it stores that interface's fluxes into F_GEF = A_P, F_UB = A_X)
+constexpr int A_AC_L = A_P, A_AC_R = A_COL, A_AC_P = A_X;
 
 // Fallback workgroups (one pencil, TP = 1) keep two more arrays: the HLL
 // density flux for the in-order fallback decision, then the two gas-energy
@@ -207,6 +214,13 @@ PPM_HD Chars chars_of(double u, double cs, double dt, double dx2i) {
   r.cp = dt * (u + cs) * dx2i;
   return r;
 }
+
+// The acoustic quantities of an interface for the learned start:
+// C_l = sqrt(gamma p_l rho_l), C_r, and the start p_ac floored at pmin
+// (twoshock.F:156-170).
+struct Acoustic {
+  double cl, cr, ps;
+};
 
 struct Regs {
   // interface averages (inteuler.F before the corrections)
@@ -769,14 +783,21 @@ PPM_HD int twoshock1(double dls, double drs, double &pls, double &prs, double ul
 // elsewhere p_ac, whose error is second order in that strength, is kept and
 // the network is not evaluated.  Where p_ac is at the floor (it predicts
 // cavitation) it is kept too.
-PPM_HD bool learn_gate(const float *w, double dls, double drs, double pls, double prs, double uls,
-                       double urs, double gamma, double pmin) {
-  const double cl = sqrt(gamma * pls * dls);
-  const double cr = sqrt(gamma * prs * drs);
-  double ps_ref = acoustic_start(dls, drs, pls, prs, uls, urs, cl, cr);
-  if (ps_ref < pmin) ps_ref = pmin;
+// The acoustic quantities of an interface (Acoustic), shared by the gate
+// and the solver.
+PPM_HD Acoustic acoustic_of(double dls, double drs, double pls, double prs, double uls, double urs,
+                            double gamma, double pmin) {
+  Acoustic a;
+  a.cl = sqrt(gamma * pls * dls);
+  a.cr = sqrt(gamma * prs * drs);
+  a.ps = acoustic_start(dls, drs, pls, prs, uls, urs, a.cl, a.cr);
+  if (a.ps < pmin) a.ps = pmin;
+  return a;
+}
+
+PPM_HD bool learn_gate(const float *w, const Acoustic &A, double pls, double prs, double pmin) {
   const double g = w[kOffGate];
-  return ps_ref > pmin && (fabs(ps_ref - pls) > g * pls || fabs(ps_ref - prs) > g * prs);
+  return A.ps > pmin && (fabs(A.ps - pls) > g * pls || fabs(A.ps - prs) > g * prs);
 }
 
 // The same solver started from the learned pressure (ppm_learn.hpp): at an
@@ -785,15 +806,14 @@ PPM_HD bool learn_gate(const float *w, double dls, double drs, double pls, doubl
 // not meet the tolerance from the learned start, p_ac starts it again
 // (restarted = 1), so a non-converging interface gets the reference
 // method's result.  Returns all updates taken.
-PPM_HD int twoshock_learned(bool have_c, float c, double dls, double drs, double &pls,
-                            double &prs, double uls, double urs, double gamma, double pmin,
-                            int ipresfree, double &pbar, double &ubar, double &ps0, int &restarted) {
+// A holds the interface's acoustic_of.
+PPM_HD int twoshock_learned(bool have_c, float c, const Acoustic &A, double dls, double drs,
+                            double &pls, double &prs, double uls, double urs, double gamma,
+                            double pmin, int ipresfree, double &pbar, double &ubar, double &ps0,
+                            int &restarted) {
   restarted = 0;
   if (ipresfree == 1) return twoshock1(dls, drs, pls, prs, uls, urs, gamma, pmin, ipresfree, pbar, ubar, ps0);
-  const double cl = sqrt(gamma * pls * dls);
-  const double cr = sqrt(gamma * prs * drs);
-  double ps_ref = acoustic_start(dls, drs, pls, prs, uls, urs, cl, cr);
-  if (ps_ref < pmin) ps_ref = pmin;
+  const double cl = A.cl, cr = A.cr, ps_ref = A.ps;
   double ps = ps_ref;
   if (have_c) {  // the interface passed the gate (learn_gate)
     ps = ps_ref * (double)mlp_exp(c);
@@ -1080,6 +1100,12 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
     r.s_pr = prs;
     r.s_ul = uls;
     r.s_ur = urs;
+    if (P.ipresfree != 1) {  // kept in LDS arrays that are dead here (A_AC_*)
+      const Acoustic a = acoustic_of(dls, drs, pls, prs, uls, urs, gamma, P.pmin);
+      LDS<TP, TS, UNI>(sm, A_AC_L, p, cb) = a.cl;
+      LDS<TP, TS, UNI>(sm, A_AC_R, p, cb) = a.cr;
+      LDS<TP, TS, UNI>(sm, A_AC_P, p, cb) = a.ps;
+    }
     return;
   }
   if constexpr (STAGE == 1) {
@@ -1094,9 +1120,14 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
   double pbar, ubar, ps0;
   const double pls_in = pls, prs_in = prs;
   int iters, restarted = 0;
-  if constexpr (LEARN != 0 && STAGE == 1)
-    iters = twoshock_learned(r.ls >= 0, r.lc, dls, drs, pls, prs, uls, urs, gamma, P.pmin,
+  if constexpr (LEARN != 0 && STAGE == 1) {
+    Acoustic a;
+    a.cl = LDS<TP, TS, UNI>(sm, A_AC_L, p, cb);
+    a.cr = LDS<TP, TS, UNI>(sm, A_AC_R, p, cb);
+    a.ps = LDS<TP, TS, UNI>(sm, A_AC_P, p, cb);
+    iters = twoshock_learned(r.ls >= 0, r.lc, a, dls, drs, pls, prs, uls, urs, gamma, P.pmin,
                              P.ipresfree, pbar, ubar, ps0, restarted);
+  }
   else
     iters = twoshock1(dls, drs, pls, prs, uls, urs, gamma, P.pmin, P.ipresfree, pbar, ubar, ps0);
   if (FB == 0 && P.harvest && i >= T.o && i < T.o + T.w) {
@@ -1707,33 +1738,44 @@ struct TileCounts {
 // Learned start inside the tile.  Between the two stages of the Riemann
 // phase, the LDS arrays A_CS .. A_Q6 are free (the reconstruction is done,
 // the flux store has not begun); as floats they hold the list of network
-// inputs of the interfaces that pass the gate ([NT][kNetStride]), the
-// hidden layers of a chunk of CH of them ([CH][32] twice) and, in the last
-// float, the list's length.  The workgroup evaluates the network on the list
+// inputs of the interfaces that pass the gate ([NT][kNetStride]) and the
+// hidden layers of a chunk of CH of them ([CH][32] twice).  The list's
+// length is an int in the first entry of A_COL (pencil 0, no interface's
+// entry), cleared in the tile's first phase.  The gate is tested in the
+// first stage's phase; the workgroup then evaluates the network on the list
 // together, one (interface, hidden unit) pair per work-item, so that the
 // network costs registers nowhere and work only where the gate passes.
 template <int TP, int TS, int UNI>
 struct NetLds {
   static constexpr int NT = TP * TS;
   static constexpr int FLOATS = 2 * (A_Q6 - A_CS + 1) * TP * Cfg<TP, TS, UNI>::LW;
-  static constexpr int fits(int ch) { return kNetStride * NT + 2 * kMlpHidden * ch + 1 <= FLOATS; }
+  static constexpr int fits(int ch) { return kNetStride * NT + 2 * kMlpHidden * ch <= FLOATS; }
   static constexpr int CH = fits(32) ? 32 : fits(16) ? 16 : fits(8) ? 8 : 4;
   static_assert(fits(CH), "the learned start's lists fit the free tile arrays");
+  static constexpr bool outside(int a) { return a < A_CS || a > A_Q6; }
+  static_assert(outside(A_AC_L) && outside(A_AC_R) && outside(A_AC_P),
+                "the acoustic quantities live outside the lists");
   PPM_HD static float *base(double *sm) {
     return reinterpret_cast<float *>(sm + (size_t)A_CS * TP * Cfg<TP, TS, UNI>::LW);
   }
-  PPM_HD static int *count(double *sm) { return reinterpret_cast<int *>(base(sm) + FLOATS - 1); }
+  PPM_HD static int *count(double *sm) {
+    return reinterpret_cast<int *>(sm + (size_t)A_COL * TP * Cfg<TP, TS, UNI>::LW);
+  }
 };
 
-// Phase: the gate at the thread's interface (learn_gate); a passing
-// interface appends its network inputs to the list and keeps its slot.
+// The gate at the thread's interface (learn_gate), after the first stage of
+// the Riemann phase in the same phase: a passing interface appends its
+// network inputs to the list and keeps its slot.
 template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_learn_gate(const SweepParams &P, const Tile &T, double *sm, int tid, Regs &r) {
   r.ls = -1;
   r.lc = 0.0f;
   const int p = tid / TS, q = tid % TS;
   if (p >= T.np || q >= T.NI || P.ipresfree == 1) return;
-  if (!learn_gate(P.learn, r.dls, r.drs, r.s_pl, r.s_pr, r.s_ul, r.s_ur, P.gamma, P.pmin)) return;
+  PPM_ASSERT(T.IA - T.L0 > 0);  // no interface's A_AC_R entry is the list's length
+  Acoustic a;
+  a.ps = LDS<TP, TS, UNI>(sm, A_AC_P, p, T.IA + q - T.L0);
+  if (!learn_gate(P.learn, a, r.s_pl, r.s_pr, P.pmin)) return;
   const int k = atomicAdd(NetLds<TP, TS, UNI>::count(sm), 1);
   r.ls = k;
   float x[kMlpIn];
@@ -1760,7 +1802,7 @@ PPM_HD void learn_eval(const SweepParams &P, double *sm, int m, X &ex) {
         const float *x = list + kNetStride * (c0 + e);
         float a = w[kOffB1 + j];
         for (int k = 0; k < kMlpIn; k++)
-          a += w[kOffW1 + j * kMlpIn + k] * ((x[k] - w[kOffMean + k]) * w[kOffInvStd + k]);
+          a += w[kOffW1T + k * H + j] * ((x[k] - w[kOffMean + k]) * w[kOffInvStd + k]);
         h1[idx] = mlp_tanh(a);
       }
     });
@@ -1769,7 +1811,10 @@ PPM_HD void learn_eval(const SweepParams &P, double *sm, int m, X &ex) {
       for (int idx = tid; idx < mc * H; idx += L::NT) {
         const int e = idx / H, j = idx % H;
         float a = w[kOffB2 + j];
-        for (int k = 0; k < H; k++) a += w[kOffW2 + j * H + k] * h1[e * H + k];
+        // a few weights in flight at a time: the interface states stay live
+        // in registers across the network
+#pragma unroll 4
+        for (int k = 0; k < H; k++) a += w[kOffW2T + k * H + j] * h1[e * H + k];
         h2[idx] = w[kOffW3 + j] * mlp_tanh(a);
       }
     });
@@ -1777,6 +1822,7 @@ PPM_HD void learn_eval(const SweepParams &P, double *sm, int m, X &ex) {
       (void)r;
       for (int e = tid; e < mc; e += L::NT) {
         float y = w[kOffB3];
+#pragma unroll 4
         for (int j = 0; j < H; j++) y += h2[e * H + j];
         list[kNetStride * (c0 + e)] = y;
       }
@@ -1806,6 +1852,8 @@ PPM_HD TileCounts sweep_body(const SweepParams &P, const Tile &T, double *sm, X 
   ex.run([&](int tid, Regs &r) {
     ph_flat_reg<TP, TS, UNI>(P, T, tid, r);
     ph_load<TP, TS, UNI>(P, T, sm, tid);
+    if constexpr (LEARN != 0)
+      if (tid == 0) *NetLds<TP, TS, UNI>::count(sm) = 0;
   });
   ex.run([&](int tid, Regs &r) { (void)r; ph_chars<TP, TS, UNI>(P, T, sm, tid); });
   ex.run([&](int tid, Regs &r) { (void)r; ph_steepen<TP, TS, UNI>(P, T, sm, tid); });
@@ -1819,9 +1867,8 @@ PPM_HD TileCounts sweep_body(const SweepParams &P, const Tile &T, double *sm, X 
     // the Riemann phase in two stages around the network (NetLds)
     ex.run([&](int tid, Regs &r) {
       ph_solve<TP, TS, UNI, FB, LEARN, 0>(P, T, sm, tid, r);
-      if (tid == 0) *NetLds<TP, TS, UNI>::count(sm) = 0;
+      ph_learn_gate<TP, TS, UNI>(P, T, sm, tid, r);
     });
-    ex.run([&](int tid, Regs &r) { ph_learn_gate<TP, TS, UNI>(P, T, sm, tid, r); });
     const int m = *NetLds<TP, TS, UNI>::count(sm);  // the same for every work-item
     if (m > 0) learn_eval<TP, TS, UNI>(P, sm, m, ex);
     ex.run([&](int tid, Regs &r) {

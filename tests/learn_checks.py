@@ -141,6 +141,28 @@ def check_sweep_uses_network(lib, n=12, seed=13):
     check_learned_run(base, moved, one_step=True)
 
 
+def check_sweep_network_is_kernels(lib, g, glob, w):
+    """The sweep's learned start (the workgroup's evaluation of the network
+    on the gated interfaces, from the device copy of the weights) is the
+    per-interface network of the statistics kernel (mlp_eval), bit for bit:
+    harvesting with the learned start on, every record's start, Newton
+    updates, restart flag and p* equal what hipPPMLearnNewtonStats computes
+    from the record's inputs.  Returns the fraction of records whose start
+    the network moved."""
+    w = np.array(w, dtype=np.float32, copy=True)
+    w[learn.OFF_GATE] = 0.0
+    rec, seen = learn.harvest(g, glob, steps=1, lib=lib, weights=w)
+    assert seen == len(rec) > 0
+    out, _ = learn.newton_stats(rec, GAMMA, PMIN, w, lib=lib)
+    assert np.array_equal(out[:, 5], rec[:, 7])
+    assert np.array_equal(out[:, 1], rec[:, 8])
+    assert np.array_equal(out[:, 2], rec[:, 9])
+    assert np.array_equal(out[:, 4], rec[:, 6])
+    moved = float(np.mean(out[:, 5] != out[:, 6]))
+    assert moved > 0.5, moved
+    return moved
+
+
 def check_level_uses_network(lib, grids, glob):
     """check_sweep_uses_network for the batched level (hipPPMLevel*): the
     constant network moves the step with gate 0 and leaves it bitwise the

@@ -83,6 +83,11 @@ def test_sweep_uses_network(lib):
     lc.check_sweep_uses_network(lib, n=32)
 
 
+def test_sweep_network_is_kernels(lib, trained):
+    g, glob, _ = problems.turbulent(32, seed=5)
+    lc.check_sweep_network_is_kernels(lib, g, glob, trained[0])
+
+
 def test_level_uses_network(lib):
     grids, glob = level_grids("blast", 24)
     lc.check_level_uses_network(lib, grids, glob)

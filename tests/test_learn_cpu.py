@@ -85,6 +85,11 @@ def test_sweep_uses_network(cpulib):
     lc.check_sweep_uses_network(cpulib)
 
 
+def test_sweep_network_is_kernels(cpulib, trained):
+    g, glob, _ = problems.turbulent(12, seed=5)
+    lc.check_sweep_network_is_kernels(cpulib, g, glob, trained[0])
+
+
 def test_level_uses_network(cpulib):
     grids, glob = level_grids("blast", 3)
     lc.check_level_uses_network(cpulib, grids, glob)
