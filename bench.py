@@ -39,8 +39,8 @@ SIMDS, VALU_CYCLES_PER_INST, CLOCK_HZ = 1024, 4, 2.4e9
 # rocprofv3 counts of one 256^3 sweep of the turbulent workload with the
 # current build (profiles/ppm_sweep_kernel_turbulent256.md): wave-level VALU
 # instructions, waves, and the FP64 add / mul / fma / transcendental parts
-VALU_PER_SWEEP_256, WAVES_PER_SWEEP_256 = 1.242e9, 345840
-F64_ADD, F64_MUL, F64_FMA, F64_TRANS = 0.173e9, 0.218e9, 0.217e9, 0.035e9
+VALU_PER_SWEEP_256, WAVES_PER_SWEEP_256 = 1.231e9, 345840
+F64_ADD, F64_MUL, F64_FMA, F64_TRANS = 0.168e9, 0.210e9, 0.218e9, 0.0355e9
 
 
 def parse():
@@ -186,6 +186,18 @@ def level_bench(lib, ngrids, steps, cfl=0.4, weights=None):
             d.pressure_and_dt(cfl, want_host=False)
             d.step(0, dt)
         shapes = lv.tune(1, dt, repeats=2)
+        ttimes, (t_plain, t_pers) = lv.tune_times()
+        shape_names = ("16x16", "8x32", "4x64")
+        classes = ("[0,20]", "[21,40]", "[41,80]", "81+")
+        tuner = {"ms_per_step": {classes[c]: {shape_names[s]: round(float(ttimes[c][s]) * 1e3, 3)
+                                              for s in range(3) if ttimes[c][s] == ttimes[c][s]}
+                                 for c in range(4)},
+                 "chosen": {classes[c]: shape_names[shapes[c]] for c in range(4)},
+                 "launch_ms_per_step": {"one_workgroup_per_tile": t_plain * 1e3,
+                                        "persistent": t_pers * 1e3},
+                 "persistent": lv.persistent(),
+                 "note": "pencil-length classes (ghosts included), each timed with the other "
+                         "classes at their current choice, in class order"}
         lv.step(1, dt, cfl)
         for d in dgs:
             d.pressure_and_dt(cfl, want_host=False)
@@ -239,7 +251,7 @@ def level_bench(lib, ngrids, steps, cfl=0.4, weights=None):
             "cells_per_step": cells, "steps": steps, "dt": dt,
             "plain_per_grid": {"value": plain, "ms_per_step": (t1 - t0) * 1e3 / steps},
             "batched": {"value": batched, "ms_per_step": (t2 - t1) * 1e3 / steps,
-                        "tile_shape_by_pencil_length": shapes},
+                        "tile_shape_by_pencil_length": shapes, "tuner": tuner},
             "unit": "cell-updates/s", "speedup": batched / plain, "bitwise_equal": same,
             **({"batched_learned": {"value": batched_learned,
                                     "ms_per_step": cells / batched_learned * 1e3,
@@ -399,7 +411,11 @@ def learned_bench(lib, n, steps, warmup, epochs, cap):
                         "seconds": t1 - t0},
             "training": {"epochs": epochs, "batch": 8192, "lr": 3e-3, "seconds": t2 - t1,
                          "mse_per_epoch": [float(x) for x in mse],
-                         "gate": float(w[learn.OFF_GATE]),
+                         # +inf: the network cannot save 1% of the updates on the
+                         # gate-fit sample, so it is switched off (plain kernels)
+                         "gate": (float(w[learn.OFF_GATE]) if np.isfinite(w[learn.OFF_GATE])
+                                  else None),
+                         "network_on": bool(np.isfinite(w[learn.OFF_GATE])),
                          "gate_choice": {str(k): {"mean_updates": v[0], "evaluated_fraction": v[1]}
                                          for k, v in gate_table.items()}},
             "held_out_newton": stats, "grid_step_learned": t}, w

@@ -1643,8 +1643,11 @@ int hipPPMSetLearnedGuess(hipPPMData *D, const float *weights) {
     D->learned = false;
     return ENZO_PPM_SUCCESS;
   }
-  for (int k = 0; k < kMlpFloats; k++)
-    if (!std::isfinite(weights[k])) return ENZO_PPM_ERR_ARGUMENT;
+  if (!learn_weights_valid(weights)) return ENZO_PPM_ERR_ARGUMENT;
+  if (learn_gate_closed(weights)) {  // no interface can pass: the plain kernels
+    D->learned = false;
+    return ENZO_PPM_SUCCESS;
+  }
   if (!D->learn && alloc_dev((void **)&D->learn, sizeof(float) * kMlpDevFloats))
     return ENZO_PPM_ERR_HIP;
   float dw[kMlpDevFloats];

@@ -44,6 +44,17 @@ constexpr int kOffW1T = kMlpFloats;
 constexpr int kOffW2T = kOffW1T + kMlpIn * kMlpHidden;
 constexpr int kMlpDevFloats = kOffW2T + kMlpHidden * kMlpHidden;
 
+// A parameter block is valid when every value is finite, except the gate,
+// which may also be +infinity: the gate closed, no interface evaluates the
+// network, every start is the reference one and the library runs its plain
+// kernels (bitwise the reference start's results, at no cost).
+inline bool learn_gate_closed(const float *w) { return w[kOffGate] == INFINITY; }
+inline bool learn_weights_valid(const float *w) {
+  for (int k = 0; k < kMlpFloats; k++)
+    if (!(std::isfinite(w[k]) || (k == kOffGate && learn_gate_closed(w)))) return false;
+  return true;
+}
+
 // the device layout of a parameter block w (kMlpFloats) into d (kMlpDevFloats)
 inline void learn_device_weights(const float *w, float *d) {
   for (int k = 0; k < kMlpFloats; k++) d[k] = w[k];

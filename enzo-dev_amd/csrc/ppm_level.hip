@@ -357,6 +357,9 @@ struct hipPPMLevel {
   bool persistent = false;
   int persistent_blocks = 1024;
   unsigned *queues = nullptr;
+  // the last hipPPMLevelTune's timings, seconds per level step: [class][shape]
+  // (NaN: not timed), then the ordinary and the persistent launch
+  double tune_times[4 * kShapes + 2];
   std::map<int, hipGraphExec_t> graphs;
   double graph_cfl = -1.0;          // cfl the Courant-step graphs were captured with
   long long fallback_total = 0;
@@ -895,6 +898,7 @@ int hipPPMLevelCreate(hipPPMLevel **Level, enzo_int NumberOfGrids, const hipPPMP
   }
   hipPPMLevel *L = new (std::nothrow) hipPPMLevel();
   if (!L) return ENZO_PPM_ERR_ARGUMENT;
+  for (double &t : L->tune_times) t = std::nan("");
   const int G = (int)NumberOfGrids;
   L->ngrids = G;
   L->para.assign(Para, Para + G);
@@ -1288,6 +1292,7 @@ int hipPPMLevelTune(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dt, enzo_in
   LCHK(hipMemcpy(save, parity0 == 0 ? L->A : L->B, sizeof(double) * tot, hipMemcpyDeviceToDevice));
   int choice[4];
   for (int c = 0; c < 4; c++) choice[c] = 1;
+  for (double &t : L->tune_times) t = std::nan("");
   auto apply = [&]() {
     for (int g = 0; g < L->ngrids; g++)
       for (int d = 0; d < L->rank; d++) {
@@ -1316,6 +1321,7 @@ int hipPPMLevelTune(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dt, enzo_in
         st = hipPPMLevelStep(L, CycleNumber, dt, 0.0);
       if (hipStreamSynchronize(L->stream) != hipSuccess) st = ENZO_PPM_ERR_HIP;
       const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (st == ENZO_PPM_SUCCESS) L->tune_times[c * kShapes + s] = t / (double)repeats;
       if (st == ENZO_PPM_SUCCESS && t < best) {
         best = t;
         best_s = s;
@@ -1336,6 +1342,7 @@ int hipPPMLevelTune(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dt, enzo_in
         st = hipPPMLevelStep(L, CycleNumber, dt, 0.0);
       if (hipStreamSynchronize(L->stream) != hipSuccess) st = ENZO_PPM_ERR_HIP;
       tp[pers] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (st == ENZO_PPM_SUCCESS) L->tune_times[4 * kShapes + pers] = tp[pers] / (double)repeats;
     }
     hipPPMLevelPersistent(L, tp[1] < tp[0] ? 1 : 0);
   }
@@ -1348,6 +1355,12 @@ int hipPPMLevelTune(hipPPMLevel *L, enzo_int CycleNumber, enzo_float dt, enzo_in
   if (shape_by_length)
     for (int c = 0; c < 4; c++) shape_by_length[c] = choice[c];
   return st;
+}
+
+int hipPPMLevelTuneTimes(const hipPPMLevel *L, enzo_float *times) {
+  if (!L || !times) return ENZO_PPM_ERR_ARGUMENT;
+  for (int k = 0; k < 4 * kShapes + 2; k++) times[k] = L->tune_times[k];
+  return ENZO_PPM_SUCCESS;
 }
 
 int hipPPMLevelSetComoving(hipPPMLevel *L, enzo_int enable) {
@@ -1645,15 +1658,16 @@ int hipPPMLevelExternalBoundary(hipPPMLevel *L, const enzo_int *BoundaryType,
 int hipPPMLevelSetLearnedGuess(hipPPMLevel *L, const float *weights) {
   if (!L) return ENZO_PPM_ERR_ARGUMENT;
   LCHK(hipStreamSynchronize(L->stream));
-  if (weights) {
-    for (int k = 0; k < kMlpFloats; k++)
-      if (!std::isfinite(weights[k])) return ENZO_PPM_ERR_ARGUMENT;
+  if (weights && !learn_weights_valid(weights)) return ENZO_PPM_ERR_ARGUMENT;
+  // a closed gate (no interface can pass) runs the plain kernels
+  const bool on = weights && !learn_gate_closed(weights);
+  if (on) {
     if (!L->learn && dev_alloc(&L->learn, kMlpDevFloats)) return ENZO_PPM_ERR_HIP;
     float dw[kMlpDevFloats];
     learn_device_weights(weights, dw);
     LCHK(hipMemcpy(L->learn, dw, sizeof(float) * kMlpDevFloats, hipMemcpyHostToDevice));
   }
-  L->learned = weights != nullptr;
+  L->learned = on;
   return build_plan(L);  // parameters carry the network pointer; graphs are recaptured
 }
 

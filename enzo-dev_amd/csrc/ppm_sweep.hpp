@@ -1749,9 +1749,15 @@ template <int TP, int TS, int UNI>
 struct NetLds {
   static constexpr int NT = TP * TS;
   static constexpr int FLOATS = 2 * (A_Q6 - A_CS + 1) * TP * Cfg<TP, TS, UNI>::LW;
-  static constexpr int fits(int ch) { return kNetStride * NT + 2 * kMlpHidden * ch <= FLOATS; }
-  static constexpr int CH = fits(32) ? 32 : fits(16) ? 16 : fits(8) ? 8 : 4;
-  static_assert(fits(CH), "the learned start's lists fit the free tile arrays");
+  // list, two hidden layers of a chunk of ch interfaces, and (w2) a copy of
+  // the transposed second-layer weights
+  static constexpr int need(int ch, bool w2) {
+    return kNetStride * NT + 2 * kMlpHidden * ch + (w2 ? kMlpHidden * kMlpHidden : 0);
+  }
+  static constexpr bool fits(int ch, bool w2) { return need(ch, w2) <= FLOATS; }
+  static constexpr bool W2 = fits(16, true);  // chunks of at least 16 with the weights in LDS
+  static constexpr int CH = fits(32, W2) ? 32 : fits(16, W2) ? 16 : fits(8, W2) ? 8 : 4;
+  static_assert(fits(CH, W2), "the learned start's lists fit the free tile arrays");
   static constexpr bool outside(int a) { return a < A_CS || a > A_Q6; }
   static_assert(outside(A_AC_L) && outside(A_AC_R) && outside(A_AC_P),
                 "the acoustic quantities live outside the lists");
@@ -1786,17 +1792,21 @@ PPM_HD void ph_learn_gate(const SweepParams &P, const Tile &T, double *sm, int t
 
 // The network (mlp_eval's arithmetic, operation for operation) on the m
 // listed interfaces, chunk by chunk; each correction replaces its inputs'
-// first float.
+// first float, where the interface's work-item reads it.
 template <int TP, int TS, int UNI, class X>
 PPM_HD void learn_eval(const SweepParams &P, double *sm, int m, X &ex) {
   using L = NetLds<TP, TS, UNI>;
   constexpr int H = kMlpHidden;
   float *list = L::base(sm), *h1 = list + kNetStride * L::NT, *h2 = h1 + L::CH * H;
+  float *w2s = h2 + L::CH * H;  // the transposed W2 in LDS when L::W2
   const float *w = P.learn;
+  const float *w2 = L::W2 ? w2s : w + kOffW2T;
   for (int c0 = 0; c0 < m; c0 += L::CH) {
     const int mc = m - c0 < L::CH ? m - c0 : L::CH;
-    ex.run([&](int tid, Regs &r) {  // hidden layer 1
+    ex.run([&](int tid, Regs &r) {  // hidden layer 1 (and the W2 copy, once)
       (void)r;
+      if (L::W2 && c0 == 0)
+        for (int idx = tid; idx < H * H; idx += L::NT) w2s[idx] = w[kOffW2T + idx];
       for (int idx = tid; idx < mc * H; idx += L::NT) {
         const int e = idx / H, j = idx % H;
         const float *x = list + kNetStride * (c0 + e);
@@ -1814,7 +1824,7 @@ PPM_HD void learn_eval(const SweepParams &P, double *sm, int m, X &ex) {
         // a few weights in flight at a time: the interface states stay live
         // in registers across the network
 #pragma unroll 4
-        for (int k = 0; k < H; k++) a += w[kOffW2T + k * H + j] * h1[e * H + k];
+        for (int k = 0; k < H; k++) a += w2[k * H + j] * h1[e * H + k];
         h2[idx] = w[kOffW3 + j] * mlp_tanh(a);
       }
     });
@@ -1828,7 +1838,9 @@ PPM_HD void learn_eval(const SweepParams &P, double *sm, int m, X &ex) {
       }
     });
   }
-  ex.run([&](int tid, Regs &r) {  // every listed interface takes its correction
+  // every listed interface takes its correction (before the second stage,
+  // whose flux store overwrites the list)
+  ex.run([&](int tid, Regs &r) {
     (void)tid;
     if (r.ls >= 0) r.lc = list[kNetStride * r.ls];
   });

@@ -217,6 +217,7 @@ PROTOTYPES = {
     "hipPPMLevelSetShape": (C.c_int, [_H, I, I, I]),
     "hipPPMLevelCommitShapes": (C.c_int, [_H]),
     "hipPPMLevelTune": (C.c_int, [_H, I, _D, I, PI]),
+    "hipPPMLevelTuneTimes": (C.c_int, [_H, PF]),
     "hipPPMLevelSetComoving": (C.c_int, [_H, I]),
     "hipPPMLevelSetExpansionFactors": (C.c_int, [_H, _D, _D]),
     "hipPPMLevelTimeStep": (C.c_int, [_H, _D, PF]),

@@ -165,8 +165,11 @@ def fit_gate(records, gamma, pmin, weights,
              candidates=(0.0, 1e-3, 3e-3, 1e-2, 3e-2, 0.1, 0.3, 1.0), rel_tol=0.01, lib=None):
     """Gate choice on `records`: the largest candidate whose mean Newton
     updates are within `rel_tol` of the fewest any candidate gives (nearly
-    the whole gain for the fewest network evaluations).  Returns (weights
-    with that gate, {gate: (mean updates, fraction evaluated)})."""
+    the whole gain for the fewest network evaluations).  The closed gate
+    (+inf: the network never evaluated, the reference start everywhere, the
+    plain kernels) is a candidate too, so a network that cannot save
+    `rel_tol` of the updates is switched off.  Returns (weights with that
+    gate, {gate: (mean updates, fraction evaluated)})."""
     w = np.array(weights, dtype=np.float32, copy=True)
     r = np.ascontiguousarray(records, dtype=np.float64)
     table = {}
@@ -175,6 +178,7 @@ def fit_gate(records, gamma, pmin, weights,
         out, s = newton_stats(r, gamma, pmin, w, lib=lib)
         table[float(c)] = (s["newton_updates_learned_start"]["mean"],
                            float(evaluated(r, out, np.float32(c), pmin).mean()))
+    table[float("inf")] = (s["newton_updates_reference_start"]["mean"], 0.0)
     best = min(v[0] for v in table.values())
     w[OFF_GATE] = max(c for c, v in table.items() if v[0] <= best * (1.0 + rel_tol))
     return w, table

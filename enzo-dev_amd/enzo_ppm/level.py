@@ -116,6 +116,14 @@ class Level:
         self._check(self.lib.hipPPMLevelTune(self.handle, cycle, dt, repeats, out), "LevelTune")
         return list(out)
 
+    def tune_times(self):
+        """The last tune's seconds per level step: ([class][shape] array,
+        NaN where not timed; (ordinary, persistent) launch)."""
+        import numpy as np
+        t = np.full(14, np.nan)
+        self._check(self.lib.hipPPMLevelTuneTimes(self.handle, abi.dptr(t)), "LevelTuneTimes")
+        return t[:12].reshape(4, 3), (float(t[12]), float(t[13]))
+
     def set_comoving(self, enable=True):
         self._check(self.lib.hipPPMLevelSetComoving(self.handle, int(enable)), "LevelSetComoving")
         self.a_sweep = 1.0

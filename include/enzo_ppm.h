@@ -421,7 +421,9 @@ int hipPPMSynchronize(hipPPMData *Data);
    W3[32] b3, input mean[6], 1/std[6], gate.
 
    hipPPMSetLearnedGuess: use the network from the next sweep on (copied;
-   weights NULL switches back to the reference start). */
+   weights NULL switches back to the reference start).  Every weight must be
+   finite, except the gate, which may be +infinity: no interface passes,
+   and the plain kernels run (the reference start, at no cost). */
 int hipPPMSetLearnedGuess(hipPPMData *Data, const float *weights);
 
 /* Harvest of interface states: the sweeps record, for one owned interface in
@@ -562,6 +564,13 @@ int hipPPMLevelCommitShapes(hipPPMLevel *Level);
    shape_by_length[4] receives the choices. */
 int hipPPMLevelTune(hipPPMLevel *Level, enzo_int CycleNumber, enzo_float dt, enzo_int repeats,
                     enzo_int *shape_by_length);
+
+/* What the last hipPPMLevelTune measured, seconds per level step:
+   times[3 * class + shape] for the four length classes and the three tile
+   shapes (NaN where a shape was not timed: no grid of the class, or a shape
+   that cannot tile it), then times[12] for one workgroup per work item and
+   times[13] for the persistent launch (NaN before any tuning). */
+int hipPPMLevelTuneTimes(const hipPPMLevel *Level, enzo_float *times);
 
 /* Comoving coordinates for the level, as hipPPMSetComoving /
    hipPPMSetExpansionFactors / hipPPMComovingExpansionTerms for one grid:

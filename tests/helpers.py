@@ -112,6 +112,22 @@ def fallback_differences(grid, fields_ref, nfb):
     return n
 
 
+def check_tune_times(lv, choice):
+    """The tuner kept, per pencil-length class, the shape it timed fastest,
+    and the faster of the two launch forms (hipPPMLevelTuneTimes)."""
+    import numpy as np
+    times, (plain, pers) = lv.tune_times()
+    for c in range(4):
+        row = times[c]
+        if np.all(np.isnan(row)):
+            continue
+        assert np.all(row[~np.isnan(row)] > 0)
+        assert choice[c] == int(np.nanargmin(row)), (c, row, choice)
+    assert plain > 0 and pers > 0
+    assert lv.persistent() == (pers < plain)
+    return times, (plain, pers)
+
+
 def level_reference(grids, glob, steps, cfl=0.4, dt=None):
     """The oracle's per-grid loop for a level: each step every grid takes the
     level's dt (the minimum over the grids of the Courant step, or `dt`) and

@@ -137,6 +137,9 @@ def check_sweep_uses_network(lib, n=12, seed=13):
     closed = run(w)
     for a, b in zip(closed.fields, base.fields):
         assert np.array_equal(a, b)
+    w[learn.OFF_GATE] = np.inf    # the closed gate: the plain kernels
+    for a, b in zip(run(w).fields, base.fields):
+        assert np.array_equal(a, b)
     assert any(not np.array_equal(a, b) for a, b in zip(moved.fields, base.fields))
     check_learned_run(base, moved, one_step=True)
 
@@ -189,6 +192,10 @@ def check_level_uses_network(lib, grids, glob):
     moved = run(w)
     w[learn.OFF_GATE] = 1.0e30
     closed = run(w)
+    w[learn.OFF_GATE] = np.inf
+    for gb, gi in zip(base, run(w)):
+        for a, b in zip(gb.fields, gi.fields):
+            assert np.array_equal(a, b)
     differs = False
     for gb, gm, gc in zip(base, moved, closed):
         for a, b, c in zip(gb.fields, gm.fields, gc.fields):

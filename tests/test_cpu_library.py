@@ -144,7 +144,7 @@ def test_comoving_steps(cpulib, dual):
 
 # ---- batched AMR level (hipPPMLevel*) --------------------------------------
 from enzo_ppm.level import Level  # noqa: E402
-from helpers import level_grids, level_reference  # noqa: E402
+from helpers import check_tune_times, level_grids, level_reference  # noqa: E402
 
 
 def level_check(lib, grids, glob, steps, shapes=None, graph=True, dt=None, persistent=False):
@@ -231,8 +231,11 @@ def test_level_tune_restores_state(cpulib):
     dt = min(oracle.compute_time_step("port", g, glob, 0.4) for g in grids)
     lv = Level([g.copy() for g in grids], glob, lib=cpulib)
     try:
+        import numpy as np
+        assert np.all(np.isnan(lv.tune_times()[0]))  # nothing timed yet
         choice = lv.tune(0, dt, repeats=1)
         assert all(0 <= c <= 2 for c in choice)
+        check_tune_times(lv, choice)
         lv.step(0, dt, 0.4)
         assert lv.status()[0] == 1
         lv.download()

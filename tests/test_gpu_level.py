@@ -7,7 +7,7 @@ import pytest
 import oracle
 from enzo_ppm import abi, problems
 from enzo_ppm.level import Level
-from helpers import fallback_case, field_report, level_grids, level_reference
+from helpers import check_tune_times, fallback_case, field_report, level_grids, level_reference
 
 pytestmark = pytest.mark.gpu
 
@@ -111,6 +111,7 @@ def test_level_tune(lib):
     try:
         choice = lv.tune(0, dt, repeats=2)
         assert all(0 <= c <= 2 for c in choice)
+        check_tune_times(lv, choice)
         lv.step(0, dt, 0.4)
         assert lv.status()[0] == 1
         lv.download()

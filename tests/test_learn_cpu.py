@@ -110,10 +110,15 @@ def test_learned_weights_rejected_when_not_finite(cpulib):
     from enzo_ppm import device
     dg = device.DeviceGrid(g, glob, lib=cpulib)
     try:
+        for k, bad in ((5, np.nan), (5, np.inf), (learn.OFF_GATE, np.nan),
+                       (learn.OFF_GATE, -np.inf)):
+            w = learn.init_weights(0)
+            w[k] = bad
+            with pytest.raises(RuntimeError):
+                dg.set_learned(w)
         w = learn.init_weights(0)
-        w[5] = np.nan
-        with pytest.raises(RuntimeError):
-            dg.set_learned(w)
+        w[learn.OFF_GATE] = np.inf    # the closed gate is accepted
+        dg.set_learned(w)
     finally:
         dg.close()
 
