@@ -144,6 +144,23 @@ def check_sweep_uses_network(lib, n=12, seed=13):
     check_learned_run(base, moved, one_step=True)
 
 
+def best_open_gate(w, table):
+    """w with the open (finite) gate of fit_gate's table that gave the
+    fewest mean updates."""
+    w = np.array(w, dtype=np.float32, copy=True)
+    w[learn.OFF_GATE] = min((c for c in table if np.isfinite(c)), key=lambda c: table[c][0])
+    return w
+
+
+def check_gate_choice(fitted, table, rel_tol=0.01):
+    """fit_gate's rule: the largest gate within rel_tol of the fewest mean
+    updates, the closed gate (+inf, the reference start) included."""
+    best = min(v[0] for v in table.values())
+    ok = [c for c, v in table.items() if v[0] <= best * (1.0 + rel_tol)]
+    assert fitted[learn.OFF_GATE] == np.float32(max(ok))  # the weights hold float32
+    assert table[float("inf")][1] == 0.0
+
+
 def check_sweep_network_is_kernels(lib, g, glob, w):
     """The sweep's learned start (the workgroup's evaluation of the network
     on the gated interfaces, from the device copy of the weights) is the
