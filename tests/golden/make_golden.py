@@ -38,6 +38,16 @@ CASES = {
     "hll_diffusion_8": (lambda: with_glob(problems.random_smooth([8, 9, 7], seed=28, ncolours=1,
                                                                 flags=(1, 1, 1)), RiemannSolver=1),
                         0.4, 2, None),
+    # PPMFlatteningParameter 2 (calcdiss.F:447-485) and 3 (:489-566) on
+    # strong shocks, where the three flattenings differ (the smooth fields
+    # above leave every flattening at zero)
+    "sod_flatten2_12": (lambda: with_glob(problems.sod_x(12), PPMFlatteningParameter=2),
+                        0.4, 4, None),
+    "sod_flatten3_12": (lambda: with_glob(problems.sod_x(12), PPMFlatteningParameter=3),
+                        0.4, 4, None),
+    "mach30_flatten3_10": (lambda: with_glob(problems.turbulent(10, seed=31, mach=30,
+                                                                sigma_lnrho=1.5),
+                                             PPMFlatteningParameter=3), 0.4, 3, None),
     # RiemannSolver HLLC (4): flux_hllc over whole slices
     "hllc_turbulent_10": (lambda: with_glob(problems.turbulent(10, seed=29), RiemannSolver=4),
                           0.4, 2, None),

@@ -79,6 +79,24 @@ FALLBACK_CASES = {
 }
 
 
+def evolved(case, steps, cfl=0.4):
+    """(grid, globals, run) of `case` after `steps` periodic oracle steps:
+    strong-shock states from smooth initial conditions, where the
+    flattening types differ."""
+    g, glob, run = case
+    oracle_steps("port", g, glob, steps, cfl, "periodic")
+    return g, glob, run
+
+
+def sod_level(flatten, sizes=(16, 20, 12)):
+    """A level of 3-D Sod tubes along x (shock, contact and rarefaction in
+    every slice's first line, where the flattening of each slice is taken;
+    diffusion on) with PPMFlatteningParameter `flatten`."""
+    from enzo_ppm import problems
+    base = [problems.sod_x(n) for n in sizes]
+    return [b[0] for b in base], dict(base[0][1], PPMFlatteningParameter=flatten)
+
+
 def fallback_case(name, ncolours=1):
     """(grid, globals, cycle) of a fallback case with grid.dt set; the first
     sweep of `cycle` runs along the streaming axis."""

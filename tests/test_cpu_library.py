@@ -14,7 +14,8 @@ import pytest
 import oracle
 from enzo_ppm import device, problems
 from enzo_ppm.grid import SubgridFluxes
-from helpers import FALLBACK_CASES, bit_equal, cpu_library, fallback_case, field_report, fill_bc
+from helpers import (FALLBACK_CASES, bit_equal, cpu_library, evolved, fallback_case, field_report,
+                     fill_bc)
 
 
 @pytest.fixture(scope="module")
@@ -61,10 +62,10 @@ CASES = {
     "longest_y": lambda: problems.random_smooth([3, 1025, 2], seed=11, ncolours=1),
     "one_cell": lambda: problems.random_smooth([1, 1, 1], seed=11, ncolours=1),
     "ragged": lambda: problems.random_smooth([5, 33, 2], seed=11, ncolours=1),
-    # flattening types 2 and 3 (calcdiss.F:447-566)
-    "flatten2_diffusion": lambda: problems.random_smooth([12, 10, 9], seed=9, ncolours=1,
-                                                         flags=(1, 2, 1)),
-    "flatten3": lambda: problems.random_smooth([12, 10, 9], seed=9, ncolours=1, flags=(1, 3, 0)),
+    # flattening types 2 and 3 (calcdiss.F:447-566) on strong shocks, where
+    # the three flattenings differ
+    "flatten2_sod_diffusion": lambda: evolved(problems.sod_x(12, flags=(1, 2, 1)), 3),
+    "flatten3_sod": lambda: evolved(problems.sod_x(12, flags=(1, 3, 0)), 3),
 }
 
 
@@ -144,7 +145,7 @@ def test_comoving_steps(cpulib, dual):
 
 # ---- batched AMR level (hipPPMLevel*) --------------------------------------
 from enzo_ppm.level import Level  # noqa: E402
-from helpers import check_tune_times, level_grids, level_reference  # noqa: E402
+from helpers import check_tune_times, level_grids, level_reference, sod_level  # noqa: E402
 
 
 def level_check(lib, grids, glob, steps, shapes=None, graph=True, dt=None, persistent=False):
@@ -186,6 +187,20 @@ def test_level_direct_launches(cpulib):
 def test_level_kinds(cpulib, kind):
     grids, glob = level_grids(kind, 3)
     level_check(cpulib, grids, glob, 2)
+
+
+@pytest.mark.parametrize("iflatten", [2, 3])
+def test_level_flattening_types(cpulib, iflatten):
+    """PPMFlatteningParameter 2 and 3 (calcdiss.F:447-566) on the batched
+    level: a level of Sod tubes (strong shocks in every slice's first line,
+    where each slice's flattening is taken; diffusion on), every grid
+    bitwise as the oracle's, whose result differs from flattening type 1's."""
+    grids, glob = sod_level(iflatten)
+    level_check(cpulib, grids, glob, 3)
+    a, b = [g.copy() for g in grids], [g.copy() for g in grids]
+    level_reference(a, glob, 3)
+    level_reference(b, dict(glob, PPMFlatteningParameter=1), 3)
+    assert any(not bit_equal(x, y) for ga, gb in zip(a, b) for x, y in zip(ga.fields, gb.fields))
 
 
 @pytest.mark.parametrize("kind", ["blast", "diffusion"])
@@ -537,3 +552,21 @@ def test_steps_lds_reads(cpulib, name, monkeypatch):
     assert run(cpulib, b, glob, 1) == (1, 0)
     assert all(r[1] for r in field_report(b, a))
     assert cpulib.hipcpu_lds_violations() == before, "LDS read before write"
+
+
+GOLD_FLATTEN = sorted(__import__("glob").glob(__import__("os").path.join(
+    __import__("os").path.dirname(__import__("os").path.abspath(__file__)), "golden",
+    "step_*flatten*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLD_FLATTEN, ids=[p.rsplit("/", 1)[-1] for p in GOLD_FLATTEN])
+def test_golden_flattening(cpulib, path):
+    """The reference's own outputs with flattening types 2 and 3 on strong
+    shocks (tests/golden), replayed bitwise by the library's source."""
+    import test_golden as tg
+    z, g, glob = tg.load_case(path)
+
+    def step(g, s, fl):
+        assert device.solve_ppm_de(g, glob, cycle=s, fluxes=fl, lib=cpulib) == 1
+
+    tg.replay(z, g, glob, step, path.rsplit("/", 1)[-1])

@@ -94,3 +94,27 @@ def test_oracle_matches_golden_fallback(path):
     st, fb = oracle.solve_ppm_de("port", g, glob, cycle=int(z["cycle"]))
     assert (st, fb) == (int(z["status"]), int(z["fallbacks"]))
     fallback_differences(g, [z[f"out_f{k}"] for k in range(g.nfields)], fb)
+
+
+FLATTEN_FILES = [p for p in STEP_FILES if "flatten" in os.path.basename(p)]
+
+
+@pytest.mark.parametrize("path", FLATTEN_FILES, ids=[os.path.basename(p) for p in FLATTEN_FILES])
+def test_flattening_fixtures_exercise_their_type(path):
+    """The flattening fixtures pin their type: the same steps with
+    PPMFlatteningParameter 1 (and with the other of 2 and 3) end elsewhere,
+    so a device or oracle that ignored the type would fail them."""
+    z, g0, glob = load_case(path)
+    own = glob["PPMFlatteningParameter"]
+    assert own in (2, 3)
+    last = int(z["nsteps"]) - 1
+    for other in sorted({1, 2, 3} - {own}):
+        g = g0.copy()
+        gl = dict(glob, PPMFlatteningParameter=other)
+        for s in range(int(z["nsteps"])):
+            g.fill_periodic()
+            g.dt = oracle.compute_time_step("port", g, gl, float(z["cfl"]))
+            assert oracle.solve_ppm_de("port", g, gl, cycle=s)[0] == 1
+        differ = sum(int(np.count_nonzero(g.fields[k] != z[f"out{last}_f{k}"]))
+                     for k in range(g.nfields))
+        assert differ > 100, (os.path.basename(path), other, differ)

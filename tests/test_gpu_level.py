@@ -7,7 +7,8 @@ import pytest
 import oracle
 from enzo_ppm import abi, problems
 from enzo_ppm.level import Level
-from helpers import check_tune_times, fallback_case, field_report, level_grids, level_reference
+from helpers import (bit_equal, check_tune_times, fallback_case, field_report, level_grids,
+                     level_reference, sod_level)
 
 pytestmark = pytest.mark.gpu
 
@@ -48,6 +49,20 @@ def level_check(lib, grids, glob, steps, shapes=None, graph=True, persistent=Fal
 def test_level_blast_shapes(lib, shape):
     grids, glob = level_grids("blast", 4)
     level_check(lib, grids, glob, 3, shapes=shape)
+
+
+@pytest.mark.parametrize("iflatten", [2, 3])
+def test_level_flattening_types(lib, iflatten):
+    """PPMFlatteningParameter 2 and 3 (calcdiss.F:447-566) on the batched
+    level: a level of Sod tubes (strong shocks in every slice's first line,
+    where each slice's flattening is taken; diffusion on), every grid
+    bitwise as the oracle's, whose result differs from flattening type 1's."""
+    grids, glob = sod_level(iflatten)
+    level_check(lib, grids, glob, 3)
+    a, b = [g.copy() for g in grids], [g.copy() for g in grids]
+    level_reference(a, glob, 3)
+    level_reference(b, dict(glob, PPMFlatteningParameter=1), 3)
+    assert any(not bit_equal(x, y) for ga, gb in zip(a, b) for x, y in zip(ga.fields, gb.fields))
 
 
 def test_level_direct_launches(lib):

@@ -15,7 +15,7 @@ import pytest
 import oracle
 from enzo_ppm import abi, device, problems
 from enzo_ppm.grid import SubgridFluxes
-from helpers import bit_equal, field_report, fill_bc, max_rel, oracle_steps
+from helpers import bit_equal, evolved, field_report, fill_bc, max_rel, oracle_steps
 
 pytestmark = pytest.mark.gpu
 
@@ -63,18 +63,19 @@ CASES = {
     "ragged": lambda: problems.random_smooth([5, 33, 2], seed=11, ncolours=1),
     "one_by_long": lambda: problems.random_smooth([1, 200, 1], seed=11, ncolours=1),
     # PPMFlatteningParameter 2 (Lagrangean, calcdiss.F:447-485) and 3
-    # (multidimensional, :489-566), the first with diffusion
-    "flatten2_diffusion": lambda: problems.random_smooth([12, 10, 9], seed=9, ncolours=1,
-                                                         flags=(1, 2, 1)),
-    "flatten3": lambda: problems.random_smooth([12, 10, 9], seed=9, ncolours=1, flags=(1, 3, 0)),
-    "flatten3_blast": lambda: _first_grid(problems.amr_level(1, seed=44, nmin=16, nmax=20,
-                                                             ncolours=1, flags=(1, 3, 0))),
+    # (multidimensional, :489-566) on strong shocks, where the three
+    # flattenings differ (tests/test_golden.py checks the same on the
+    # reference's fixtures): a Sod tube with diffusion, a Mach-30 box
+    "flatten2_sod_diffusion": lambda: evolved(problems.sod_x(16, flags=(1, 2, 1)), 3),
+    "flatten3_sod": lambda: evolved(problems.sod_x(16, flags=(1, 3, 0)), 3),
+    "flatten3_mach30": lambda: evolved(_flatten(problems.turbulent(16, seed=4, mach=30,
+                                                                   sigma_lnrho=1.5), 3), 2),
 }
 
 
-def _first_grid(level):
-    grids, glob, run = level
-    return grids[0], glob, run
+def _flatten(case, iflatten):
+    g, glob, run = case
+    return g, dict(glob, PPMFlatteningParameter=iflatten), run
 
 
 @pytest.mark.parametrize("name", sorted(CASES))

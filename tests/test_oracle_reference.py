@@ -24,13 +24,21 @@ CASES = {
     "gravity": (lambda: problems.random_smooth([10, 12, 8], seed=2, ncolours=1, gravity=True), 0.4, "periodic", 3),
     "no_dual_energy": (lambda: problems.random_smooth([10, 12, 8], seed=3, dual_energy=False), 0.4, "periodic", 3),
     "diffusion_3d": (lambda: problems.random_smooth([10, 12, 8], seed=4, ncolours=1, flags=(1, 1, 1)), 0.4, "periodic", 3),
-    "flatten2": (lambda: problems.random_smooth([12, 10, 9], seed=9, ncolours=1, flags=(1, 2, 0)), 0.4, "periodic", 2),
-    "flatten3": (lambda: problems.random_smooth([12, 10, 9], seed=9, ncolours=1, flags=(1, 3, 0)), 0.4, "periodic", 2),
+    # flattening types 2 and 3 on strong shocks (where the three differ)
+    "flatten2_sod": (lambda: problems.sod_x(16, flags=(1, 2, 1)), 0.4, "periodic", 4),
+    "flatten3_sod": (lambda: problems.sod_x(16, flags=(1, 3, 0)), 0.4, "periodic", 4),
+    "flatten3_mach30": (lambda: flattening(problems.turbulent(16, seed=4, mach=30, sigma_lnrho=1.5), 3),
+                        0.4, "periodic", 3),
     "turbulent": (lambda: problems.turbulent(24, seed=3), 0.4, "periodic", 3),
     "turbulent_mach30": (lambda: problems.turbulent(20, seed=4, mach=30, sigma_lnrho=1.5), 0.4, "periodic", 3),
     "sod1d_gamma14": (lambda: problems.sod_x(40, rank=1, gamma=1.4, dual_energy=False, flags=(0, 0, 0)), 0.9, "outflow", 8),
     "sod2d": (lambda: problems.sod_x(16, rank=2, gamma=1.4, flags=(1, 1, 1)), 0.9, "periodic", 4),
 }
+
+
+def flattening(case, iflatten):
+    g, glob, run = case
+    return g, dict(glob, PPMFlatteningParameter=iflatten), run
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
