@@ -240,6 +240,33 @@ def level_bench(lib, ngrids, steps, cfl=0.4, weights=None):
                 raise RuntimeError(f"learned level step status {st}")
             batched_learned = cells * steps / (t4 - t3)
             lv.set_learned(None)
+        # what the learned-start path itself costs: a network whose output is
+        # zero (every start the acoustic one, so the Newton work is the plain
+        # kernels') evaluated wherever the gate passes, against the plain
+        # kernels, at several gates
+        from enzo_ppm import learn
+        wz = np.zeros(learn.N_FLOATS, np.float32)
+        wz[learn.OFF_INV_STD:learn.OFF_GATE] = 1.0
+        cost = {}
+        c = 3 + 2 * steps
+        for name, gate in (("plain", None), ("gate_1e30", 1e30), ("gate_1", 1.0),
+                           ("gate_0.1", 0.1), ("gate_0", 0.0)):
+            if gate is None:
+                lv.set_learned(None)
+            else:
+                wz[learn.OFF_GATE] = gate
+                lv.set_learned(wz)
+            lv.step(c, dt, cfl)
+            c += 1
+            lv.status()
+            t5 = time.perf_counter()
+            for _ in range(steps):
+                lv.step(c, dt, cfl)
+                c += 1
+            if lv.status()[0] != 1:
+                raise RuntimeError("learned-path cost step failed")
+            cost[name] = (time.perf_counter() - t5) * 1e3 / steps
+        lv.set_learned(None)
     finally:
         lv.close()
         for d in dgs:
@@ -256,7 +283,11 @@ def level_bench(lib, ngrids, steps, cfl=0.4, weights=None):
             **({"batched_learned": {"value": batched_learned,
                                     "ms_per_step": cells / batched_learned * 1e3,
                                     "speedup_vs_plain_per_grid": batched_learned / plain}}
-               if batched_learned else {})}
+               if batched_learned else {}),
+            "learned_path_cost_ms_per_step": {
+                **cost, "note": "zero-output network (every start the acoustic one) evaluated "
+                                "where the gate passes (1e30: nowhere, 0: everywhere above "
+                                "the floor) against the plain kernels"}}
 
 
 def time_grid(lib, n, steps, warmup, weights=None, solver=None):
