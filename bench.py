@@ -85,8 +85,8 @@ def copy_calibration():
 
 def host_cpu_share():
     """(threads, nproc, affinity): the CPUs this process may run on
-    (sched_getaffinity), at most 16, the host CPUs a one-GPU box allots to
-    its GPU; nproc is os.cpu_count() for the record."""
+    (sched_getaffinity), at most 16 (the host CPUs that go with one GPU
+    of the machine); nproc is os.cpu_count() for the record."""
     nproc = os.cpu_count() or 1
     try:
         aff = len(os.sched_getaffinity(0))
@@ -547,7 +547,7 @@ def main():
         return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+                                   "(multi-GPU sharding is designed, not built)"}))
         sys.exit(2)
     from enzo_ppm import abi
     lib = abi.load_library()

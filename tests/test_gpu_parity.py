@@ -3,9 +3,9 @@
 The oracle's C restatement is itself pinned bit-for-bit to the reference
 Fortran (tests/test_oracle_reference.py and the golden fixtures); when the
 reference build travels with the tree (oracle/_ref), it is compared directly
-as well.  Bar: bitwise equality of every baryon field and captured face flux;
-the north-star tolerances (max relative 1e-11 after one step) are asserted
-as the hard limit and bitwise equality is reported.
+as well.  Bar: bitwise equality of every baryon field and captured face flux
+(a maximum relative difference of 1e-11 after one step is also asserted,
+and reported, so that a failure shows its size).
 """
 import ctypes as C
 
