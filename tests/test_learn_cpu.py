@@ -144,3 +144,37 @@ def test_level_learned(cpulib, trained):
     for a, b in zip(lv.grids, ref):
         for x, y in zip(a.fields, b.fields):
             assert lc.l1_rel(x, y) <= 1e-9
+
+
+def test_learned_work_item_order_and_lds_reads(cpulib, trained, monkeypatch):
+    """The learned start's in-tile list (slots taken in work-item order by
+    an LDS counter), the workgroup's network evaluation and the acoustic
+    quantities kept in dead LDS arrays: with the work-items of every
+    barrier interval run forward, in reverse and in two shuffled orders, the
+    step is bitwise the same, and no LDS cell is read before the workgroup
+    wrote it (every __shared__ array starts as NaN in the CPU build)."""
+    import ctypes as C
+    from enzo_ppm import device
+    cpulib.hipcpu_lds_violations.restype = C.c_ulonglong
+    monkeypatch.setenv("HIPCPU_LDS_CHECK", "1")
+    monkeypatch.setenv("HIPCPU_LDS_CONTINUE", "1")
+    before = cpulib.hipcpu_lds_violations()
+    g0, glob, _ = problems.turbulent(12, seed=21)
+    out = {}
+    for order in ("fwd", "rev", "shuf:1", "shuf:5"):
+        monkeypatch.setenv("HIPCPU_ORDER", order)
+        g = g0.copy()
+        dg = device.DeviceGrid(g, glob, lib=cpulib)
+        try:
+            dg.set_learned(trained[0])
+            for cycle in range(2):
+                dg.periodic_boundary()
+                dg.pressure_and_dt(0.4)
+                dg.step(cycle)
+            assert dg.download() == 1
+        finally:
+            dg.close()
+        out[order] = g
+    assert cpulib.hipcpu_lds_violations() == before, "LDS read before write"
+    for order, g in out.items():
+        assert all(np.array_equal(a, b) for a, b in zip(g.fields, out["fwd"].fields)), order
