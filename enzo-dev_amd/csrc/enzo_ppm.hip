@@ -21,6 +21,7 @@
 #include "../../include/enzo_ppm.h"
 #include "ppm_device.hpp"
 #include "ppm_geometry.hpp"
+#include "ppm_stage.hpp"
 #include "ppm_sweep.hpp"
 
 using namespace ppm;
@@ -253,6 +254,13 @@ struct hipPPMData {
   int profile = 0;
   std::vector<hipEvent_t> ev;  // pairs
   size_t ev_used = 0;
+  // pinned staging ring of the host-array transfers (stage_upload /
+  // stage_download), allocated on first use; stage_busy[b]: a copy into or
+  // out of buffer b was queued and stage_ev[b] marks its end
+  void *stage[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t stage_ev[3] = {nullptr, nullptr, nullptr};
+  bool stage_busy[3] = {false, false, false};
+  int stage_next = 0;
 };
 
 namespace {
@@ -334,6 +342,118 @@ void free_data(hipPPMData *D) {
                   D->faces[2], D->face_out[0], D->face_out[1], D->face_out[2]};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  for (int b = 0; b < 3; b++) {
+    if (D->stage_ev[b]) (void)hipEventDestroy(D->stage_ev[b]);
+    if (D->stage[b]) (void)hipHostFree(D->stage[b]);
+    D->stage_ev[b] = nullptr;
+    D->stage[b] = nullptr;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Host-array transfers.  hipMemcpy from the caller's pageable arrays moves
+// ~28 GB/s to the device and ~18 GB/s back (the runtime stages them
+// itself); here a field goes in chunks through a ring of pinned buffers,
+// the host copy of each chunk (CopyPool, several threads) overlapping the
+// DMA of the previous one, which runs at ~57 GB/s.  Smaller transfers, or a
+// host that cannot pin the ring, take the direct path.
+constexpr int kStageBufs = 3;
+constexpr size_t kStageBytes = size_t(32) << 20;  // per pinned buffer
+constexpr size_t kStageMin = size_t(8) << 20;     // direct path below this total
+
+struct HostXfer {
+  char *dst;
+  const char *src;
+  size_t bytes;
+};
+
+bool stage_ready(hipPPMData *D) {
+  if (D->stage[kStageBufs - 1]) return true;
+  for (int b = 0; b < kStageBufs; b++) {
+    if (!D->stage[b] && hipHostMalloc(&D->stage[b], kStageBytes, hipHostMallocDefault) != hipSuccess) {
+      D->stage[b] = nullptr;
+      return false;
+    }
+    if (!D->stage_ev[b] && hipEventCreateWithFlags(&D->stage_ev[b], hipEventDisableTiming) != hipSuccess) {
+      D->stage_ev[b] = nullptr;
+      (void)hipHostFree(D->stage[b]);
+      D->stage[b] = nullptr;
+      return false;
+    }
+  }
+  return true;
+}
+
+size_t xfer_total(const std::vector<HostXfer> &xs) {
+  size_t t = 0;
+  for (const HostXfer &x : xs) t += x.bytes;
+  return t;
+}
+
+// host -> device on D->stream; the caller's arrays may be reused on return
+// (the DMA reads only the pinned ring)
+int stage_upload(hipPPMData *D, const std::vector<HostXfer> &xs) {
+  if (xfer_total(xs) < kStageMin || !stage_ready(D)) {
+    for (const HostXfer &x : xs)
+      HIPCHK(hipMemcpyAsync(x.dst, x.src, x.bytes, hipMemcpyHostToDevice, D->stream));
+    return ENZO_PPM_SUCCESS;
+  }
+  int b = D->stage_next;
+  for (const HostXfer &x : xs)
+    for (size_t off = 0; off < x.bytes; off += kStageBytes) {
+      const size_t len = std::min(kStageBytes, x.bytes - off);
+      if (D->stage_busy[b]) HIPCHK(hipEventSynchronize(D->stage_ev[b]));
+      CopyPool::get().copy(D->stage[b], x.src + off, len);
+      HIPCHK(hipMemcpyAsync(x.dst + off, D->stage[b], len, hipMemcpyHostToDevice, D->stream));
+      HIPCHK(hipEventRecord(D->stage_ev[b], D->stream));
+      D->stage_busy[b] = true;
+      b = (b + 1) % kStageBufs;
+    }
+  D->stage_next = b;
+  return ENZO_PPM_SUCCESS;
+}
+
+// device -> host after the work queued on D->stream; complete on return
+int stage_download(hipPPMData *D, const std::vector<HostXfer> &xs) {
+  if (xfer_total(xs) < kStageMin || !stage_ready(D)) {
+    for (const HostXfer &x : xs)
+      HIPCHK(hipMemcpyAsync(x.dst, x.src, x.bytes, hipMemcpyDeviceToHost, D->stream));
+    HIPCHK(hipStreamSynchronize(D->stream));
+    return ENZO_PPM_SUCCESS;
+  }
+  char *pend_dst[kStageBufs] = {};
+  size_t pend_len[kStageBufs] = {};
+  auto drain = [&](int c) -> int {  // buffer c's chunk to the caller's array
+    if (!pend_dst[c]) return ENZO_PPM_SUCCESS;
+    HIPCHK(hipEventSynchronize(D->stage_ev[c]));
+    CopyPool::get().copy(pend_dst[c], D->stage[c], pend_len[c]);
+    pend_dst[c] = nullptr;
+    D->stage_busy[c] = false;
+    return ENZO_PPM_SUCCESS;
+  };
+  int b = D->stage_next;
+  for (const HostXfer &x : xs)
+    for (size_t off = 0; off < x.bytes; off += kStageBytes) {
+      const size_t len = std::min(kStageBytes, x.bytes - off);
+      if (pend_dst[b]) {
+        const int st = drain(b);
+        if (st != ENZO_PPM_SUCCESS) return st;
+      } else if (D->stage_busy[b]) {
+        HIPCHK(hipEventSynchronize(D->stage_ev[b]));
+      }
+      HIPCHK(hipMemcpyAsync(D->stage[b], x.src + off, len, hipMemcpyDeviceToHost, D->stream));
+      HIPCHK(hipEventRecord(D->stage_ev[b], D->stream));
+      D->stage_busy[b] = true;
+      pend_dst[b] = x.dst + off;
+      pend_len[b] = len;
+      b = (b + 1) % kStageBufs;
+    }
+  for (int i = 0; i < kStageBufs; i++) {
+    const int st = drain((b + i) % kStageBufs);
+    if (st != ENZO_PPM_SUCCESS) return st;
+  }
+  D->stage_next = b;
+  return ENZO_PPM_SUCCESS;
 }
 
 int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const double *dtp,
@@ -755,11 +875,12 @@ int hipPPMSetBaryon(hipPPMData *D, const hipPPMParameter *P, enzo_float *BaryonF
   std::vector<long long> f;
   slot_fields(P, f);
   const size_t fb = sizeof(double) * (size_t)D->cells;
+  std::vector<HostXfer> up;
   for (int s = 0; s < D->nslot; s++) {
     double *dst = D->A + (size_t)s * D->cells;
     if (f[s] >= 0) {
       if (!BaryonField[f[s]]) return ENZO_PPM_ERR_ARGUMENT;
-      HIPCHK(hipMemcpyAsync(dst, BaryonField[f[s]], fb, hipMemcpyHostToDevice, D->stream));
+      up.push_back({(char *)dst, (const char *)BaryonField[f[s]], fb});
     } else {
       HIPCHK(hipMemsetAsync(dst, 0, fb, D->stream));
     }
@@ -768,15 +889,15 @@ int hipPPMSetBaryon(hipPPMData *D, const hipPPMParameter *P, enzo_float *BaryonF
     if (!AccelerationField || !D->accel) return ENZO_PPM_ERR_ARGUMENT;
     for (int d = 0; d < 3; d++) {
       if (d < P->GridRank && AccelerationField[d])
-        HIPCHK(hipMemcpyAsync(D->accel + (size_t)d * D->cells, AccelerationField[d], fb,
-                              hipMemcpyHostToDevice, D->stream));
+        up.push_back({(char *)(D->accel + (size_t)d * D->cells), (const char *)AccelerationField[d], fb});
       else
         HIPCHK(hipMemsetAsync(D->accel + (size_t)d * D->cells, 0, fb, D->stream));
     }
   }
-  if (Pressure) {
-    HIPCHK(hipMemcpyAsync(D->pres, Pressure, fb, hipMemcpyHostToDevice, D->stream));
-  } else {
+  if (Pressure) up.push_back({(char *)D->pres, (const char *)Pressure, fb});
+  const int sup = stage_upload(D, up);
+  if (sup != ENZO_PPM_SUCCESS) return sup;
+  if (!Pressure) {
     int st = launch_pressure(D, P, 0.0, 0, 1.0);
     if (st != ENZO_PPM_SUCCESS) return st;
   }
@@ -855,12 +976,13 @@ int hipPPMGetBaryon(hipPPMData *D, const hipPPMParameter *P, enzo_float *BaryonF
   std::vector<long long> f;
   slot_fields(P, f);
   const size_t fb = sizeof(double) * (size_t)D->cells;
+  std::vector<HostXfer> down;
   for (int s = 0; s < D->nslot; s++) {
     if (f[s] < 0 || !BaryonField[f[s]]) continue;
-    HIPCHK(hipMemcpyAsync(BaryonField[f[s]], D->A + (size_t)s * D->cells, fb,
-                          hipMemcpyDeviceToHost, D->stream));
+    down.push_back({(char *)BaryonField[f[s]], (const char *)(D->A + (size_t)s * D->cells), fb});
   }
-  HIPCHK(hipStreamSynchronize(D->stream));
+  const int sdn = stage_download(D, down);
+  if (sdn != ENZO_PPM_SUCCESS) return sdn;
   unsigned long long c[4];
   HIPCHK(hipMemcpy(c, D->counters, sizeof(c), hipMemcpyDeviceToHost));
   D->fallback_total = (long long)c[0];

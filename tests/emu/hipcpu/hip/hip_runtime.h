@@ -364,6 +364,13 @@ inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind
   return hipMemcpy(d, s, n, k);
 }
 inline hipError_t hipMemset(void *d, int v, size_t n) { memset(d, v, n); return hipSuccess; }
+// pinned host memory: ordinary memory on the CPU
+constexpr unsigned hipHostMallocDefault = 0;
+inline hipError_t hipHostMalloc(void **p, size_t n, unsigned = 0) {
+  *p = malloc(n ? n : 8);
+  return *p ? hipSuccess : hipErrorMemoryAllocation;
+}
+inline hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
 inline hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t = nullptr) { return hipMemset(d, v, n); }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
@@ -385,6 +392,8 @@ inline hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int) { *v 
 
 struct hipcpu_event { std::chrono::steady_clock::time_point t; };
 inline hipError_t hipEventCreate(hipEvent_t *e) { *e = new hipcpu_event(); return hipSuccess; }
+constexpr unsigned hipEventDisableTiming = 2;
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
 inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t = nullptr) {
   e->t = std::chrono::steady_clock::now();

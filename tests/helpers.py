@@ -97,6 +97,38 @@ def sod_level(flatten, sizes=(16, 20, 12)):
     return [b[0] for b in base], dict(base[0][1], PPMFlatteningParameter=flatten)
 
 
+def check_host_round_trip(lib, n, seed=3):
+    """hipPPMSetBaryon then hipPPMGetBaryon of an n^3 grid with gravity and
+    colours (the pinned staging ring: fields of more than one 32 MiB chunk
+    when n >= 160, more fields than ring buffers): every array comes back
+    bit for bit, and an upload after a download (the ring's buffers reused
+    in the other direction) too."""
+    import numpy as np
+    from enzo_ppm import device, problems
+    g, glob, _ = problems.random_smooth([n, n, n], seed=seed, ncolours=2, gravity=True)
+    rng = np.random.default_rng(seed)
+    for f in g.fields:
+        f[...] = rng.standard_normal(f.shape)
+    want = [f.copy() for f in g.fields]
+    dg = device.DeviceGrid(g, glob, lib=lib)
+    try:
+        for f in g.fields:
+            f[...] = 0.0
+        assert dg.download() == 1
+        for a, b in zip(g.fields, want):
+            assert bit_equal(a, b)
+        for f in g.fields:
+            f *= 2.0
+        dg.upload()
+        for f in g.fields:
+            f[...] = 0.0
+        assert dg.download() == 1
+        for a, b in zip(g.fields, want):
+            assert bit_equal(a, 2.0 * b)
+    finally:
+        dg.close()
+
+
 def fallback_case(name, ncolours=1):
     """(grid, globals, cycle) of a fallback case with grid.dt set; the first
     sweep of `cycle` runs along the streaming axis."""

@@ -570,3 +570,12 @@ def test_golden_flattening(cpulib, path):
         assert device.solve_ppm_de(g, glob, cycle=s, fluxes=fl, lib=cpulib) == 1
 
     tg.replay(z, g, glob, step, path.rsplit("/", 1)[-1])
+
+
+@pytest.mark.parametrize("n", [24, 72])
+def test_host_round_trip(cpulib, n):
+    """The host-array transfers: the direct path (24^3) and the pinned
+    staging ring (72^3: 13 fields of 3.3 MB, more than the ring's three
+    buffers)."""
+    from helpers import check_host_round_trip
+    check_host_round_trip(cpulib, n)

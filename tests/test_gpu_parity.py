@@ -266,3 +266,12 @@ def test_device_matches_golden_fallback(lib, path):
             lib.hipPPMDestroy(h)
     assert (st, fb) == (int(z["status"]), int(z["fallbacks"]))
     fallback_differences(g, [z[f"out_f{k}"] for k in range(g.nfields)], fb)
+
+
+@pytest.mark.parametrize("n", [24, 72, 170])
+def test_host_round_trip(lib, n):
+    """The host-array transfers: the direct path (24^3), the pinned staging
+    ring with one chunk per field (72^3) and with two (170^3, 45 MB per
+    field): every array back bit for bit, in both directions."""
+    from helpers import check_host_round_trip
+    check_host_round_trip(lib, n)
