@@ -146,6 +146,23 @@ struct SweepParams {
   unsigned harvest_keep;
 };
 
+// Kernels that read their SweepParams from memory (the batched level's)
+// see generic pointers there, which the compiler addresses with flat
+// instructions (counted with the LDS operations too, so that LDS waits also
+// wait for them); each global access below casts its pointer through the
+// global address space and accesses memory through the global pointer, so
+// that it compiles to a global instruction as for the per-grid kernel's
+// by-value parameters.  Identity outside the device pass.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PPM_GLOBAL __attribute__((address_space(1)))
+#else
+#define PPM_GLOBAL
+#endif
+template <class T>
+PPM_HD PPM_GLOBAL T *as_global(T *p) {
+  return (PPM_GLOBAL T *)p;
+}
+
 // Learned start in the sweep tile: the network's inputs of the interfaces
 // that pass the gate are listed in LDS, kNetStride floats each (their
 // corrections then overwrite the first), and the workgroup evaluates the
@@ -292,9 +309,10 @@ PPM_HD PPM_LDS_REF LDSH(double *sm, int arr, int c) {
   return PPM_LDS_AT(sm[arr * Cfg<TP, TS, UNI>::LW + c]);
 }
 
-PPM_HD double dtval(const SweepParams &P) { return P.dtp ? *P.dtp : P.dt; }
+PPM_HD double dtval(const SweepParams &P) { return P.dtp ? *as_global(P.dtp) : P.dt; }
 
-PPM_HD double tabv(const double *tab, int N, int t, int cell) {
+template <class Ptr>
+PPM_HD double tabv(Ptr tab, int N, int t, int cell) {
   PPM_ASSERT(cell >= 1 && cell <= N && t >= 0 && t < NTAB);
   return tab[t * N + (cell - 1)];
 }
@@ -304,13 +322,13 @@ PPM_HD double tabv(const double *tab, int N, int t, int cell) {
 template <int UNI>
 PPM_HD double TS_(const SweepParams &P, int t, int cell) {
   if (UNI) return P.kt[t];
-  return tabv(P.tab, P.N, t, cell);
+  return tabv(as_global(P.tab), P.N, t, cell);
 }
 PPM_HD double TR_(const SweepParams &P, int t, int cell, int n) {
-  return P.uni[1] ? P.kt_row[t] : tabv(P.tab_row, n, t, cell);
+  return P.uni[1] ? P.kt_row[t] : tabv(as_global(P.tab_row), n, t, cell);
 }
 PPM_HD double TO_(const SweepParams &P, int t, int cell, int n) {
-  return P.uni[2] ? P.kt_out[t] : tabv(P.tab_out, n, t, cell);
+  return P.uni[2] ? P.kt_out[t] : tabv(as_global(P.tab_out), n, t, cell);
 }
 // whether x is a power of two (then 1/x is exact)
 inline bool is_pow2(double x) {
@@ -336,7 +354,7 @@ PPM_HD double qc_uniform(const SweepParams &P, double dt) {
 template <int UNI>
 PPM_HD double qc_of(const SweepParams &P, const Tile &T, int cell) {
   if (UNI) return T.qcu;
-  return P.qc[cell - 1];
+  return as_global(P.qc)[cell - 1];
 }
 
 // grid strides of the axes of a sweep (x fastest); after dim, ax_t1, ax_t2
@@ -402,8 +420,8 @@ PPM_HD Tile make_tile(const SweepParams &P, int block) {
   T.t2 = P.t2_0 + r / P.nblk_t1;
   T.t1_0 = tb * TP;
   T.np = P.n_t1 - T.t1_0 < TP ? P.n_t1 - T.t1_0 : TP;
-  T.o = P.seg_o[seg];
-  T.w = P.seg_w[seg];
+  T.o = as_global(P.seg_o)[seg];
+  T.w = as_global(P.seg_w)[seg];
   tile_ranges(P, T);
   T.dt = dtval(P);
   T.qcu = qc_uniform(P, T.dt);
@@ -468,7 +486,7 @@ PPM_HD void ph_load(const SweepParams &P, const Tile &T, double *sm, int tid) {
   per_cell_coalesced<TP, TS>(P, T, tid, nc, [&](int p, int c) {
     int cell = T.L0 + c;
     long long g = gidx(P, T.t1_0 + p, T.t2, cell);
-    const double *A = P.A;
+    const auto A = as_global(P.A);
     LDS<TP, TS, UNI>(sm, A_D, p, c) = A[S_D * P.cells + g];
     LDS<TP, TS, UNI>(sm, A_E, p, c) = A[S_E * P.cells + g];
     const double uu = A[(S_VX + P.dim) * P.cells + g];
@@ -477,8 +495,8 @@ PPM_HD void ph_load(const SweepParams &P, const Tile &T, double *sm, int tid) {
     LDS<TP, TS, UNI>(sm, A_V, p, c) = vax < P.rank ? A[(S_VX + vax) * P.cells + g] : 0.0;
     LDS<TP, TS, UNI>(sm, A_W, p, c) = wax < P.rank ? A[(S_VX + wax) * P.cells + g] : 0.0;
     if (P.idual) LDS<TP, TS, UNI>(sm, A_GE, p, c) = A[S_GE * P.cells + g];
-    LDS<TP, TS, UNI>(sm, A_P, p, c) = P.pres[g];
-    if (P.gravity) LDS<TP, TS, UNI>(sm, A_GR, p, c) = P.gr[g];
+    LDS<TP, TS, UNI>(sm, A_P, p, c) = as_global(P.pres)[g];
+    if (P.gravity) LDS<TP, TS, UNI>(sm, A_GR, p, c) = as_global(P.gr)[g];
   });
 }
 
@@ -570,7 +588,7 @@ PPM_HD void ph_slope(const SweepParams &P, const Tile &T, double *sm, int tid, i
 template <int TP, int TS, int UNI>
 PPM_HD double flat_of(const SweepParams &P, const Tile &T, int p, int cell) {
   int outer = coord_of(P, (P.dim + 2) % 3, T.t1_0 + p, T.t2);
-  return P.flat[(long long)outer * P.N + (cell - 1)];
+  return as_global(P.flat)[(long long)outer * P.N + (cell - 1)];
 }
 
 // Phase: interface values, steepening, monotonisation, flattening, CHECK_LR
@@ -836,7 +854,7 @@ PPM_HD void harvest_record(const SweepParams &P, int t1, int t2, int i, const do
   if (P.harvest_keep > 1 && (h % P.harvest_keep) != 0) return;
   const unsigned long long k = PPM_ATOMIC_ADD_U64(P.harvest_n, 1);
   if ((long long)k >= P.harvest_cap) return;
-  for (int m = 0; m < kHarvestDoubles; m++) P.harvest[k * kHarvestDoubles + m] = v[m];
+  for (int m = 0; m < kHarvestDoubles; m++) as_global(P.harvest)[k * kHarvestDoubles + m] = v[m];
 }
 
 // diffusion coefficient at interface i (calcdiss.F:207-236, 377-404);
@@ -859,14 +877,14 @@ PPM_HD double diffcoef1(const SweepParams &P, const Tile &T, double *sm, int p, 
     return ((long long)c[2] * P.GD[1] + c[1]) * P.GD[0] + c[0];
   };
   if (jdim > 1 && 1 < j && j < jdim) {  // nyz > 1 .and. lj1 (dimy > 1 .and. ljy1)
-    const double *v = P.A + (long long)(S_VX + rax) * P.cells;
+    const auto v = as_global(P.A) + (long long)(S_VX + rax) * P.cells;
     double vdiff1 = (v[at(row - 1, outer, i)] + v[at(row - 1, outer, i - 1)]) -
                     (v[at(row + 1, outer, i)] + v[at(row + 1, outer, i - 1)]);
     dc = dc + (TS_<UNI>(P, T_DIFFX, i) / TR_(P, T_DDEN, j, jdim)) * vdiff1;
   }
   if (P.nact_out > 1 && 1 < k && k < kdim) {  // nzz > 1 .and. lk1 (dimz > 1 .and. lkz1)
-    const double *wo = P.A + (long long)(S_VX + oax) * P.cells;
-    const double *wn = P.B + (long long)(S_VX + oax) * P.cells;
+    const auto wo = as_global(P.A) + (long long)(S_VX + oax) * P.cells;
+    const auto wn = as_global(P.B) + (long long)(S_VX + oax) * P.cells;
     double wdiff1 = (wn[at(row, outer - 1, i)] + wn[at(row, outer - 1, i - 1)]) -
                     (wo[at(row, outer + 1, i)] + wo[at(row, outer + 1, i - 1)]);
     dc = dc + (TS_<UNI>(P, T_DIFFX, i) / TO_(P, T_DDEN, k, kdim)) * wdiff1;
@@ -1517,12 +1535,13 @@ PPM_HD int ph_pgas_write(const SweepParams &P, const Tile &T, double *sm, int ti
       e = e_new;
     }
     long long g = gidx(P, T.t1_0 + p, T.t2, cell);
-    double *B = P.B;
+    const auto B = as_global(P.B);
+    const auto A = as_global(P.A);
     B[S_D * P.cells + g] = d;
     B[S_E * P.cells + g] = e;
     B[(S_VX + P.dim) * P.cells + g] = u;
-    B[(S_VX + vax) * P.cells + g] = vax < P.rank ? v : P.A[(S_VX + vax) * P.cells + g];
-    B[(S_VX + wax) * P.cells + g] = wax < P.rank ? w : P.A[(S_VX + wax) * P.cells + g];
+    B[(S_VX + vax) * P.cells + g] = vax < P.rank ? v : A[(S_VX + vax) * P.cells + g];
+    B[(S_VX + wax) * P.cells + g] = wax < P.rank ? w : A[(S_VX + wax) * P.cells + g];
     if (P.idual) B[S_GE * P.cells + g] = ge;
   });
   return chains;
@@ -1546,22 +1565,22 @@ PPM_HD void ph_capture(const SweepParams &P, const Tile &T, double *sm, int tid)
   if (p >= T.np || q >= T.NI) return;
   int i = T.IA + q;
   if (i < T.o || i >= T.o + T.w) return;  // owner of interface i = owner of cell i
-  int h0 = P.face_head[i], h1 = P.face_head[i + 1];
+  int h0 = as_global(P.face_head)[i], h1 = as_global(P.face_head)[i + 1];
   if (h0 == h1) return;
   int ci, cj;
   face_coords(P, T.t1_0 + p, T.t2, ci, cj);
   int cb = i - T.L0;
   const int vax = (P.dim + 1) % 3, wax = (P.dim + 2) % 3;
   for (int h = h0; h < h1; h++) {
-    const FaceRec &F = P.faces[h];
+    const FaceRec F = as_global(P.faces)[h];
     if (ci < F.lo_i || ci > F.hi_i || cj < F.lo_j || cj > F.hi_j) continue;
     long long off = F.base + (ci - F.lo_i) + (long long)(cj - F.lo_j) * (F.hi_i - F.lo_i + 1);
-    P.face_out[off + 0 * F.nface] = LDS<TP, TS, UNI>(sm, F_DF, p, cb);
-    P.face_out[off + 1 * F.nface] = LDS<TP, TS, UNI>(sm, F_EF, p, cb);
-    P.face_out[off + (2 + P.dim) * F.nface] = LDS<TP, TS, UNI>(sm, F_UF, p, cb);
-    P.face_out[off + (2 + vax) * F.nface] = LDS<TP, TS, UNI>(sm, F_VF, p, cb);
-    P.face_out[off + (2 + wax) * F.nface] = LDS<TP, TS, UNI>(sm, F_WF, p, cb);
-    if (P.idual) P.face_out[off + 5 * F.nface] = LDS<TP, TS, UNI>(sm, F_GEF, p, cb);
+    as_global(P.face_out)[off + 0 * F.nface] = LDS<TP, TS, UNI>(sm, F_DF, p, cb);
+    as_global(P.face_out)[off + 1 * F.nface] = LDS<TP, TS, UNI>(sm, F_EF, p, cb);
+    as_global(P.face_out)[off + (2 + P.dim) * F.nface] = LDS<TP, TS, UNI>(sm, F_UF, p, cb);
+    as_global(P.face_out)[off + (2 + vax) * F.nface] = LDS<TP, TS, UNI>(sm, F_VF, p, cb);
+    as_global(P.face_out)[off + (2 + wax) * F.nface] = LDS<TP, TS, UNI>(sm, F_WF, p, cb);
+    if (P.idual) as_global(P.face_out)[off + 5 * F.nface] = LDS<TP, TS, UNI>(sm, F_GEF, p, cb);
   }
 }
 
@@ -1569,7 +1588,7 @@ PPM_HD void ph_capture(const SweepParams &P, const Tile &T, double *sm, int tid)
 template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_load_col(const SweepParams &P, const Tile &T, double *sm, int tid, int n) {
   const int nc = T.L1 - T.L0 + 1;
-  const double *src = P.A + (long long)(S_COL0 + n) * P.cells;
+  const auto src = as_global(P.A) + (long long)(S_COL0 + n) * P.cells;
   per_cell_coalesced<TP, TS>(P, T, tid, nc, [&](int p, int c) {
     LDS<TP, TS, UNI>(sm, A_COL, p, c) = src[gidx(P, T.t1_0 + p, T.t2, T.L0 + c)];
   });
@@ -1604,14 +1623,14 @@ PPM_HD void ph_col_flux(const SweepParams &P, const Tile &T, double *sm, int tid
   colf = (UNI && P.dx_pow2) ? colf * P.inv_dx : colf / TS_<UNI>(P, T_DX, i);
   LDS<TP, TS, UNI>(sm, F_COLF, p, cb) = colf;
   if (P.face_head && i >= T.o && i < T.o + T.w) {
-    int h0 = P.face_head[i], h1 = P.face_head[i + 1];
+    int h0 = as_global(P.face_head)[i], h1 = as_global(P.face_head)[i + 1];
     int ci, cj;
     face_coords(P, T.t1_0 + p, T.t2, ci, cj);
     for (int h = h0; h < h1; h++) {
-      const FaceRec &F = P.faces[h];
+      const FaceRec F = as_global(P.faces)[h];
       if (ci < F.lo_i || ci > F.hi_i || cj < F.lo_j || cj > F.hi_j) continue;
       long long off = F.base + (ci - F.lo_i) + (long long)(cj - F.lo_j) * (F.hi_i - F.lo_i + 1);
-      P.face_out[off + (6 + n) * F.nface] = colf;
+      as_global(P.face_out)[off + (6 + n) * F.nface] = colf;
     }
   }
 }
@@ -1620,7 +1639,7 @@ PPM_HD void ph_col_flux(const SweepParams &P, const Tile &T, double *sm, int tid
 template <int TP, int TS, int UNI = 0>
 PPM_HD void ph_col_update(const SweepParams &P, const Tile &T, double *sm, int tid, int n) {
   const double min_color = 1e-5 * kTiny;
-  double *dst = P.B + (long long)(S_COL0 + n) * P.cells;
+  const auto dst = as_global(P.B) + (long long)(S_COL0 + n) * P.cells;
   per_cell_coalesced<TP, TS>(P, T, tid, T.w, [&](int p, int k) {
     int cell = T.o + k, c = cell - T.L0;
     double col = LDS<TP, TS, UNI>(sm, A_COL, p, c);
