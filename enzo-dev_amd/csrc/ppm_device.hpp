@@ -172,7 +172,7 @@ struct PresParams {
 };
 
 __device__ inline double pressure_of(const PresParams &Q, long long g, double min_support) {
-  const double *A = Q.A;
+  const auto A = as_global(Q.A);
   double density = A[S_D * Q.cells + g];
   double gas_energy;
   if (Q.idual == 0) {
@@ -194,7 +194,7 @@ __device__ inline double pressure_of(const PresParams &Q, long long g, double mi
 // ghost cells or without the time step)
 __device__ inline double pressure_dt_cell(const PresParams &Q, long long g) {
   double p = pressure_of(Q, g, Q.min_support);
-  Q.pres[g] = p;
+  as_global(Q.pres)[g] = p;
   if (!Q.want_dt) return 1.0e20;
   int i = (int)(g % Q.GD[0]);
   long long r = g / Q.GD[0];
@@ -204,17 +204,19 @@ __device__ inline double pressure_dt_cell(const PresParams &Q, long long g) {
     return 1.0e20;
   // calc_dt's pressure uses minimum-support coefficient 0
   double pdt = Q.min_support != 0.0 ? pressure_of(Q, g, 0.0) : p;
-  double d = Q.A[S_D * Q.cells + g];
+  const auto A = as_global(Q.A);
+  const auto cw0 = as_global(Q.cw[0]), cw1 = as_global(Q.cw[1]), cw2 = as_global(Q.cw[2]);
+  double d = A[S_D * Q.cells + g];
   double cs = fmx(sqrt(Q.gamma * pdt / d), kTiny);
-  double u = Q.A[S_VX * Q.cells + g];
-  if (Q.rank == 1) return Q.cw[0][i] * Q.aye / (cs + fabs(u - 0.0));
+  double u = A[S_VX * Q.cells + g];
+  if (Q.rank == 1) return cw0[i] * Q.aye / (cs + fabs(u - 0.0));
   if (Q.rank == 2) {
-    double v = Q.A[S_VY * Q.cells + g];
-    return Q.aye / ((cs + fabs(u - 0.0)) / Q.cw[0][i] + (cs + fabs(v - 0.0)) / Q.cw[1][j]);
+    double v = A[S_VY * Q.cells + g];
+    return Q.aye / ((cs + fabs(u - 0.0)) / cw0[i] + (cs + fabs(v - 0.0)) / cw1[j]);
   }
-  double v = Q.A[S_VY * Q.cells + g], w = Q.A[S_VZ * Q.cells + g];
-  return Q.aye / ((cs + fabs(u - 0.0)) / Q.cw[0][i] + (cs + fabs(v - 0.0)) / Q.cw[1][j] +
-                  (cs + fabs(w - 0.0)) / Q.cw[2][k]);
+  double v = A[S_VY * Q.cells + g], w = A[S_VZ * Q.cells + g];
+  return Q.aye / ((cs + fabs(u - 0.0)) / cw0[i] + (cs + fabs(v - 0.0)) / cw1[j] +
+                  (cs + fabs(w - 0.0)) / cw2[k]);
 }
 
 // wavefront then workgroup minimum of `local` into *dtmin (blockDim a
@@ -357,12 +359,12 @@ PPM_HD int expansion_old_slots(int idual, int rank, int s[5]) {
 }
 
 __device__ inline void expand_cell(const ExpandParams &X, long long g) {
-  double *A = X.A;
+  const auto A = as_global(X.A);
   const long long n = X.cells;
   const double d = A[S_D * n + g];
   double density, gas_energy;
   if (X.interp) {
-    const double *O = X.old;
+    const auto O = as_global(X.old);
     const double c = X.coef, co = 1 - X.coef;
     const double total_energy = c * A[S_E * n + g] + co * O[S_E * n + g];
     density = c * d + co * O[S_D * n + g];
@@ -497,8 +499,8 @@ __device__ inline void fluxcorr_side(const FluxCorrParams &F, long long x, int s
 // sweep needs.  Items: N + (i2 - i1 + 3) * n_outer.
 __device__ inline void prep_item(const FlatParams &F, long long gid) {
   if (gid < F.N) {
-    const double dt = F.dtp ? *F.dtp : F.dt;
-    F.qc[gid] = dt / F.dx[gid];
+    const double dt = F.dtp ? *as_global(F.dtp) : F.dt;
+    as_global(F.qc)[gid] = dt / as_global(F.dx)[gid];
     return;
   }
   if (!F.iflatten) return;
@@ -507,7 +509,7 @@ __device__ inline void prep_item(const FlatParams &F, long long gid) {
   if (k >= (long long)ncell * F.n_outer) return;
   const int outer = (int)(k / ncell);
   const int cell = F.i1 - 1 + (int)(k % ncell);
-  F.flat[(long long)outer * F.N + (cell - 1)] = flat_value(F, outer, cell);
+  as_global(F.flat)[(long long)outer * F.N + (cell - 1)] = flat_value(F, outer, cell);
 }
 
 

@@ -814,7 +814,7 @@ PPM_HD Acoustic acoustic_of(double dls, double drs, double pls, double prs, doub
 }
 
 PPM_HD bool learn_gate(const float *w, const Acoustic &A, double pls, double prs, double pmin) {
-  const double g = w[kOffGate];
+  const double g = as_global(w)[kOffGate];
   return A.ps > pmin && (fabs(A.ps - pls) > g * pls || fabs(A.ps - prs) > g * prs);
 }
 
@@ -1686,10 +1686,11 @@ PPM_HD double flat_value(const FlatParams &F, int outer, int cell) {
     c[oax] = outer;
     long long g = ((long long)c[2] * F.GD[1] + c[1]) * F.GD[0] + c[0];
     PPM_ASSERT(g >= 0 && g < F.cells);
-    pv[k + 4] = F.pres[g];
-    uv[k + 4] = F.A[(S_VX + F.dim) * F.cells + g];
-    dv[k + 4] = de ? F.A[S_D * F.cells + g] : 1.0;
-    ev[k + 4] = de ? F.A[S_E * F.cells + g] : 0.0;
+    const auto A = as_global(F.A);
+    pv[k + 4] = as_global(F.pres)[g];
+    uv[k + 4] = A[(S_VX + F.dim) * F.cells + g];
+    dv[k + 4] = de ? A[S_D * F.cells + g] : 1.0;
+    ev[k + 4] = de ? A[S_E * F.cells + g] : 0.0;
   }
   auto P = [&](int x) { PPM_ASSERT(x - cell >= -4 && x - cell <= 4); return pv[x - cell + 4]; };
   auto U = [&](int x) { PPM_ASSERT(x - cell >= -4 && x - cell <= 4); return uv[x - cell + 4]; };
@@ -1818,8 +1819,8 @@ PPM_HD void learn_eval(const SweepParams &P, double *sm, int m, X &ex) {
   constexpr int H = kMlpHidden;
   float *list = L::base(sm), *h1 = list + kNetStride * L::NT, *h2 = h1 + L::CH * H;
   float *w2s = h2 + L::CH * H;  // the transposed W2 in LDS when L::W2
-  const float *w = P.learn;
-  const float *w2 = L::W2 ? w2s : w + kOffW2T;
+  const auto w = as_global(P.learn);
+  const float *w2 = L::W2 ? w2s : P.learn + kOffW2T;
   for (int c0 = 0; c0 < m; c0 += L::CH) {
     const int mc = m - c0 < L::CH ? m - c0 : L::CH;
     ex.run([&](int tid, Regs &r) {  // hidden layer 1 (and the W2 copy, once)
