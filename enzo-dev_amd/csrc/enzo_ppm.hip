@@ -35,7 +35,9 @@ using SweepCfg = Cfg<kTP, kTS>;
 // ------------------------------------------------------------------------
 // Fused sweep kernel: see ppm_sweep.hpp for the stages.
 template <int UNI, int LEARN>
-__global__ __launch_bounds__(kTP *kTS) __attribute__((amdgpu_waves_per_eu(4)))
+// 5 wavefronts per SIMD: 96 VGPRs (a few spilled) and 31 KB of LDS per
+// workgroup, 5 workgroups per CU
+__global__ __launch_bounds__(kTP *kTS) __attribute__((amdgpu_waves_per_eu(5)))
 void ppm_sweep_kernel(SweepParams P) {
   __shared__ double sm[SweepCfg::SMEM];
   PPM_LDS_ENTER(sm);

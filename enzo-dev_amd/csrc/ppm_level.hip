@@ -59,7 +59,7 @@ constexpr int kShapeTP[kShapes] = {16, 8, 4};
 constexpr int kShapeTS[kShapes] = {16, 32, 64};
 
 template <int TP, int TS, int UNI, int LEARN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 void level_sweep_kernel(const SweepParams *__restrict__ params, const Work *__restrict__ work) {
   __shared__ double sm[Cfg<TP, TS>::SMEM];
   PPM_LDS_ENTER(sm);

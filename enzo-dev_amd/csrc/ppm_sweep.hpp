@@ -176,20 +176,20 @@ constexpr int kHarvestDoubles = 10;
 // ---------------------------------------------------------------------------
 // LDS layout: NARR arrays of [TP][LW] doubles.
 enum Arr {
-  A_D = 0, A_E, A_U, A_V, A_W, A_GE, A_P, A_GR, A_CS, A_U0,
+  A_D = 0, A_E, A_U, A_V, A_W, A_GE, A_P, A_CS,
   A_ST, A_DQ, A_QL, A_QR, A_Q6, A_COL, A_X, NARR
 };
 // flux arrays alias arrays that are dead after the interface solve
 constexpr int F_DF = A_DQ, F_EF = A_QL, F_UF = A_QR, F_VF = A_Q6, F_WF = A_ST,
               F_GEF = A_P, F_UB = A_X, F_COLF = A_ST, A_EOUT = A_X, A_GE1 = A_DQ,
               A_HOT = A_QR;
-// the learned start's acoustic quantities (Acoustic) of each interface,
-// between the two stages of the Riemann phase, in arrays dead then and
-// outside the network's lists (A_CS .. A_Q6): A_P after the pressure's
-// reconstruction, A_COL before the fallback flags and the colours, A_X
-// before the flux store (each work-item reads its own interface's entries
-// before it stores that interface's fluxes into F_GEF = A_P, F_UB = A_X)
-constexpr int A_AC_L = A_P, A_AC_R = A_COL, A_AC_P = A_X;
+// the learned start's acoustic quantities C_r and p_ac (Acoustic) of each
+// interface, between the two stages of the Riemann phase, in arrays dead
+// then and outside the network's lists (A_P .. A_Q6): A_COL before the
+// fallback flags and the colours, A_X before the flux store (each
+// work-item reads its own interface's entries before it stores that
+// interface's fluxes into F_UB = A_X); C_l is computed again
+constexpr int A_AC_R = A_COL, A_AC_P = A_X;
 
 // Fallback workgroups (one pencil, TP = 1) keep two more arrays: the HLL
 // density flux for the in-order fallback decision, then the two gas-energy
@@ -201,9 +201,12 @@ constexpr int A_HA = NARR, A_HB = NARR + 1, NARR_FB = NARR + 2;
 // 57 owned cells); on the 256^3 workload 4 x 64 beat 8 x 32 and 2 x 128.
 constexpr int kGridTP = 4, kGridTS = 64;
 
+// LW: cells per pencil in LDS.  A tile loads at most NI + 5 cells (two
+// halo cells below its first interface and three above its last) and
+// build_segments keeps NI + 4 <= TS, so TS + 1 cells hold any tile.
 template <int TP, int TS, int UNI = 0>
 struct Cfg {
-  static constexpr int LW = TS + 4;
+  static constexpr int LW = TS + 1;
   static constexpr int NT = TP * TS;
   static constexpr int SMEM = NARR * TP * LW;  // doubles
 };
@@ -430,7 +433,7 @@ PPM_HD Tile make_tile(const SweepParams &P, int block) {
 
 // one whole pencil (t1, t2) as a single tile owning cells [1, N]: the tile
 // of the fallback workgroups.  Fits when N - 1 <= TS (interfaces and slope
-// items) and N <= TS + 4 (loaded cells).
+// items) and N <= Cfg::LW (loaded cells).
 PPM_HD Tile make_pencil_tile(const SweepParams &P, int t1, int t2) {
   Tile T;
   T.t2 = t2;
@@ -489,14 +492,11 @@ PPM_HD void ph_load(const SweepParams &P, const Tile &T, double *sm, int tid) {
     const auto A = as_global(P.A);
     LDS<TP, TS, UNI>(sm, A_D, p, c) = A[S_D * P.cells + g];
     LDS<TP, TS, UNI>(sm, A_E, p, c) = A[S_E * P.cells + g];
-    const double uu = A[(S_VX + P.dim) * P.cells + g];
-    LDS<TP, TS, UNI>(sm, A_U, p, c) = uu;
-    LDS<TP, TS, UNI>(sm, A_U0, p, c) = uu;
+    LDS<TP, TS, UNI>(sm, A_U, p, c) = A[(S_VX + P.dim) * P.cells + g];
     LDS<TP, TS, UNI>(sm, A_V, p, c) = vax < P.rank ? A[(S_VX + vax) * P.cells + g] : 0.0;
     LDS<TP, TS, UNI>(sm, A_W, p, c) = wax < P.rank ? A[(S_VX + wax) * P.cells + g] : 0.0;
     if (P.idual) LDS<TP, TS, UNI>(sm, A_GE, p, c) = A[S_GE * P.cells + g];
     LDS<TP, TS, UNI>(sm, A_P, p, c) = as_global(P.pres)[g];
-    if (P.gravity) LDS<TP, TS, UNI>(sm, A_GR, p, c) = as_global(P.gr)[g];
   });
 }
 
@@ -709,9 +709,9 @@ PPM_HD void ph_iface(const SweepParams &P, const Tile &T, double *sm, int tid, R
   if constexpr (VAR == 0) {
     // the characteristic distances of cells i-1 and i, kept in registers
     // for the other variables, the Riemann solve and the colours
-    r.ca = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, ca), LDS<TP, TS, UNI>(sm, A_CS, p, ca), T.dt,
+    r.ca = chars_of(LDS<TP, TS, UNI>(sm, A_U, p, ca), LDS<TP, TS, UNI>(sm, A_CS, p, ca), T.dt,
                     TS_<UNI>(P, T_DX2I, i - 1));
-    r.cb = chars_of(LDS<TP, TS, UNI>(sm, A_U0, p, cb), LDS<TP, TS, UNI>(sm, A_CS, p, cb), T.dt,
+    r.cb = chars_of(LDS<TP, TS, UNI>(sm, A_U, p, cb), LDS<TP, TS, UNI>(sm, A_CS, p, cb), T.dt,
                     TS_<UNI>(P, T_DX2I, i));
   }
   if constexpr (VAR >= 3) {
@@ -1047,7 +1047,8 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
   double betal0 = (r.pla - r.pl0) * (f1 * f1) + 1.0 / r.dla - 1.0 / r.dl0;
   double grs = 0.0;
   if (P.gravity == 1) {
-    grs = 0.25 * dt * (LDS<TP, TS, UNI>(sm, A_GR, p, ca) + LDS<TP, TS, UNI>(sm, A_GR, p, cb));
+    const auto gr = as_global(P.gr);
+    grs = 0.25 * dt * (gr[gidx(P, T.t1_0 + p, T.t2, i - 1)] + gr[gidx(P, T.t1_0 + p, T.t2, i)]);
     betalp = betalp - grs;
     betalm = betalm - grs;
   }
@@ -1120,7 +1121,6 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
     r.s_ur = urs;
     if (P.ipresfree != 1) {  // kept in LDS arrays that are dead here (A_AC_*)
       const Acoustic a = acoustic_of(dls, drs, pls, prs, uls, urs, gamma, P.pmin);
-      LDS<TP, TS, UNI>(sm, A_AC_L, p, cb) = a.cl;
       LDS<TP, TS, UNI>(sm, A_AC_R, p, cb) = a.cr;
       LDS<TP, TS, UNI>(sm, A_AC_P, p, cb) = a.ps;
     }
@@ -1140,7 +1140,7 @@ PPM_HD void ph_solve(const SweepParams &P, const Tile &T, double *sm, int tid, R
   int iters, restarted = 0;
   if constexpr (LEARN != 0 && STAGE == 1) {
     Acoustic a;
-    a.cl = LDS<TP, TS, UNI>(sm, A_AC_L, p, cb);
+    a.cl = sqrt(gamma * pls * dls);  // as acoustic_of
     a.cr = LDS<TP, TS, UNI>(sm, A_AC_R, p, cb);
     a.ps = LDS<TP, TS, UNI>(sm, A_AC_P, p, cb);
     iters = twoshock_learned(r.ls >= 0, r.lc, a, dls, drs, pls, prs, uls, urs, gamma, P.pmin,
@@ -1449,7 +1449,7 @@ PPM_HD int ph_euler(const SweepParams &P, const Tile &T, double *sm, int tid, bo
       LDS<TP, TS, UNI>(sm, A_GE, p, c) = ge;
     }
     if (P.gravity == 1) {
-      double gr = LDS<TP, TS, UNI>(sm, A_GR, p, c);
+      const double gr = as_global(P.gr)[gidx(P, T.t1_0 + p, T.t2, cell)];
       u = u + dt * gr * 0.5 * (d * dnuinv + 1.0);
       e = e + dt * gr * 0.5 * (u + uold * d * dnuinv);
       e = fmx(e, kTiny);
@@ -1756,7 +1756,7 @@ struct TileCounts {
 
 // ---------------------------------------------------------------------------
 // Learned start inside the tile.  Between the two stages of the Riemann
-// phase, the LDS arrays A_CS .. A_Q6 are free (the reconstruction is done,
+// phase, the LDS arrays A_P .. A_Q6 are free (the reconstruction is done,
 // the flux store has not begun); as floats they hold the list of network
 // inputs of the interfaces that pass the gate ([NT][kNetStride]) and the
 // hidden layers of a chunk of CH of them ([CH][32] twice).  The list's
@@ -1768,7 +1768,7 @@ struct TileCounts {
 template <int TP, int TS, int UNI>
 struct NetLds {
   static constexpr int NT = TP * TS;
-  static constexpr int FLOATS = 2 * (A_Q6 - A_CS + 1) * TP * Cfg<TP, TS, UNI>::LW;
+  static constexpr int FLOATS = 2 * (A_Q6 - A_P + 1) * TP * Cfg<TP, TS, UNI>::LW;
   // list, two hidden layers of a chunk of ch interfaces, and (w2) a copy of
   // the transposed second-layer weights
   static constexpr int need(int ch, bool w2) {
@@ -1778,11 +1778,11 @@ struct NetLds {
   static constexpr bool W2 = fits(16, true);  // chunks of at least 16 with the weights in LDS
   static constexpr int CH = fits(32, W2) ? 32 : fits(16, W2) ? 16 : fits(8, W2) ? 8 : 4;
   static_assert(fits(CH, W2), "the learned start's lists fit the free tile arrays");
-  static constexpr bool outside(int a) { return a < A_CS || a > A_Q6; }
-  static_assert(outside(A_AC_L) && outside(A_AC_R) && outside(A_AC_P),
+  static constexpr bool outside(int a) { return a < A_P || a > A_Q6; }
+  static_assert(outside(A_AC_R) && outside(A_AC_P),
                 "the acoustic quantities live outside the lists");
   PPM_HD static float *base(double *sm) {
-    return reinterpret_cast<float *>(sm + (size_t)A_CS * TP * Cfg<TP, TS, UNI>::LW);
+    return reinterpret_cast<float *>(sm + (size_t)A_P * TP * Cfg<TP, TS, UNI>::LW);
   }
   PPM_HD static int *count(double *sm) {
     return reinterpret_cast<int *>(sm + (size_t)A_COL * TP * Cfg<TP, TS, UNI>::LW);
@@ -1962,7 +1962,7 @@ PPM_HD TileCounts sweep_tile(const SweepParams &P, int block, double *sm, X &ex)
 }
 
 // One whole pencil in a one-pencil workgroup of TS threads, with the
-// in-order per-cell fallback; sm holds NARR_FB arrays of TS + 4 doubles.
+// in-order per-cell fallback; sm holds NARR_FB arrays of Cfg<1, TS>::LW doubles.
 template <int TS, int UNI, class X, int LEARN = 0>
 PPM_HD TileCounts sweep_pencil_fb(const SweepParams &P, int t1, int t2, double *sm, X &ex) {
   return sweep_body<1, TS, UNI, 1, X, LEARN>(P, make_pencil_tile(P, t1, t2), sm, ex);
@@ -1975,7 +1975,7 @@ PPM_HD bool pencil_fits(const SweepParams &P) {
   T.o = 1;
   T.w = P.N;
   tile_ranges(P, T);
-  return T.NI + 4 <= TS && T.L1 - T.L0 + 1 <= TS + 4 && T.L0 >= 1 && T.L1 <= P.N;
+  return T.NI + 4 <= TS && T.L1 - T.L0 + 1 <= Cfg<1, TS>::LW && T.L0 >= 1 && T.L1 <= P.N;
 }
 
 }  // namespace ppm
