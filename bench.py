@@ -39,7 +39,7 @@ SIMDS, VALU_CYCLES_PER_INST, CLOCK_HZ = 1024, 4, 2.4e9
 # rocprofv3 counts of one 256^3 sweep of the turbulent workload with the
 # current build (profiles/ppm_sweep_kernel_turbulent256.md): wave-level VALU
 # instructions, waves, and the FP64 add / mul / fma / transcendental parts
-VALU_PER_SWEEP_256, WAVES_PER_SWEEP_256 = 1.231e9, 345840
+VALU_PER_SWEEP_256, WAVES_PER_SWEEP_256 = 1.232e9, 345840
 F64_ADD, F64_MUL, F64_FMA, F64_TRANS = 0.168e9, 0.210e9, 0.218e9, 0.0355e9
 
 
