@@ -30,6 +30,13 @@ namespace ppm {
 #define PPM_OPAQUE_VGPR(x) asm volatile("" : "+v"(x))
 #endif
 
+// LDS of the executor's votes, one per kernel (a local __shared__ array in
+// the member templates below would be one per instantiation)
+__device__ inline int *exec_lds() {
+  __shared__ int slots[17];  // [0, 16): a count per wavefront, [16]: first()
+  return slots;
+}
+
 // Workgroup executor of the tile phases (ppm_sweep.hpp): run = phase then
 // barrier; count = number of work-items with a nonzero result; first = the
 // result of work-item 0, broadcast.
@@ -45,7 +52,7 @@ struct DevExec {
   // over the workgroup's wavefronts (at most 16: 1024 work-items)
   template <class F>
   __device__ int count(F f) {
-    __shared__ int part[16];
+    int *part = exec_lds();
     int v = f(tid, r);
     const unsigned long long b = __ballot(v != 0);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = __popcll(b);
@@ -57,7 +64,7 @@ struct DevExec {
   }
   template <class F>
   __device__ int first(F f) {
-    __shared__ int v0;
+    int &v0 = exec_lds()[16];
     int v = f(tid, r);
     if (threadIdx.x == 0) v0 = v;
     __syncthreads();
