@@ -16,7 +16,6 @@
 #include <cstring>
 #include <mutex>
 #include <new>
-#include <type_traits>
 #include <vector>
 
 #include "../../include/enzo_ppm.h"
@@ -30,23 +29,21 @@ using namespace ppm;
 namespace {
 
 constexpr int kTP = kGridTP;  // pencils per workgroup
-// interfaces per pencil segment: kGridTS, or kGridTSLong on the axes where
-// its segments take fewer work-items (plan_segments)
-constexpr int kTS = kGridTS, kTSLong = kGridTSLong;
+constexpr int kTS = kGridTS;  // interfaces per pencil segment
+using SweepCfg = Cfg<kTP, kTS>;
 
 // ------------------------------------------------------------------------
 // Fused sweep kernel: see ppm_sweep.hpp for the stages.
-template <int TS, int UNI, int LEARN>
-// 96 VGPRs (a few spilled) for 5 wavefronts per SIMD; LDS per workgroup
-// 31 KB at TS = 64 (5 workgroups, 20 wavefronts per CU), 47 KB at TS = 96
-// (3 workgroups, 18 wavefronts per CU)
-__global__ __launch_bounds__(kTP *TS) __attribute__((amdgpu_waves_per_eu(5)))
+template <int UNI, int LEARN>
+// 5 wavefronts per SIMD: 96 VGPRs (a few spilled) and 31 KB of LDS per
+// workgroup, 5 workgroups per CU
+__global__ __launch_bounds__(kTP *kTS) __attribute__((amdgpu_waves_per_eu(5)))
 void ppm_sweep_kernel(SweepParams P) {
-  __shared__ double sm[Cfg<kTP, TS>::SMEM];
+  __shared__ double sm[SweepCfg::SMEM];
   PPM_LDS_ENTER(sm);
   DevExec ex;
   const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  TileCounts tc = sweep_tile<kTP, TS, UNI, DevExec, LEARN>(P, tile, sm, ex);
+  TileCounts tc = sweep_tile<kTP, kTS, UNI, DevExec, LEARN>(P, tile, sm, ex);
   if (threadIdx.x == 0) {
     // a failing cell or an error: the fallback kernel redoes the group's
     // pencils whole (the fallback chain can cross segments) and settles them
@@ -228,7 +225,6 @@ struct hipPPMData {
   int uni[3] = {0, 0, 0};  // axis has one cell width
   double kt[3][NTAB];      // its coefficient entries
   int segoff[3] = {0, 0, 0}, nseg[3] = {0, 0, 0};
-  int seg_ts[3] = {kTS, kTS, kTS};  // interfaces per segment (tile width) of each axis
   // face capture
   int *face_head[3] = {nullptr, nullptr, nullptr};
   FaceRec *faces[3] = {nullptr, nullptr, nullptr};
@@ -556,18 +552,13 @@ int launch_sweep(hipPPMData *D, const hipPPMParameter *P, double dtFixed, const 
   // instantiation by (one cell width, learned start)
   const int variant = (S.uni[0] ? 1 : 0) | (D->learned ? 2 : 0);
   auto launch_main = [&](unsigned nblk) {
-    auto go = [&](auto W) {
-      constexpr int TS = decltype(W)::value;
-      const dim3 g(nblk), b(kTP * TS);
-      switch (variant) {
-        case 0: hipLaunchKernelGGL((ppm_sweep_kernel<TS, 0, 0>), g, b, 0, D->stream, S); break;
-        case 1: hipLaunchKernelGGL((ppm_sweep_kernel<TS, 1, 0>), g, b, 0, D->stream, S); break;
-        case 2: hipLaunchKernelGGL((ppm_sweep_kernel<TS, 0, 1>), g, b, 0, D->stream, S); break;
-        default: hipLaunchKernelGGL((ppm_sweep_kernel<TS, 1, 1>), g, b, 0, D->stream, S); break;
-      }
-    };
-    if (D->seg_ts[dim] == kTSLong) go(std::integral_constant<int, kTSLong>());
-    else go(std::integral_constant<int, kTS>());
+    const dim3 g(nblk), b(kTP * kTS);
+    switch (variant) {
+      case 0: hipLaunchKernelGGL((ppm_sweep_kernel<0, 0>), g, b, 0, D->stream, S); break;
+      case 1: hipLaunchKernelGGL((ppm_sweep_kernel<1, 0>), g, b, 0, D->stream, S); break;
+      case 2: hipLaunchKernelGGL((ppm_sweep_kernel<0, 1>), g, b, 0, D->stream, S); break;
+      default: hipLaunchKernelGGL((ppm_sweep_kernel<1, 1>), g, b, 0, D->stream, S); break;
+    }
     return hipGetLastError();
   };
   // the two-shock tiles, or with the HLL solver every group to the
@@ -686,16 +677,8 @@ int plan_segments(hipPPMData *D, const hipPPMParameter *Para, int *segs) {
     const int N = D->GD[d];
     const int i1 = (int)Para->GridStartIndex[d] + 1, i2 = (int)Para->GridEndIndex[d] + 1;
     int ns = 0;
-    D->seg_ts[d] = kTS;
     if (d < Para->GridRank && N > 1) {
-      // the longer segments where they take fewer work-items (the same
-      // count goes to the shorter ones, whose workgroups fit more per CU)
-      const int nl = build_segments<kTP, kTSLong>(N, i1, i2, segs + n, segs + n + N, N);
       ns = build_segments<kTP, kTS>(N, i1, i2, segs + n, segs + n + N, N);
-      if (nl > 0 && (ns == 0 || nl * kTSLong < ns * kTS)) {
-        ns = build_segments<kTP, kTSLong>(N, i1, i2, segs + n, segs + n + N, N);
-        D->seg_ts[d] = kTSLong;
-      }
       if (ns == 0) return -1;
       for (int k = 0; k < ns; k++) segs[n + ns + k] = segs[n + N + k];  // w after o
     }

@@ -196,13 +196,13 @@ constexpr int A_AC_R = A_COL, A_AC_P = A_X;
 // source terms sl*bm0 and sr*bp0 of each interface (flux_hll.F:297-308).
 constexpr int A_HA = NARR, A_HB = NARR + 1, NARR_FB = NARR + 2;
 
-// Tile shapes of the per-grid sweep kernel: 4 pencils x 64 interfaces (256
-// work-items; segments of up to 57 owned cells), or 4 x 96 (384 work-items,
-// up to 89 owned cells) on an axis where its segments take fewer
-// work-items: a 262-cell pencil (256 active cells) is 5 x 64 = 320
-// work-items in the first, 3 x 96 = 288 in the second.  On the 256^3
-// workload 4 x 64 beat 8 x 32 and 2 x 128.
-constexpr int kGridTP = 4, kGridTS = 64, kGridTSLong = 96;
+// Tile shape of the per-grid sweep kernel: 4 pencils x 64 interfaces (256
+// work-items).  Longer segments cut the halo recomputation (segments of up to
+// 57 owned cells); on the 256^3 workload 4 x 64 beat 8 x 32, 2 x 128 and
+// 4 x 96 (3.26 ms against 2.40 per sweep, though its 3 segments of a
+// 262-cell pencil take 10% fewer work-items than 5 x 64: pencils that
+// straddle wavefronts cost more than the idle lanes saved).
+constexpr int kGridTP = 4, kGridTS = 64;
 
 // LW: cells per pencil in LDS.  A tile loads at most NI + 5 cells (two
 // halo cells below its first interface and three above its last) and
