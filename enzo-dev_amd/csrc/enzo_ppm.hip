@@ -186,7 +186,7 @@ __global__ void scatter_slots_kernel(const double *A, long long cells, int nslot
 // the external boundary).  Thread per (slot, ghost cell).
 __global__ void periodic_axis_kernel(BCParams B) {
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid < periodic_items(B)) periodic_item(B, gid);
+  if (gid < periodic_items(B)) periodic_item(B, (int)gid);
 }
 
 __global__ void ext_bc_kernel(ExtBCParams B) {
@@ -1662,7 +1662,7 @@ int hipPPMPeriodicBoundary(hipPPMData *D, const hipPPMParameter *P) {
       B.GD[d] = D->GD[d];
       B.act[d] = (int)(P->GridEndIndex[d] - P->GridStartIndex[d] + 1);
     }
-    if (P->GridDimension[a] - P->GridEndIndex[a] - 1 != B.ghost) return ENZO_PPM_ERR_ARGUMENT;
+    if (!periodic_fits(B, P->GridDimension[a] - P->GridEndIndex[a] - 1)) return ENZO_PPM_ERR_ARGUMENT;
     B.cells = D->cells;
     B.S = D->A;
     long long n = (long long)2 * B.ghost * B.GD[(a + 1) % 3] * B.GD[(a + 2) % 3] * B.nslot;

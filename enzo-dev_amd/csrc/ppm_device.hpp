@@ -251,31 +251,45 @@ struct BCParams {
   double *S;
 };
 
+// Whether the periodic fill of B's axis is defined and fits its items:
+// as many ghost cells on the right (`right_ghost`) as on the left, an
+// active extent of at least the ghost count (each ghost cell is then one
+// period from an active cell, the images CopyZonesFromGrid takes; a
+// narrower grid would also read ghost cells the same launch writes), and
+// fewer than 2^31 items.
+inline bool periodic_fits(const BCParams &B, long long right_ghost) {
+  const int a = B.axis;
+  const long long n = 2LL * B.ghost * B.GD[(a + 1) % 3] * B.GD[(a + 2) % 3] * B.nslot;
+  return right_ghost == B.ghost && B.ghost >= 0 && B.act[a] >= B.ghost && n < (1LL << 31);
+}
+
 __device__ inline long long periodic_items(const BCParams &B) {
   return (long long)2 * B.ghost * B.GD[(B.axis + 1) % 3] * B.GD[(B.axis + 2) % 3] * B.nslot;
 }
 
-__device__ inline void periodic_item(const BCParams &B, long long gid) {
-  const int a = B.axis;
-  const int b = (a + 1) % 3, c = (a + 2) % 3;
-  const long long per_slot = (long long)2 * B.ghost * B.GD[b] * B.GD[c];
-  const int slot = (int)(gid / per_slot);
-  long long r = gid % per_slot;
-  const int gl = (int)(r % (2 * B.ghost));
-  r /= 2 * B.ghost;
-  const int cb = (int)(r % B.GD[b]);
-  const int cc = (int)(r / B.GD[b]);
-  int dst = gl < B.ghost ? gl : B.ghost + B.act[a] + (gl - B.ghost);
-  int src = gl < B.ghost ? gl + B.act[a] : B.ghost + (gl - B.ghost);
-  int x[3];
-  x[b] = cb;
-  x[c] = cc;
-  x[a] = dst;
-  long long gd = ((long long)x[2] * B.GD[1] + x[1]) * B.GD[0] + x[0];
-  x[a] = src;
-  long long gs = ((long long)x[2] * B.GD[1] + x[1]) * B.GD[0] + x[0];
+// Item gid walks the box of the axis' ghost layers (extent 2 * ghost along
+// the axis, the grid's extent across it) with x fastest, so that the loads
+// and stores of a wavefront are contiguous along x (runs of `ghost` cells
+// when the axis is x).  Layer l < ghost is left ghost cell l, taken from
+// active cell l + act; layer l >= ghost is right ghost cell l + act, taken
+// from active cell l (act >= ghost, checked on the host).  The host also
+// keeps the item count below 2^31, so 32-bit arithmetic.
+__device__ inline void periodic_item(const BCParams &B, int gid) {
+  const int a = B.axis, g = B.ghost, act = B.act[a];
+  const int n0 = a == 0 ? 2 * g : B.GD[0], n1 = a == 1 ? 2 * g : B.GD[1],
+            n2 = a == 2 ? 2 * g : B.GD[2];
+  const int per_slot = n0 * n1 * n2;
+  const int slot = gid / per_slot;
+  int r = gid - slot * per_slot;
+  const int x0 = r % n0;
+  r /= n0;
+  const int x1 = r % n1, x2 = r / n1;
+  const int l = a == 0 ? x0 : a == 1 ? x1 : x2;
+  const int sa = a == 0 ? 1 : a == 1 ? B.GD[0] : B.GD[0] * B.GD[1];  // stride of the axis
+  const int base = (x2 * B.GD[1] + x1) * B.GD[0] + x0 - l * sa;     // cell 0 of the line
+  const int dst = l < g ? l : l + act, src = l < g ? l + act : l;
   double *S = B.S + (long long)slot * B.cells;
-  S[gd] = S[gs];
+  S[base + dst * sa] = S[base + src * sa];
 }
 
 // ------------------------------------------------------------------------

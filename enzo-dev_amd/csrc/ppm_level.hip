@@ -223,9 +223,8 @@ __global__ void level_copy_kernel(const CopyBox *__restrict__ boxes,
 // periodic ghost fill of every grid along one axis: one workgroup row per grid
 __global__ void level_periodic_kernel(const BCParams *__restrict__ B) {
   const BCParams &b = B[blockIdx.y];
-  const long long n = periodic_items(b);
-  for (long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x; gid < n;
-       gid += (long long)gridDim.x * blockDim.x)
+  const int n = (int)periodic_items(b);  // below 2^31 (periodic_fits)
+  for (int gid = blockIdx.x * blockDim.x + threadIdx.x; gid < n; gid += gridDim.x * blockDim.x)
     periodic_item(b, gid);
 }
 
@@ -1539,7 +1538,7 @@ int hipPPMLevelPeriodicBoundary(hipPPMLevel *L) {
         B.GD[d] = L->GD[3 * g + d];
         B.act[d] = (int)(P.GridEndIndex[d] - P.GridStartIndex[d] + 1);
       }
-      if (P.GridDimension[a] - P.GridEndIndex[a] - 1 != B.ghost) return ENZO_PPM_ERR_ARGUMENT;
+      if (!periodic_fits(B, P.GridDimension[a] - P.GridEndIndex[a] - 1)) return ENZO_PPM_ERR_ARGUMENT;
       B.cells = L->cells[g];
       B.S = (L->parity == 0 ? L->A : L->B) + L->off[g];
       nmax = std::max(nmax, (long long)2 * B.ghost * B.GD[(a + 1) % 3] * B.GD[(a + 2) % 3] *

@@ -428,3 +428,75 @@ class DeviceArrays:
         for p in self.ptrs:
             self.hip.hipFree(p)
         self.ptrs = []
+
+
+def _carried(g):
+    """Indices of the fields the device state holds: all but the velocity
+    components beyond the grid's rank (fields d, e, vx, vy, vz, ge, colours)."""
+    return [k for k in range(len(g.fields)) if not (2 + g.rank <= k <= 4)]
+
+
+def _scramble_ghosts(g, seed):
+    """Ghost cells of every field the device holds set to distinct seeded
+    values (so that a fill that misses or misplaces a cell shows)."""
+    rng = np.random.default_rng(seed)
+    sl = g.active_slice()
+    for k in _carried(g):
+        f = g.fields[k]
+        keep = f[sl].copy()
+        f[...] = rng.uniform(-1e3, -1.0, f.shape)
+        f[sl] = keep
+
+
+def check_periodic_fill(lib):
+    """Device periodic ghost fill, per grid and on a level of ragged grids,
+    against the host fill (ExternalBoundary periodic case), every cell of
+    every field bitwise; a grid whose active extent along an axis is below
+    its ghost count is refused."""
+    import pytest
+
+    from enzo_ppm import device, problems
+    from enzo_ppm.level import Level
+    cases = [problems.random_smooth([7, 5, 4], seed=31, ncolours=2),
+             problems.random_smooth([9, 6, 1], seed=32, ncolours=1, rank=2),
+             problems.random_smooth([12, 1, 1], seed=33, rank=1)]
+    for k, (g, glob, _) in enumerate(cases):
+        _scramble_ghosts(g, 40 + k)
+        ref = g.copy()
+        ref.fill_periodic()
+        dg = device.DeviceGrid(g, glob, lib=lib)
+        try:
+            dg.periodic_boundary()
+            assert dg.download() == abi.SUCCESS
+        finally:
+            dg.close()
+        assert all(bit_equal(g.fields[i], ref.fields[i]) for i in _carried(g)), k
+    grids = [problems.random_smooth(a, seed=34 + i, ncolours=1)[0]
+             for i, a in enumerate(([6, 4, 5], [3, 8, 4], [5, 5, 9]))]
+    glob = problems.random_smooth([6, 4, 5], seed=34, ncolours=1)[1]
+    for i, g in enumerate(grids):
+        _scramble_ghosts(g, 50 + i)
+    refs = [g.copy() for g in grids]
+    for r in refs:
+        r.fill_periodic()
+    lv = Level(grids, glob, lib=lib)
+    try:
+        lv.periodic_boundary()
+        lv.download()
+    finally:
+        lv.close()
+    for i, (g, r) in enumerate(zip(grids, refs)):
+        assert all(bit_equal(x, y) for x, y in zip(g.fields, r.fields)), i
+    g, glob, _ = problems.random_smooth([2, 5, 4], seed=35)
+    dg = device.DeviceGrid(g, glob, lib=lib)
+    try:
+        with pytest.raises(RuntimeError):
+            dg.periodic_boundary()
+    finally:
+        dg.close()
+    lv = Level([g], glob, lib=lib)
+    try:
+        with pytest.raises(RuntimeError):
+            lv.periodic_boundary()
+    finally:
+        lv.close()

@@ -579,3 +579,8 @@ def test_host_round_trip(cpulib, n):
     buffers)."""
     from helpers import check_host_round_trip
     check_host_round_trip(cpulib, n)
+
+
+def test_periodic_fill(cpulib):
+    from helpers import check_periodic_fill
+    check_periodic_fill(cpulib)

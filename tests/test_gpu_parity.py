@@ -275,3 +275,10 @@ def test_host_round_trip(lib, n):
     field): every array back bit for bit, in both directions."""
     from helpers import check_host_round_trip
     check_host_round_trip(lib, n)
+
+
+def test_periodic_fill(lib):
+    """Device periodic ghost fill against the host fill, bitwise, per grid
+    and on a level; narrow grids refused."""
+    from helpers import check_periodic_fill
+    check_periodic_fill(lib)
