@@ -294,7 +294,10 @@ int hipPPMStep(hipPPMData *Data, const hipPPMParameter *Para, enzo_int CycleNumb
    SetBoundaryConditions gives a grid that spans a periodic domain, through
    CopyZonesFromGrid from its own periodic images (CheckForOverlap's
    EdgeOffsets; ExternalBoundary::SetExternalBoundary leaves periodic faces
-   alone in the reference build, which does not define USE_PERIODIC). */
+   alone in the reference build, which does not define USE_PERIODIC).
+   ENZO_PPM_ERR_ARGUMENT when the ghost zones of an axis differ in width or
+   its active extent is below the ghost count (ghost cells more than one
+   period from every active cell). */
 int hipPPMPeriodicBoundary(hipPPMData *Data, const hipPPMParameter *Para);
 
 /* grid::SetExternalBoundaryValues (Grid_SetExternalBoundaryValues.C:91-117
@@ -586,7 +589,7 @@ int hipPPMLevelComovingExpansionTerms(hipPPMLevel *Level, enzo_float coef, enzo_
 /* Periodic ghost fill of every grid of the level (for grids that span a
    periodic domain alone, as a root grid does: the copies from their own
    periodic images that SetBoundaryConditions makes, see
-   hipPPMPeriodicBoundary). */
+   hipPPMPeriodicBoundary, whose argument checks apply to every grid). */
 int hipPPMLevelPeriodicBoundary(hipPPMLevel *Level);
 
 /* hipPPMExternalBoundary for every grid of the level
