@@ -382,46 +382,44 @@ PPM_HD int expansion_old_slots(int idual, int rank, int s[5]) {
 __device__ inline void expand_cell(const ExpandParams &X, long long g) {
   const auto A = as_global(X.A);
   const long long n = X.cells;
-  const double d = A[S_D * n + g];
+  // every field of the cell read before any is written, so that its loads
+  // are in flight together (each field is written once, from these values)
+  const double d = A[S_D * n + g], e0 = A[S_E * n + g], v1 = A[S_VX * n + g];
+  const double v2 = X.rank > 1 ? A[S_VY * n + g] : 0.0;
+  const double v3 = X.rank > 2 ? A[S_VZ * n + g] : 0.0;
+  const double ge0 = X.idual != 0 ? A[S_GE * n + g] : 0.0;
   double density, gas_energy;
   if (X.interp) {
     const auto O = as_global(X.old);
     const double c = X.coef, co = 1 - X.coef;
-    const double total_energy = c * A[S_E * n + g] + co * O[S_E * n + g];
     density = c * d + co * O[S_D * n + g];
-    const double velocity1 = c * A[S_VX * n + g] + co * O[S_VX * n + g];
-    double velocity2 = 0, velocity3 = 0;
-    if (X.rank > 1) velocity2 = c * A[S_VY * n + g] + co * O[S_VY * n + g];
-    if (X.rank > 2) velocity3 = c * A[S_VZ * n + g] + co * O[S_VZ * n + g];
     if (X.idual == 0) {
+      const double total_energy = c * e0 + co * O[S_E * n + g];
+      const double velocity1 = c * v1 + co * O[S_VX * n + g];
+      double velocity2 = 0, velocity3 = 0;
+      if (X.rank > 1) velocity2 = c * v2 + co * O[S_VY * n + g];
+      if (X.rank > 2) velocity3 = c * v3 + co * O[S_VZ * n + g];
       const double kineticE =
           0.5 * (velocity1 * velocity1 + velocity2 * velocity2 + velocity3 * velocity3);
       gas_energy = total_energy - kineticE;
     } else {
-      gas_energy = c * A[S_GE * n + g] + co * O[S_GE * n + g];
+      gas_energy = c * ge0 + co * O[S_GE * n + g];
     }
   } else {
     density = d;
-    if (X.idual == 0) {
-      const double v1 = A[S_VX * n + g];
-      const double v2 = X.rank > 1 ? A[S_VY * n + g] : 0.0;
-      const double v3 = X.rank > 2 ? A[S_VZ * n + g] : 0.0;
-      gas_energy = A[S_E * n + g] - 0.5 * (v1 * v1 + v2 * v2 + v3 * v3);
-    } else {
-      gas_energy = A[S_GE * n + g];
-    }
+    gas_energy = X.idual == 0 ? e0 - 0.5 * (v1 * v1 + v2 * v2 + v3 * v3) : ge0;
   }
   double p = (X.gamma - 1.0) * density * gas_energy;
   if (p < X.tiny_number) p = X.tiny_number;
   // expand_terms.F:66-166
   const double cf = X.Coefficient;
-  if (X.idual == 1) A[S_GE * n + g] = A[S_GE * n + g] * (1.0 - cf) / (1.0 + cf);
-  double e = A[S_E * n + g] * (1.0 - cf) / (1.0 + cf);
+  if (X.idual == 1) A[S_GE * n + g] = ge0 * (1.0 - cf) / (1.0 + cf);
+  double e = e0 * (1.0 - cf) / (1.0 + cf);
   e = fmx(e - cf * (3.0 - 2.0 / (X.gamma - 1.0)) * p / d, 0.5 * e);
   A[S_E * n + g] = e;
-  A[S_VX * n + g] = A[S_VX * n + g] * (1.0 - 0.5 * cf) / (1.0 + 0.5 * cf);
-  if (X.rank > 1) A[S_VY * n + g] = A[S_VY * n + g] * (1.0 - 0.5 * cf) / (1.0 + 0.5 * cf);
-  if (X.rank > 2) A[S_VZ * n + g] = A[S_VZ * n + g] * (1.0 - 0.5 * cf) / (1.0 + 0.5 * cf);
+  A[S_VX * n + g] = v1 * (1.0 - 0.5 * cf) / (1.0 + 0.5 * cf);
+  if (X.rank > 1) A[S_VY * n + g] = v2 * (1.0 - 0.5 * cf) / (1.0 + 0.5 * cf);
+  if (X.rank > 2) A[S_VZ * n + g] = v3 * (1.0 - 0.5 * cf) / (1.0 + 0.5 * cf);
 }
 
 // ------------------------------------------------------------------------
