@@ -236,7 +236,12 @@ __device__ inline void block_min_dt(double local, unsigned long long *dtmin) {
   if (threadIdx.x == 0) {
     double m = wmin[0];
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmn(m, wmin[w]);
-    atomicMin(dtmin, (unsigned long long)__double_as_longlong(m));
+    // the device-wide minimum only decreases: a workgroup whose minimum is
+    // not below the value it reads skips the atomic (one address, updated
+    // by every workgroup of the launch, serialises them)
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+    if (bits < __hip_atomic_load(dtmin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMin(dtmin, bits);
   }
 }
 

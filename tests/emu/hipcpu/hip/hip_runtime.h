@@ -340,6 +340,10 @@ template <class T>
 inline T atomicMin(T *p, T v) { T o = *p; if (v < o) *p = v; return o; }
 template <class T>
 inline T atomicExch(T *p, T v) { T o = *p; *p = v; return o; }
+// work-items run one at a time: a relaxed atomic load is a plain load
+#define __HIP_MEMORY_SCOPE_AGENT 3
+template <class T>
+inline T __hip_atomic_load(T *p, int, int) { return *p; }
 
 inline long long __double_as_longlong(double d) { long long r; memcpy(&r, &d, 8); return r; }
 inline double __longlong_as_double(long long l) { double r; memcpy(&r, &l, 8); return r; }
